@@ -1,0 +1,218 @@
+/*
+ * isce3_amd -- MI355X-native time-domain backprojection, C ABI.
+ *
+ * This is the drop-in boundary for isce3's backprojection focuser.  Plain C
+ * types only: pointers, sizes and POD structs; no C++ or torch types.  All
+ * times are seconds relative to the orbit's reference epoch (the caller
+ * aligns radar-grid epochs to the orbit epoch, as RadarGeometry does in
+ * cxx/isce3/container/RadarGeometry.icc:15-26).
+ *
+ * Return codes: values 0..11 mirror isce3::error::ErrorCode
+ * (cxx/isce3/error/ErrorCode.h:8-21).  Negative values are argument/runtime
+ * errors that a host binding raises as exceptions; bp_last_error_message()
+ * returns the message for the calling thread.
+ *
+ * Threading: every call is synchronous with respect to the host unless a
+ * non-null HIP stream is passed, in which case device work is enqueued on
+ * that stream and the call returns after enqueueing (outputs that are read
+ * back to the host, such as any_failed and the update count, are then
+ * synchronised on that stream before return).  A context may be used from
+ * one host thread at a time.
+ */
+#ifndef ISCE3_AMD_BACKPROJECT_H
+#define ISCE3_AMD_BACKPROJECT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+enum {
+    BP_SUCCESS = 0,
+    BP_ORBIT_INTERP_SIZE_ERROR = 1,
+    BP_ORBIT_INTERP_DOMAIN_ERROR = 2,
+    BP_ORBIT_INTERP_UNKNOWN_METHOD = 3,
+    BP_OUT_OF_BOUNDS_DEM = 4,
+    BP_INVALID_DEM = 5,
+    BP_FAILED_TO_CONVERGE = 6,
+    BP_WRONG_LOOK_SIDE = 7,
+    BP_OUT_OF_BOUNDS_LOOKUP = 8,
+    BP_NULL_DEREFERENCE = 9,
+    BP_INVALID_TOLERANCE = 10,
+    BP_INVALID_INTERVAL = 11,
+    BP_ERR_INVALID_ARGUMENT = -1, /* isce3::except::InvalidArgument */
+    BP_ERR_OUT_OF_RANGE = -2,     /* isce3::except::OutOfRange */
+    BP_ERR_DOMAIN = -3,           /* isce3::except::DomainError */
+    BP_ERR_HIP = -4,              /* HIP runtime failure */
+    BP_ERR_RUNTIME = -5,          /* isce3::except::RuntimeError */
+    BP_ERR_OUT_OF_MEMORY = -6
+};
+
+enum { BP_INTERP_HERMITE = 0, BP_INTERP_LEGENDRE = 1 };   /* OrbitInterpMethod */
+enum { BP_LOOK_LEFT = 1, BP_LOOK_RIGHT = -1 };            /* isce3 LookSide */
+enum { BP_TROPO_NODELAY = 0, BP_TROPO_TSX = 1 };          /* DryTroposphereModel */
+enum { BP_DEM_BILINEAR = 0 };                             /* dataInterpMethod */
+enum {
+    BP_PRECISION_PLAIN = 0, /* FP64 delay/phase/taps index, complex<double> sum */
+    BP_PRECISION_SPLIT = 1  /* FP64 per-pixel reference + FP32 deltas, Kahan sum */
+};
+
+/* Uniformly spaced orbit state vectors (isce3::core::Orbit, core/Orbit.h). */
+typedef struct {
+    double t0;          /* time of first state vector (s) */
+    double dt;          /* state vector spacing (s) */
+    int32_t n;          /* number of state vectors */
+    int32_t method;     /* BP_INTERP_* */
+    const double* pos;  /* host, n x 3 ECEF (m) */
+    const double* vel;  /* host, n x 3 ECEF (m/s) */
+} bp_orbit;
+
+/* Bilinear LUT2d<double> (core/LUT2d.h); have_data == 0 -> constant ref_value. */
+typedef struct {
+    int32_t have_data;
+    int32_t bounds_error;
+    double ref_value;
+    double xstart, ystart, dx, dy;  /* x = slant range (m), y = azimuth time (s) */
+    int64_t width, length;
+    const double* data;             /* host, length x width */
+} bp_lut2d;
+
+/* RadarGridParameters subset used by the path (product/RadarGridParameters.h). */
+typedef struct {
+    double sensing_start;        /* s, relative to the orbit epoch */
+    double prf;                  /* Hz; azimuth spacing is 1/prf */
+    double starting_range;       /* m */
+    double range_pixel_spacing;  /* m */
+    int64_t length;              /* lines (azimuth) */
+    int64_t width;               /* samples (range) */
+    int32_t look_side;           /* BP_LOOK_* */
+    int32_t pad_;
+} bp_grid;
+
+/* isce3::container::RadarGeometry: grid + orbit + Doppler. */
+typedef struct {
+    bp_grid grid;
+    bp_orbit orbit;
+    bp_lut2d doppler;
+} bp_radar_geometry;
+
+/* DEMInterpolator (geometry/DEMInterpolator.h): constant height or an
+ * EPSG:4326 float raster with bilinear interpolation. */
+typedef struct {
+    int32_t have_raster;
+    int32_t epsg;
+    int32_t method;      /* BP_DEM_BILINEAR */
+    int32_t pad_;
+    double ref_height;
+    double xstart, ystart, dx, dy;  /* degrees */
+    int64_t width, length;
+    const float* data;   /* host, length x width */
+} bp_dem;
+
+/* TabulatedKernel<float> (core/Kernels.h:133-151). */
+typedef struct {
+    double halfwidth;    /* kernel.width() / 2 */
+    int32_t n;           /* table size (>= 2) */
+    int32_t pad_;
+    const float* table;  /* host, n samples of K(x), x in [0, halfwidth] */
+} bp_kernel;
+
+/* Rdr2GeoBracketParams (geometry/detail/Rdr2Geo.h:87-97). */
+typedef struct {
+    double tol_height, look_min, look_max;
+} bp_rdr2geo_params;
+
+/* Geo2RdrBracketParams (geometry/detail/Geo2Rdr.h:57-68); the optional
+ * times are present when have_* is nonzero. */
+typedef struct {
+    double tol_aztime;
+    int32_t have_time_start, have_time_end;
+    double time_start, time_end;
+} bp_geo2rdr_params;
+
+/* Launch/tiling options; zero-initialise for defaults. */
+typedef struct {
+    int32_t tile_rows;     /* output rows per 64-pixel wavefront tile: 1,2,4,8 (0 = auto) */
+    int32_t chunk;         /* pulses per compensated FP32 partial sum (split; 0 = 32) */
+    int32_t fit_degree;    /* split: delay polynomial degree 2..6 (0 = 4) */
+    int32_t reserved[5];
+} bp_options;
+
+/* Optional per-call measurements (filled when non-null). */
+typedef struct {
+    int64_t updates;       /* sum over pixels of (kstop - kstart) */
+    double geometry_ms;    /* device time of the geometry pass (HIP events) */
+    double sum_ms;         /* device time of the coherent-sum kernel */
+    int32_t any_failed;    /* some pixel failed to converge */
+    int32_t pad_;
+} bp_stats;
+
+typedef struct bp_context bp_context;
+
+/* Create a context bound to a HIP device (holds device scratch buffers). */
+int bp_context_create(int device, bp_context** ctx);
+int bp_context_destroy(bp_context* ctx);
+
+/* Thread-local message of the last failing call. */
+const char* bp_last_error_message(void);
+
+/*
+ * Focus in azimuth by time-domain backprojection.
+ *
+ * Replaces isce3::focus::backproject (cxx/isce3/focus/Backproject.h:37-47,
+ * Backproject.cpp:66-212) and isce3::cuda::focus::backproject
+ * (cxx/isce3/cuda/focus/Backproject.h:45-56).
+ *
+ *   in_dev      device, complex64 [in.grid.length x in.grid.width], row-major
+ *   out_dev     device, complex64 [out.grid.length x out.grid.width], written
+ *   height_dev  device, float32 [out shape] or NULL
+ *   any_failed  host int, set nonzero when some pixel failed to converge
+ *               (those pixels are NaN), as the reference's return value
+ *   stream      hipStream_t or NULL for the null stream (synchronous)
+ *
+ * Returns BP_SUCCESS / BP_FAILED_TO_CONVERGE, or a negative error code.
+ */
+int bp_backproject(bp_context* ctx, const bp_radar_geometry* out_geometry,
+                   const bp_radar_geometry* in_geometry, const bp_dem* dem, double fc,
+                   double ds, const bp_kernel* kernel, int dry_tropo_model,
+                   const bp_rdr2geo_params* rdr2geo, const bp_geo2rdr_params* geo2rdr,
+                   const void* in_dev, void* out_dev, float* height_dev, int precision,
+                   const bp_options* options, void* stream, int* any_failed,
+                   bp_stats* stats);
+
+/* ---------------------------------------------------- host-side helpers
+ * Host implementations of the same FP64 primitives the kernels use; they back
+ * the Python mirror's Orbit.interpolate, Ellipsoid and DEM/LUT lookups. */
+
+/* isce3::core::Orbit::interpolate (core/Orbit.cpp:71-86); border: 0 error,
+ * 1 extrapolate, 2 fill NaN.  Returns an ErrorCode value. */
+int bp_orbit_interpolate(const bp_orbit* orbit, double t, int border, double* pos,
+                         double* vel);
+/* Ellipsoid::lonLatToXyz / xyzToLonLat (core/Ellipsoid.h:176-224). */
+void bp_lonlat_to_xyz(double a, double e2, const double* llh, double* xyz);
+void bp_xyz_to_lonlat(double a, double e2, const double* xyz, double* llh);
+/* DEMInterpolator::interpolateLonLat / interpolateXY (DEMInterpolator.cpp:593-662). */
+double bp_dem_interpolate_lonlat(const bp_dem* dem, double lon, double lat);
+double bp_dem_interpolate_xy(const bp_dem* dem, double x, double y);
+/* LUT2d<double>::eval (core/LUT2d.cpp:127-159). */
+double bp_lut2d_eval(const bp_lut2d* lut, double y, double x);
+/* KnabKernel<double>::operator() (core/Kernels.icc:29-52). */
+double bp_knab_eval(double width, double bandwidth, double x);
+/* TabulatedKernel<float> table fill from a Knab kernel (core/Kernels.icc:113-135). */
+int bp_tabulate_knab(double width, double bandwidth, int n, float* table);
+/* rdr2geo_bracket / geo2rdr_bracket on the host (geometry/rdr2geo_roots.h:34-40,
+ * geometry/geo2rdr_roots.h:28-33); return ErrorCode values. */
+int bp_rdr2geo_bracket(double aztime, double slant_range, double doppler,
+                       const bp_orbit* orbit, const bp_dem* dem, double wavelength,
+                       int side, const bp_rdr2geo_params* params, double* xyz);
+int bp_geo2rdr_bracket(const double* xyz, const bp_orbit* orbit, const bp_lut2d* doppler,
+                       double wavelength, int side, const bp_geo2rdr_params* params,
+                       double* aztime, double* range);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISCE3_AMD_BACKPROJECT_H */

@@ -1,0 +1,110 @@
+// Internal device-side data layout shared by the geometry pass, the
+// coherent-sum kernels and the host launcher.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bp_math.h"
+
+namespace bp {
+
+// Per-pixel status values written by the geometry pass.
+enum PixStatus : int32_t {
+    kPixOk = 0,
+    kPixRdr2GeoFailed = 1,   // output NaN, height NaN (Backproject.cpp:146-152)
+    kPixGeo2RdrFailed = 2,   // output NaN, height valid (Backproject.cpp:167-170)
+    kPixOutOfRange = 3,      // orbit domain error; the call raises OutOfRange
+    kPixOutside = 4,         // padding lane outside the output grid
+};
+
+// Result of the per-pixel geometry pass consumed by the plain FP64 sum:
+// target ECEF position, dry-troposphere delay and pulse window [kstart, kstop).
+struct alignas(16) PixGeom {
+    double x[3];
+    double tau_atm;
+    int32_t kstart, kstop;
+    int32_t status;
+    int32_t pad_;
+};
+
+// Per-pixel record of the split-precision sum.  The pulse phase in cycles is
+// c_ref + sum_{n=1..4} b[n-1] s^n with s = (k - kc) * inv_h (FP64 Horner on
+// the FP64 fitted coefficients); the fractional range-bin position is
+// f_ref + (phase - c_ref) * inv_k evaluated in FP32, relative to bin n_ref.
+struct alignas(16) PixSplit {
+    double b[4];
+    double c_ref;
+    double kc;
+    double inv_h;
+    float f_ref;
+    int32_t n_ref;
+    int32_t kstart, kstop;
+    int32_t status;
+    int32_t pad_;
+};
+
+constexpr int kFitDegree = 4;        // delay polynomial degree of the split path
+constexpr int kPolyphaseRows = 512;  // fractional-offset rows of the split weight table
+constexpr int kMaxTableLds = 8192;   // plain path keeps tables up to this size in LDS
+
+// Pixel tiling: each 64-lane wavefront owns a tile of tile_rows x (64/tile_rows)
+// output pixels; four wavefronts form a 256-thread workgroup.
+struct TileMap {
+    int32_t tile_rows, tile_cols;
+    int32_t tiles_x, tiles_y;
+    int64_t length, width;      // output grid
+    int64_t n_tiles;
+    int32_t n_blocks;
+    int32_t pad_;
+};
+
+struct GeomArgs {
+    TileMap map;
+    double out_t0, out_dt, out_r0, out_dr;
+    int32_t out_side, in_side;
+    double in_t0, in_dt;
+    int32_t na, tropo;
+    double swst, dtau;
+    double wvl, fc, ds;
+    Orbit out_orbit, in_orbit;  // device pointers
+    Lut2d out_dop, in_dop;      // device pointers
+    Dem dem;                    // device pointers
+    R2GParams r2g;
+    G2RParams g2r;
+    int32_t want_split;
+    int32_t pad_;
+    PixGeom* geom;              // plain records (always written)
+    PixSplit* split;            // split records (when want_split)
+    float* height;              // optional user height buffer
+    int32_t* flags;             // [0] any failed, [1] out of range
+    unsigned long long* updates;
+};
+
+struct SumArgs {
+    TileMap map;
+    int64_t nr;                 // range samples per pulse
+    double swst, dtau, fc;
+    double inv_k;               // 1 / (fc * dtau): cycles -> range bins
+    double halfwidth;           // kernel half width
+    float rdx;                  // float(1/dx) of the tabulated kernel
+    int32_t ntab;
+    int32_t chunk;
+    int32_t pad_;
+    const float2* rc;           // range-compressed pulses [na x nr]
+    const double* pulse_pv;     // per-pulse position/velocity [na x 6]
+    const PixGeom* geom;
+    const PixSplit* split;
+    const float* table;         // tabulated kernel (plain)
+    const float* poly;          // polyphase weights (split): (rows+1) x (2W)
+    float2* out;
+};
+
+// Host launchers (defined in bp_kernels.hip).
+hipError_t launch_pulse_orbit(const Orbit& orbit, double t0, double dt, int na, double* pv,
+                              int32_t* flags, hipStream_t stream);
+hipError_t launch_geometry(const GeomArgs& args, hipStream_t stream);
+hipError_t launch_sum_plain(const SumArgs& args, int width, hipStream_t stream);
+hipError_t launch_sum_split(const SumArgs& args, int width, hipStream_t stream);
+
+}  // namespace bp

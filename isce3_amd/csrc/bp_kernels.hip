@@ -1,0 +1,481 @@
+// HIP kernels of the backprojection path for gfx950 (wave64):
+//   pulse_orbit_kernel  FP64 platform position/velocity at every pulse
+//   geometry_kernel     FP64 per-pixel rdr2geo/geo2rdr, CPI window, TSX delay,
+//                       and (split mode) the per-pixel delay polynomial fit
+//   sum_plain_kernel    reference-semantics coherent sum, FP64 delay & phase
+//   sum_split_kernel    split-precision coherent sum: FP64 polynomial phase,
+//                       FP32 range-bin delta, polyphase weights from LDS,
+//                       FP32 complex MACs with Kahan-compensated chunk sums
+#include <hip/hip_runtime.h>
+#include <limits.h>
+
+#include "bp_internal.h"
+
+namespace bp {
+namespace {
+
+__device__ inline int wave_min_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline int wave_max_i(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline long long wave_sum_ll(long long v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
+// XCD (own L2) walks a contiguous run of tiles, which share pulse data.
+__device__ inline int64_t wave_tile(const TileMap& m, int* lane)
+{
+    const int nb = m.n_blocks;
+    const int b = blockIdx.x;
+    const int per = nb / 8, rem = nb % 8;
+    const int xcd = b % 8, idx = b / 8;
+    const int lb = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    *lane = threadIdx.x & 63;
+    return (int64_t)lb * 4 + (threadIdx.x >> 6);
+}
+
+__device__ inline bool tile_pixel(const TileMap& m, int64_t tile, int lane, int64_t* row,
+                                  int64_t* col)
+{
+    if (tile >= m.n_tiles) return false;
+    const int64_t ty = tile / m.tiles_x, tx = tile % m.tiles_x;
+    *row = ty * m.tile_rows + lane / m.tile_cols;
+    *col = tx * m.tile_cols + lane % m.tile_cols;
+    return *row < m.length && *col < m.width;
+}
+
+// TabulatedKernel<float>::operator() with the reference's FP64 index math
+// (core/Kernels.icc:137-154).
+__device__ inline float tab_eval(const float* tab, int ntab, double hw, float rdx, double x)
+{
+    const double ax = fabs(x);
+    if (ax > hw) return 0.f;
+    const double axn = ax * (double)rdx;
+    int i = (int)floor(axn);
+    i = min(i, ntab - 2);
+    const float t0 = tab[i];
+    const float dt = tab[i + 1] - t0;
+    return (float)((double)t0 + (axn - i) * (double)dt);
+}
+
+}  // namespace
+
+__global__ void pulse_orbit_kernel(Orbit orbit, double t0, double dt, int na, double* pv,
+                                   int32_t* flags)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= na) return;
+    V3 p, v;
+    const int st = orbit_interpolate(orbit, t0 + k * dt, kBorderError, &p, &v);
+    if (st != kSuccess) atomicOr(&flags[1], 1);
+    double* o = pv + 6 * (int64_t)k;
+    o[0] = p.x;
+    o[1] = p.y;
+    o[2] = p.z;
+    o[3] = v.x;
+    o[4] = v.y;
+    o[5] = v.z;
+}
+
+// Per-pixel geometry, following Backproject.cpp:127-199.
+__global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
+{
+    int lane;
+    const int64_t tile = wave_tile(a.map, &lane);
+    int64_t row = 0, col = 0;
+    const bool valid = tile_pixel(a.map, tile, lane, &row, &col);
+    const int64_t pix = row * a.map.width + col;
+
+    PixGeom g;
+    g.x[0] = g.x[1] = g.x[2] = qnan();
+    g.tau_atm = 0.;
+    g.kstart = 0;
+    g.kstop = 0;
+    g.status = kPixOutside;
+    g.pad_ = 0;
+    V3 x{qnan(), qnan(), qnan()};
+    V3 p{0., 0., 0.};
+    double t = 0., r = 0.;
+
+    if (valid) {
+        const double ta = a.out_t0 + row * a.out_dt;
+        const double ra = a.out_r0 + col * a.out_dr;
+        const double fd = lut2d_eval(a.out_dop, ta, ra);
+        int st = rdr2geo_bracket(ta, ra, fd, a.out_orbit, a.dem, a.wvl, a.out_side, a.r2g, &x);
+        if (st == kOrbitInterpDomainError || st == kOrbitInterpSizeError ||
+            st == kOrbitInterpUnknownMethod) {
+            atomicOr(&a.flags[1], 1);
+            g.status = kPixOutOfRange;
+        } else {
+            const V3 llh = xyz_to_lonlat(kWgsA, kWgsE2, x);
+            if (a.height) a.height[pix] = (float)llh.z;
+            if (st != kSuccess) {
+                g.status = kPixRdr2GeoFailed;
+                if (a.height) a.height[pix] = __builtin_nanf("");
+            } else {
+                int st2 = geo2rdr_bracket(x, a.in_orbit, a.in_dop, a.wvl, a.in_side, a.g2r, &t, &r);
+                if (st2 != kSuccess) {
+                    g.status = kPixGeo2RdrFailed;
+                } else {
+                    V3 v;
+                    if (orbit_interpolate(a.in_orbit, t, kBorderError, &p, &v) != kSuccess) {
+                        atomicOr(&a.flags[1], 1);
+                        g.status = kPixOutOfRange;
+                    } else {
+                        const double l = a.wvl * r * (norm(p) / norm(x)) / (2. * a.ds);
+                        const double cpi = l / norm(v);
+                        const double tstart = t - 0.5 * cpi;
+                        const double tstop = t + 0.5 * cpi;
+                        int kstart = (int)floor((tstart - a.in_t0) / a.in_dt);
+                        int kstop = (int)ceil((tstop - a.in_t0) / a.in_dt);
+                        kstart = max(kstart, 0);
+                        kstop = min(kstop, a.na);
+                        g.kstart = kstart;
+                        g.kstop = kstop;
+                        if (a.tropo == 1) g.tau_atm = dry_tropo_tsx(p, llh);
+                        g.status = kPixOk;
+                    }
+                }
+            }
+            if (g.status == kPixRdr2GeoFailed || g.status == kPixGeo2RdrFailed)
+                atomicOr(&a.flags[0], 1);
+        }
+        g.x[0] = x.x;
+        g.x[1] = x.y;
+        g.x[2] = x.z;
+        a.geom[pix] = g;
+    }
+
+    const bool has_win = valid && g.status == kPixOk && g.kstop > g.kstart;
+    const long long nupd = has_win ? (long long)(g.kstop - g.kstart) : 0;
+    const long long wsum = wave_sum_ll(nupd);
+    if (lane == 0 && wsum > 0) atomicAdd(a.updates, (unsigned long long)wsum);
+
+    if (!a.want_split) return;
+
+    // Split mode: fit the pulse phase (cycles) over the wavefront's union
+    // window with a degree-4 Chebyshev interpolant evaluated in FP64.
+    const int kw0 = wave_min_i(has_win ? g.kstart : INT_MAX);
+    const int kw1 = wave_max_i(has_win ? g.kstop : INT_MIN);
+    if (!valid) return;
+    PixSplit sp;
+    sp.b[0] = sp.b[1] = sp.b[2] = sp.b[3] = 0.;
+    sp.c_ref = 0.;
+    sp.kc = 0.;
+    sp.inv_h = 0.;
+    sp.f_ref = 0.f;
+    sp.n_ref = 0;
+    sp.kstart = has_win ? g.kstart : 0;
+    sp.kstop = has_win ? g.kstop : 0;
+    sp.status = g.status;
+    sp.pad_ = 0;
+    if (has_win) {
+        const double kc = 0.5 * ((double)kw0 + (double)(kw1 - 1));
+        const double h = fmax(0.5 * (double)(kw1 - 1 - kw0), 0.5);
+        sp.kc = kc;
+        sp.inv_h = 1.0 / h;
+        // delay at the reference pulse kc
+        V3 pc, vc;
+        orbit_interpolate(a.in_orbit, a.in_t0 + kc * a.in_dt, kBorderExtrapolate, &pc, &vc);
+        const double tau_c = g.tau_atm + bistatic_delay(pc, vc, x);
+        constexpr int M = kFitDegree + 1;
+        double gm[M], sm[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            sm[m] = cos(kPi * (m + 0.5) / M);
+            const double km = kc + h * sm[m];
+            V3 pm, vm;
+            orbit_interpolate(a.in_orbit, a.in_t0 + km * a.in_dt, kBorderExtrapolate, &pm, &vm);
+            const double tau_m = g.tau_atm + bistatic_delay(pm, vm, x);
+            gm[m] = a.fc * (tau_m - tau_c);  // cycles relative to the reference pulse
+        }
+        // Chebyshev coefficients c_j = 2/M sum_m g_m T_j(s_m), c_0 halved.
+        double cc[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            double acc = 0.;
+#pragma unroll
+            for (int m = 0; m < M; ++m) acc += gm[m] * cos(j * kPi * (m + 0.5) / M);
+            cc[j] = acc * (2.0 / M);
+        }
+        cc[0] *= 0.5;
+        // Chebyshev -> monomial: T_{j+1} = 2 s T_j - T_{j-1}
+        double mono[M], tjm1[M], tj[M], tjp1[M];
+#pragma unroll
+        for (int n = 0; n < M; ++n) {
+            mono[n] = 0.;
+            tjm1[n] = 0.;
+            tj[n] = 0.;
+        }
+        tjm1[0] = 1.;  // T_0
+        tj[1] = 1.;    // T_1
+        mono[0] += cc[0];
+        mono[1] += cc[1];
+#pragma unroll
+        for (int j = 1; j < M - 1; ++j) {
+#pragma unroll
+            for (int n = 0; n < M; ++n) tjp1[n] = -tjm1[n] + (n > 0 ? 2.0 * tj[n - 1] : 0.);
+#pragma unroll
+            for (int n = 0; n < M; ++n) {
+                mono[n] += cc[j + 1] * tjp1[n];
+                tjm1[n] = tj[n];
+                tj[n] = tjp1[n];
+            }
+        }
+        sp.b[0] = mono[1];
+        sp.b[1] = mono[2];
+        sp.b[2] = mono[3];
+        sp.b[3] = mono[4];
+        // Reference phase (cycles, reduced) and range-bin position at s = 0.
+        const double cyc_c = a.fc * tau_c;
+        sp.c_ref = (cyc_c - floor(cyc_c)) + mono[0];
+        const double u_c = (tau_c - a.swst) / a.dtau + mono[0] / (a.fc * a.dtau);
+        const double n_ref = floor(u_c);
+        sp.n_ref = (int32_t)n_ref;
+        sp.f_ref = (float)(u_c - n_ref);
+    }
+    a.split[pix] = sp;
+}
+
+// Reference-semantics coherent sum (Backproject.cpp:30-63): FP64 bistatic
+// delay, range-bin position and phase per pulse; float tabulated taps with
+// FP64 index math and a complex<float> inner product; complex<double> sum.
+template<int W>
+__global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
+{
+    extern __shared__ float s_tab[];
+    const bool tab_in_lds = a.ntab <= kMaxTableLds;
+    if (tab_in_lds)
+        for (int i = threadIdx.x; i < a.ntab; i += blockDim.x) s_tab[i] = a.table[i];
+    __syncthreads();
+    const float* tab = tab_in_lds ? s_tab : a.table;
+
+    int lane;
+    const int64_t tile = wave_tile(a.map, &lane);
+    int64_t row = 0, col = 0;
+    const bool valid = tile_pixel(a.map, tile, lane, &row, &col);
+    const int64_t pix = row * a.map.width + col;
+
+    PixGeom g;
+    if (valid) g = a.geom[pix];
+    const bool ok = valid && g.status == kPixOk;
+    const int ks = ok ? g.kstart : INT_MAX;
+    const int ke = ok ? g.kstop : INT_MIN;
+    const int kw0 = __builtin_amdgcn_readfirstlane(wave_min_i(ks));
+    const int kw1 = __builtin_amdgcn_readfirstlane(wave_max_i(ke));
+    const V3 x{g.x[0], g.x[1], g.x[2]};
+    const double tau_atm = ok ? g.tau_atm : 0.;
+
+    double acc_re = 0., acc_im = 0.;
+    for (int k = kw0; k < kw1; ++k) {
+        if (k < ks || k >= ke) continue;
+        const double* pv = a.pulse_pv + 6 * (int64_t)k;
+        const V3 p{pv[0], pv[1], pv[2]};
+        const V3 v{pv[3], pv[4], pv[5]};
+        const double tau = tau_atm + bistatic_delay(p, v, x);
+        const double u = (tau - a.swst) / a.dtau;
+        const long i0 = (W % 2 == 0) ? (long)ceil(u) : (long)round(u);
+        const long low = i0 - W / 2;
+        const float2* line = a.rc + (int64_t)k * a.nr;
+        float sre = 0.f, sim = 0.f;
+        if (low >= 0 && low + W <= a.nr) {
+#pragma unroll
+            for (int m = 0; m < W; ++m) {
+                const float w = tab_eval(tab, a.ntab, a.halfwidth, a.rdx, (double)(m + low) - u);
+                const float2 d = line[low + m];
+                sre += w * d.x;
+                sim += w * d.y;
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < W; ++m) {
+                const long j = low + m;
+                const float w = tab_eval(tab, a.ntab, a.halfwidth, a.rdx, (double)j - u);
+                const float2 d = (j >= 0 && j < a.nr) ? line[j] : make_float2(0.f, 0.f);
+                sre += w * d.x;
+                sim += w * d.y;
+            }
+        }
+        double sn, cs;
+        sincospi(2.0 * a.fc * tau, &sn, &cs);
+        acc_re += (double)sre * cs - (double)sim * sn;
+        acc_im += (double)sre * sn + (double)sim * cs;
+    }
+    if (valid) {
+        const float nanf_ = __builtin_nanf("");
+        a.out[pix] = ok ? make_float2((float)acc_re, (float)acc_im) : make_float2(nanf_, nanf_);
+    }
+}
+
+// Split-precision coherent sum.  Per pulse: FP64 Horner of the fitted phase
+// polynomial (full-rate FP64 FMA on CDNA), FP32 range-bin position, weights
+// linearly interpolated between polyphase rows held in LDS, FP32 complex MACs
+// and rotation, FP32 chunk sums folded into a Kahan-compensated accumulator.
+template<int W>
+__global__ __launch_bounds__(256) void sum_split_kernel(SumArgs a)
+{
+    constexpr int WP = (W + 3) / 4 * 4;  // padded tap count (float4 rows)
+    constexpr int ROW = 2 * WP;          // weights then deltas
+    constexpr int Q = kPolyphaseRows;
+    constexpr float TMIN = (W % 2 == 0) ? 0.0f : -0.5f;
+    extern __shared__ float4 s_poly4[];
+    float* s_poly = reinterpret_cast<float*>(s_poly4);
+    for (int i = threadIdx.x; i < Q * ROW; i += blockDim.x) s_poly[i] = a.poly[i];
+    __syncthreads();
+
+    int lane;
+    const int64_t tile = wave_tile(a.map, &lane);
+    int64_t row = 0, col = 0;
+    const bool valid = tile_pixel(a.map, tile, lane, &row, &col);
+    const int64_t pix = row * a.map.width + col;
+
+    PixSplit sp;
+    if (valid) sp = a.split[pix];
+    const bool ok = valid && sp.status == kPixOk;
+    const int ks = (ok && sp.kstop > sp.kstart) ? sp.kstart : INT_MAX;
+    const int ke = (ok && sp.kstop > sp.kstart) ? sp.kstop : INT_MIN;
+    const int kw0 = __builtin_amdgcn_readfirstlane(wave_min_i(ks));
+    const int kw1 = __builtin_amdgcn_readfirstlane(wave_max_i(ke));
+    const double b0 = ok ? sp.b[0] : 0., b1 = ok ? sp.b[1] : 0.;
+    const double b2 = ok ? sp.b[2] : 0., b3 = ok ? sp.b[3] : 0.;
+    const double c_ref = ok ? sp.c_ref : 0., kc = ok ? sp.kc : 0., inv_h = ok ? sp.inv_h : 0.;
+    const float f_ref = ok ? sp.f_ref : 0.f;
+    const int n_ref = ok ? sp.n_ref : 0;
+    const double inv_k = a.inv_k;
+    const int64_t nr = a.nr;
+    const int chunk = a.chunk;
+
+    float acc_re = 0.f, acc_im = 0.f, cmp_re = 0.f, cmp_im = 0.f;
+    for (int k0 = kw0; k0 < kw1; k0 += chunk) {
+        const int kend = min(k0 + chunk, kw1);
+        float cre = 0.f, cim = 0.f;
+        for (int k = k0; k < kend; ++k) {
+            if (k < ks || k >= ke) continue;
+            const double s = ((double)k - kc) * inv_h;
+            const double dc = fma(fma(fma(b3, s, b2), s, b1), s, b0) * s;
+            double cy = c_ref + dc;
+            cy -= rint(cy);
+            const float ph = (float)cy;  // revolutions in [-0.5, 0.5]
+            const float ul = f_ref + (float)(dc * inv_k);
+            const float fi = (W % 2 == 0) ? ceilf(ul) : roundf(ul);
+            const float t0 = fi - ul;
+            const int c0 = n_ref + (int)fi - W / 2;
+            const float q = (t0 - TMIN) * (float)Q;
+            const int qi = max(min((int)q, Q - 1), 0);
+            const float qf = q - (float)qi;
+            const float* rowp = s_poly + qi * ROW;
+            float w[W];
+#pragma unroll
+            for (int m = 0; m < W; ++m) w[m] = fmaf(qf, rowp[WP + m], rowp[m]);
+            const float2* line = a.rc + (int64_t)k * nr;
+            float sre = 0.f, sim = 0.f;
+            if (c0 >= 0 && c0 + W <= nr) {
+#pragma unroll
+                for (int m = 0; m < W; ++m) {
+                    const float2 d = line[c0 + m];
+                    sre = fmaf(w[m], d.x, sre);
+                    sim = fmaf(w[m], d.y, sim);
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < W; ++m) {
+                    const int64_t j = (int64_t)c0 + m;
+                    const float2 d = (j >= 0 && j < nr) ? line[j] : make_float2(0.f, 0.f);
+                    sre = fmaf(w[m], d.x, sre);
+                    sim = fmaf(w[m], d.y, sim);
+                }
+            }
+            const float sn = __builtin_amdgcn_sinf(ph);
+            const float cs = __builtin_amdgcn_cosf(ph);
+            cre += sre * cs - sim * sn;
+            cim += sre * sn + sim * cs;
+        }
+        // Kahan-compensated fold of the chunk sum.
+        const float yr = cre - cmp_re;
+        const float tr = acc_re + yr;
+        cmp_re = (tr - acc_re) - yr;
+        acc_re = tr;
+        const float yi = cim - cmp_im;
+        const float ti = acc_im + yi;
+        cmp_im = (ti - acc_im) - yi;
+        acc_im = ti;
+    }
+    if (valid) {
+        const float nanf_ = __builtin_nanf("");
+        a.out[pix] = ok ? make_float2(acc_re - cmp_re, acc_im - cmp_im) : make_float2(nanf_, nanf_);
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+
+hipError_t launch_pulse_orbit(const Orbit& orbit, double t0, double dt, int na, double* pv,
+                              int32_t* flags, hipStream_t stream)
+{
+    if (na <= 0) return hipSuccess;
+    const int threads = 256;
+    const int blocks = (na + threads - 1) / threads;
+    hipLaunchKernelGGL(pulse_orbit_kernel, dim3(blocks), dim3(threads), 0, stream, orbit, t0, dt,
+                       na, pv, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_geometry(const GeomArgs& args, hipStream_t stream)
+{
+    if (args.map.n_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(geometry_kernel, dim3(args.map.n_blocks), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_plain(const SumArgs& args, int width, hipStream_t stream)
+{
+    if (args.map.n_blocks <= 0) return hipSuccess;
+    const size_t lds = args.ntab <= kMaxTableLds ? sizeof(float) * args.ntab : 0;
+    switch (width) {
+    case 8:
+        hipLaunchKernelGGL(sum_plain_kernel<8>, dim3(args.map.n_blocks), dim3(256), lds, stream, args);
+        break;
+    case 9:
+        hipLaunchKernelGGL(sum_plain_kernel<9>, dim3(args.map.n_blocks), dim3(256), lds, stream, args);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+size_t split_lds_bytes(int width)
+{
+    const int wp = (width + 3) / 4 * 4;
+    return sizeof(float) * (size_t)kPolyphaseRows * 2 * wp;
+}
+
+hipError_t launch_sum_split(const SumArgs& args, int width, hipStream_t stream)
+{
+    if (args.map.n_blocks <= 0) return hipSuccess;
+    const size_t lds = split_lds_bytes(width);
+    switch (width) {
+    case 8:
+        hipLaunchKernelGGL(sum_split_kernel<8>, dim3(args.map.n_blocks), dim3(256), lds, stream, args);
+        break;
+    case 9:
+        hipLaunchKernelGGL(sum_split_kernel<9>, dim3(args.map.n_blocks), dim3(256), lds, stream, args);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace bp

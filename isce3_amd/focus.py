@@ -1,0 +1,264 @@
+"""Drop-in ``backproject`` for isce3.focus.backproject / isce3.cuda.focus.backproject.
+
+Signature, argument checks, error types and the return value follow
+python/extensions/pybind_isce3/focus/Backproject.cpp:29-166 (CPU) and
+python/extensions/pybind_isce3/cuda/focus/Backproject.cpp:25-116 (GPU, which
+adds ``batch``).  The work runs in HIP kernels on the current GPU through
+the C-ABI in include/isce3_amd_backproject.h; host arrays are staged through
+device buffers, device tensors (torch, ROCm) are used in place.
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes as C
+import threading
+from typing import Optional
+
+import numpy as np
+
+from . import _native, errors
+from .container import RadarGeometry
+from .core import KernelF32, TabulatedKernelF32
+from .geometry import DEMInterpolator
+
+_TROPO = {"nodelay": 0, "tsx": 1}
+_PRECISION = {"plain": _native.PRECISION_PLAIN, "split": _native.PRECISION_SPLIT}
+
+DEFAULT_TOL_HEIGHT = 1e-5    # geometry/detail/Rdr2Geo.h:84
+DEFAULT_TOL_AZTIME = 1e-7    # geometry/detail/Geo2Rdr.h:54
+
+
+def parse_dry_tropo_model(s: str) -> int:
+    """focus/DryTroposphereModel.cpp:20-33."""
+    if s in _TROPO:
+        return _TROPO[s]
+    raise errors.InvalidArgument(
+        f"expected one of {{'nodelay', 'tsx'}}, instead got '{s}'")
+
+
+def parse_rdr2geo_params(params: dict) -> _native.Rdr2GeoParams_t:
+    """pybind focus/Backproject.cpp:29-49."""
+    out = _native.Rdr2GeoParams_t(DEFAULT_TOL_HEIGHT, 0.0, np.pi / 2)
+    for key, val in dict(params).items():
+        if key == "tol_height":
+            out.tol_height = float(val)
+        elif key == "look_min":
+            out.look_min = float(val)
+        elif key == "look_max":
+            out.look_max = float(val)
+        else:
+            raise errors.InvalidArgument(f"unexpected rdr2geo_bracket keyword: {key}")
+    return out
+
+
+def parse_geo2rdr_params(params: dict) -> _native.Geo2RdrParams_t:
+    """pybind focus/Backproject.cpp:52-78 (a None time means 'use the default')."""
+    out = _native.Geo2RdrParams_t(DEFAULT_TOL_AZTIME, 0, 0, 0.0, 0.0)
+    for key, val in dict(params).items():
+        if key == "tol_aztime":
+            out.tol_aztime = float(val)
+        elif key == "time_start":
+            if val is not None:
+                out.have_time_start = 1
+                out.time_start = float(val)
+        elif key == "time_end":
+            if val is not None:
+                out.have_time_end = 1
+                out.time_end = float(val)
+        else:
+            raise errors.InvalidArgument(f"unexpected geo2rdr_bracket keyword: {key}")
+    return out
+
+
+def geometry_struct(g: RadarGeometry) -> _native.RadarGeometry_t:
+    grid = g.radar_grid
+    gs = _native.Grid_t(grid.sensing_start, grid.prf, grid.starting_range,
+                        grid.range_pixel_spacing, grid.length, grid.width, int(grid.lookside), 0)
+    return _native.RadarGeometry_t(gs, g.orbit._struct(), g.doppler._struct())
+
+
+class _Contexts:
+    """One native context per HIP device, created on first use."""
+
+    def __init__(self):
+        self._lock = threading.Lock()
+        self._ctx = {}
+
+    def get(self, device: int):
+        with self._lock:
+            if device not in self._ctx:
+                h = C.c_void_p()
+                _native.check(_native.lib().bp_context_create(int(device), C.byref(h)))
+                self._ctx[device] = h
+            return self._ctx[device]
+
+    def close(self):
+        with self._lock:
+            for h in self._ctx.values():
+                _native.lib().bp_context_destroy(h)
+            self._ctx.clear()
+
+
+_contexts = _Contexts()
+atexit.register(_contexts.close)
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _torch():
+    import torch  # deferred: importing torch is slow and not needed for checks
+    if not torch.cuda.is_available():
+        raise errors.HipError("isce3_amd.focus.backproject needs a ROCm GPU (none visible)")
+    return torch
+
+
+def _shape_of(a):
+    return tuple(int(s) for s in a.shape)
+
+
+def backproject(out, out_geometry: RadarGeometry, signal_data, in_geometry: RadarGeometry,
+                dem: DEMInterpolator, fc: float, ds: float, kernel: KernelF32,
+                dry_tropo_model: str = "tsx", rdr2geo_params: Optional[dict] = None,
+                geo2rdr_params: Optional[dict] = None, batch=1024, height=None, *,
+                precision: str = "plain", options: Optional[dict] = None, stream=None,
+                stats: Optional[dict] = None) -> bool:
+    """Focus in azimuth via time-domain backprojection.
+
+    Parameters follow isce3.focus.backproject: ``out`` (complex64, written in
+    place), ``out_geometry``, ``signal_data`` (range-compressed complex64),
+    ``in_geometry``, ``dem``, ``fc`` (Hz), ``ds`` (desired azimuth resolution,
+    m), ``kernel`` (TabulatedKernelF32), ``dry_tropo_model`` ("tsx" or
+    "nodelay"), ``rdr2geo_params`` {tol_height, look_min, look_max},
+    ``geo2rdr_params`` {tol_aztime, time_start, time_end}, ``batch`` (accepted
+    for the GPU signature; all pulses are resident so it only has to be >= 1)
+    and ``height`` (optional float32 output).  Arrays may be numpy arrays or
+    ROCm torch tensors.
+
+    Extensions: ``precision`` "plain" (reference semantics, FP64 inner loop)
+    or "split" (FP64 per-pixel reference + FP32 deltas), ``options`` tiling
+    dict {tile_rows, chunk}, ``stream`` (torch stream or raw hipStream_t int)
+    and ``stats`` (dict filled with updates, geometry_ms, sum_ms).
+
+    Returns True if any pixel failed to converge (those pixels are NaN).
+    """
+    # The CPU binding takes height as the 12th positional argument.
+    if batch is not None and not isinstance(batch, (int, np.integer)) and height is None:
+        height, batch = batch, 1024
+    if out.ndim != 2:
+        raise errors.InvalidArgument("output array must be 2-D")
+    if _shape_of(out) != (out_geometry.grid_length, out_geometry.grid_width):
+        raise errors.InvalidArgument("output array shape must match output radar grid shape")
+    if signal_data.ndim != 2:
+        raise errors.InvalidArgument("input signal data must be 2-D")
+    if _shape_of(signal_data) != (in_geometry.grid_length, in_geometry.grid_width):
+        raise errors.InvalidArgument("input signal data shape must match input radar grid shape")
+    if height is not None and _shape_of(height) != (out_geometry.grid_length,
+                                                    out_geometry.grid_width):
+        raise errors.InvalidArgument("height array shape must match output radar grid shape")
+    tropo = parse_dry_tropo_model(dry_tropo_model)
+    r2g = parse_rdr2geo_params(rdr2geo_params or {})
+    g2r = parse_geo2rdr_params(geo2rdr_params or {})
+    if batch is None or int(batch) < 1:
+        raise errors.DomainError("batch size must be > 0")
+    if out_geometry.reference_epoch != in_geometry.reference_epoch:
+        raise errors.RuntimeError_("input reference epoch must match output reference epoch")
+    if not isinstance(kernel, TabulatedKernelF32):
+        raise errors.InvalidArgument("kernel must be a TabulatedKernelF32")
+    if precision not in _PRECISION:
+        raise errors.InvalidArgument(f"precision must be 'plain' or 'split', got {precision!r}")
+    _check_out_array(out, "output")
+    if height is not None:
+        _check_out_array(height, "height", complex_=False)
+
+    torch = _torch()
+    device = _device_of(torch, out, signal_data, height)
+    with torch.cuda.device(device):
+        return _run(torch, device, out, out_geometry, signal_data, in_geometry, dem, fc, ds,
+                    kernel, tropo, r2g, g2r, height, _PRECISION[precision], options, stream,
+                    stats)
+
+
+def _check_out_array(a, what, complex_=True):
+    if _is_torch(a):
+        import torch
+        want = torch.complex64 if complex_ else torch.float32
+        if a.dtype != want or not a.is_contiguous():
+            raise errors.InvalidArgument(f"{what} must be a contiguous {want} tensor")
+        return
+    a = np.asarray(a) if not isinstance(a, np.ndarray) else a
+    want = np.complex64 if complex_ else np.float32
+    if a.dtype != want or not a.flags.c_contiguous or not a.flags.writeable:
+        raise errors.InvalidArgument(
+            f"{what} must be a writeable C-contiguous {np.dtype(want).name} array")
+
+
+def _device_of(torch, *arrays) -> int:
+    for a in arrays:
+        if a is not None and _is_torch(a) and a.is_cuda:
+            return a.device.index if a.device.index is not None else torch.cuda.current_device()
+    return torch.cuda.current_device()
+
+
+def _run(torch, device, out, ogeom, sig, igeom, dem, fc, ds, kernel, tropo, r2g, g2r, height,
+         precision, options, stream, stats):
+    if stream is None:
+        hstream = torch.cuda.current_stream().cuda_stream
+    elif isinstance(stream, int):
+        hstream = stream
+    else:
+        hstream = stream.cuda_stream
+    # Device views of the arrays (host arrays are staged on the device).
+    if _is_torch(sig):
+        d_in = sig.contiguous() if sig.is_cuda else sig.contiguous().to(f"cuda:{device}")
+        if d_in.dtype != torch.complex64:
+            d_in = d_in.to(torch.complex64)
+    else:
+        h = np.ascontiguousarray(sig, dtype=np.complex64)
+        d_in = torch.from_numpy(h).to(f"cuda:{device}", non_blocking=False)
+    out_is_dev = _is_torch(out) and out.is_cuda
+    d_out = out if out_is_dev else torch.empty(_shape_of(out), dtype=torch.complex64,
+                                               device=f"cuda:{device}")
+    d_h = None
+    if height is not None:
+        h_is_dev = _is_torch(height) and height.is_cuda
+        d_h = height if h_is_dev else torch.empty(_shape_of(height), dtype=torch.float32,
+                                                  device=f"cuda:{device}")
+    og = geometry_struct(ogeom)
+    ig = geometry_struct(igeom)
+    dm = dem._struct()
+    kr = kernel._struct()
+    opt = _native.Options_t()
+    for k, v in (options or {}).items():
+        if k not in ("tile_rows", "chunk", "fit_degree"):
+            raise errors.InvalidArgument(f"unknown option {k!r}")
+        setattr(opt, k, int(v))
+    st = _native.Stats_t()
+    failed = C.c_int(0)
+    ctx = _contexts.get(device)
+    # keep the host arrays referenced by the structs alive for the call
+    _keep = (ogeom, igeom, dem, kernel)
+    rc = _native.lib().bp_backproject(
+        ctx, C.byref(og), C.byref(ig), C.byref(dm), float(fc), float(ds), C.byref(kr),
+        int(tropo), C.byref(r2g), C.byref(g2r), C.c_void_p(d_in.data_ptr()),
+        C.c_void_p(d_out.data_ptr()), C.c_void_p(d_h.data_ptr()) if d_h is not None else None,
+        int(precision), C.byref(opt), C.c_void_p(hstream), C.byref(failed), C.byref(st))
+    del _keep
+    _native.check(rc)
+    if not out_is_dev:
+        res = d_out.cpu().numpy()
+        if _is_torch(out):
+            out.copy_(torch.from_numpy(res))
+        else:
+            out[...] = res
+    if height is not None and not (_is_torch(height) and height.is_cuda):
+        res = d_h.cpu().numpy()
+        if _is_torch(height):
+            height.copy_(torch.from_numpy(res))
+        else:
+            height[...] = res
+    if stats is not None:
+        stats.update(updates=int(st.updates), geometry_ms=float(st.geometry_ms),
+                     sum_ms=float(st.sum_ms), any_failed=bool(st.any_failed))
+    return bool(failed.value)

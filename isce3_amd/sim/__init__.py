@@ -1,0 +1,258 @@
+"""Seeded synthetic L-band stripmap scenes for tests and benchmarks.
+
+The radar is a straight-line ECEF track at 747 km altitude and 7.5 km/s
+(state vectors every 10 s), right looking, zero Doppler, wavelength 0.2384 m,
+PRF 1520 Hz and 6.25 m range spacing with near range 850 km.  Scenes hold
+unit point targets and, for the large configurations, a grid of clutter
+scatterers with complex Gaussian reflectivity, placed on the DEM surface by
+rdr2geo so that each scatterer sits at a known radar coordinate.  Range-
+compressed pulses are written by the native generator in
+include/isce3_amd_sim.h (sinc range response of bandwidth 0.8 of the sample
+rate, bistatic delay, phase exp(-j 2 pi fc tau)).
+
+Configurations (pulses x range bins -> output grid):
+  c0   512 x 2048 -> 256 x 256, 3 targets, flat DEM, noise 0.01
+  c1   16384 x 16384 -> 4096 x 4096, 200 targets + clutter, fractal DEM
+  c3   8192 x 16384 raw set (tile requests are cut from its swath)
+  pt   1024 x 1024 -> 129 x 129 chip around one target, Knab width 9
+  tiny 256 x 512 -> 48 x 40, 2 targets, fractal DEM (fast parity checks)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from ..container import RadarGeometry
+from ..core import (DateTime, KnabKernel, LUT2d, LookSide, Orbit, TabulatedKernelF32,
+                    speed_of_light)
+from ..geometry import DEMInterpolator, rdr2geo_bracket
+from ..product import RadarGridParameters
+
+WAVELENGTH = 0.2384
+FC = speed_of_light / WAVELENGTH
+PRF = 1520.0
+DR = 6.25
+R0 = 850e3
+ALTITUDE = 747e3
+SPEED = 7500.0
+WGS84_A = 6378137.0
+EPOCH = DateTime(2025, 1, 1, 0, 0, 0)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libisce3_amd_sim.so")
+
+
+class _SimParams(C.Structure):
+    _fields_ = [("na", C.c_int64), ("nr", C.c_int64), ("r0", C.c_double), ("dr", C.c_double),
+                ("fc", C.c_double), ("bandwidth", C.c_double), ("half_taps", C.c_int32),
+                ("nthreads", C.c_int32), ("beam_sin", C.c_double), ("noise_var", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"scene generator library missing: {LIB_PATH} "
+                              "(build with `python -m isce3_amd._build`)")
+        h = C.CDLL(LIB_PATH)
+        h.sim_range_compressed.restype = C.c_int
+        h.sim_range_compressed.argtypes = [C.POINTER(_SimParams), C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_int64, C.c_void_p]
+        h.sim_noise.restype = C.c_int
+        h.sim_noise.argtypes = [C.c_uint64, C.c_double, C.c_int64, C.c_int64, C.c_void_p]
+        _lib = h
+    return _lib
+
+
+def straight_orbit(half_span: float = 50.0, spacing: float = 10.0,
+                   epoch: DateTime = EPOCH) -> Orbit:
+    """p(t) = p0 + v t with p0 on the equator at 747 km altitude, v northward."""
+    n = int(round(2 * half_span / spacing)) + 1
+    t = -half_span + spacing * np.arange(n)
+    p0 = np.array([WGS84_A + ALTITUDE, 0.0, 0.0])
+    v = np.array([0.0, 0.0, SPEED])
+    pos = p0[None, :] + t[:, None] * v[None, :]
+    vel = np.repeat(v[None, :], n, axis=0)
+    return Orbit.from_arrays(epoch, float(t[0]), spacing, pos, vel)
+
+
+def fractal_dem(n: int, hurst: float, sigma: float, seed: int) -> np.ndarray:
+    """Fractional-Brownian surface by spectral synthesis (amplitude ~ k^-(H+1)),
+    zero mean, standard deviation sigma (m)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kx = np.fft.fftfreq(n)
+    k = np.sqrt(kx[None, :] ** 2 + kx[:, None] ** 2)
+    k[0, 0] = np.inf
+    spec = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * k ** (-(hurst + 1))
+    z = np.real(np.fft.ifft2(spec))
+    z -= z.mean()
+    z *= sigma / z.std()
+    return z.astype(np.float32)
+
+
+@dataclass
+class Scene:
+    name: str
+    orbit: Orbit
+    in_grid: RadarGridParameters
+    out_grid: RadarGridParameters
+    dem: DEMInterpolator
+    fc: float
+    ds: float
+    kernel: TabulatedKernelF32
+    targets: np.ndarray            # [nt, 3] ECEF
+    amplitudes: np.ndarray         # [nt] complex128
+    target_radar: np.ndarray       # [nt, 2] (azimuth time, slant range)
+    n_point_targets: int
+    noise_var: float
+    seed: int
+    doppler: LUT2d = field(default_factory=LUT2d)
+    dry_tropo_model: str = "nodelay"
+    bandwidth: float = 0.8
+
+    def in_geometry(self) -> RadarGeometry:
+        return RadarGeometry(self.in_grid, self.orbit, self.doppler)
+
+    def out_geometry(self, grid: Optional[RadarGridParameters] = None) -> RadarGeometry:
+        return RadarGeometry(grid or self.out_grid, self.orbit, self.doppler)
+
+
+def _grid(t0, prf, r0, dr, length, width):
+    return RadarGridParameters(t0, WAVELENGTH, prf, r0, dr, LookSide.Right, length, width, EPOCH)
+
+
+def _center_grid(na, nr, lout, wout):
+    in_grid = _grid(-0.5 * na / PRF, PRF, R0, DR, na, nr)
+    r0_out = R0 + 0.5 * (nr - wout) * DR
+    out_grid = _grid(-0.5 * lout / PRF, PRF, r0_out, DR, lout, wout)
+    return in_grid, out_grid
+
+
+def _locate(orbit, dem, radar_coords):
+    xyz = np.empty((len(radar_coords), 3))
+    for i, (t, r) in enumerate(radar_coords):
+        xyz[i] = rdr2geo_bracket(t, r, orbit, LookSide.Right, 0.0, WAVELENGTH, dem)
+    return xyz
+
+
+def _dem_around(orbit, grid, n, posting_m, hurst, sigma, seed):
+    """Fractal DEM of n x n postings centred on the ground point of the grid centre."""
+    flat = DEMInterpolator(0.0)
+    tc = grid.sensing_time((grid.length - 1) / 2.0)
+    rc = grid.slant_range((grid.width - 1) / 2.0)
+    xyz = rdr2geo_bracket(tc, rc, orbit, LookSide.Right, 0.0, WAVELENGTH, flat)
+    lon = math.degrees(math.atan2(xyz[1], xyz[0]))
+    lat = math.degrees(math.atan2(xyz[2], math.hypot(xyz[0], xyz[1])))
+    d = posting_m / (WGS84_A * math.pi / 180.0)
+    z = fractal_dem(n, hurst, sigma, seed)
+    dem = DEMInterpolator(0.0)
+    dem.load_dem_array(z, lon - 0.5 * n * d, lat - 0.5 * n * d, d, d)
+    return dem
+
+
+def make_scene(config: str, seed: Optional[int] = None, *, ds: Optional[float] = None,
+               n_targets: Optional[int] = None, clutter_step: Optional[int] = None,
+               kernel_width: Optional[float] = None) -> Scene:
+    config = config.lower()
+    orbit = straight_orbit()
+    kw = kernel_width
+    clutter_var = 0.05
+    if config == "c0":
+        seed = 0 if seed is None else seed
+        in_grid, out_grid = _center_grid(512, 2048, 256, 256)
+        dem = DEMInterpolator(0.0)
+        rel = [(64.3, 70.2), (128.0, 128.0), (190.6, 180.4)]
+        ntg, step, noise, kw = 3, None, 0.01, kw or 8.0
+    elif config == "c1":
+        seed = 1 if seed is None else seed
+        in_grid, out_grid = _center_grid(16384, 16384, 4096, 4096)
+        dem = _dem_around(orbit, out_grid, 1024, 30.0, 0.8, 200.0, seed)
+        ntg, step, noise, kw = n_targets or 200, clutter_step or 32, 0.01, kw or 8.0
+        rel = None
+    elif config == "c3":
+        seed = 3 if seed is None else seed
+        in_grid, out_grid = _center_grid(8192, 16384, 4096, 4096)
+        dem = _dem_around(orbit, out_grid, 1024, 30.0, 0.8, 200.0, seed)
+        ntg, step, noise, kw = n_targets or 200, clutter_step or 32, 0.01, kw or 8.0
+        rel = None
+    elif config == "pt":
+        seed = 7 if seed is None else seed
+        in_grid, _ = _center_grid(1024, 1024, 129, 129)
+        dem = DEMInterpolator(25.0)
+        tgt_t = in_grid.sensing_time(511.3)
+        tgt_r = in_grid.slant_range(500.6)
+        out_grid = _grid(tgt_t - 64 / PRF, PRF, tgt_r - 64 * DR, DR, 129, 129)
+        rel = [(64.0, 64.0)]
+        ntg, step, noise, kw = 1, None, 0.0, kw or 9.0
+        ds = ds or 30.0
+    elif config == "tiny":
+        seed = 11 if seed is None else seed
+        in_grid, out_grid = _center_grid(256, 512, 48, 40)
+        dem = _dem_around(orbit, out_grid, 128, 30.0, 0.8, 200.0, seed)
+        rel = [(12.4, 9.7), (35.2, 30.1)]
+        ntg, step, noise, kw = 2, None, 0.01, kw or 8.0
+    else:
+        raise ValueError(f"unknown scene config {config!r}")
+    ds = 6.0 if ds is None else float(ds)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if rel is None:
+        rel = np.column_stack([rng.uniform(0, out_grid.length - 1, ntg),
+                               rng.uniform(0, out_grid.width - 1, ntg)])
+    radar = [(out_grid.sensing_time(a), out_grid.slant_range(b)) for a, b in rel]
+    amps = [1.0 + 0j] * len(radar)
+    if step:
+        rows = np.arange(step / 2, out_grid.length, step)
+        cols = np.arange(step / 2, out_grid.width, step)
+        rr, cc = np.meshgrid(rows, cols, indexing="ij")
+        rr = rr.ravel() + rng.uniform(-step / 2, step / 2, rr.size)
+        cc = cc.ravel() + rng.uniform(-step / 2, step / 2, cc.size)
+        radar += [(out_grid.sensing_time(a), out_grid.slant_range(b)) for a, b in zip(rr, cc)]
+        g = rng.standard_normal((rr.size, 2)) * math.sqrt(clutter_var / 2)
+        amps += list(g[:, 0] + 1j * g[:, 1])
+    xyz = _locate(orbit, dem, radar)
+    kernel = TabulatedKernelF32(KnabKernel(kw, 0.8), 2048)
+    return Scene(config, orbit, in_grid, out_grid, dem, FC, ds, kernel, xyz,
+                 np.asarray(amps, dtype=np.complex128), np.asarray(radar), len(rel), noise,
+                 int(seed))
+
+
+def pulse_state(orbit: Orbit, grid: RadarGridParameters) -> np.ndarray:
+    """Platform position/velocity [na, 6] at every pulse of `grid`."""
+    pv = np.empty((grid.length, 6))
+    for k in range(grid.length):
+        p, v = orbit.interpolate(grid.sensing_start + k / grid.prf)
+        pv[k, :3] = p
+        pv[k, 3:] = v
+    return pv
+
+
+def simulate(scene: Scene, out: Optional[np.ndarray] = None, nthreads: int = 0,
+             beam_factor: float = 1.5) -> np.ndarray:
+    """Range-compressed pulses [na, nr] complex64 for the scene (written to `out`
+    when given; it must be a C-contiguous complex64 array of that shape)."""
+    g = scene.in_grid
+    if out is None:
+        out = np.empty((g.length, g.width), dtype=np.complex64)
+    if out.shape != (g.length, g.width) or out.dtype != np.complex64 or not out.flags.c_contiguous:
+        raise ValueError("out must be a C-contiguous complex64 array of the input grid shape")
+    pv = np.ascontiguousarray(pulse_state(scene.orbit, g))
+    tg = np.ascontiguousarray(scene.targets, dtype=np.float64)
+    amp = np.ascontiguousarray(np.column_stack([scene.amplitudes.real, scene.amplitudes.imag]))
+    beam = beam_factor * WAVELENGTH / (4.0 * scene.ds)
+    p = _SimParams(g.length, g.width, g.starting_range, g.range_pixel_spacing, scene.fc,
+                   scene.bandwidth, 16, int(nthreads), beam, scene.noise_var,
+                   np.uint64(scene.seed * 7919 + 17))
+    rc = lib().sim_range_compressed(C.byref(p), pv.ctypes.data, tg.ctypes.data, amp.ctypes.data,
+                                    tg.shape[0], out.ctypes.data)
+    if rc != 0:
+        raise ValueError("scene generator rejected its arguments")
+    return out
