@@ -1,0 +1,320 @@
+"""Pins of the CPU oracle: every oracle function is checked against the
+reference's own known answers (tests/golden/, made from the reference test
+sources and from the reference building blocks compiled in oracle/_ref),
+against oracle/_ref directly when it is built, and against closed forms and
+invariants chosen so that a dropped term, a sign or an index error fails."""
+import ctypes as C
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+REF = np.load(os.path.join(GOLDEN, "ref_blocks.npz"))
+D3 = C.c_double * 3
+
+
+def _d3(v):
+    return D3(*[float(x) for x in v])
+
+
+# ------------------------------------------------------------ ellipsoid
+def test_ellipsoid_known_answers():
+    kat = json.load(open(os.path.join(GOLDEN, "ellipsoid_kat.json")))
+    a, e2 = kat["a"], kat["e2"]
+    assert len(kat["cases"]) == 21
+    for case in kat["cases"]:
+        xyz, llh = D3(), D3()
+        oracle.lib().orc_lonlat_to_xyz(a, e2, _d3(case["llh"]), xyz)
+        np.testing.assert_allclose(list(xyz), case["xyz"], atol=kat["tol_xyz"], rtol=0,
+                                   err_msg=case["name"])
+        oracle.lib().orc_xyz_to_lonlat(a, e2, _d3(case["xyz"]), llh)
+        assert abs(llh[0] - case["llh"][0]) <= kat["tol_lonlat"] or case["name"] in (
+            "NorthPole", "SouthPole"), case["name"]  # longitude is arbitrary at the poles
+        assert abs(llh[1] - case["llh"][1]) <= kat["tol_lonlat"], case["name"]
+        assert abs(llh[2] - case["llh"][2]) <= kat["tol_h"], case["name"]
+
+
+def test_ellipsoid_round_trip():
+    rng = np.random.Generator(np.random.PCG64(5))
+    for _ in range(200):
+        llh = [rng.uniform(-np.pi, np.pi), rng.uniform(-1.5, 1.5), rng.uniform(-500, 9000)]
+        xyz, back = D3(), D3()
+        oracle.lib().orc_lonlat_to_xyz(6378137.0, 0.006694379990141317, _d3(llh), xyz)
+        oracle.lib().orc_xyz_to_lonlat(6378137.0, 0.006694379990141317, xyz, back)
+        np.testing.assert_allclose(list(back), llh, atol=1e-6)
+
+
+# ------------------------------------------------------------ kernels
+@pytest.mark.parametrize("w", [8.0, 9.0])
+@pytest.mark.parametrize("bw", [0.8, 0.83])
+def test_knab_table_matches_reference(w, bw):
+    t32 = np.empty(2048, np.float32)
+    t64 = np.empty(2048, np.float64)
+    oracle.lib().orc_tabulate_knab_f32(w, bw, 2048, t32.ctypes.data)
+    oracle.lib().orc_tabulate_knab_f64(w, bw, 2048, t64.ctypes.data)
+    np.testing.assert_array_equal(t32, REF[f"table_f32_w{w:g}_bw{bw:g}"])
+    np.testing.assert_allclose(t64, REF[f"table_f64_w{w:g}_bw{bw:g}"], rtol=0, atol=1e-15)
+
+
+def test_knab_closed_form():
+    """K(0) = 1, K(n) = 0 for integers 0 < |n| < w/2 (sinc zeros), even symmetry."""
+    lib = oracle.lib()
+    assert lib.orc_knab(0.0, 4.0, 0.8) == 1.0
+    for n in (1, 2, 3):
+        assert abs(lib.orc_knab(float(n), 4.0, 0.8)) < 1e-16
+        assert lib.orc_knab(n + 0.37, 4.0, 0.8) == lib.orc_knab(-(n + 0.37), 4.0, 0.8)
+    # window value at t = 0.5 from the definition sinc(t) cosh(c sqrt(1-(t/hw)^2))/cosh(c)
+    c = math.pi * 4.0 * 0.2
+    expect = math.sin(math.pi * 0.5) / (math.pi * 0.5) * math.cosh(
+        c * math.sqrt(1 - (0.5 / 4.0) ** 2)) / math.cosh(c)
+    assert abs(lib.orc_knab(0.5, 4.0, 0.8) - expect) < 1e-15
+    np.testing.assert_allclose([lib.orc_knab(x, 4.0, 0.8) for x in REF["knab_x"]],
+                               REF["knab_w8"], rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("w", [8.0, 9.0])
+def test_tabulated_eval_matches_reference(w):
+    tab = np.empty(2048, np.float32)
+    oracle.lib().orc_tabulate_knab_f32(w, 0.8, 2048, tab.ctypes.data)
+    k = oracle.Kernel_t(w / 2, 2048, 0, tab.ctypes.data, None)
+    got = np.array([oracle.lib().orc_tab_eval_f32(C.byref(k), float(x)) for x in REF["tab_eval_x"]],
+                   dtype=np.float32)
+    np.testing.assert_array_equal(got, REF[f"tab_eval_w{w:g}"])
+
+
+@pytest.mark.parametrize("w", [8.0, 9.0])
+def test_interp1d_matches_reference(w):
+    """Zero padding beyond [0, n), ceil/round window start and the float tap
+    inner product (core/detail/Interp1d.h:23-106), bit for bit."""
+    tab = np.empty(2048, np.float32)
+    oracle.lib().orc_tabulate_knab_f32(w, 0.8, 2048, tab.ctypes.data)
+    k = oracle.Kernel_t(w / 2, 2048, 0, tab.ctypes.data, None)
+    line = np.ascontiguousarray(REF["interp_line"])
+    out = np.empty(2, np.float32)
+    got = np.empty(REF["interp_t"].size, np.complex64)
+    for i, t in enumerate(REF["interp_t"]):
+        oracle.lib().orc_interp1d_c64(C.byref(k), line.ctypes.data, line.size, float(t),
+                                      out.ctypes.data)
+        got[i] = out[0] + 1j * out[1]
+    np.testing.assert_array_equal(got, REF[f"interp_w{w:g}"])
+
+
+def test_interp1d_quality_band_limited():
+    """Quality bar of tests/cxx/isce3/core/interp1d.cpp:270-309 (Knab 9/0.8,
+    tabulated 2048) on a seeded band-limited signal: correlation >= 0.998,
+    phase spread <= 5 deg, power bias and spread <= 0.5 dB; offset 0 returns
+    the samples."""
+    rng = np.random.Generator(np.random.PCG64(1234))
+    n, bw = 512, 0.8
+    tk = rng.uniform(0, n - 1, 4 * n)
+    wk = (rng.standard_normal(4 * n) + 1j * rng.standard_normal(4 * n)) * n / (4 * n)
+
+    def sig(t):
+        return (wk[None, :] * np.sinc(bw * (np.asarray(t)[:, None] - tk[None, :]))).sum(1)
+
+    data = sig(np.arange(n)).astype(np.complex64)
+    tab = np.empty(2048, np.float32)
+    oracle.lib().orc_tabulate_knab_f32(9.0, 0.8, 2048, tab.ctypes.data)
+    k = oracle.Kernel_t(4.5, 2048, 0, tab.ctypes.data, None)
+    out = np.empty(2, np.float32)
+
+    def interp(ts):
+        res = np.empty(len(ts), np.complex128)
+        for i, t in enumerate(ts):
+            oracle.lib().orc_interp1d_c64(C.byref(k), data.ctypes.data, n, float(t),
+                                          out.ctypes.data)
+            res[i] = out[0] + 1j * out[1]
+        return res
+
+    pad = 8
+    t0 = np.arange(pad, n - pad, dtype=float)
+    np.testing.assert_allclose(interp(t0), data[pad:n - pad], rtol=2e-6, atol=2e-6)
+    ts = np.arange(n) + rng.uniform(-0.5, 0.5, n)
+    ts = ts[(ts >= pad) & (ts <= n - 1 - pad)]
+    ref, got = sig(ts), interp(ts)
+    cor = np.abs((ref * np.conj(got))).sum() / np.sqrt(
+        (np.abs(ref) ** 2).sum() * (np.abs(got) ** 2).sum())
+    ph = np.angle(got * np.conj(ref))
+    ratio = np.abs(got) / np.abs(ref)
+    db = 20 * np.log10(ratio)
+    assert cor >= 0.998
+    assert np.degrees(np.std(ph, ddof=1)) <= 5.0
+    assert abs(20 * np.log10(ratio.mean())) <= 0.5
+    assert np.std(db, ddof=1) <= 0.5
+
+
+def test_linspace_search_matches_reference():
+    got = [oracle.lib().orc_linspace_search(0.0, 10.0, 11, float(v)) for v in REF["search_vals"]]
+    np.testing.assert_array_equal(got, REF["search_idx"])
+
+
+# ------------------------------------------------------------ Brent
+def test_brent_reference_functions():
+    """Brent-1973 problems of tests/cxx/isce3/math/root_find1d.cpp:200-230:
+    expected roots/status, and identical iterates to the reference's Brent."""
+    intervals = [(4.0, 5.0), (-9.0, 31.0), (-1.0, 4.0), (-1.0, 1.1), (-5.0, 5.0)]
+    for fid, (a, b) in enumerate(intervals, start=1):
+        r = C.c_double()
+        st = oracle.lib().orc_brent_testfn(fid, a, b, 1e-12, C.byref(r))
+        assert st == REF["brent_status"][fid - 1]
+        if st == 0:
+            assert r.value == REF["brent_roots"][fid - 1]
+    r = C.c_double()
+    assert oracle.lib().orc_brent_testfn(1, 4.0, 5.0, 1e-12, C.byref(r)) == 0
+    assert abs(r.value - 3 * math.pi / 2) <= 1e-12
+    assert oracle.lib().orc_brent_testfn(2, -9.0, 31.0, 1e-12, C.byref(r)) == 0
+    assert abs(r.value) <= 1e-12
+    assert oracle.lib().orc_brent_testfn(5, -5.0, 5.0, 1e-12, C.byref(r)) == 6  # NaN -> fail
+    # an interval that does not bracket a root
+    assert oracle.lib().orc_brent_testfn(1, 5.0, 6.0, 1e-12, C.byref(r)) == 11
+
+
+@pytest.mark.skipif(oracle.ref_lib() is None, reason="oracle/_ref not built (no reference tree)")
+def test_brent_matches_reference_build_on_geometry_like_function():
+    ref = oracle.ref_lib()
+    f = lambda x: math.cos(3 * x) - x / 7 + 0.1  # noqa: E731
+    cb = ref.FN(lambda x, _c: f(x))
+    r = C.c_double()
+    assert ref.ref_find_zero_brent(0.0, 1.0, cb, None, 1e-13, C.byref(r)) == 0
+    assert abs(f(r.value)) < 1e-12
+
+
+# ------------------------------------------------------------ orbit
+def _orbit(times, pos, vel, method=0):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    s = oracle.Orbit_t(times[0], times[1] - times[0], len(times), method, pos.ctypes.data,
+                       vel.ctypes.data)
+    s._keep = (pos, vel)
+    return s
+
+
+def _interp(orb, t, border=0):
+    p, v = D3(), D3()
+    st = oracle.lib().orc_orbit_interpolate(C.byref(orb), t, border, p, v)
+    return st, np.array(p), np.array(v)
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_orbit_exact_for_reference_trajectories(method):
+    """tests/cxx/isce3/core/orbit/orbit.cpp:384-552: linear, polynomial and
+    circular reference orbits are reproduced to 1e-8."""
+    t = 10.0 * np.arange(11)
+    p0, v0 = np.array([0., 0., 0.]), np.array([4000., -1000., 4500.])
+    orb = _orbit(t, p0 + t[:, None] * v0, np.repeat(v0[None], 11, 0), method)
+    for tt in (23.3, 36.7, 54.5, 89.3):
+        st, p, v = _interp(orb, tt)
+        assert st == 0
+        np.testing.assert_allclose(p, p0 + tt * v0, atol=1e-8, rtol=0)
+        np.testing.assert_allclose(v, v0, atol=1e-8, rtol=0)
+    coeffs = np.array([[-7000000.0, 5400000.0, 0.0], [5435.0, -4257.0, 7000.0],
+                       [-45.0, 23.0, 11.0], [7.3, 3.9, 0.0], [0.0, 0.01, 0.0]])
+    pos = np.array([sum(coeffs[i] * tt ** i for i in range(5)) for tt in t])
+    vel = np.array([sum(i * coeffs[i] * tt ** (i - 1) for i in range(1, 5)) for tt in t])
+    orb = _orbit(t, pos, vel, method)
+    for tt in (23.3, 36.7, 54.5, 89.3):
+        st, p, v = _interp(orb, tt)
+        np.testing.assert_allclose(p, sum(coeffs[i] * tt ** i for i in range(5)), atol=1e-8, rtol=0)
+        np.testing.assert_allclose(v, sum(i * coeffs[i] * tt ** (i - 1) for i in range(1, 5)),
+                                   atol=1e-8, rtol=0)
+    th0, ph0, dth, dph, r = 2 * np.pi / 8, 2 * np.pi / 12, 2 * np.pi / 7000, 2 * np.pi / 4000, 8e6
+    tc = 5.0 * np.arange(11)
+
+    def cpos(tt):
+        th, ph = th0 + tt * dth, ph0 + tt * dph
+        return np.array([r * np.cos(th), r * (np.sin(th) + np.cos(ph)), r * np.sin(ph)])
+
+    def cvel(tt):
+        th, ph = th0 + tt * dth, ph0 + tt * dph
+        return np.array([-r * dth * np.sin(th), r * (dth * np.cos(th) - dph * np.sin(ph)),
+                         r * dph * np.cos(ph)])
+
+    orb = _orbit(tc, [cpos(x) for x in tc], [cvel(x) for x in tc], method)
+    for tt in (11.65, 18.35, 27.25, 44.65):
+        st, p, v = _interp(orb, tt)
+        np.testing.assert_allclose(p, cpos(tt), atol=1e-8, rtol=0)
+        np.testing.assert_allclose(v, cvel(tt), atol=1e-8, rtol=0)
+
+
+def test_orbit_border_modes():
+    t = 10.0 * np.arange(11)
+    v0 = np.array([1., 2., 3.])
+    orb = _orbit(t, t[:, None] * v0, np.repeat(v0[None], 11, 0))
+    st, p, v = _interp(orb, 101.0, border=2)
+    assert st == 2 and np.isnan(p).all() and np.isnan(v).all()
+    st, p, v = _interp(orb, 101.0, border=1)
+    assert st == 0
+    np.testing.assert_allclose(p, 101.0 * v0, atol=1e-8)
+    assert _interp(orb, -1.0, border=0)[0] == 2
+    small = _orbit(t[:3], t[:3, None] * v0, np.repeat(v0[None], 3, 0))
+    assert _interp(small, 5.0)[0] == 1  # needs 4 state vectors
+
+
+# ------------------------------------------------------------ delay and tropo
+def test_bistatic_delay_solves_light_path():
+    """tau = 2 (r.v - c|r|)/(|v|^2 - c^2) is the exact solution of
+    |x - p| + |x - (p + v tau)| = c tau (stationary target, moving antenna)."""
+    rng = np.random.Generator(np.random.PCG64(9))
+    c = 299792458.0
+    for _ in range(50):
+        p = rng.uniform(-7e6, 7e6, 3)
+        v = rng.uniform(-8e3, 8e3, 3)
+        x = p + rng.uniform(-9e5, 9e5, 3)
+        tau = oracle.lib().orc_bistatic_delay(_d3(p), _d3(v), _d3(x))
+        lhs = np.linalg.norm(x - p) + np.linalg.norm(x - p - v * tau)
+        assert abs(lhs - c * tau) <= 1e-7 * c * tau
+        # with v = 0 it is the monostatic 2R/c
+        tau0 = oracle.lib().orc_bistatic_delay(_d3(p), _d3([0, 0, 0]), _d3(x))
+        assert abs(tau0 - 2 * np.linalg.norm(x - p) / c) <= 4 * np.spacing(tau0)
+
+
+def test_dry_tropo_tsx_closed_form():
+    """2 * 2.3 m * exp(-h/6000) / (c cos(incidence)); at nadir cos = 1."""
+    c = 299792458.0
+    for h in (0.0, 1000.0, 4000.0):
+        llh = [0.3, 0.2, h]
+        xyz = D3()
+        oracle.lib().orc_lonlat_to_xyz(6378137.0, 0.006694379990141317, _d3(llh), xyz)
+        n = np.array([np.cos(0.2) * np.cos(0.3), np.cos(0.2) * np.sin(0.3), np.sin(0.2)])
+        p = np.array(xyz) + 7e5 * n  # straight above the target
+        tau = oracle.lib().orc_dry_tropo_tsx(_d3(p), _d3(llh))
+        assert abs(tau - 2 * 2.3 * np.exp(-h / 6000) / c) < 1e-20
+        p2 = np.array(xyz) + 7e5 * n + 4e5 * np.cross(n, [0, 0, 1]) / np.linalg.norm(np.cross(n, [0, 0, 1]))
+        cosi = np.dot((p2 - np.array(xyz)) / np.linalg.norm(p2 - np.array(xyz)), n)
+        tau2 = oracle.lib().orc_dry_tropo_tsx(_d3(p2), _d3(llh))
+        assert abs(tau2 - 2 * 2.3 * np.exp(-h / 6000) / (c * cosi)) < 1e-18
+
+
+# ------------------------------------------------------------ DEM / LUT
+def test_dem_bilinear_and_borders():
+    data = np.arange(100, dtype=np.float32).reshape(10, 10) * 3.0
+    dem = oracle.Dem_t(1, 4326, 7.0, 10.0, 20.0, 0.1, 0.1, 10, 10, data.ctypes.data)
+    f = lambda x, y: oracle.lib().orc_dem_interpolate_xy(C.byref(dem), x, y)  # noqa: E731
+    # inside: bilinear of a plane is exact: z = 3*(10*row + col)
+    for col, row in ((2.25, 3.5), (5.0, 5.0), (7.75, 2.5)):
+        assert abs(f(10.0 + col * 0.1, 20.0 + row * 0.1) - 3 * (10 * row + col)) < 1e-3
+    # border rule of DEMInterpolator.cpp:650-656: rows/cols < 2 or >= n-1 -> ref height
+    assert f(10.0 + 1.9 * 0.1, 20.5) == 7.0
+    assert f(10.0 + 9.0 * 0.1, 20.5) == 7.0
+    assert f(10.5, 20.0 + 1.0 * 0.1) == 7.0
+    assert f(5.0, 20.5) == 7.0  # west of the raster
+    # lon/lat path converts radians to degrees
+    g = oracle.lib().orc_dem_interpolate_lonlat(C.byref(dem), math.radians(10.5), math.radians(20.5))
+    assert abs(g - f(10.5, 20.5)) < 1e-4
+
+
+def test_lut2d_bilinear_clamped():
+    data = np.array([[1.0, 2.0, 3.0], [4.0, 5.0, 6.0]])
+    lut = oracle.Lut_t(1, 1, 1.0, 100.0, 0.0, 10.0, 1.0, 3, 2, data.ctypes.data)
+    ev = lambda y, x: oracle.lib().orc_lut2d_eval(C.byref(lut), y, x)  # noqa: E731
+    assert ev(0.5, 105.0) == pytest.approx(3.0)
+    assert ev(0.0, 120.0) == 3.0
+    assert ev(5.0, 500.0) == 6.0  # clamped
+    assert ev(-3.0, -50.0) == 1.0
+    zero = oracle.Lut_t(0, 1, 0.0, 0, 0, 1, 1, 0, 0, None)
+    assert oracle.lib().orc_lut2d_eval(C.byref(zero), 3.0, 4.0) == 0.0
