@@ -1,0 +1,101 @@
+"""Full benchmark size (16384 x 16384 pulses -> 4096 x 4096) on the GPU, checked
+through size-independent properties: exact scaling by two, split vs plain on
+every pixel, a seeded pixel sample against the oracle, and the focused point
+targets standing out of the clutter at their known positions."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import complex_rel_error, phase_errors, rms
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1(torch_gpu):
+    torch = torch_gpu
+    from isce3_amd import sim
+    print("[fullsize] simulating c1", flush=True)
+    sc = sim.make_scene("c1")
+    raw = torch.empty((sc.in_grid.length, sc.in_grid.width), dtype=torch.complex64,
+                      pin_memory=True)
+    sim.simulate(sc, out=raw.numpy(), nthreads=min(16, os.cpu_count() or 1))
+    d_raw = raw.cuda()
+    res = {}
+    from isce3_amd.focus import backproject
+    for precision in ("plain", "split"):
+        out = torch.empty(sc.out_grid.shape, dtype=torch.complex64, device="cuda")
+        st = {}
+        err = backproject(out, sc.out_geometry(), d_raw, sc.in_geometry(), sc.dem, sc.fc, sc.ds,
+                          sc.kernel, sc.dry_tropo_model, precision=precision, stats=st)
+        print(f"[fullsize] {precision}: {st}", flush=True)
+        res[precision] = (out.cpu().numpy(), err, st)
+    return sc, raw, d_raw, res
+
+
+def test_no_failures_and_update_count(c1):
+    sc, _, _, res = c1
+    for precision in ("plain", "split"):
+        z, err, st = res[precision]
+        assert not err and np.isfinite(z).all()
+        # every pixel integrates a full ~2.66 s aperture (about 4046 pulses)
+        assert 3900 * z.size < st["updates"] < 4200 * z.size
+    assert res["plain"][2]["updates"] == res["split"][2]["updates"]
+
+
+def test_split_agrees_with_plain_everywhere(c1):
+    _, _, _, res = c1
+    a, b = res["split"][0], res["plain"][0]
+    assert complex_rel_error(a, b) <= 1e-4
+    assert rms(phase_errors(a, b)) <= 1e-4
+
+
+def test_exact_scaling_by_two(c1, torch_gpu):
+    """Doubling the input doubles every output bit for bit (taps, rotation and
+    accumulation are linear and scaling by 2 is exact)."""
+    torch = torch_gpu
+    sc, _, d_raw, res = c1
+    from isce3_amd.focus import backproject
+    d2 = d_raw * 2
+    for precision in ("plain", "split"):
+        out = torch.empty(sc.out_grid.shape, dtype=torch.complex64, device="cuda")
+        backproject(out, sc.out_geometry(), d2, sc.in_geometry(), sc.dem, sc.fc, sc.ds,
+                    sc.kernel, sc.dry_tropo_model, precision=precision)
+        np.testing.assert_array_equal(out.cpu().numpy(), 2 * res[precision][0])
+    del d2
+
+
+def test_pixel_sample_against_oracle(c1):
+    sc, raw, _, res = c1
+    rng = np.random.Generator(np.random.PCG64(77))
+    L, W = sc.out_grid.shape
+    sub = np.sort(rng.choice(L * W, 384, replace=False))
+    for precision, mode, tol in (("plain", oracle.MODE_REFERENCE, 1e-5),
+                                 ("split", oracle.MODE_FP64, 1e-3)):
+        ref, _, _, rc = oracle.backproject(sc.out_geometry(), raw.numpy(), sc.in_geometry(),
+                                           sc.dem, sc.fc, sc.ds, sc.kernel, sc.dry_tropo_model,
+                                           mode=mode, subset=sub, kernel_bandwidth=0.8,
+                                           nthreads=min(16, os.cpu_count() or 1))
+        z = res[precision][0].ravel()[sub]
+        assert complex_rel_error(z, ref) <= tol
+        assert rms(phase_errors(z, ref)) <= tol
+
+
+def test_point_targets_focus_at_known_positions(c1):
+    sc, _, _, res = c1
+    g = sc.out_grid
+    a = np.abs(res["split"][0])
+    med = np.median(a)
+    hits = 0
+    for t, r in sc.target_radar[:sc.n_point_targets]:
+        row = (t - g.sensing_start) * g.prf
+        col = (r - g.starting_range) / g.range_pixel_spacing
+        i, j = int(round(row)), int(round(col))
+        win = a[max(i - 2, 0):i + 3, max(j - 2, 0):j + 3]
+        pk = np.unravel_index(np.argmax(win), win.shape)
+        if win.max() > 20 * med and abs(pk[0] + max(i - 2, 0) - row) <= 1 and \
+                abs(pk[1] + max(j - 2, 0) - col) <= 1:
+            hits += 1
+    assert hits >= 0.95 * sc.n_point_targets
