@@ -48,6 +48,11 @@ def main():
     ap.add_argument("--sample-pixels", type=int, default=2048,
                     help="pixels checked against the oracle")
     ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--only", choices=["plain", "split"], default=None,
+                    help="time one precision only (profiling runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one device-to-device copy of this many MiB first, as the "
+                         "known-size reference for HBM byte counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -75,6 +80,14 @@ def main():
     torch.cuda.synchronize()
     out = torch.empty(sc.out_grid.shape, dtype=torch.complex64, device=dev)
     opts = {"tile_rows": args.tile_rows} if args.tile_rows else None
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"calibration copy of {args.calibrate_copy} MiB done")
 
     def step(precision):
         st = {}
@@ -83,7 +96,8 @@ def main():
         return st
 
     results = {}
-    for precision in ("plain", "split"):
+    precisions = (args.only,) if args.only else ("plain", "split")
+    for precision in precisions:
         for _ in range(args.warmup):
             step(precision)
         torch.cuda.synchronize()
@@ -112,6 +126,8 @@ def main():
     nth = min(16, os.cpu_count() or 1)
     acc = {}
     for precision, mode in (("plain", oracle.MODE_REFERENCE), ("split", oracle.MODE_FP64)):
+        if precision not in results or args.sample_pixels <= 0:
+            continue
         ref, _, _, _ = oracle.backproject(sc.out_geometry(), raw_np, sc.in_geometry(), sc.dem,
                                           sc.fc, sc.ds, sc.kernel, sc.dry_tropo_model,
                                           mode=mode, subset=subset, nthreads=nth,
@@ -156,6 +172,12 @@ def main():
                          f"{t_total:.1f} s"}
         log(f"cpu baseline: {cpu['value']:.3e} updates/s on {nth} threads")
 
+    if args.only:
+        r = results[args.only]
+        print(json.dumps({"only": args.only, "ms_per_step": r["ms_per_step"],
+                          "sum_ms": r["sum_ms"], "updates": r["updates"],
+                          "updates_per_s": r["updates_per_s"]}))
+        return
     sp, pl = results["split"], results["plain"]
     achieved = sp["updates"] * FLOPS_PER_UPDATE / (sp["sum_ms"] * 1e-3) / 1e12
     line = {

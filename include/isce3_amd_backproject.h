@@ -134,10 +134,12 @@ typedef struct {
 
 /* Launch/tiling options; zero-initialise for defaults. */
 typedef struct {
-    int32_t tile_rows;     /* output rows per 64-pixel wavefront tile: 1,2,4,8 (0 = auto) */
+    int32_t tile_rows;     /* output rows per 64-pixel wavefront tile: 1,2,4,8 (0 = 1) */
     int32_t chunk;         /* pulses per compensated FP32 partial sum (split; 0 = 32) */
-    int32_t fit_degree;    /* split: delay polynomial degree 2..6 (0 = 4) */
-    int32_t reserved[5];
+    int32_t fit_degree;    /* split: delay polynomial degree (0 = 4; only 4 is built) */
+    int32_t order;         /* tile walk of the sum kernels: 0 auto, 1 row-major, 2 column-major */
+    int32_t blocks_per_cu; /* persistent 512-thread workgroups per CU (0 = occupancy limit) */
+    int32_t reserved[3];
 } bp_options;
 
 /* Optional per-call measurements (filled when non-null). */

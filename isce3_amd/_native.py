@@ -76,7 +76,7 @@ class Geo2RdrParams_t(C.Structure):
 
 class Options_t(C.Structure):
     _fields_ = [("tile_rows", C.c_int32), ("chunk", C.c_int32), ("fit_degree", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("order", C.c_int32), ("blocks_per_cu", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Stats_t(C.Structure):

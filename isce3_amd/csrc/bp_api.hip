@@ -15,10 +15,6 @@
 #include "../../include/isce3_amd_backproject.h"
 #include "bp_internal.h"
 
-namespace bp {
-size_t split_lds_bytes(int width);
-}
-
 namespace {
 
 thread_local std::string g_msg;
@@ -38,6 +34,7 @@ struct DevBuf {
 
 struct bp_context {
     int device = 0;
+    int n_cu = 0;
     DevBuf table, poly, in_pos, in_vel, out_pos, out_vel, in_dop, out_dop, dem;
     DevBuf pulse_pv, geom, split, flags;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -185,7 +182,7 @@ bp::Dem host_dem(const bp_dem& m)
     return d;
 }
 
-bp::TileMap make_map(int64_t length, int64_t width, int tile_rows)
+bp::TileMap make_map(int64_t length, int64_t width, int tile_rows, int order)
 {
     bp::TileMap m;
     m.tile_rows = tile_rows;
@@ -196,7 +193,7 @@ bp::TileMap make_map(int64_t length, int64_t width, int tile_rows)
     m.width = width;
     m.n_tiles = (int64_t)m.tiles_x * m.tiles_y;
     m.n_blocks = (int32_t)((m.n_tiles + 3) / 4);
-    m.pad_ = 0;
+    m.order = order;
     return m;
 }
 
@@ -223,11 +220,13 @@ void build_polyphase(const bp_kernel& k, int width, std::vector<float>& out)
         const double t0 = tmin + (double)q / Q;
         for (int m = 0; m < width; ++m) w[(size_t)q * width + m] = weight((m - width / 2) + t0);
     }
-    out.assign((size_t)Q * 2 * wp, 0.f);
-    for (int q = 0; q < Q; ++q) {
+    // Q + 1 rows so that offsets t0 = tmin + 1 (q = Q) index a valid row
+    // (whose deltas are zero) without clamping in the kernel.
+    out.assign((size_t)(Q + 1) * 2 * wp, 0.f);
+    for (int q = 0; q <= Q; ++q) {
         for (int m = 0; m < width; ++m) {
             const float w0 = (float)w[(size_t)q * width + m];
-            const float w1 = (float)w[(size_t)(q + 1) * width + m];
+            const float w1 = q < Q ? (float)w[(size_t)(q + 1) * width + m] : w0;
             out[(size_t)q * 2 * wp + m] = w0;
             out[(size_t)q * 2 * wp + wp + m] = w1 - w0;
         }
@@ -251,6 +250,9 @@ int bp_context_create(int device, bp_context** ctx)
     BP_HIP_CHECK(hipSetDevice(device));
     auto* c = new bp_context();
     c->device = device;
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+        c->n_cu < 1)
+        c->n_cu = 256;
     for (auto& ev : c->ev) {
         if (hipEventCreate(&ev) != hipSuccess) {
             delete c;
@@ -324,6 +326,9 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
     if (chunk < 1 || chunk > 4096) return fail(BP_ERR_INVALID_ARGUMENT, "chunk must be in [1, 4096]");
     if (opt.fit_degree != 0 && opt.fit_degree != bp::kFitDegree)
         return fail(BP_ERR_INVALID_ARGUMENT, "fit_degree must be 4");
+    if (opt.order < 0 || opt.order > 2) return fail(BP_ERR_INVALID_ARGUMENT, "order must be 0, 1 or 2");
+    if (opt.blocks_per_cu < 0 || opt.blocks_per_cu > 16)
+        return fail(BP_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [0, 16]");
 
     const int64_t na = gi.length, nr = gi.width;
     const int64_t npix = go.length * go.width;
@@ -378,7 +383,11 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
 
     const double swst = 2. * gi.starting_range / bp::kC;
     const double dtau = 2. * gi.range_pixel_spacing / bp::kC;
-    const bp::TileMap map = make_map(go.length, go.width, tile_rows);
+    const bp::TileMap map = make_map(go.length, go.width, tile_rows, opt.order == 1 ? 0 : 1);
+    const int per_cu = opt.blocks_per_cu > 0 ? opt.blocks_per_cu
+                                             : bp::sum_blocks_per_cu(precision, width);
+    int grid = c->n_cu * per_cu;
+    grid = (grid + 7) / 8 * 8;
 
     bp::Dem ddem = host_dem(*dem);
     ddem.data = static_cast<const float*>(c->dem.ptr);
@@ -449,9 +458,9 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
     sa.poly = static_cast<const float*>(c->poly.ptr);
     sa.out = static_cast<float2*>(out_dev);
     if (precision == BP_PRECISION_PLAIN)
-        BP_HIP_CHECK(bp::launch_sum_plain(sa, width, s));
+        BP_HIP_CHECK(bp::launch_sum_plain(sa, width, grid, s));
     else
-        BP_HIP_CHECK(bp::launch_sum_split(sa, width, s));
+        BP_HIP_CHECK(bp::launch_sum_split(sa, width, grid, s));
     if (stats) BP_HIP_CHECK(hipEventRecord(c->ev[2], s));
 
     int32_t hflags[8];

@@ -55,8 +55,8 @@ struct TileMap {
     int32_t tiles_x, tiles_y;
     int64_t length, width;      // output grid
     int64_t n_tiles;
-    int32_t n_blocks;
-    int32_t pad_;
+    int32_t n_blocks;           // 256-thread blocks of the geometry pass
+    int32_t order;              // sum-kernel tile sequence: 0 row-major, 1 column-major
 };
 
 struct GeomArgs {
@@ -104,7 +104,9 @@ struct SumArgs {
 hipError_t launch_pulse_orbit(const Orbit& orbit, double t0, double dt, int na, double* pv,
                               int32_t* flags, hipStream_t stream);
 hipError_t launch_geometry(const GeomArgs& args, hipStream_t stream);
-hipError_t launch_sum_plain(const SumArgs& args, int width, hipStream_t stream);
-hipError_t launch_sum_split(const SumArgs& args, int width, hipStream_t stream);
+hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_t stream);
+hipError_t launch_sum_split(const SumArgs& args, int width, int grid, hipStream_t stream);
+int sum_blocks_per_cu(int precision, int width);
+size_t split_lds_bytes(int width);
 
 }  // namespace bp

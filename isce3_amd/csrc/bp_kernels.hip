@@ -249,21 +249,45 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
     a.split[pix] = sp;
 }
 
-// Reference-semantics coherent sum (Backproject.cpp:30-63): FP64 bistatic
-// delay, range-bin position and phase per pulse; float tabulated taps with
-// FP64 index math and a complex<float> inner product; complex<double> sum.
-template<int W>
-__global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
-{
-    extern __shared__ float s_tab[];
-    const bool tab_in_lds = a.ntab <= kMaxTableLds;
-    if (tab_in_lds)
-        for (int i = threadIdx.x; i < a.ntab; i += blockDim.x) s_tab[i] = a.table[i];
-    __syncthreads();
-    const float* tab = tab_in_lds ? s_tab : a.table;
+// Persistent tile schedule of the sum kernels.  A workgroup holds 8
+// wavefronts; the grid is a multiple of 8 workgroups, spread round-robin over
+// the 8 XCDs.  Each XCD walks a contiguous chunk of the tile sequence, and with
+// column-major order (map.order == 1) the wavefronts running together cover a
+// narrow band of range columns across many output lines: their pulse windows
+// overlap, so the pulse lines they gather stay resident in that XCD's L2.
+constexpr int kSumWaves = 8;
 
-    int lane;
-    const int64_t tile = wave_tile(a.map, &lane);
+__device__ inline int64_t seq_to_tile(const TileMap& m, int64_t s)
+{
+    if (m.order == 0) return s;
+    const int64_t tx = s / m.tiles_y, ty = s % m.tiles_y;
+    return ty * m.tiles_x + tx;
+}
+
+template<typename Body>
+__device__ inline void for_each_tile(const TileMap& m, Body&& body)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int xcd = blockIdx.x & 7;
+    const int64_t j = blockIdx.x >> 3;
+    const int64_t n_local = gridDim.x >> 3;
+    const int64_t n_wgseq = (m.n_tiles + kSumWaves - 1) / kSumWaves;
+    const int64_t chunk = (n_wgseq + 7) / 8;
+    const int64_t end = min(n_wgseq, (xcd + 1) * chunk);
+    for (int64_t w = xcd * chunk + j; w < end; w += n_local) {
+        const int64_t s = w * kSumWaves + wave;
+        const int64_t tile = s < m.n_tiles ? seq_to_tile(m, s) : m.n_tiles;
+        body(tile, lane);
+    }
+}
+
+// Reference-semantics coherent sum of one wavefront tile (Backproject.cpp:30-63):
+// FP64 bistatic delay, range-bin position and phase per pulse; float tabulated
+// taps with FP64 index math and a complex<float> inner product; complex<double> sum.
+template<int W>
+__device__ inline void plain_tile(const SumArgs& a, const float* tab, int64_t tile, int lane)
+{
     int64_t row = 0, col = 0;
     const bool valid = tile_pixel(a.map, tile, lane, &row, &col);
     const int64_t pix = row * a.map.width + col;
@@ -319,24 +343,128 @@ __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
     }
 }
 
-// Split-precision coherent sum.  Per pulse: FP64 Horner of the fitted phase
-// polynomial (full-rate FP64 FMA on CDNA), FP32 range-bin position, weights
-// linearly interpolated between polyphase rows held in LDS, FP32 complex MACs
-// and rotation, FP32 chunk sums folded into a Kahan-compensated accumulator.
+// One 256-thread workgroup per four tiles (measured faster than the
+// persistent walk for this FP64-bound kernel, which needs ~100 VGPRs).
 template<int W>
-__global__ __launch_bounds__(256) void sum_split_kernel(SumArgs a)
+__global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
 {
-    constexpr int WP = (W + 3) / 4 * 4;  // padded tap count (float4 rows)
-    constexpr int ROW = 2 * WP;          // weights then deltas
-    constexpr int Q = kPolyphaseRows;
-    constexpr float TMIN = (W % 2 == 0) ? 0.0f : -0.5f;
-    extern __shared__ float4 s_poly4[];
-    float* s_poly = reinterpret_cast<float*>(s_poly4);
-    for (int i = threadIdx.x; i < Q * ROW; i += blockDim.x) s_poly[i] = a.poly[i];
+    extern __shared__ float s_tab[];
+    const bool tab_in_lds = a.ntab <= kMaxTableLds;
+    if (tab_in_lds)
+        for (int i = threadIdx.x; i < a.ntab; i += blockDim.x) s_tab[i] = a.table[i];
     __syncthreads();
-
+    const float* tab = tab_in_lds ? s_tab : a.table;
     int lane;
     const int64_t tile = wave_tile(a.map, &lane);
+    plain_tile<W>(a, tab, tile, lane);
+}
+
+// Split-precision coherent sum of one wavefront tile.
+//
+// Per pixel (FP64, from the geometry pass): phase polynomial coefficients
+// b0..b3 of s = (k - kc)/h, the reduced reference phase c_ref and the
+// reference range bin n_ref + f_ref.  Per pulse: s by one FP64 FMA, the phase
+// offset dc by FP64 Horner (full-rate FP64 FMA on CDNA), the phase fraction by
+// v_fract_f64, the fractional range-bin position f_ref + dc/(fc dtau) in FP32,
+// tap weights linearly interpolated between polyphase rows in LDS, FP32 complex
+// MACs, hardware sin/cos of the phase in revolutions, and a fused
+// rotate-accumulate into an FP32 partial sum that is folded every `chunk`
+// pulses into a Kahan-compensated accumulator.
+//
+// The pulses every lane of the wavefront integrates, [max kstart, min kstop),
+// run in an unmasked loop whose tap loads are issued one pulse ahead; the
+// ragged ends of the lanes' windows run in a masked loop.
+template<int W>
+struct SplitPulse {
+    float ph;   // phase, revolutions in [0, 1)
+    float t0;   // fractional offset of the first tap
+    int c0;     // first tap's range bin
+};
+
+template<int W>
+__device__ inline SplitPulse<W> split_pulse(double kd, double s_off, double inv_h, double b0,
+                                            double b1, double b2, double b3, double c_ref,
+                                            float inv_kf, float f_ref, int n_ref)
+{
+    const double s = fma(kd, inv_h, s_off);
+    const double dc = fma(fma(fma(b3, s, b2), s, b1), s, b0) * s;
+    SplitPulse<W> r;
+    r.ph = (float)__builtin_amdgcn_fract(c_ref + dc);
+    const float ul = fmaf((float)dc, inv_kf, f_ref);
+    const float fi = (W % 2 == 0) ? ceilf(ul) : roundf(ul);
+    r.t0 = fi - ul;
+    r.c0 = n_ref + (int)fi - W / 2;
+    return r;
+}
+
+template<int W>
+__device__ inline void split_taps(const float2* line, int c0, int nr, float2 (&d)[W])
+{
+    // clamp the start so the load stays inside the line; lanes whose window
+    // reaches past an end are fixed up in split_update
+    if (nr < W) {
+#pragma unroll
+        for (int m = 0; m < W; ++m) d[m] = make_float2(0.f, 0.f);
+        return;
+    }
+    const int cb = min(max(c0, 0), nr - W);
+    const float2* p = line + cb;
+#pragma unroll
+    for (int m = 0; m < W; ++m) d[m] = p[m];
+}
+
+template<int W>
+__device__ inline void split_update(const float* s_poly, const float2* line, int nr,
+                                    const SplitPulse<W>& pu, const float2 (&d)[W], float& cre,
+                                    float& cim)
+{
+    constexpr int WP = (W + 3) / 4 * 4;
+    constexpr int ROW = 2 * WP;
+    constexpr float TMIN = (W % 2 == 0) ? 0.0f : -0.5f;
+    const float q = (pu.t0 - TMIN) * (float)kPolyphaseRows;
+    const int qi = (int)q;  // in [0, Q]: the table holds Q + 1 rows
+    const float qf = __builtin_amdgcn_fractf(q);
+    const float* rowp = s_poly + qi * ROW;
+    float sre = 0.f, sim = 0.f;
+    const int cb = min(max(pu.c0, 0), nr - W);
+    if (nr >= W && cb == pu.c0) {
+#pragma unroll
+        for (int m = 0; m < W; ++m) {
+            const float w = fmaf(qf, rowp[WP + m], rowp[m]);
+            sre = fmaf(w, d[m].x, sre);
+            sim = fmaf(w, d[m].y, sim);
+        }
+    } else {
+        // taps outside [0, nr) are zero (core/detail/Interp1d.h:62-83)
+#pragma unroll
+        for (int m = 0; m < W; ++m) {
+            const int j = pu.c0 + m;
+            const float w = fmaf(qf, rowp[WP + m], rowp[m]);
+            const float2 v = (j >= 0 && j < nr) ? line[j] : make_float2(0.f, 0.f);
+            sre = fmaf(w, v.x, sre);
+            sim = fmaf(w, v.y, sim);
+        }
+    }
+    const float sn = __builtin_amdgcn_sinf(pu.ph);
+    const float cs = __builtin_amdgcn_cosf(pu.ph);
+    cre = fmaf(sre, cs, cre);
+    cre = fmaf(-sim, sn, cre);
+    cim = fmaf(sre, sn, cim);
+    cim = fmaf(sim, cs, cim);
+}
+
+__device__ inline void kahan_fold(float& acc, float& cmp, float& part)
+{
+    const float y = part - cmp;
+    const float t = acc + y;
+    cmp = (t - acc) - y;
+    acc = t;
+    part = 0.f;
+}
+
+template<int W>
+__device__ inline void split_tile(const SumArgs& a, const float* s_poly, int64_t tile, int lane)
+{
     int64_t row = 0, col = 0;
     const bool valid = tile_pixel(a.map, tile, lane, &row, &col);
     const int64_t pix = row * a.map.width + col;
@@ -344,78 +472,96 @@ __global__ __launch_bounds__(256) void sum_split_kernel(SumArgs a)
     PixSplit sp;
     if (valid) sp = a.split[pix];
     const bool ok = valid && sp.status == kPixOk;
-    const int ks = (ok && sp.kstop > sp.kstart) ? sp.kstart : INT_MAX;
-    const int ke = (ok && sp.kstop > sp.kstart) ? sp.kstop : INT_MIN;
+    const bool win = ok && sp.kstop > sp.kstart;
+    const int ks = win ? sp.kstart : INT_MAX;
+    const int ke = win ? sp.kstop : INT_MIN;
+    // union window [kw0, kw1) of the wavefront; empty (kw0 >= kw1) when no
+    // lane has pulses, including wavefronts whose tile lies past the grid
     const int kw0 = __builtin_amdgcn_readfirstlane(wave_min_i(ks));
     const int kw1 = __builtin_amdgcn_readfirstlane(wave_max_i(ke));
+    // pulses integrated by every lane of the wavefront, [kf0, kf1) inside
+    // [kw0, kw1): run unmasked only when every valid lane has a window
+    int kf0 = __builtin_amdgcn_readfirstlane(wave_max_i(win ? ks : INT_MIN));
+    int kf1 = __builtin_amdgcn_readfirstlane(wave_min_i(win ? ke : INT_MAX));
+    const bool all_win = __builtin_amdgcn_readfirstlane(wave_min_i(valid ? (int)win : 1)) != 0;
+    if (kw0 >= kw1 || !all_win || kf0 < kw0 || kf1 > kw1 || kf0 >= kf1) {
+        kf0 = kw1;
+        kf1 = kw1;
+    }
     const double b0 = ok ? sp.b[0] : 0., b1 = ok ? sp.b[1] : 0.;
     const double b2 = ok ? sp.b[2] : 0., b3 = ok ? sp.b[3] : 0.;
-    const double c_ref = ok ? sp.c_ref : 0., kc = ok ? sp.kc : 0., inv_h = ok ? sp.inv_h : 0.;
+    const double c_ref = ok ? sp.c_ref : 0.;
+    const double inv_h = ok ? sp.inv_h : 0.;
+    const double s_off = ok ? -sp.kc * sp.inv_h : 0.;
     const float f_ref = ok ? sp.f_ref : 0.f;
     const int n_ref = ok ? sp.n_ref : 0;
-    const double inv_k = a.inv_k;
-    const int64_t nr = a.nr;
+    const float inv_kf = (float)a.inv_k;
+    const int nr = (int)a.nr;
     const int chunk = a.chunk;
+    const float2* rc = a.rc;
 
     float acc_re = 0.f, acc_im = 0.f, cmp_re = 0.f, cmp_im = 0.f;
-    for (int k0 = kw0; k0 < kw1; k0 += chunk) {
-        const int kend = min(k0 + chunk, kw1);
-        float cre = 0.f, cim = 0.f;
-        for (int k = k0; k < kend; ++k) {
-            if (k < ks || k >= ke) continue;
-            const double s = ((double)k - kc) * inv_h;
-            const double dc = fma(fma(fma(b3, s, b2), s, b1), s, b0) * s;
-            double cy = c_ref + dc;
-            cy -= rint(cy);
-            const float ph = (float)cy;  // revolutions in [-0.5, 0.5]
-            const float ul = f_ref + (float)(dc * inv_k);
-            const float fi = (W % 2 == 0) ? ceilf(ul) : roundf(ul);
-            const float t0 = fi - ul;
-            const int c0 = n_ref + (int)fi - W / 2;
-            const float q = (t0 - TMIN) * (float)Q;
-            const int qi = max(min((int)q, Q - 1), 0);
-            const float qf = q - (float)qi;
-            const float* rowp = s_poly + qi * ROW;
-            float w[W];
-#pragma unroll
-            for (int m = 0; m < W; ++m) w[m] = fmaf(qf, rowp[WP + m], rowp[m]);
-            const float2* line = a.rc + (int64_t)k * nr;
-            float sre = 0.f, sim = 0.f;
-            if (c0 >= 0 && c0 + W <= nr) {
-#pragma unroll
-                for (int m = 0; m < W; ++m) {
-                    const float2 d = line[c0 + m];
-                    sre = fmaf(w[m], d.x, sre);
-                    sim = fmaf(w[m], d.y, sim);
-                }
-            } else {
-#pragma unroll
-                for (int m = 0; m < W; ++m) {
-                    const int64_t j = (int64_t)c0 + m;
-                    const float2 d = (j >= 0 && j < nr) ? line[j] : make_float2(0.f, 0.f);
-                    sre = fmaf(w[m], d.x, sre);
-                    sim = fmaf(w[m], d.y, sim);
-                }
+    float cre = 0.f, cim = 0.f;
+
+    // masked loop over [k_lo, k_hi): ragged window ends
+    auto masked = [&](int k_lo, int k_hi) {
+        for (int k0 = k_lo; k0 < k_hi; k0 += chunk) {
+            const int k1 = min(k0 + chunk, k_hi);
+            for (int k = k0; k < k1; ++k) {
+                if (k < ks || k >= ke) continue;
+                const SplitPulse<W> pu = split_pulse<W>((double)k, s_off, inv_h, b0, b1, b2, b3,
+                                                        c_ref, inv_kf, f_ref, n_ref);
+                const float2* line = rc + (int64_t)k * nr;
+                float2 d[W];
+                split_taps<W>(line, pu.c0, nr, d);
+                split_update<W>(s_poly, line, nr, pu, d, cre, cim);
             }
-            const float sn = __builtin_amdgcn_sinf(ph);
-            const float cs = __builtin_amdgcn_cosf(ph);
-            cre += sre * cs - sim * sn;
-            cim += sre * sn + sim * cs;
+            kahan_fold(acc_re, cmp_re, cre);
+            kahan_fold(acc_im, cmp_im, cim);
         }
-        // Kahan-compensated fold of the chunk sum.
-        const float yr = cre - cmp_re;
-        const float tr = acc_re + yr;
-        cmp_re = (tr - acc_re) - yr;
-        acc_re = tr;
-        const float yi = cim - cmp_im;
-        const float ti = acc_im + yi;
-        cmp_im = (ti - acc_im) - yi;
-        acc_im = ti;
+    };
+
+    if (kw0 < kw1) masked(kw0, kf0);
+    // unmasked, software-pipelined loop over [kf0, kf1) (empty unless
+    // kw0 <= kf0 < kf1 <= kw1, so every pulse index is some lane's)
+    for (int k0 = kf0; k0 < kf1; k0 += chunk) {
+        const int k1 = min(k0 + chunk, kf1);
+        SplitPulse<W> pn = split_pulse<W>((double)k0, s_off, inv_h, b0, b1, b2, b3, c_ref, inv_kf,
+                                          f_ref, n_ref);
+        float2 dn[W];
+        split_taps<W>(rc + (int64_t)k0 * nr, pn.c0, nr, dn);
+        for (int k = k0; k < k1; ++k) {
+            const SplitPulse<W> pu = pn;
+            float2 d[W];
+#pragma unroll
+            for (int m = 0; m < W; ++m) d[m] = dn[m];
+            if (k + 1 < k1) {
+                pn = split_pulse<W>((double)(k + 1), s_off, inv_h, b0, b1, b2, b3, c_ref, inv_kf,
+                                    f_ref, n_ref);
+                split_taps<W>(rc + (int64_t)(k + 1) * nr, pn.c0, nr, dn);
+            }
+            split_update<W>(s_poly, rc + (int64_t)k * nr, nr, pu, d, cre, cim);
+        }
+        kahan_fold(acc_re, cmp_re, cre);
+        kahan_fold(acc_im, cmp_im, cim);
     }
+    if (kw0 < kw1) masked(max(kf1, kw0), kw1);
+
     if (valid) {
         const float nanf_ = __builtin_nanf("");
         a.out[pix] = ok ? make_float2(acc_re - cmp_re, acc_im - cmp_im) : make_float2(nanf_, nanf_);
     }
+}
+
+template<int W>
+__global__ __launch_bounds__(512) void sum_split_kernel(SumArgs a)
+{
+    constexpr int ROW = 2 * ((W + 3) / 4 * 4);
+    extern __shared__ float4 s_poly4[];
+    float* s_poly = reinterpret_cast<float*>(s_poly4);
+    for (int i = threadIdx.x; i < (kPolyphaseRows + 1) * ROW; i += blockDim.x) s_poly[i] = a.poly[i];
+    __syncthreads();
+    for_each_tile(a.map, [&](int64_t tile, int lane) { split_tile<W>(a, s_poly, tile, lane); });
 }
 
 // ------------------------------------------------------------------ launchers
@@ -438,9 +584,10 @@ hipError_t launch_geometry(const GeomArgs& args, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t launch_sum_plain(const SumArgs& args, int width, hipStream_t stream)
+hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_t stream)
 {
-    if (args.map.n_blocks <= 0) return hipSuccess;
+    (void)grid;
+    if (args.map.n_tiles <= 0) return hipSuccess;
     const size_t lds = args.ntab <= kMaxTableLds ? sizeof(float) * args.ntab : 0;
     switch (width) {
     case 8:
@@ -455,22 +602,45 @@ hipError_t launch_sum_plain(const SumArgs& args, int width, hipStream_t stream)
     return hipGetLastError();
 }
 
+// Workgroups of 512 threads one CU can hold for the chosen sum kernel.
+int sum_blocks_per_cu(int precision, int width)
+{
+    int n = 0;
+    hipError_t e = hipSuccess;
+    if (precision == 0) {
+        const size_t lds = sizeof(float) * kMaxTableLds;
+        e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_plain_kernel<8>, 256, lds)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_plain_kernel<9>, 256, lds);
+    } else {
+        const size_t lds = split_lds_bytes(width);
+        e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<8>, 512, lds)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<9>, 512, lds);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n < 1 ? 1 : n;
+}
+
 size_t split_lds_bytes(int width)
 {
     const int wp = (width + 3) / 4 * 4;
-    return sizeof(float) * (size_t)kPolyphaseRows * 2 * wp;
+    return sizeof(float) * (size_t)(kPolyphaseRows + 1) * 2 * wp;
 }
 
-hipError_t launch_sum_split(const SumArgs& args, int width, hipStream_t stream)
+// Persistent grid: `grid` workgroups of 512 threads (a multiple of 8, one
+// share per XCD), sized by the caller from the device's CU count and occupancy.
+hipError_t launch_sum_split(const SumArgs& args, int width, int grid, hipStream_t stream)
 {
-    if (args.map.n_blocks <= 0) return hipSuccess;
+    if (args.map.n_tiles <= 0) return hipSuccess;
     const size_t lds = split_lds_bytes(width);
     switch (width) {
     case 8:
-        hipLaunchKernelGGL(sum_split_kernel<8>, dim3(args.map.n_blocks), dim3(256), lds, stream, args);
+        hipLaunchKernelGGL(sum_split_kernel<8>, dim3(grid), dim3(512), lds, stream, args);
         break;
     case 9:
-        hipLaunchKernelGGL(sum_split_kernel<9>, dim3(args.map.n_blocks), dim3(256), lds, stream, args);
+        hipLaunchKernelGGL(sum_split_kernel<9>, dim3(grid), dim3(512), lds, stream, args);
         break;
     default:
         return hipErrorInvalidValue;

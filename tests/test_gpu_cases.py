@@ -225,3 +225,17 @@ def test_context_reuse_across_sizes(torch_gpu):
     b2, _, _ = gpu(sc2, raw2, precision="split")
     np.testing.assert_array_equal(a1, a2)
     np.testing.assert_array_equal(b1, b2)
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (3, 65), (17, 129), (9, 200)])
+@pytest.mark.parametrize("precision", ["plain", "split"])
+def test_ragged_output_grids(torch_gpu, shape, precision):
+    """Output grids whose tile counts are not multiples of the workgroup's
+    wavefront count leave wavefronts past the grid; they must do nothing."""
+    sc, raw = scene_and_raw("tiny")
+    g = sc.out_grid.offset_and_resize(3.0, 2.0, shape[0], shape[1])
+    g.range_pixel_spacing = sc.out_grid.range_pixel_spacing * 0.37
+    z, _, err = gpu(sc, raw, out_grid=g, precision=precision)
+    ref, _, rc = orc(sc, raw, out_grid=g, mode=0 if precision == "plain" else 1)
+    tol = 1e-5 if precision == "plain" else 1e-3
+    assert complex_rel_error(z, ref) <= tol
