@@ -397,23 +397,19 @@ __device__ inline SplitPulse<W> split_pulse(double kd, double s_off, double inv_
     return r;
 }
 
+// Requires nr >= W (split_tile routes narrower lines to the guarded loads).
 template<int W>
 __device__ inline void split_taps(const float2* line, int c0, int nr, float2 (&d)[W])
 {
     // clamp the start so the load stays inside the line; lanes whose window
     // reaches past an end are fixed up in split_update
-    if (nr < W) {
-#pragma unroll
-        for (int m = 0; m < W; ++m) d[m] = make_float2(0.f, 0.f);
-        return;
-    }
     const int cb = min(max(c0, 0), nr - W);
     const float2* p = line + cb;
 #pragma unroll
     for (int m = 0; m < W; ++m) d[m] = p[m];
 }
 
-template<int W>
+template<int W, bool kGuarded = false>
 __device__ inline void split_update(const float* s_poly, const float2* line, int nr,
                                     const SplitPulse<W>& pu, const float2 (&d)[W], float& cre,
                                     float& cim)
@@ -424,25 +420,34 @@ __device__ inline void split_update(const float* s_poly, const float2* line, int
     const float q = (pu.t0 - TMIN) * (float)kPolyphaseRows;
     const int qi = (int)q;  // in [0, Q]: the table holds Q + 1 rows
     const float qf = __builtin_amdgcn_fractf(q);
-    const float* rowp = s_poly + qi * ROW;
+    // one polyphase row: WP weights then WP deltas, read as float4
+    const float4* row4 = reinterpret_cast<const float4*>(s_poly + qi * ROW);
+    float w[WP];
+#pragma unroll
+    for (int i = 0; i < WP / 4; ++i) {
+        const float4 a = row4[i];
+        const float4 b = row4[WP / 4 + i];
+        w[4 * i + 0] = fmaf(qf, b.x, a.x);
+        w[4 * i + 1] = fmaf(qf, b.y, a.y);
+        w[4 * i + 2] = fmaf(qf, b.z, a.z);
+        w[4 * i + 3] = fmaf(qf, b.w, a.w);
+    }
     float sre = 0.f, sim = 0.f;
     const int cb = min(max(pu.c0, 0), nr - W);
-    if (nr >= W && cb == pu.c0) {
+    if (!kGuarded && cb == pu.c0) {
 #pragma unroll
         for (int m = 0; m < W; ++m) {
-            const float w = fmaf(qf, rowp[WP + m], rowp[m]);
-            sre = fmaf(w, d[m].x, sre);
-            sim = fmaf(w, d[m].y, sim);
+            sre = fmaf(w[m], d[m].x, sre);
+            sim = fmaf(w[m], d[m].y, sim);
         }
     } else {
         // taps outside [0, nr) are zero (core/detail/Interp1d.h:62-83)
 #pragma unroll
         for (int m = 0; m < W; ++m) {
             const int j = pu.c0 + m;
-            const float w = fmaf(qf, rowp[WP + m], rowp[m]);
             const float2 v = (j >= 0 && j < nr) ? line[j] : make_float2(0.f, 0.f);
-            sre = fmaf(w, v.x, sre);
-            sim = fmaf(w, v.y, sim);
+            sre = fmaf(w[m], v.x, sre);
+            sim = fmaf(w[m], v.y, sim);
         }
     }
     const float sn = __builtin_amdgcn_sinf(pu.ph);
@@ -503,6 +508,28 @@ __device__ inline void split_tile(const SumArgs& a, const float* s_poly, int64_t
     float acc_re = 0.f, acc_im = 0.f, cmp_re = 0.f, cmp_im = 0.f;
     float cre = 0.f, cim = 0.f;
 
+    if (nr < W) {
+        // lines narrower than the kernel: every tap through the guarded path
+        for (int k = kw0; k < kw1; ++k) {
+            if (k < ks || k >= ke) continue;
+            const SplitPulse<W> pu = split_pulse<W>((double)k, s_off, inv_h, b0, b1, b2, b3,
+                                                    c_ref, inv_kf, f_ref, n_ref);
+            float2 d[W];
+            split_update<W, true>(s_poly, rc + (int64_t)k * nr, nr, pu, d, cre, cim);
+            if ((k - kw0) % chunk == chunk - 1) {
+                kahan_fold(acc_re, cmp_re, cre);
+                kahan_fold(acc_im, cmp_im, cim);
+            }
+        }
+        kahan_fold(acc_re, cmp_re, cre);
+        kahan_fold(acc_im, cmp_im, cim);
+        if (valid) {
+            const float nanf_ = __builtin_nanf("");
+            a.out[pix] = ok ? make_float2(acc_re - cmp_re, acc_im - cmp_im) : make_float2(nanf_, nanf_);
+        }
+        return;
+    }
+
     // masked loop over [k_lo, k_hi): ragged window ends
     auto masked = [&](int k_lo, int k_hi) {
         for (int k0 = k_lo; k0 < k_hi; k0 += chunk) {
@@ -524,24 +551,29 @@ __device__ inline void split_tile(const SumArgs& a, const float* s_poly, int64_t
     if (kw0 < kw1) masked(kw0, kf0);
     // unmasked, software-pipelined loop over [kf0, kf1) (empty unless
     // kw0 <= kf0 < kf1 <= kw1, so every pulse index is some lane's)
+    auto pulse = [&](int k) {
+        return split_pulse<W>((double)k, s_off, inv_h, b0, b1, b2, b3, c_ref, inv_kf, f_ref,
+                              n_ref);
+    };
     for (int k0 = kf0; k0 < kf1; k0 += chunk) {
         const int k1 = min(k0 + chunk, kf1);
-        SplitPulse<W> pn = split_pulse<W>((double)k0, s_off, inv_h, b0, b1, b2, b3, c_ref, inv_kf,
-                                          f_ref, n_ref);
-        float2 dn[W];
-        split_taps<W>(rc + (int64_t)k0 * nr, pn.c0, nr, dn);
-        for (int k = k0; k < k1; ++k) {
-            const SplitPulse<W> pu = pn;
-            float2 d[W];
-#pragma unroll
-            for (int m = 0; m < W; ++m) d[m] = dn[m];
-            if (k + 1 < k1) {
-                pn = split_pulse<W>((double)(k + 1), s_off, inv_h, b0, b1, b2, b3, c_ref, inv_kf,
-                                    f_ref, n_ref);
-                split_taps<W>(rc + (int64_t)(k + 1) * nr, pn.c0, nr, dn);
+        // ping-pong buffers: the taps of pulse k+1 are in flight while pulse k
+        // is accumulated
+        SplitPulse<W> pa = pulse(k0), pb;
+        float2 da[W], db[W];
+        split_taps<W>(rc + (int64_t)k0 * nr, pa.c0, nr, da);
+        int k = k0;
+        for (; k + 1 < k1; k += 2) {
+            pb = pulse(k + 1);
+            split_taps<W>(rc + (int64_t)(k + 1) * nr, pb.c0, nr, db);
+            split_update<W>(s_poly, rc + (int64_t)k * nr, nr, pa, da, cre, cim);
+            if (k + 2 < k1) {
+                pa = pulse(k + 2);
+                split_taps<W>(rc + (int64_t)(k + 2) * nr, pa.c0, nr, da);
             }
-            split_update<W>(s_poly, rc + (int64_t)k * nr, nr, pu, d, cre, cim);
+            split_update<W>(s_poly, rc + (int64_t)(k + 1) * nr, nr, pb, db, cre, cim);
         }
+        if (k < k1) split_update<W>(s_poly, rc + (int64_t)k * nr, nr, pa, da, cre, cim);
         kahan_fold(acc_re, cmp_re, cre);
         kahan_fold(acc_im, cmp_im, cim);
     }

@@ -239,3 +239,22 @@ def test_ragged_output_grids(torch_gpu, shape, precision):
     ref, _, rc = orc(sc, raw, out_grid=g, mode=0 if precision == "plain" else 1)
     tol = 1e-5 if precision == "plain" else 1e-3
     assert complex_rel_error(z, ref) <= tol
+
+
+@pytest.mark.parametrize("precision", ["plain", "split"])
+def test_lines_narrower_than_the_kernel(torch_gpu, precision):
+    """Range lines of 5 samples (fewer than the 8 taps): every tap goes through
+    the zero-padded path, as in core/detail/Interp1d.h:62-83."""
+    from isce3_amd.container import RadarGeometry
+    sc, raw = scene_and_raw("tiny")
+    ig = sc.in_grid.copy()
+    ig.width = 5
+    ig.starting_range = sc.out_grid.starting_range + 3 * sc.out_grid.range_pixel_spacing
+    narrow = np.ascontiguousarray(raw[:, 100:105])
+    geom = RadarGeometry(ig, sc.orbit, sc.doppler)
+    g = sc.out_grid.offset_and_resize(0, 0, 16, 12)
+    z, _, _ = gpu(sc, narrow, out_grid=g, precision=precision, in_geom=geom)
+    ref, _, _ = orc(sc, narrow, out_grid=g, mode=0 if precision == "plain" else 1, in_geom=geom)
+    assert np.abs(ref).max() > 0
+    tol = 1e-5 if precision == "plain" else 1e-3
+    assert complex_rel_error(z, ref) <= tol
