@@ -184,6 +184,40 @@ int bp_backproject(bp_context* ctx, const bp_radar_geometry* out_geometry,
                    const bp_options* options, void* stream, int* any_failed,
                    bp_stats* stats);
 
+/* One output block of bp_backproject_blocks. */
+typedef struct {
+    bp_grid grid;        /* output radar grid of this block */
+    void* out_dev;       /* device, complex64 [grid.length x grid.width], written */
+    float* height_dev;   /* device, float32 [same shape] or NULL */
+    int32_t failed;      /* set on return: some pixel of this block failed to converge */
+    int32_t pad_;
+} bp_out_block;
+
+/*
+ * Focus many output blocks of one range-compressed raw set in one pass: one
+ * geometry launch and one coherent-sum launch over the pixels of all blocks
+ * (blocks may differ in shape, start time, range and spacing).
+ *
+ * Replaces the per-block loop of isce3.focus.backproject calls in the RSLC
+ * workflow (python/packages/nisar/workflows/focus.py:1986-2004), whose blocks
+ * share the raw data, input geometry, output orbit and Doppler, DEM, kernel
+ * and parameters.  Each block's result equals bp_backproject on that block
+ * alone (the per-pixel arithmetic does not depend on the batch).
+ *
+ *   out_orbit, out_doppler  orbit and Doppler of every output block
+ *   blocks[n_blocks]        output grids and device buffers; failed is set
+ *
+ * Returns BP_SUCCESS / BP_FAILED_TO_CONVERGE (some block failed), or a
+ * negative error code.
+ */
+int bp_backproject_blocks(bp_context* ctx, const bp_orbit* out_orbit,
+                          const bp_lut2d* out_doppler, bp_out_block* blocks, int n_blocks,
+                          const bp_radar_geometry* in_geometry, const bp_dem* dem, double fc,
+                          double ds, const bp_kernel* kernel, int dry_tropo_model,
+                          const bp_rdr2geo_params* rdr2geo, const bp_geo2rdr_params* geo2rdr,
+                          const void* in_dev, int precision, const bp_options* options,
+                          void* stream, int* any_failed, bp_stats* stats);
+
 /* ---------------------------------------------------- host-side helpers
  * Host implementations of the same FP64 primitives the kernels use; they back
  * the Python mirror's Orbit.interpolate, Ellipsoid and DEM/LUT lookups. */

@@ -79,6 +79,11 @@ class Options_t(C.Structure):
                 ("order", C.c_int32), ("blocks_per_cu", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
+class OutBlock_t(C.Structure):
+    _fields_ = [("grid", Grid_t), ("out_dev", C.c_void_p), ("height_dev", C.c_void_p),
+                ("failed", C.c_int32), ("pad_", C.c_int32)]
+
+
 class Stats_t(C.Structure):
     _fields_ = [("updates", C.c_int64), ("geometry_ms", C.c_double), ("sum_ms", C.c_double),
                 ("any_failed", C.c_int32), ("pad_", C.c_int32)]
@@ -94,6 +99,11 @@ _SIGNATURES = {
                                  _P(Rdr2GeoParams_t), _P(Geo2RdrParams_t), C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_int, _P(Options_t), C.c_void_p,
                                  _P(C.c_int), _P(Stats_t)]),
+    "bp_backproject_blocks": (C.c_int, [C.c_void_p, _P(Orbit_t), _P(Lut2d_t), _P(OutBlock_t),
+                                        C.c_int, _P(RadarGeometry_t), _P(Dem_t), C.c_double,
+                                        C.c_double, _P(Kernel_t), C.c_int, _P(Rdr2GeoParams_t),
+                                        _P(Geo2RdrParams_t), C.c_void_p, C.c_int, _P(Options_t),
+                                        C.c_void_p, _P(C.c_int), _P(Stats_t)]),
     "bp_orbit_interpolate": (C.c_int, [_P(Orbit_t), C.c_double, C.c_int, _P(C.c_double),
                                        _P(C.c_double)]),
     "bp_lonlat_to_xyz": (None, [C.c_double, C.c_double, _P(C.c_double), _P(C.c_double)]),

@@ -36,9 +36,12 @@ struct bp_context {
     int device = 0;
     int n_cu = 0;
     DevBuf table, poly, in_pos, in_vel, out_pos, out_vel, in_dop, out_dop, dem;
-    DevBuf pulse_pv, geom, split, flags;
+    DevBuf pulse_pv, geom, split, flags, desc;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     std::vector<float> poly_host;
+    std::vector<bp::OutDesc> desc_host;
+    std::vector<int> desc_block;
+    std::vector<int32_t> flags_host;
 };
 
 namespace {
@@ -192,8 +195,8 @@ bp::TileMap make_map(int64_t length, int64_t width, int tile_rows, int order)
     m.length = length;
     m.width = width;
     m.n_tiles = (int64_t)m.tiles_x * m.tiles_y;
-    m.n_blocks = (int32_t)((m.n_tiles + 3) / 4);
     m.order = order;
+    m.pad_ = 0;
     return m;
 }
 
@@ -270,7 +273,7 @@ int bp_context_destroy(bp_context* c)
     (void)hipDeviceSynchronize();
     DevBuf* bufs[] = {&c->table, &c->poly, &c->in_pos, &c->in_vel, &c->out_pos, &c->out_vel,
                       &c->in_dop, &c->out_dop, &c->dem, &c->pulse_pv, &c->geom, &c->split,
-                      &c->flags};
+                      &c->flags, &c->desc};
     for (DevBuf* b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
     for (auto& ev : c->ev)
@@ -279,26 +282,46 @@ int bp_context_destroy(bp_context* c)
     return BP_SUCCESS;
 }
 
-int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_geometry* ig,
-                   const bp_dem* dem, double fc, double ds, const bp_kernel* kernel,
-                   int dry_tropo_model, const bp_rdr2geo_params* r2g,
-                   const bp_geo2rdr_params* g2r, const void* in_dev, void* out_dev,
-                   float* height_dev, int precision, const bp_options* options, void* stream,
-                   int* any_failed, bp_stats* stats)
+}  // extern "C"
+
+namespace {
+
+// Shared implementation of bp_backproject and bp_backproject_blocks: every
+// output block shares the output orbit and Doppler, the input geometry, DEM,
+// kernel and parameters; the pixels of all blocks are focused by one geometry
+// launch and one coherent-sum launch.
+int run_blocks(bp_context* c, const bp_orbit* oo_p, const bp_lut2d* odop_p, bp_out_block* blocks,
+               int n_blocks, const bp_radar_geometry* ig, const bp_dem* dem, double fc, double ds,
+               const bp_kernel* kernel, int dry_tropo_model, const bp_rdr2geo_params* r2g,
+               const bp_geo2rdr_params* g2r, const void* in_dev, int precision,
+               const bp_options* options, void* stream, int* any_failed, bp_stats* stats)
 {
-    if (!c || !og || !ig || !dem || !kernel || !r2g || !g2r)
+    if (any_failed) *any_failed = 0;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (!c || !oo_p || !odop_p || !ig || !dem || !kernel || !r2g || !g2r)
         return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    if (n_blocks < 0 || (n_blocks > 0 && blocks == nullptr))
+        return fail(BP_ERR_INVALID_ARGUMENT, "invalid output block list");
     if (dry_tropo_model != BP_TROPO_NODELAY && dry_tropo_model != BP_TROPO_TSX)
         return fail(BP_ERR_INVALID_ARGUMENT, "unexpected dry troposphere model");
     if (precision != BP_PRECISION_PLAIN && precision != BP_PRECISION_SPLIT)
         return fail(BP_ERR_INVALID_ARGUMENT, "unknown precision mode");
-    const bp_grid& go = og->grid;
     const bp_grid& gi = ig->grid;
-    if (go.length < 0 || go.width < 0 || gi.length < 0 || gi.width < 0)
-        return fail(BP_ERR_INVALID_ARGUMENT, "negative grid size");
-    if (gi.length > INT32_MAX || go.length > INT32_MAX || go.width > INT32_MAX)
-        return fail(BP_ERR_INVALID_ARGUMENT, "grid size exceeds int range");
-    if (!(gi.prf > 0) || !(go.prf > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "PRF must be positive");
+    if (gi.length < 0 || gi.width < 0) return fail(BP_ERR_INVALID_ARGUMENT, "negative grid size");
+    if (gi.length > INT32_MAX) return fail(BP_ERR_INVALID_ARGUMENT, "grid size exceeds int range");
+    if (!(gi.prf > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "PRF must be positive");
+    int64_t npix = 0;
+    for (int b = 0; b < n_blocks; ++b) {
+        const bp_grid& go = blocks[b].grid;
+        blocks[b].failed = 0;
+        if (go.length < 0 || go.width < 0) return fail(BP_ERR_INVALID_ARGUMENT, "negative grid size");
+        if (go.length > INT32_MAX || go.width > INT32_MAX)
+            return fail(BP_ERR_INVALID_ARGUMENT, "grid size exceeds int range");
+        if (!(go.prf > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "PRF must be positive");
+        if (go.length * go.width > 0 && blocks[b].out_dev == nullptr)
+            return fail(BP_ERR_INVALID_ARGUMENT, "out is null");
+        npix += go.length * go.width;
+    }
     if (!(fc > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "center frequency must be positive");
     if (!(ds > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "azimuth resolution must be positive");
     if (kernel->n < 2 || kernel->table == nullptr || !(kernel->halfwidth > 0))
@@ -313,9 +336,9 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
             return fail(BP_ERR_INVALID_ARGUMENT, "DEM raster must be at least 4x4");
     }
     int rc;
-    if ((rc = check_orbit(og->orbit, "output geometry")) != BP_SUCCESS) return rc;
+    if ((rc = check_orbit(*oo_p, "output geometry")) != BP_SUCCESS) return rc;
     if ((rc = check_orbit(ig->orbit, "input geometry")) != BP_SUCCESS) return rc;
-    if ((rc = check_lut(og->doppler, "output geometry")) != BP_SUCCESS) return rc;
+    if ((rc = check_lut(*odop_p, "output geometry")) != BP_SUCCESS) return rc;
     if ((rc = check_lut(ig->doppler, "input geometry")) != BP_SUCCESS) return rc;
     bp_options opt{};
     if (options) opt = *options;
@@ -331,10 +354,6 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
         return fail(BP_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [0, 16]");
 
     const int64_t na = gi.length, nr = gi.width;
-    const int64_t npix = go.length * go.width;
-    if (any_failed) *any_failed = 0;
-    if (stats) std::memset(stats, 0, sizeof(*stats));
-    if (npix > 0 && out_dev == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "out is null");
     if (npix > 0 && na * nr > 0 && in_dev == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "in is null");
 
     // Pulse times must lie in the input orbit: Orbit::interpolate with the
@@ -351,15 +370,48 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
     BP_HIP_CHECK(hipSetDevice(c->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
 
+    // Output descriptors: blocks with pixels, tiles numbered consecutively.
+    const int order = opt.order == 1 ? 0 : 1;
+    c->desc_host.clear();
+    c->desc_block.clear();
+    int64_t n_seq = 0, n_rec = 0;
+    for (int b = 0; b < n_blocks; ++b) {
+        const bp_grid& go = blocks[b].grid;
+        if (go.length * go.width == 0) continue;
+        bp::OutDesc d;
+        std::memset(&d, 0, sizeof(d));
+        d.map = make_map(go.length, go.width, tile_rows, order);
+        d.t0 = go.sensing_start;
+        d.dt = 1.0 / go.prf;
+        d.r0 = go.starting_range;
+        d.dr = go.range_pixel_spacing;
+        d.side = go.look_side;
+        d.seq_base = n_seq;
+        d.rec_base = n_rec;
+        d.out = static_cast<float2*>(blocks[b].out_dev);
+        d.height = blocks[b].height_dev;
+        n_seq += d.map.n_tiles;
+        n_rec += go.length * go.width;
+        c->desc_host.push_back(d);
+        c->desc_block.push_back(b);
+    }
+    const int n_grids = (int)c->desc_host.size();
+    if ((n_seq + 3) / 4 > INT32_MAX) return fail(BP_ERR_INVALID_ARGUMENT, "too many output pixels");
+    if ((rc = ensure(c->flags, 64 + sizeof(int32_t) * n_grids))) return rc;
+    int32_t* d_flags = static_cast<int32_t*>(c->flags.ptr);
+    auto* d_updates = reinterpret_cast<unsigned long long*>(static_cast<char*>(c->flags.ptr) + 32);
+    int32_t* d_failed = reinterpret_cast<int32_t*>(static_cast<char*>(c->flags.ptr) + 64);
+    for (int g = 0; g < n_grids; ++g) c->desc_host[g].failed = d_failed + g;
+
     // Uploads.
-    const bp_orbit& oo = og->orbit;
+    const bp_orbit& oo = *oo_p;
     const bp_orbit& io = ig->orbit;
     if ((rc = upload(c->out_pos, oo.pos, sizeof(double) * 3 * oo.n, s))) return rc;
     if ((rc = upload(c->out_vel, oo.vel, sizeof(double) * 3 * oo.n, s))) return rc;
     if ((rc = upload(c->in_pos, io.pos, sizeof(double) * 3 * io.n, s))) return rc;
     if ((rc = upload(c->in_vel, io.vel, sizeof(double) * 3 * io.n, s))) return rc;
-    if ((rc = upload(c->out_dop, og->doppler.have_data ? og->doppler.data : nullptr,
-                     og->doppler.have_data ? sizeof(double) * og->doppler.width * og->doppler.length : 0, s)))
+    if ((rc = upload(c->out_dop, odop_p->have_data ? odop_p->data : nullptr,
+                     odop_p->have_data ? sizeof(double) * odop_p->width * odop_p->length : 0, s)))
         return rc;
     if ((rc = upload(c->in_dop, ig->doppler.have_data ? ig->doppler.data : nullptr,
                      ig->doppler.have_data ? sizeof(double) * ig->doppler.width * ig->doppler.length : 0, s)))
@@ -372,22 +424,21 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
         build_polyphase(*kernel, width, c->poly_host);
         if ((rc = upload(c->poly, c->poly_host.data(), sizeof(float) * c->poly_host.size(), s))) return rc;
     }
-    if ((rc = ensure(c->flags, 64))) return rc;
-    if ((rc = ensure(c->geom, sizeof(bp::PixGeom) * npix))) return rc;
-    if (precision == BP_PRECISION_SPLIT && (rc = ensure(c->split, sizeof(bp::PixSplit) * npix))) return rc;
+    if ((rc = upload(c->desc, c->desc_host.data(), sizeof(bp::OutDesc) * n_grids, s))) return rc;
+    if ((rc = ensure(c->geom, sizeof(bp::PixGeom) * n_rec))) return rc;
+    if (precision == BP_PRECISION_SPLIT && (rc = ensure(c->split, sizeof(bp::PixSplit) * n_rec))) return rc;
     if (precision == BP_PRECISION_PLAIN && (rc = ensure(c->pulse_pv, sizeof(double) * 6 * std::max<int64_t>(na, 1))))
         return rc;
-    BP_HIP_CHECK(hipMemsetAsync(c->flags.ptr, 0, 64, s));
-    int32_t* d_flags = static_cast<int32_t*>(c->flags.ptr);
-    auto* d_updates = reinterpret_cast<unsigned long long*>(static_cast<char*>(c->flags.ptr) + 32);
+    BP_HIP_CHECK(hipMemsetAsync(c->flags.ptr, 0, 64 + sizeof(int32_t) * n_grids, s));
 
     const double swst = 2. * gi.starting_range / bp::kC;
     const double dtau = 2. * gi.range_pixel_spacing / bp::kC;
-    const bp::TileMap map = make_map(go.length, go.width, tile_rows, opt.order == 1 ? 0 : 1);
     const int per_cu = opt.blocks_per_cu > 0 ? opt.blocks_per_cu
                                              : bp::sum_blocks_per_cu(precision, width);
     int grid = c->n_cu * per_cu;
     grid = (grid + 7) / 8 * 8;
+    const auto* d_desc = static_cast<const bp::OutDesc*>(c->desc.ptr);
+    const int32_t wblocks = (int32_t)((n_seq + 3) / 4);
 
     bp::Dem ddem = host_dem(*dem);
     ddem.data = static_cast<const float*>(c->dem.ptr);
@@ -400,12 +451,10 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
 
     bp::GeomArgs ga;
     std::memset(&ga, 0, sizeof(ga));
-    ga.map = map;
-    ga.out_t0 = go.sensing_start;
-    ga.out_dt = 1.0 / go.prf;
-    ga.out_r0 = go.starting_range;
-    ga.out_dr = go.range_pixel_spacing;
-    ga.out_side = go.look_side;
+    ga.grids = d_desc;
+    ga.n_grids = n_grids;
+    ga.n_seq = n_seq;
+    ga.n_blocks = wblocks;
     ga.in_side = gi.look_side;
     ga.in_t0 = gi.sensing_start;
     ga.in_dt = in_dt;
@@ -418,7 +467,7 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
     ga.ds = ds;
     ga.out_orbit = dev_orbit(oo, c->out_pos, c->out_vel);
     ga.in_orbit = dev_orbit(io, c->in_pos, c->in_vel);
-    ga.out_dop = dev_lut(og->doppler, c->out_dop);
+    ga.out_dop = dev_lut(*odop_p, c->out_dop);
     ga.in_dop = dev_lut(ig->doppler, c->in_dop);
     ga.dem = ddem;
     ga.r2g = {r2g->tol_height, r2g->look_min, r2g->look_max};
@@ -430,7 +479,6 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
     ga.want_split = precision == BP_PRECISION_SPLIT;
     ga.geom = static_cast<bp::PixGeom*>(c->geom.ptr);
     ga.split = static_cast<bp::PixSplit*>(c->split.ptr);
-    ga.height = height_dev;
     ga.flags = d_flags;
     ga.updates = d_updates;
 
@@ -440,7 +488,10 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
 
     bp::SumArgs sa;
     std::memset(&sa, 0, sizeof(sa));
-    sa.map = map;
+    sa.grids = d_desc;
+    sa.n_grids = n_grids;
+    sa.n_seq = n_seq;
+    sa.n_blocks = wblocks;
     sa.nr = nr;
     sa.swst = swst;
     sa.dtau = dtau;
@@ -456,18 +507,20 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
     sa.split = static_cast<const bp::PixSplit*>(c->split.ptr);
     sa.table = static_cast<const float*>(c->table.ptr);
     sa.poly = static_cast<const float*>(c->poly.ptr);
-    sa.out = static_cast<float2*>(out_dev);
     if (precision == BP_PRECISION_PLAIN)
-        BP_HIP_CHECK(bp::launch_sum_plain(sa, width, grid, s));
+        BP_HIP_CHECK(bp::launch_sum_plain(sa, width, s));
     else
         BP_HIP_CHECK(bp::launch_sum_split(sa, width, grid, s));
     if (stats) BP_HIP_CHECK(hipEventRecord(c->ev[2], s));
 
-    int32_t hflags[8];
-    unsigned long long hupd = 0;
-    BP_HIP_CHECK(hipMemcpyAsync(hflags, c->flags.ptr, sizeof(hflags), hipMemcpyDeviceToHost, s));
-    BP_HIP_CHECK(hipMemcpyAsync(&hupd, d_updates, sizeof(hupd), hipMemcpyDeviceToHost, s));
+    c->flags_host.assign(16 + n_grids, 0);
+    BP_HIP_CHECK(hipMemcpyAsync(c->flags_host.data(), c->flags.ptr, 64 + sizeof(int32_t) * n_grids,
+                                hipMemcpyDeviceToHost, s));
     BP_HIP_CHECK(hipStreamSynchronize(s));
+    const int32_t* hflags = c->flags_host.data();
+    unsigned long long hupd = 0;
+    std::memcpy(&hupd, hflags + 8, sizeof(hupd));
+    for (int g = 0; g < n_grids; ++g) blocks[c->desc_block[g]].failed = hflags[16 + g] != 0;
     if (stats) {
         float ms = 0.f;
         BP_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
@@ -480,6 +533,42 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
     if (hflags[1]) return fail(BP_ERR_OUT_OF_RANGE, orbit_err_string(BP_ORBIT_INTERP_DOMAIN_ERROR));
     if (any_failed) *any_failed = hflags[0] != 0;
     return hflags[0] ? BP_FAILED_TO_CONVERGE : BP_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_geometry* ig,
+                   const bp_dem* dem, double fc, double ds, const bp_kernel* kernel,
+                   int dry_tropo_model, const bp_rdr2geo_params* r2g,
+                   const bp_geo2rdr_params* g2r, const void* in_dev, void* out_dev,
+                   float* height_dev, int precision, const bp_options* options, void* stream,
+                   int* any_failed, bp_stats* stats)
+{
+    if (any_failed) *any_failed = 0;
+    if (!og) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    bp_out_block blk;
+    std::memset(&blk, 0, sizeof(blk));
+    blk.grid = og->grid;
+    blk.out_dev = out_dev;
+    blk.height_dev = height_dev;
+    return run_blocks(c, &og->orbit, &og->doppler, &blk, 1, ig, dem, fc, ds, kernel,
+                      dry_tropo_model, r2g, g2r, in_dev, precision, options, stream, any_failed,
+                      stats);
+}
+
+int bp_backproject_blocks(bp_context* c, const bp_orbit* out_orbit, const bp_lut2d* out_doppler,
+                          bp_out_block* blocks, int n_blocks, const bp_radar_geometry* ig,
+                          const bp_dem* dem, double fc, double ds, const bp_kernel* kernel,
+                          int dry_tropo_model, const bp_rdr2geo_params* r2g,
+                          const bp_geo2rdr_params* g2r, const void* in_dev, int precision,
+                          const bp_options* options, void* stream, int* any_failed,
+                          bp_stats* stats)
+{
+    return run_blocks(c, out_orbit, out_doppler, blocks, n_blocks, ig, dem, fc, ds, kernel,
+                      dry_tropo_model, r2g, g2r, in_dev, precision, options, stream, any_failed,
+                      stats);
 }
 
 // ------------------------------------------------------------ host helpers

@@ -48,21 +48,39 @@ constexpr int kFitDegree = 4;        // delay polynomial degree of the split pat
 constexpr int kPolyphaseRows = 512;  // fractional-offset rows of the split weight table
 constexpr int kMaxTableLds = 8192;   // plain path keeps tables up to this size in LDS
 
-// Pixel tiling: each 64-lane wavefront owns a tile of tile_rows x (64/tile_rows)
-// output pixels; four wavefronts form a 256-thread workgroup.
+// Pixel tiling of one output grid: each 64-lane wavefront owns a tile of
+// tile_rows x (64/tile_rows) output pixels.
 struct TileMap {
     int32_t tile_rows, tile_cols;
     int32_t tiles_x, tiles_y;
     int64_t length, width;      // output grid
-    int64_t n_tiles;
-    int32_t n_blocks;           // 256-thread blocks of the geometry pass
+    int64_t n_tiles;            // wavefront tiles of this grid
     int32_t order;              // sum-kernel tile sequence: 0 row-major, 1 column-major
+    int32_t pad_;
+};
+
+// One output grid of a (batched) call.  Wavefront tiles of all grids are
+// numbered consecutively: grid g owns [seq_base, seq_base + map.n_tiles).
+struct OutDesc {
+    TileMap map;
+    double t0, dt, r0, dr;      // sensing start, 1/PRF, starting range, spacing
+    int32_t side;               // look side
+    int32_t pad_;
+    int64_t seq_base;           // first global wavefront tile of this grid
+    int64_t rec_base;           // first per-pixel record of this grid
+    float2* out;                // complex64 [length x width]
+    float* height;              // optional float32 [length x width]
+    int32_t* failed;            // per-grid flag: some pixel failed to converge
+    void* pad2_;
 };
 
 struct GeomArgs {
-    TileMap map;
-    double out_t0, out_dt, out_r0, out_dr;
-    int32_t out_side, in_side;
+    const OutDesc* grids;       // device array
+    int32_t n_grids;
+    int32_t want_split;
+    int64_t n_seq;              // total wavefront tiles
+    int32_t n_blocks;           // 256-thread blocks (4 tiles each)
+    int32_t in_side;
     double in_t0, in_dt;
     int32_t na, tropo;
     double swst, dtau;
@@ -72,24 +90,24 @@ struct GeomArgs {
     Dem dem;                    // device pointers
     R2GParams r2g;
     G2RParams g2r;
-    int32_t want_split;
-    int32_t pad_;
     PixGeom* geom;              // plain records (always written)
     PixSplit* split;            // split records (when want_split)
-    float* height;              // optional user height buffer
     int32_t* flags;             // [0] any failed, [1] out of range
     unsigned long long* updates;
 };
 
 struct SumArgs {
-    TileMap map;
+    const OutDesc* grids;       // device array
+    int32_t n_grids;
+    int32_t chunk;
+    int64_t n_seq;              // total wavefront tiles
+    int32_t n_blocks;           // 256-thread blocks of the plain kernel
+    int32_t ntab;
     int64_t nr;                 // range samples per pulse
     double swst, dtau, fc;
     double inv_k;               // 1 / (fc * dtau): cycles -> range bins
     double halfwidth;           // kernel half width
     float rdx;                  // float(1/dx) of the tabulated kernel
-    int32_t ntab;
-    int32_t chunk;
     int32_t pad_;
     const float2* rc;           // range-compressed pulses [na x nr]
     const double* pulse_pv;     // per-pulse position/velocity [na x 6]
@@ -97,14 +115,13 @@ struct SumArgs {
     const PixSplit* split;
     const float* table;         // tabulated kernel (plain)
     const float* poly;          // polyphase weights (split): (rows+1) x (2W)
-    float2* out;
 };
 
 // Host launchers (defined in bp_kernels.hip).
 hipError_t launch_pulse_orbit(const Orbit& orbit, double t0, double dt, int na, double* pv,
                               int32_t* flags, hipStream_t stream);
 hipError_t launch_geometry(const GeomArgs& args, hipStream_t stream);
-hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_t stream);
+hipError_t launch_sum_plain(const SumArgs& args, int width, hipStream_t stream);
 hipError_t launch_sum_split(const SumArgs& args, int width, int grid, hipStream_t stream);
 int sum_blocks_per_cu(int precision, int width);
 size_t split_lds_bytes(int width);

@@ -35,15 +35,29 @@ __device__ inline long long wave_sum_ll(long long v)
 
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
 // XCD (own L2) walks a contiguous run of tiles, which share pulse data.
-__device__ inline int64_t wave_tile(const TileMap& m, int* lane)
+// Returns the global wavefront-tile index of this wavefront (4 per block).
+__device__ inline int64_t wave_tile(int nb, int* lane)
 {
-    const int nb = m.n_blocks;
     const int b = blockIdx.x;
     const int per = nb / 8, rem = nb % 8;
     const int xcd = b % 8, idx = b / 8;
     const int lb = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
     *lane = threadIdx.x & 63;
-    return (int64_t)lb * 4 + (threadIdx.x >> 6);
+    return (int64_t)lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
+// Output grid owning global wavefront tile s (s uniform across the wavefront).
+__device__ inline int find_grid(const OutDesc* g, int n, int64_t s)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g[mid].seq_base <= s)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
 }
 
 __device__ inline bool tile_pixel(const TileMap& m, int64_t tile, int lane, int64_t* row,
@@ -93,10 +107,13 @@ __global__ void pulse_orbit_kernel(Orbit orbit, double t0, double dt, int na, do
 __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
 {
     int lane;
-    const int64_t tile = wave_tile(a.map, &lane);
+    const int64_t gseq = wave_tile(a.n_blocks, &lane);
+    const bool in_seq = gseq < a.n_seq;
+    const OutDesc& d = a.grids[in_seq ? find_grid(a.grids, a.n_grids, gseq) : 0];
     int64_t row = 0, col = 0;
-    const bool valid = tile_pixel(a.map, tile, lane, &row, &col);
-    const int64_t pix = row * a.map.width + col;
+    const bool valid = in_seq && tile_pixel(d.map, gseq - d.seq_base, lane, &row, &col);
+    const int64_t pix = row * d.map.width + col;
+    const int64_t rec = d.rec_base + pix;
 
     PixGeom g;
     g.x[0] = g.x[1] = g.x[2] = qnan();
@@ -110,20 +127,20 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
     double t = 0., r = 0.;
 
     if (valid) {
-        const double ta = a.out_t0 + row * a.out_dt;
-        const double ra = a.out_r0 + col * a.out_dr;
+        const double ta = d.t0 + row * d.dt;
+        const double ra = d.r0 + col * d.dr;
         const double fd = lut2d_eval(a.out_dop, ta, ra);
-        int st = rdr2geo_bracket(ta, ra, fd, a.out_orbit, a.dem, a.wvl, a.out_side, a.r2g, &x);
+        int st = rdr2geo_bracket(ta, ra, fd, a.out_orbit, a.dem, a.wvl, d.side, a.r2g, &x);
         if (st == kOrbitInterpDomainError || st == kOrbitInterpSizeError ||
             st == kOrbitInterpUnknownMethod) {
             atomicOr(&a.flags[1], 1);
             g.status = kPixOutOfRange;
         } else {
             const V3 llh = xyz_to_lonlat(kWgsA, kWgsE2, x);
-            if (a.height) a.height[pix] = (float)llh.z;
+            if (d.height) d.height[pix] = (float)llh.z;
             if (st != kSuccess) {
                 g.status = kPixRdr2GeoFailed;
-                if (a.height) a.height[pix] = __builtin_nanf("");
+                if (d.height) d.height[pix] = __builtin_nanf("");
             } else {
                 int st2 = geo2rdr_bracket(x, a.in_orbit, a.in_dop, a.wvl, a.in_side, a.g2r, &t, &r);
                 if (st2 != kSuccess) {
@@ -149,13 +166,15 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
                     }
                 }
             }
-            if (g.status == kPixRdr2GeoFailed || g.status == kPixGeo2RdrFailed)
+            if (g.status == kPixRdr2GeoFailed || g.status == kPixGeo2RdrFailed) {
                 atomicOr(&a.flags[0], 1);
+                atomicOr(d.failed, 1);
+            }
         }
         g.x[0] = x.x;
         g.x[1] = x.y;
         g.x[2] = x.z;
-        a.geom[pix] = g;
+        a.geom[rec] = g;
     }
 
     const bool has_win = valid && g.status == kPixOk && g.kstop > g.kstart;
@@ -246,7 +265,7 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
         sp.n_ref = (int32_t)n_ref;
         sp.f_ref = (float)(u_c - n_ref);
     }
-    a.split[pix] = sp;
+    a.split[rec] = sp;
 }
 
 // Persistent tile schedule of the sum kernels.  A workgroup holds 8
@@ -264,21 +283,29 @@ __device__ inline int64_t seq_to_tile(const TileMap& m, int64_t s)
     return ty * m.tiles_x + tx;
 }
 
+// Calls body(grid descriptor, tile of that grid, lane) for every global
+// wavefront tile this wavefront owns; tiles past the last grid are passed as
+// tile == map.n_tiles (no valid pixel).
 template<typename Body>
-__device__ inline void for_each_tile(const TileMap& m, Body&& body)
+__device__ inline void for_each_tile(const OutDesc* grids, int n_grids, int64_t n_seq, Body&& body)
 {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int xcd = blockIdx.x & 7;
     const int64_t j = blockIdx.x >> 3;
     const int64_t n_local = gridDim.x >> 3;
-    const int64_t n_wgseq = (m.n_tiles + kSumWaves - 1) / kSumWaves;
+    const int64_t n_wgseq = (n_seq + kSumWaves - 1) / kSumWaves;
     const int64_t chunk = (n_wgseq + 7) / 8;
     const int64_t end = min(n_wgseq, (xcd + 1) * chunk);
     for (int64_t w = xcd * chunk + j; w < end; w += n_local) {
         const int64_t s = w * kSumWaves + wave;
-        const int64_t tile = s < m.n_tiles ? seq_to_tile(m, s) : m.n_tiles;
-        body(tile, lane);
+        if (s < n_seq) {
+            const OutDesc& d = grids[find_grid(grids, n_grids, s)];
+            body(d, seq_to_tile(d.map, s - d.seq_base), lane);
+        } else {
+            const OutDesc& d = grids[n_grids - 1];
+            body(d, d.map.n_tiles, lane);
+        }
     }
 }
 
@@ -286,14 +313,15 @@ __device__ inline void for_each_tile(const TileMap& m, Body&& body)
 // FP64 bistatic delay, range-bin position and phase per pulse; float tabulated
 // taps with FP64 index math and a complex<float> inner product; complex<double> sum.
 template<int W>
-__device__ inline void plain_tile(const SumArgs& a, const float* tab, int64_t tile, int lane)
+__device__ inline void plain_tile(const SumArgs& a, const float* tab, const OutDesc& od,
+                                  int64_t tile, int lane)
 {
     int64_t row = 0, col = 0;
-    const bool valid = tile_pixel(a.map, tile, lane, &row, &col);
-    const int64_t pix = row * a.map.width + col;
+    const bool valid = tile_pixel(od.map, tile, lane, &row, &col);
+    const int64_t pix = row * od.map.width + col;
 
     PixGeom g;
-    if (valid) g = a.geom[pix];
+    if (valid) g = a.geom[od.rec_base + pix];
     const bool ok = valid && g.status == kPixOk;
     const int ks = ok ? g.kstart : INT_MAX;
     const int ke = ok ? g.kstop : INT_MIN;
@@ -339,7 +367,7 @@ __device__ inline void plain_tile(const SumArgs& a, const float* tab, int64_t ti
     }
     if (valid) {
         const float nanf_ = __builtin_nanf("");
-        a.out[pix] = ok ? make_float2((float)acc_re, (float)acc_im) : make_float2(nanf_, nanf_);
+        od.out[pix] = ok ? make_float2((float)acc_re, (float)acc_im) : make_float2(nanf_, nanf_);
     }
 }
 
@@ -355,8 +383,11 @@ __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
     __syncthreads();
     const float* tab = tab_in_lds ? s_tab : a.table;
     int lane;
-    const int64_t tile = wave_tile(a.map, &lane);
-    plain_tile<W>(a, tab, tile, lane);
+    const int64_t gseq = wave_tile(a.n_blocks, &lane);
+    if (gseq < a.n_seq) {
+        const OutDesc& d = a.grids[find_grid(a.grids, a.n_grids, gseq)];
+        plain_tile<W>(a, tab, d, gseq - d.seq_base, lane);
+    }
 }
 
 // Split-precision coherent sum of one wavefront tile.
@@ -468,14 +499,15 @@ __device__ inline void kahan_fold(float& acc, float& cmp, float& part)
 }
 
 template<int W>
-__device__ inline void split_tile(const SumArgs& a, const float* s_poly, int64_t tile, int lane)
+__device__ inline void split_tile(const SumArgs& a, const float* s_poly, const OutDesc& od,
+                                  int64_t tile, int lane)
 {
     int64_t row = 0, col = 0;
-    const bool valid = tile_pixel(a.map, tile, lane, &row, &col);
-    const int64_t pix = row * a.map.width + col;
+    const bool valid = tile_pixel(od.map, tile, lane, &row, &col);
+    const int64_t pix = row * od.map.width + col;
 
     PixSplit sp;
-    if (valid) sp = a.split[pix];
+    if (valid) sp = a.split[od.rec_base + pix];
     const bool ok = valid && sp.status == kPixOk;
     const bool win = ok && sp.kstop > sp.kstart;
     const int ks = win ? sp.kstart : INT_MAX;
@@ -525,7 +557,7 @@ __device__ inline void split_tile(const SumArgs& a, const float* s_poly, int64_t
         kahan_fold(acc_im, cmp_im, cim);
         if (valid) {
             const float nanf_ = __builtin_nanf("");
-            a.out[pix] = ok ? make_float2(acc_re - cmp_re, acc_im - cmp_im) : make_float2(nanf_, nanf_);
+            od.out[pix] = ok ? make_float2(acc_re - cmp_re, acc_im - cmp_im) : make_float2(nanf_, nanf_);
         }
         return;
     }
@@ -581,7 +613,7 @@ __device__ inline void split_tile(const SumArgs& a, const float* s_poly, int64_t
 
     if (valid) {
         const float nanf_ = __builtin_nanf("");
-        a.out[pix] = ok ? make_float2(acc_re - cmp_re, acc_im - cmp_im) : make_float2(nanf_, nanf_);
+        od.out[pix] = ok ? make_float2(acc_re - cmp_re, acc_im - cmp_im) : make_float2(nanf_, nanf_);
     }
 }
 
@@ -593,7 +625,9 @@ __global__ __launch_bounds__(512) void sum_split_kernel(SumArgs a)
     float* s_poly = reinterpret_cast<float*>(s_poly4);
     for (int i = threadIdx.x; i < (kPolyphaseRows + 1) * ROW; i += blockDim.x) s_poly[i] = a.poly[i];
     __syncthreads();
-    for_each_tile(a.map, [&](int64_t tile, int lane) { split_tile<W>(a, s_poly, tile, lane); });
+    for_each_tile(a.grids, a.n_grids, a.n_seq, [&](const OutDesc& d, int64_t tile, int lane) {
+        split_tile<W>(a, s_poly, d, tile, lane);
+    });
 }
 
 // ------------------------------------------------------------------ launchers
@@ -611,22 +645,21 @@ hipError_t launch_pulse_orbit(const Orbit& orbit, double t0, double dt, int na, 
 
 hipError_t launch_geometry(const GeomArgs& args, hipStream_t stream)
 {
-    if (args.map.n_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(geometry_kernel, dim3(args.map.n_blocks), dim3(256), 0, stream, args);
+    if (args.n_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(geometry_kernel, dim3(args.n_blocks), dim3(256), 0, stream, args);
     return hipGetLastError();
 }
 
-hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_t stream)
+hipError_t launch_sum_plain(const SumArgs& args, int width, hipStream_t stream)
 {
-    (void)grid;
-    if (args.map.n_tiles <= 0) return hipSuccess;
+    if (args.n_seq <= 0) return hipSuccess;
     const size_t lds = args.ntab <= kMaxTableLds ? sizeof(float) * args.ntab : 0;
     switch (width) {
     case 8:
-        hipLaunchKernelGGL(sum_plain_kernel<8>, dim3(args.map.n_blocks), dim3(256), lds, stream, args);
+        hipLaunchKernelGGL(sum_plain_kernel<8>, dim3(args.n_blocks), dim3(256), lds, stream, args);
         break;
     case 9:
-        hipLaunchKernelGGL(sum_plain_kernel<9>, dim3(args.map.n_blocks), dim3(256), lds, stream, args);
+        hipLaunchKernelGGL(sum_plain_kernel<9>, dim3(args.n_blocks), dim3(256), lds, stream, args);
         break;
     default:
         return hipErrorInvalidValue;
@@ -665,7 +698,7 @@ size_t split_lds_bytes(int width)
 // share per XCD), sized by the caller from the device's CU count and occupancy.
 hipError_t launch_sum_split(const SumArgs& args, int width, int grid, hipStream_t stream)
 {
-    if (args.map.n_tiles <= 0) return hipSuccess;
+    if (args.n_seq <= 0) return hipSuccess;
     const size_t lds = split_lds_bytes(width);
     switch (width) {
     case 8:

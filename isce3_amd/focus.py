@@ -201,39 +201,72 @@ def _device_of(torch, *arrays) -> int:
     return torch.cuda.current_device()
 
 
-def _run(torch, device, out, ogeom, sig, igeom, dem, fc, ds, kernel, tropo, r2g, g2r, height,
-         precision, options, stream, stats):
+def _stream_handle(torch, stream):
     if stream is None:
-        hstream = torch.cuda.current_stream().cuda_stream
-    elif isinstance(stream, int):
-        hstream = stream
-    else:
-        hstream = stream.cuda_stream
-    # Device views of the arrays (host arrays are staged on the device).
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def _device_input(torch, device, sig):
     if _is_torch(sig):
         d_in = sig.contiguous() if sig.is_cuda else sig.contiguous().to(f"cuda:{device}")
         if d_in.dtype != torch.complex64:
             d_in = d_in.to(torch.complex64)
+        return d_in
+    h = np.ascontiguousarray(sig, dtype=np.complex64)
+    if not h.flags.writeable:  # torch tensors over read-only numpy memory are undefined
+        h = h.copy()
+    return torch.from_numpy(h).to(f"cuda:{device}", non_blocking=False)
+
+
+def _device_output(torch, device, a, dtype):
+    """Device buffer written by the kernels: the array itself when it is a
+    device tensor, else a staging tensor copied back by _copy_back."""
+    if a is None:
+        return None
+    if _is_torch(a) and a.is_cuda:
+        return a
+    return torch.empty(_shape_of(a), dtype=dtype, device=f"cuda:{device}")
+
+
+def _copy_back(torch, a, d):
+    if a is None or d is a:
+        return
+    res = d.cpu().numpy()
+    if _is_torch(a):
+        a.copy_(torch.from_numpy(res))
     else:
-        h = np.ascontiguousarray(sig, dtype=np.complex64)
-        d_in = torch.from_numpy(h).to(f"cuda:{device}", non_blocking=False)
-    out_is_dev = _is_torch(out) and out.is_cuda
-    d_out = out if out_is_dev else torch.empty(_shape_of(out), dtype=torch.complex64,
-                                               device=f"cuda:{device}")
-    d_h = None
-    if height is not None:
-        h_is_dev = _is_torch(height) and height.is_cuda
-        d_h = height if h_is_dev else torch.empty(_shape_of(height), dtype=torch.float32,
-                                                  device=f"cuda:{device}")
-    og = geometry_struct(ogeom)
-    ig = geometry_struct(igeom)
-    dm = dem._struct()
-    kr = kernel._struct()
+        a[...] = res
+
+
+def _options_struct(options) -> _native.Options_t:
     opt = _native.Options_t()
     for k, v in (options or {}).items():
         if k not in ("tile_rows", "chunk", "fit_degree", "order", "blocks_per_cu"):
             raise errors.InvalidArgument(f"unknown option {k!r}")
         setattr(opt, k, int(v))
+    return opt
+
+
+def _fill_stats(stats, st):
+    if stats is not None:
+        stats.update(updates=int(st.updates), geometry_ms=float(st.geometry_ms),
+                     sum_ms=float(st.sum_ms), any_failed=bool(st.any_failed))
+
+
+def _run(torch, device, out, ogeom, sig, igeom, dem, fc, ds, kernel, tropo, r2g, g2r, height,
+         precision, options, stream, stats):
+    hstream = _stream_handle(torch, stream)
+    d_in = _device_input(torch, device, sig)
+    d_out = _device_output(torch, device, out, torch.complex64)
+    d_h = _device_output(torch, device, height, torch.float32)
+    og = geometry_struct(ogeom)
+    ig = geometry_struct(igeom)
+    dm = dem._struct()
+    kr = kernel._struct()
+    opt = _options_struct(options)
     st = _native.Stats_t()
     failed = C.c_int(0)
     ctx = _contexts.get(device)
@@ -246,19 +279,117 @@ def _run(torch, device, out, ogeom, sig, igeom, dem, fc, ds, kernel, tropo, r2g,
         int(precision), C.byref(opt), C.c_void_p(hstream), C.byref(failed), C.byref(st))
     del _keep
     _native.check(rc)
-    if not out_is_dev:
-        res = d_out.cpu().numpy()
-        if _is_torch(out):
-            out.copy_(torch.from_numpy(res))
-        else:
-            out[...] = res
-    if height is not None and not (_is_torch(height) and height.is_cuda):
-        res = d_h.cpu().numpy()
-        if _is_torch(height):
-            height.copy_(torch.from_numpy(res))
-        else:
-            height[...] = res
-    if stats is not None:
-        stats.update(updates=int(st.updates), geometry_ms=float(st.geometry_ms),
-                     sum_ms=float(st.sum_ms), any_failed=bool(st.any_failed))
+    _copy_back(torch, out, d_out)
+    _copy_back(torch, height, d_h)
+    _fill_stats(stats, st)
     return bool(failed.value)
+
+
+def _same_orbit(a, b) -> bool:
+    if a is b:
+        return True
+    sa, sb = a._struct(), b._struct()
+    return (sa.t0 == sb.t0 and sa.dt == sb.dt and sa.n == sb.n and sa.method == sb.method
+            and a.reference_epoch == b.reference_epoch
+            and np.array_equal(a._pos, b._pos) and np.array_equal(a._vel, b._vel))
+
+
+def _same_lut(a, b) -> bool:
+    if a is b:
+        return True
+    sa, sb = a._struct(), b._struct()
+    same = all(getattr(sa, f) == getattr(sb, f) for f, _ in _native.Lut2d_t._fields_
+               if f != "data")
+    return same and (not a.have_data or np.array_equal(a.data, b.data))
+
+
+def backproject_blocks(outs, out_geometries, signal_data, in_geometry: RadarGeometry,
+                       dem: DEMInterpolator, fc: float, ds: float, kernel: KernelF32,
+                       dry_tropo_model: str = "tsx", rdr2geo_params: Optional[dict] = None,
+                       geo2rdr_params: Optional[dict] = None, heights=None, *,
+                       precision: str = "plain", options: Optional[dict] = None, stream=None,
+                       stats: Optional[dict] = None):
+    """Focus several output blocks of one raw data set in one pass.
+
+    Equivalent to calling :func:`backproject` once per block, as the RSLC
+    workflow does over its processing blocks
+    (python/packages/nisar/workflows/focus.py:1986-2004), but the pixels of
+    all blocks go through one geometry launch and one coherent-sum launch.
+    ``outs`` and ``out_geometries`` are equal-length lists; every output
+    geometry must share one orbit and one Doppler LUT.  ``heights`` is None
+    or a list (entries may be None).  Returns a list of per-block flags, True
+    where some pixel of that block failed to converge.
+    """
+    outs = list(outs)
+    out_geometries = list(out_geometries)
+    if len(outs) != len(out_geometries):
+        raise errors.InvalidArgument("need one output geometry per output array")
+    heights = [None] * len(outs) if heights is None else list(heights)
+    if len(heights) != len(outs):
+        raise errors.InvalidArgument("need one height entry per output array")
+    if signal_data.ndim != 2:
+        raise errors.InvalidArgument("input signal data must be 2-D")
+    if _shape_of(signal_data) != (in_geometry.grid_length, in_geometry.grid_width):
+        raise errors.InvalidArgument("input signal data shape must match input radar grid shape")
+    for out, og, h in zip(outs, out_geometries, heights):
+        if out.ndim != 2 or _shape_of(out) != (og.grid_length, og.grid_width):
+            raise errors.InvalidArgument("output array shape must match output radar grid shape")
+        if h is not None and _shape_of(h) != (og.grid_length, og.grid_width):
+            raise errors.InvalidArgument("height array shape must match output radar grid shape")
+        if og.reference_epoch != in_geometry.reference_epoch:
+            raise errors.RuntimeError_("input reference epoch must match output reference epoch")
+        _check_out_array(out, "output")
+        if h is not None:
+            _check_out_array(h, "height", complex_=False)
+    if out_geometries:
+        o0, d0 = out_geometries[0].orbit, out_geometries[0].doppler
+        for og in out_geometries[1:]:
+            if not _same_orbit(og.orbit, o0) or not _same_lut(og.doppler, d0):
+                raise errors.InvalidArgument(
+                    "all output blocks must share one orbit and one Doppler LUT")
+    tropo = parse_dry_tropo_model(dry_tropo_model)
+    r2g = parse_rdr2geo_params(rdr2geo_params or {})
+    g2r = parse_geo2rdr_params(geo2rdr_params or {})
+    if not isinstance(kernel, TabulatedKernelF32):
+        raise errors.InvalidArgument("kernel must be a TabulatedKernelF32")
+    if precision not in _PRECISION:
+        raise errors.InvalidArgument(f"precision must be 'plain' or 'split', got {precision!r}")
+    if not outs:
+        return []
+
+    torch = _torch()
+    device = _device_of(torch, signal_data, *outs, *heights)
+    with torch.cuda.device(device):
+        hstream = _stream_handle(torch, stream)
+        d_in = _device_input(torch, device, signal_data)
+        d_outs = [_device_output(torch, device, o, torch.complex64) for o in outs]
+        d_hs = [_device_output(torch, device, h, torch.float32) for h in heights]
+        blocks = (_native.OutBlock_t * len(outs))()
+        for i, og in enumerate(out_geometries):
+            blocks[i].grid = geometry_struct(og).grid
+            blocks[i].out_dev = d_outs[i].data_ptr()
+            blocks[i].height_dev = d_hs[i].data_ptr() if d_hs[i] is not None else None
+        g0 = out_geometries[0]
+        oorb = g0.orbit._struct()
+        odop = g0.doppler._struct()
+        ig = geometry_struct(in_geometry)
+        dm = dem._struct()
+        kr = kernel._struct()
+        opt = _options_struct(options)
+        st = _native.Stats_t()
+        failed = C.c_int(0)
+        ctx = _contexts.get(device)
+        _keep = (out_geometries, in_geometry, dem, kernel)
+        rc = _native.lib().bp_backproject_blocks(
+            ctx, C.byref(oorb), C.byref(odop), blocks, len(outs), C.byref(ig), C.byref(dm),
+            float(fc), float(ds), C.byref(kr), int(tropo), C.byref(r2g), C.byref(g2r),
+            C.c_void_p(d_in.data_ptr()), _PRECISION[precision], C.byref(opt),
+            C.c_void_p(hstream), C.byref(failed), C.byref(st))
+        del _keep
+        _native.check(rc)
+        for o, d in zip(outs, d_outs):
+            _copy_back(torch, o, d)
+        for h, d in zip(heights, d_hs):
+            _copy_back(torch, h, d)
+        _fill_stats(stats, st)
+        return [bool(blocks[i].failed) for i in range(len(outs))]

@@ -91,3 +91,25 @@ def test_parsers_accept_reference_keys():
     q = parse_geo2rdr_params({"tol_aztime": 1e-6, "time_start": None, "time_end": 5.0})
     assert q.tol_aztime == 1e-6 and q.have_time_start == 0 and q.have_time_end == 1
     assert q.time_end == 5.0
+
+
+def test_block_list_checks(setup):
+    """backproject_blocks rejects inconsistent block lists before device work."""
+    from isce3_amd.focus import backproject_blocks
+    from isce3_amd.sim import straight_orbit
+    sc, out, raw = setup
+    og = sc.out_geometry()
+    args = (raw, sc.in_geometry(), sc.dem, sc.fc, sc.ds, sc.kernel, "nodelay")
+    assert backproject_blocks([], [], *args) == []
+    with pytest.raises(ValueError, match="one output geometry per output array"):
+        backproject_blocks([out], [og, og], *args)
+    with pytest.raises(ValueError, match="one height entry"):
+        backproject_blocks([out], [og], *args, heights=[None, None])
+    with pytest.raises(ValueError, match="output array shape must match"):
+        backproject_blocks([out[:-1]], [og], *args)
+    other = RadarGeometry(sc.out_grid, straight_orbit(60.0, 10.0, sc.orbit.reference_epoch),
+                          sc.doppler)
+    with pytest.raises(ValueError, match="share one orbit"):
+        backproject_blocks([out, out], [og, other], *args)
+    with pytest.raises(ValueError, match="precision"):
+        backproject_blocks([out], [og], *args, precision="fp16")
