@@ -1,18 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark: pixel-pulse updates/s of the backprojector on one MI355X.
 
-Workload (one "step" = one full backprojection of one block, inputs resident
-in HBM): 16384 pulses x 16384 range bins complex64 (2 GiB) of a seeded
-synthetic L-band stripmap scene (200 point targets + clutter, 1024^2 fractal
-DEM), focused to a 4096 x 4096 complex64 grid with a tabulated Knab kernel
-(width 8, bandwidth 0.8, 2048 entries) at 6 m azimuth resolution.
-
-`value` is the split-precision path (FP64 per-pixel reference + FP32 deltas);
-the plain all-FP64-inner-loop kernel is timed on the same inputs beside it.
+Headline workload (one "step" = one full backprojection of one block, inputs
+resident in HBM): 16384 pulses x 16384 range bins complex64 (2 GiB) of a
+seeded synthetic L-band stripmap scene (200 point targets + clutter, 1024^2
+fractal DEM), focused to a 4096 x 4096 complex64 grid with a tabulated Knab
+kernel (width 8, bandwidth 0.8, 2048 entries) at 6 m azimuth resolution.
+`value` is the split-precision path (FP64 per-pixel reference + FP32
+deltas); the plain all-FP64-inner-loop kernel is timed on the same inputs.
 Accuracy of both is stated on a seeded pixel sample against the CPU oracle
 (reference mode for plain, all-FP64 mode for split).  The CPU baseline is the
-oracle in reference mode on a bounded sample of output rows, timed on the
-host's cores.
+oracle in reference mode on a bounded sample of output rows.
+
+Beside it, in the same JSON line:
+  "batch"  64 output tiles (side log-uniform in [64, 1024] px, aspect ratio
+           uniform in [0.5, 2], random positions in the illuminated swath)
+           cut from one resident 8192 x 16384 raw set, focused by one batched
+           call (one persistent sum launch, launch options from the
+           autotuner) vs 64 sequential plain calls;
+  "serve"  3 resident 8192 x 16384 raw sets, Poisson tile requests (square,
+           side log-uniform in [32, 2048] px) at 20, 50 and 100 req/s,
+           split precision with continuous batching vs the plain kernel
+           served first-come-first-served; p50/p99 latency and updates/s.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -21,6 +30,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -30,64 +40,45 @@ sys.path.insert(0, ROOT)
 FLOPS_PER_UPDATE = 101  # algorithmic flops per pixel-pulse update (FMA = 2)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sum_split_kernel_full_4096x4096_pmc.md")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c1")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
-                    help="target CPU time of the oracle baseline sample")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sample-pixels", type=int, default=2048,
-                    help="pixels checked against the oracle")
-    ap.add_argument("--tile-rows", type=int, default=0)
-    ap.add_argument("--only", choices=["plain", "split"], default=None,
-                    help="time one precision only (profiling runs)")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="run one device-to-device copy of this many MiB first, as the "
-                         "known-size reference for HBM byte counters")
-    args = ap.parse_args()
+def hbm_bytes_per_launch():
+    """HBM bytes per launch of the split sum kernel on the headline workload,
+    from the committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE
+    scaled by a copy-kernel calibration); None when it is absent."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"HBM bytes per launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
 
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not "
-                          "part of it (output tiles shard with no exchange step)"}))
-        sys.exit(2)
 
-    import numpy as np
-    import torch
-
-    from isce3_amd import sim
-    from isce3_amd.focus import backproject
-
-    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
-    dev = "cuda:0"
-    t0 = time.time()
-    sc = sim.make_scene(args.config)
-    log(f"scene {args.config}: {len(sc.targets)} scatterers ({time.time() - t0:.1f}s)")
-    t0 = time.time()
+def make_raw(torch, sim, sc, dev):
     raw_host = torch.empty((sc.in_grid.length, sc.in_grid.width), dtype=torch.complex64,
                            pin_memory=True)
     sim.simulate(sc, out=raw_host.numpy(), nthreads=min(16, os.cpu_count() or 1))
-    log(f"simulated {tuple(raw_host.shape)} in {time.time() - t0:.1f}s")
     raw = raw_host.to(dev)
     torch.cuda.synchronize()
+    return raw_host, raw
+
+
+# ------------------------------------------------------------- full block
+def run_full(args, torch, np, dev):
+    from isce3_amd import sim
+    from isce3_amd.focus import backproject
+    t0 = time.time()
+    sc = sim.make_scene("full")
+    raw_host, raw = make_raw(torch, sim, sc, dev)
+    log(f"full scene: {len(sc.targets)} scatterers, {tuple(raw.shape)} simulated in "
+        f"{time.time() - t0:.1f}s")
     out = torch.empty(sc.out_grid.shape, dtype=torch.complex64, device=dev)
     opts = {"tile_rows": args.tile_rows} if args.tile_rows else None
-    if args.calibrate_copy:
-        n = args.calibrate_copy * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-        log(f"calibration copy of {args.calibrate_copy} MiB done")
 
     def step(precision):
         st = {}
@@ -115,11 +106,12 @@ def main():
                                   out=out.cpu().numpy())
         log(f"{precision}: {dt * 1e3:.1f} ms/step, sum {sum_ms:.1f} ms, geometry {geo_ms:.1f} ms,"
             f" {upd / dt:.3e} updates/s")
+    return sc, raw_host, results
 
-    # accuracy on a seeded pixel sample against the oracle
+
+def accuracy(args, np, sc, raw_np, results):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    raw_np = raw_host.numpy()
     rng = np.random.Generator(np.random.PCG64(1234))
     npx = sc.out_grid.length * sc.out_grid.width
     subset = np.sort(rng.choice(npx, size=min(args.sample_pixels, npx), replace=False))
@@ -140,46 +132,273 @@ def main():
         ph = np.angle(z[strong] * np.conj(ref[strong]))
         acc[precision] = dict(rel_error=rel, rms_phase_rad=float(np.sqrt(np.mean(ph ** 2))),
                               max_phase_rad=float(np.abs(ph).max()),
-                              vs="oracle-reference" if mode == 0 else "oracle-fp64")
+                              vs="oracle-reference" if mode == 0 else "oracle-fp64",
+                              pixels=int(subset.size))
         log(f"{precision} accuracy vs {acc[precision]['vs']}: rel {rel:.2e}, "
             f"rms phase {acc[precision]['rms_phase_rad']:.2e} rad")
+    return acc
 
-    cpu = None
-    if not args.no_cpu:
-        W = sc.out_grid.width
-        rows = 1
-        t_total = 0.0
-        upd_total = 0
-        row0 = sc.out_grid.length // 2
-        while True:
-            sub = np.arange(row0 * W, (row0 + rows) * W, dtype=np.int64)
-            t0 = time.perf_counter()
-            _, _, diag, _ = oracle.backproject(sc.out_geometry(), raw_np, sc.in_geometry(),
-                                               sc.dem, sc.fc, sc.ds, sc.kernel,
-                                               sc.dry_tropo_model, subset=sub, nthreads=nth,
-                                               want_diag=True)
-            dt = time.perf_counter() - t0
-            t_total += dt
-            upd_total += int(sum(d.kstop - d.kstart for d in diag if d.status == 0))
-            row0 += rows
-            if t_total >= args.cpu_seconds or row0 >= sc.out_grid.length:
-                break
-            rows = max(1, min(64, int(rows * max(1.0, (args.cpu_seconds - t_total) / max(dt, 1e-3)))))
-        cpu = {"value": upd_total / t_total, "unit": "pixel-pulse updates/s", "cores": nth,
-               "kind": "port",
-               "sample": f"{upd_total} updates ({upd_total / sc.out_grid.width / 4046:.0f}-ish "
-                         f"rows of 4096 px) of the same workload, oracle reference mode, "
-                         f"{t_total:.1f} s"}
-        log(f"cpu baseline: {cpu['value']:.3e} updates/s on {nth} threads")
 
+def cpu_baseline(args, np, sc, raw_np):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    nth = min(16, os.cpu_count() or 1)
+    W = sc.out_grid.width
+    rows = 1
+    t_total = 0.0
+    upd_total = 0
+    row0 = sc.out_grid.length // 2
+    first = row0
+    while True:
+        sub = np.arange(row0 * W, (row0 + rows) * W, dtype=np.int64)
+        t0 = time.perf_counter()
+        _, _, diag, _ = oracle.backproject(sc.out_geometry(), raw_np, sc.in_geometry(),
+                                           sc.dem, sc.fc, sc.ds, sc.kernel,
+                                           sc.dry_tropo_model, subset=sub, nthreads=nth,
+                                           want_diag=True)
+        dt = time.perf_counter() - t0
+        t_total += dt
+        upd_total += int(sum(d.kstop - d.kstart for d in diag if d.status == 0))
+        row0 += rows
+        if t_total >= args.cpu_seconds or row0 >= sc.out_grid.length:
+            break
+        rows = max(1, min(64, int(rows * max(1.0, (args.cpu_seconds - t_total) / max(dt, 1e-3)))))
+    cpu = {"value": upd_total / t_total, "unit": "pixel-pulse updates/s", "cores": nth,
+           "kind": "port",
+           "sample": f"output rows {first}-{row0 - 1} of the 4096 x 4096 grid "
+                     f"({upd_total} updates), oracle reference mode, {t_total:.1f} s"}
+    log(f"cpu baseline: {cpu['value']:.3e} updates/s on {nth} threads")
+    return cpu
+
+
+# ------------------------------------------------------------ batched tiles
+def device_name(torch):
+    try:
+        return torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
+    except Exception:  # noqa: BLE001 - naming only
+        return "gpu"
+
+
+def run_batch(args, torch, np, dev, tuner):
+    from isce3_amd import autotune, sim
+    from isce3_amd.focus import backproject, backproject_blocks
+    t0 = time.time()
+    sc = sim.make_scene("swath")
+    _, raw = make_raw(torch, sim, sc, dev)
+    tiles = sim.random_tiles(sc.out_grid, 64, 64, 1024, (0.5, 2.0), seed=64)
+    px = sum(t.length * t.width for t in tiles)
+    log(f"batch: swath raw {tuple(raw.shape)} simulated in {time.time() - t0:.1f}s; "
+        f"64 tiles, {px} pixels")
+    outs = [torch.empty(t.shape, dtype=torch.complex64, device=dev) for t in tiles]
+    geoms = [sc.out_geometry(t) for t in tiles]
+
+    def batched(precision, options):
+        st = {}
+        backproject_blocks(outs, geoms, raw, sc.in_geometry(), sc.dem, sc.fc, sc.ds, sc.kernel,
+                           sc.dry_tropo_model, precision=precision, options=options, stats=st)
+        return st
+
+    def sequential(precision, options):
+        upd = 0
+        for o, g in zip(outs, geoms):
+            st = {}
+            backproject(o, g, raw, sc.in_geometry(), sc.dem, sc.fc, sc.ds, sc.kernel,
+                        sc.dry_tropo_model, precision=precision, options=options, stats=st)
+            upd += st["updates"]
+        return {"updates": upd}
+
+    key = autotune.workload_key(device_name(torch), "split", 8, px,
+                                float(np.mean([t.width for t in tiles])))
+    t0 = time.time()
+
+    def device_ms(options):
+        st = batched("split", options)
+        return st["geometry_ms"] + st["sum_ms"]
+
+    best = tuner.tune(key, "split", device_ms, force=True)
+    n_cand = len(tuner.trials.get(key, []))
+    log(f"batch: autotuned {best} over {n_cand} candidates in {time.time() - t0:.1f}s")
+
+    def timed(fn, precision, options):
+        for _ in range(args.warmup):
+            fn(precision, options)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = fn(precision, options)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        return {"ms": dt * 1e3, "updates": st["updates"], "updates_per_s": st["updates"] / dt}
+
+    res = {"split_batched": timed(batched, "split", best),
+           "plain_sequential": timed(sequential, "plain", None),
+           "split_sequential": timed(sequential, "split", None),
+           "plain_batched": timed(batched, "plain", None)}
+    for k, v in res.items():
+        log(f"batch {k}: {v['ms']:.1f} ms, {v['updates_per_s']:.3e} updates/s")
+    return {"workload": "64 tiles (side log-U[64,1024] px, aspect U[0.5,2]) of one resident "
+                        "8192x16384 c64 raw set",
+            "pixels": int(px), "tile_px_min": int(min(t.length * t.width for t in tiles)),
+            "tile_px_max": int(max(t.length * t.width for t in tiles)),
+            "options": best, "tuning_candidates": n_cand, **res,
+            "speedup_vs_plain_sequential": res["plain_sequential"]["ms"] / res["split_batched"]["ms"],
+            "speedup_vs_split_sequential": res["split_sequential"]["ms"] / res["split_batched"]["ms"]}
+
+
+# ----------------------------------------------------------------- serving
+def run_serve(args, torch, np, dev, tuner):
+    from isce3_amd import autotune, serve, sim
+    t0 = time.time()
+    sets, scenes = [], []
+    for seed in (41, 42, 43):
+        sc = sim.make_scene("swath", seed=seed)
+        _, raw = make_raw(torch, sim, sc, dev)
+        scenes.append(sc)
+        sets.append(serve.RawSet(raw, sc.in_geometry(), sc.orbit, sc.doppler, sc.dem, sc.fc,
+                                 sc.ds, sc.kernel, sc.dry_tropo_model))
+    log(f"serve: 3 raw sets {tuple(sets[0].signal.shape)} simulated in {time.time() - t0:.1f}s")
+    dname = device_name(torch)
+
+    def key_of(batch):
+        px = sum(r.pixels for r in batch)
+        mw = float(np.mean([r.grid.width for r in batch]))
+        return autotune.workload_key(dname, "split", 8, px, mw)
+
+    def split_options(set_id, batch):
+        return tuner.options_for(key_of(batch), "split")
+
+    def make_requests(n, seed, set_id=None):
+        rng = np.random.Generator(np.random.PCG64(seed))
+        reqs = []
+        for i in range(n):
+            sid = int(rng.integers(0, len(sets))) if set_id is None else set_id
+            (tile,) = sim.random_tiles(scenes[sid].out_grid, 1, 32, 2048, (1.0, 1.0),
+                                       seed=seed * 100003 + i)
+            reqs.append(serve.Request(sid, tile, tag=i))
+        return reqs
+
+    # tune the split launch options on three representative batch classes of
+    # the request mix (1, 4 and 16 requests of one set)
+    t0 = time.time()
+    current = {}
+    with serve.Server(sets, precision="split", workers=1, budget_bytes=args.serve_budget,
+                      options=lambda s, b: current) as srv:
+        for nb in (1, 4, 16):
+            proto = make_requests(nb, seed=7000 + nb, set_id=0)
+
+            def run(opts, proto=proto):
+                current.clear()
+                current.update(opts)
+                n0 = len(srv.batches)
+                fresh = srv.submit_many([serve.Request(0, r.grid) for r in proto])
+                for r in fresh:
+                    r.wait()
+                return sum(b["geometry_ms"] + b["sum_ms"] for b in srv.batches[n0:])
+
+            tuner.tune(key_of(proto), "split", run, force=True)
+    log(f"serve: tuned 3 batch classes in {time.time() - t0:.1f}s")
+
+    rows = []
+    for rate in args.rates:
+        n = max(20, int(args.serve_seconds * rate))
+        row = {"rate_per_s": rate, "requests": n}
+        for label, precision, policy in (("split_batched", "split", "batch"),
+                                         ("plain_fcfs", "plain", "fcfs")):
+            reqs = make_requests(n, seed=int(rate))
+            opts = split_options if precision == "split" else {}
+            with serve.Server(sets, precision=precision, policy=policy, workers=3,
+                              budget_bytes=args.serve_budget, options=opts) as srv:
+                srv.submit(serve.Request(0, reqs[0].grid)).wait()  # warm contexts
+                n0 = len(srv.batches)
+                res = serve.poisson_load(srv, reqs, rate, seed=int(rate) + 1)
+                batches = srv.batches[n0:]
+                upd = sum(b.get("updates", 0) for b in batches)
+                res.update(updates=upd, updates_per_s=upd / res["makespan_s"],
+                           batches=len(batches),
+                           mean_batch=float(np.mean([b["requests"] for b in batches])),
+                           peak_bytes=srv.peak_in_use)
+            row[label] = res
+            log(f"serve {rate:g}/s {label}: p50 {res['p50_ms']:.1f} ms, p99 {res['p99_ms']:.1f} ms,"
+                f" {res['updates_per_s']:.3e} updates/s, {res['batches']} batches")
+        rows.append(row)
+    return {"workload": "3 resident 8192x16384 c64 raw sets; Poisson tile requests, square side "
+                        "log-U[32,2048] px, random set and position; "
+                        f"{args.serve_budget / 1e9:.0f} GB admission budget",
+            "rates": rows}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the oracle baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sample-pixels", type=int, default=2048,
+                    help="pixels checked against the oracle")
+    ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--only", choices=["plain", "split"], default=None,
+                    help="time one precision of the full block only (profiling runs)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--rates", type=lambda s: [float(x) for x in s.split(",")],
+                    default=[20.0, 50.0, 100.0])
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="requests per rate = rate x this (at least 20)")
+    ap.add_argument("--serve-budget", type=float, default=200e9)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one device-to-device copy of this many MiB first, as the "
+                         "known-size reference for HBM byte counters")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark measures one GPU; runs on several GPUs are "
+                          "not part of it (output tiles shard with no exchange step)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from isce3_amd import autotune
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    dev = "cuda:0"
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"calibration copy of {args.calibrate_copy} MiB done")
+
+    sc, raw_host, results = run_full(args, torch, np, dev)
     if args.only:
         r = results[args.only]
         print(json.dumps({"only": args.only, "ms_per_step": r["ms_per_step"],
                           "sum_ms": r["sum_ms"], "updates": r["updates"],
                           "updates_per_s": r["updates_per_s"]}))
         return
+    raw_np = raw_host.numpy()
+    acc = accuracy(args, np, sc, raw_np, results)
+    cpu = None if args.no_cpu else cpu_baseline(args, np, sc, raw_np)
+    del raw_host, raw_np
+
+    out_dir = os.path.join(ROOT, "gpurun_out")
+    tuner = autotune.Autotuner(os.path.join(out_dir, "autotune.json")
+                               if os.path.isdir(out_dir) else None)
+    batch = None if args.no_batch else run_batch(args, torch, np, dev, tuner)
+    torch.cuda.empty_cache()
+    serve_res = None if args.no_serve else run_serve(args, torch, np, dev, tuner)
+
     sp, pl = results["split"], results["plain"]
     achieved = sp["updates"] * FLOPS_PER_UPDATE / (sp["sum_ms"] * 1e-3) / 1e12
+    latency = None
+    if serve_res:
+        latency = {f"{r['rate_per_s']:g}/s": {"p50_ms": r["split_batched"]["p50_ms"],
+                                              "p99_ms": r["split_batched"]["p99_ms"]}
+                   for r in serve_res["rates"]}
     line = {
         "metric": "pixel-pulse updates/s (split precision) + p50/p99 tile latency; split vs plain, 1 GPU",
         "value": sp["updates_per_s"],
@@ -193,20 +412,22 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64 (split)",
         "data": "synthetic (seeded scene generator)",
-        "config": {"workload": "c1/c2: 16384x16384 c64 pulses -> 4096x4096 c64, Knab 8/0.8/2048,"
-                               " ds=6 m, 200 targets + clutter, fractal DEM 1024^2",
+        "config": {"workload": "16384x16384 c64 pulses resident in HBM -> 4096x4096 c64 output, "
+                               "Knab 8/0.8/2048, ds=6 m, 200 targets + clutter, fractal DEM 1024^2",
                    "updates_per_step": sp["updates"], "parallelism": "1 GPU"},
         "plain": {"updates_per_s": pl["updates_per_s"], "ms_per_step": pl["ms_per_step"],
                   "sum_ms": pl["sum_ms"], "geometry_ms": pl["geometry_ms"],
-                  "dtype": "f64", "accuracy": acc["plain"]},
+                  "dtype": "f64", "accuracy": acc.get("plain")},
         "split": {"updates_per_s": sp["updates_per_s"], "ms_per_step": sp["ms_per_step"],
                   "sum_ms": sp["sum_ms"], "geometry_ms": sp["geometry_ms"],
-                  "accuracy": acc["split"]},
+                  "accuracy": acc.get("split")},
         "split_over_plain": sp["updates_per_s"] / pl["updates_per_s"],
         "roofline": {"bound": "valu", "kernel": "sum_split_kernel<8>",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},
-        "latency": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": hbm_bytes_per_launch()},
+        "latency": latency,
+        "batch": batch,
+        "serve": serve_res,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -46,18 +46,40 @@ __device__ inline int64_t wave_tile(int nb, int* lane)
     return (int64_t)lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 }
 
+// Descriptor words read through vector loads are not known to be uniform;
+// readfirstlane moves them into scalar registers (every lane of the
+// wavefront reads the same descriptor).
+__device__ inline int64_t uniform64(int64_t v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Output grid owning global wavefront tile s (s uniform across the wavefront).
 __device__ inline int find_grid(const OutDesc* g, int n, int64_t s)
 {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (g[mid].seq_base <= s)
+        if (uniform64(g[mid].seq_base) <= s)
             lo = mid;
         else
             hi = mid - 1;
     }
     return lo;
+}
+
+// Copy of descriptor i held in scalar registers.
+__device__ inline OutDesc load_desc(const OutDesc* g, int i)
+{
+    static_assert(sizeof(OutDesc) % 4 == 0, "descriptor must be whole dwords");
+    OutDesc d;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(g + i);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&d);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(OutDesc) / 4); ++k) dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+    return d;
 }
 
 __device__ inline bool tile_pixel(const TileMap& m, int64_t tile, int lane, int64_t* row,
@@ -109,7 +131,7 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
     int lane;
     const int64_t gseq = wave_tile(a.n_blocks, &lane);
     const bool in_seq = gseq < a.n_seq;
-    const OutDesc& d = a.grids[in_seq ? find_grid(a.grids, a.n_grids, gseq) : 0];
+    const OutDesc d = load_desc(a.grids, in_seq ? find_grid(a.grids, a.n_grids, gseq) : 0);
     int64_t row = 0, col = 0;
     const bool valid = in_seq && tile_pixel(d.map, gseq - d.seq_base, lane, &row, &col);
     const int64_t pix = row * d.map.width + col;
@@ -300,10 +322,10 @@ __device__ inline void for_each_tile(const OutDesc* grids, int n_grids, int64_t 
     for (int64_t w = xcd * chunk + j; w < end; w += n_local) {
         const int64_t s = w * kSumWaves + wave;
         if (s < n_seq) {
-            const OutDesc& d = grids[find_grid(grids, n_grids, s)];
+            const OutDesc d = load_desc(grids, find_grid(grids, n_grids, s));
             body(d, seq_to_tile(d.map, s - d.seq_base), lane);
         } else {
-            const OutDesc& d = grids[n_grids - 1];
+            const OutDesc d = load_desc(grids, n_grids - 1);
             body(d, d.map.n_tiles, lane);
         }
     }
@@ -385,7 +407,7 @@ __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
     int lane;
     const int64_t gseq = wave_tile(a.n_blocks, &lane);
     if (gseq < a.n_seq) {
-        const OutDesc& d = a.grids[find_grid(a.grids, a.n_grids, gseq)];
+        const OutDesc d = load_desc(a.grids, find_grid(a.grids, a.n_grids, gseq));
         plain_tile<W>(a, tab, d, gseq - d.seq_base, lane);
     }
 }

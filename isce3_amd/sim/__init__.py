@@ -10,12 +10,14 @@ compressed pulses are written by the native generator in
 include/isce3_amd_sim.h (sinc range response of bandwidth 0.8 of the sample
 rate, bistatic delay, phase exp(-j 2 pi fc tau)).
 
-Configurations (pulses x range bins -> output grid):
-  c0   512 x 2048 -> 256 x 256, 3 targets, flat DEM, noise 0.01
-  c1   16384 x 16384 -> 4096 x 4096, 200 targets + clutter, fractal DEM
-  c3   8192 x 16384 raw set (tile requests are cut from its swath)
-  pt   1024 x 1024 -> 129 x 129 chip around one target, Knab width 9
-  tiny 256 x 512 -> 48 x 40, 2 targets, fractal DEM (fast parity checks)
+Scenes (pulses x range bins -> output grid):
+  small  512 x 2048 -> 256 x 256, 3 targets, flat DEM, noise 0.01
+  full   16384 x 16384 -> 4096 x 4096, 200 targets + clutter, fractal DEM
+  swath  8192 x 16384 raw set; its output grid (4096 lines x 12288 range
+         samples, the illuminated part of the swath) is where tile
+         requests are cut (random_tiles)
+  pt     1024 x 1024 -> 129 x 129 chip around one target, Knab width 9
+  tiny   256 x 512 -> 48 x 40, 2 targets, fractal DEM (fast parity checks)
 """
 from __future__ import annotations
 
@@ -166,21 +168,21 @@ def make_scene(config: str, seed: Optional[int] = None, *, ds: Optional[float] =
     orbit = straight_orbit()
     kw = kernel_width
     clutter_var = 0.05
-    if config == "c0":
+    if config == "small":
         seed = 0 if seed is None else seed
         in_grid, out_grid = _center_grid(512, 2048, 256, 256)
         dem = DEMInterpolator(0.0)
         rel = [(64.3, 70.2), (128.0, 128.0), (190.6, 180.4)]
         ntg, step, noise, kw = 3, None, 0.01, kw or 8.0
-    elif config == "c1":
+    elif config == "full":
         seed = 1 if seed is None else seed
         in_grid, out_grid = _center_grid(16384, 16384, 4096, 4096)
         dem = _dem_around(orbit, out_grid, 1024, 30.0, 0.8, 200.0, seed)
         ntg, step, noise, kw = n_targets or 200, clutter_step or 32, 0.01, kw or 8.0
         rel = None
-    elif config == "c3":
+    elif config == "swath":
         seed = 3 if seed is None else seed
-        in_grid, out_grid = _center_grid(8192, 16384, 4096, 4096)
+        in_grid, out_grid = _center_grid(8192, 16384, 4096, 12288)
         dem = _dem_around(orbit, out_grid, 1024, 30.0, 0.8, 200.0, seed)
         ntg, step, noise, kw = n_targets or 200, clutter_step or 32, 0.01, kw or 8.0
         rel = None
@@ -223,6 +225,24 @@ def make_scene(config: str, seed: Optional[int] = None, *, ds: Optional[float] =
     return Scene(config, orbit, in_grid, out_grid, dem, FC, ds, kernel, xyz,
                  np.asarray(amps, dtype=np.complex128), np.asarray(radar), len(rel), noise,
                  int(seed))
+
+
+def random_tiles(grid: RadarGridParameters, n: int, side_min: float, side_max: float,
+                 aspect: tuple = (1.0, 1.0), seed: int = 0) -> list:
+    """n sub-grids of `grid` at random positions: side log-uniform in
+    [side_min, side_max] pixels, aspect ratio (lines / samples) uniform in
+    `aspect`, both sides kept within [1, grid size]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    tiles = []
+    for _ in range(n):
+        side = math.exp(rng.uniform(math.log(side_min), math.log(side_max)))
+        a = rng.uniform(aspect[0], aspect[1])
+        nl = int(min(grid.length, max(1, round(side * math.sqrt(a)))))
+        nw = int(min(grid.width, max(1, round(side / math.sqrt(a)))))
+        r0 = int(rng.integers(0, grid.length - nl + 1))
+        c0 = int(rng.integers(0, grid.width - nw + 1))
+        tiles.append(grid.offset_and_resize(r0, c0, nl, nw))
+    return tiles
 
 
 def pulse_state(orbit: Orbit, grid: RadarGridParameters) -> np.ndarray:

@@ -42,7 +42,7 @@ def run_blocks(sc, raw, grids, precision, **kw):
 @pytest.mark.parametrize("precision", ["plain", "split"])
 @pytest.mark.parametrize("tile_rows", [1, 2, 8])
 def test_blocks_equal_single_calls(torch_gpu, precision, tile_rows):
-    sc, raw = scene_and_raw("c0")
+    sc, raw = scene_and_raw("small")
     grids = uneven_blocks(sc)
     opts = {"tile_rows": tile_rows}
     outs, hs, flags, stats = run_blocks(sc, raw, grids, precision, options=opts)
@@ -58,7 +58,7 @@ def test_blocks_equal_single_calls(torch_gpu, precision, tile_rows):
 
 
 def test_blocks_match_oracle(torch_gpu):
-    sc, raw = scene_and_raw("c0")
+    sc, raw = scene_and_raw("small")
     grids = uneven_blocks(sc)
     outs, hs, flags, _ = run_blocks(sc, raw, grids, "plain")
     for b, z, h in zip(grids, outs, hs):
@@ -74,7 +74,7 @@ def test_blocks_match_oracle(torch_gpu):
 
 
 def test_split_blocks_match_fp64_oracle(torch_gpu):
-    sc, raw = scene_and_raw("c0")
+    sc, raw = scene_and_raw("small")
     grids = uneven_blocks(sc)[3:6]
     outs, _, _, _ = run_blocks(sc, raw, grids, "split")
     for b, z in zip(grids, outs):

@@ -1,6 +1,6 @@
 """GPU edge cases against the oracle: failure rules, exceptions, TSX delay,
 non-zero Doppler LUTs, outputs reaching past the data, resampled output grids,
-device-tensor I/O, the config-0 golden fixture and point-target quality."""
+device-tensor I/O, the small-scene golden fixture and point-target quality."""
 import os
 
 import numpy as np
@@ -145,7 +145,7 @@ def test_output_grid_past_the_data(torch_gpu, precision):
 
 @pytest.mark.parametrize("precision", ["plain", "split"])
 def test_resampled_output_grid(torch_gpu, precision):
-    sc, raw = scene_and_raw("c0")
+    sc, raw = scene_and_raw("small")
     g = sc.out_grid.offset_and_resize(10.3, 20.7, 77, 53)
     g.prf = sc.out_grid.prf * 1.37
     g.range_pixel_spacing = sc.out_grid.range_pixel_spacing * 0.61
@@ -163,9 +163,9 @@ def test_empty_output_grid(torch_gpu):
     assert z.shape == (0, 5) and not err
 
 
-def test_c0_golden_fixture(torch_gpu):
-    fx = np.load(os.path.join(GOLDEN, "c0_oracle.npz"))
-    sc, raw = scene_and_raw("c0")
+def test_small_scene_golden_fixture(torch_gpu):
+    fx = np.load(os.path.join(GOLDEN, "small_oracle.npz"))
+    sc, raw = scene_and_raw("small")
     stats = {}
     z, h, err = gpu(sc, raw, stats=stats)
     assert not err and stats["updates"] == int(fx["updates"])
@@ -217,7 +217,7 @@ def test_device_tensor_io_matches_host_arrays(torch_gpu, precision):
 def test_context_reuse_across_sizes(torch_gpu):
     """Consecutive calls of different sizes on one context give the results of
     fresh calls (scratch buffers are grown and reused, never stale)."""
-    sc1, raw1 = scene_and_raw("c0")
+    sc1, raw1 = scene_and_raw("small")
     sc2, raw2 = scene_and_raw("tiny")
     a1, _, _ = gpu(sc1, raw1, precision="split")
     b1, _, _ = gpu(sc2, raw2, precision="split")

@@ -18,7 +18,7 @@ def c1(torch_gpu):
     torch = torch_gpu
     from isce3_amd import sim
     print("[fullsize] simulating c1", flush=True)
-    sc = sim.make_scene("c1")
+    sc = sim.make_scene("full")
     raw = torch.empty((sc.in_grid.length, sc.in_grid.width), dtype=torch.complex64,
                       pin_memory=True)
     sim.simulate(sc, out=raw.numpy(), nthreads=min(16, os.cpu_count() or 1))

@@ -29,7 +29,7 @@ def run_gpu(config, precision, **kw):
     return out, h, err, stats
 
 
-@pytest.mark.parametrize("config", ["tiny", "c0", "pt"])
+@pytest.mark.parametrize("config", ["tiny", "small", "pt"])
 def test_plain_matches_reference_oracle(torch_gpu, config):
     print(f"[plain parity] config={config}", flush=True)
     out, h, err, stats = run_gpu(config, "plain")
@@ -44,7 +44,7 @@ def test_plain_matches_reference_oracle(torch_gpu, config):
     assert stats["updates"] > 0
 
 
-@pytest.mark.parametrize("config", ["tiny", "c0", "pt"])
+@pytest.mark.parametrize("config", ["tiny", "small", "pt"])
 def test_split_matches_fp64_oracle(torch_gpu, config):
     print(f"[split parity] config={config}", flush=True)
     out, h, err, stats = run_gpu(config, "split")

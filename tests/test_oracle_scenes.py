@@ -1,6 +1,6 @@
 """The oracle on whole seeded scenes: point-target focusing quality (the
 acceptance test of tests/python/extensions/pybind/focus/backproject.py:126-170,
-re-created on a synthetic scene), the config-0 golden fixture, agreement of
+re-created on a synthetic scene), the small-scene golden fixture, agreement of
 the reference and all-FP64 modes, and the reference's per-pixel failure and
 exception rules (Backproject.cpp:88-92,141-171)."""
 import os
@@ -50,18 +50,18 @@ def test_point_target_quality_oracle():
 
 
 def test_reference_and_fp64_modes_agree():
-    for cfg in ("tiny", "c0"):
+    for cfg in ("tiny", "small"):
         a, _, _, _ = oracle_run(cfg, oracle.MODE_REFERENCE)
         b, _, _, _ = oracle_run(cfg, oracle.MODE_FP64)
         assert complex_rel_error(a, b) < 1e-6
         assert rms(phase_errors(a, b)) < 1e-6
 
 
-def test_c0_golden_fixture():
+def test_small_scene_golden_fixture():
     """Config 0 (512 x 2048 pulses -> 256 x 256, 3 targets, flat DEM, noise 0.01)
     against the committed oracle output (every 4th pixel)."""
-    fx = np.load(os.path.join(GOLDEN, "c0_oracle.npz"))
-    out, h, _, rc = oracle_run("c0", oracle.MODE_REFERENCE)
+    fx = np.load(os.path.join(GOLDEN, "small_oracle.npz"))
+    out, h, _, rc = oracle_run("small", oracle.MODE_REFERENCE)
     assert rc == 0
     idx = fx["index"]
     np.testing.assert_allclose(out.ravel()[idx], fx["out"], rtol=1e-5, atol=1e-4)
