@@ -69,8 +69,8 @@ def make_raw(torch, sim, sc, dev):
 
 
 # ------------------------------------------------------------- full block
-def run_full(args, torch, np, dev):
-    from isce3_amd import sim
+def run_full(args, torch, np, dev, tuner):
+    from isce3_amd import autotune, sim
     from isce3_amd.focus import backproject
     t0 = time.time()
     sc = sim.make_scene("full")
@@ -78,9 +78,8 @@ def run_full(args, torch, np, dev):
     log(f"full scene: {len(sc.targets)} scatterers, {tuple(raw.shape)} simulated in "
         f"{time.time() - t0:.1f}s")
     out = torch.empty(sc.out_grid.shape, dtype=torch.complex64, device=dev)
-    opts = {"tile_rows": args.tile_rows} if args.tile_rows else None
 
-    def step(precision):
+    def step(precision, opts):
         st = {}
         backproject(out, sc.out_geometry(), raw, sc.in_geometry(), sc.dem, sc.fc, sc.ds,
                     sc.kernel, sc.dry_tropo_model, precision=precision, stats=st, options=opts)
@@ -89,13 +88,30 @@ def run_full(args, torch, np, dev):
     results = {}
     precisions = (args.only,) if args.only else ("plain", "split")
     for precision in precisions:
+        if args.options is not None:
+            opts = dict(args.options)
+        elif args.no_tune:
+            opts = None
+        else:
+            # launch options from the autotuner (untimed, part of the warmup)
+            t0 = time.time()
+            key = autotune.workload_key(device_name(torch), precision, 8,
+                                        sc.out_grid.length * sc.out_grid.width,
+                                        float(sc.out_grid.width))
+
+            def device_ms(o, precision=precision):
+                st = step(precision, o)
+                return st["geometry_ms"] + st["sum_ms"]
+
+            opts = tuner.tune(key, precision, device_ms, force=True)
+            log(f"{precision}: autotuned {opts} in {time.time() - t0:.1f}s")
         for _ in range(args.warmup):
-            step(precision)
+            step(precision, opts)
         torch.cuda.synchronize()
         stats = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            stats.append(step(precision))
+            stats.append(step(precision, opts))
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
         upd = stats[-1]["updates"]
@@ -103,7 +119,7 @@ def run_full(args, torch, np, dev):
         geo_ms = float(np.mean([s["geometry_ms"] for s in stats]))
         results[precision] = dict(ms_per_step=dt * 1e3, updates=upd, sum_ms=sum_ms,
                                   geometry_ms=geo_ms, updates_per_s=upd / dt,
-                                  out=out.cpu().numpy())
+                                  options=opts or {}, out=out.cpu().numpy())
         log(f"{precision}: {dt * 1e3:.1f} ms/step, sum {sum_ms:.1f} ms, geometry {geo_ms:.1f} ms,"
             f" {upd / dt:.3e} updates/s")
     return sc, raw_host, results
@@ -337,7 +353,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sample-pixels", type=int, default=2048,
                     help="pixels checked against the oracle")
-    ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--options", type=json.loads, default=None,
+                    help="fixed launch options (JSON) for the full block instead of the "
+                         "autotuner's, e.g. '{\"tile_rows\": 4}'")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default launch options for the full block (profiling runs)")
     ap.add_argument("--only", choices=["plain", "split"], default=None,
                     help="time one precision of the full block only (profiling runs)")
     ap.add_argument("--no-batch", action="store_true")
@@ -373,7 +393,10 @@ def main():
         del src, dst
         log(f"calibration copy of {args.calibrate_copy} MiB done")
 
-    sc, raw_host, results = run_full(args, torch, np, dev)
+    out_dir = os.path.join(ROOT, "gpurun_out")
+    tuner = autotune.Autotuner(os.path.join(out_dir, "autotune.json")
+                               if os.path.isdir(out_dir) else None)
+    sc, raw_host, results = run_full(args, torch, np, dev, tuner)
     if args.only:
         r = results[args.only]
         print(json.dumps({"only": args.only, "ms_per_step": r["ms_per_step"],
@@ -385,9 +408,6 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(args, np, sc, raw_np)
     del raw_host, raw_np
 
-    out_dir = os.path.join(ROOT, "gpurun_out")
-    tuner = autotune.Autotuner(os.path.join(out_dir, "autotune.json")
-                               if os.path.isdir(out_dir) else None)
     batch = None if args.no_batch else run_batch(args, torch, np, dev, tuner)
     torch.cuda.empty_cache()
     serve_res = None if args.no_serve else run_serve(args, torch, np, dev, tuner)
@@ -417,10 +437,10 @@ def main():
                    "updates_per_step": sp["updates"], "parallelism": "1 GPU"},
         "plain": {"updates_per_s": pl["updates_per_s"], "ms_per_step": pl["ms_per_step"],
                   "sum_ms": pl["sum_ms"], "geometry_ms": pl["geometry_ms"],
-                  "dtype": "f64", "accuracy": acc.get("plain")},
+                  "options": pl["options"], "dtype": "f64", "accuracy": acc.get("plain")},
         "split": {"updates_per_s": sp["updates_per_s"], "ms_per_step": sp["ms_per_step"],
                   "sum_ms": sp["sum_ms"], "geometry_ms": sp["geometry_ms"],
-                  "accuracy": acc.get("split")},
+                  "options": sp["options"], "accuracy": acc.get("split")},
         "split_over_plain": sp["updates_per_s"] / pl["updates_per_s"],
         "roofline": {"bound": "valu", "kernel": "sum_split_kernel<8>",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
