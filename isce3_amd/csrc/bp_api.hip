@@ -308,7 +308,8 @@ int run_blocks(bp_context* c, const bp_orbit* oo_p, const bp_lut2d* odop_p, bp_o
         return fail(BP_ERR_INVALID_ARGUMENT, "unknown precision mode");
     const bp_grid& gi = ig->grid;
     if (gi.length < 0 || gi.width < 0) return fail(BP_ERR_INVALID_ARGUMENT, "negative grid size");
-    if (gi.length > INT32_MAX) return fail(BP_ERR_INVALID_ARGUMENT, "grid size exceeds int range");
+    if (gi.length > INT32_MAX || gi.width > INT32_MAX)
+        return fail(BP_ERR_INVALID_ARGUMENT, "grid size exceeds int range");
     if (!(gi.prf > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "PRF must be positive");
     int64_t npix = 0;
     for (int b = 0; b < n_blocks; ++b) {
@@ -456,6 +457,8 @@ int run_blocks(bp_context* c, const bp_orbit* oo_p, const bp_lut2d* odop_p, bp_o
     ga.n_seq = n_seq;
     ga.n_blocks = wblocks;
     ga.in_side = gi.look_side;
+    ga.kwidth = width;
+    ga.nr = (int32_t)nr;
     ga.in_t0 = gi.sensing_start;
     ga.in_dt = in_dt;
     ga.na = (int32_t)na;
@@ -497,6 +500,7 @@ int run_blocks(bp_context* c, const bp_orbit* oo_p, const bp_lut2d* odop_p, bp_o
     sa.dtau = dtau;
     sa.fc = fc;
     sa.inv_k = 1.0 / (fc * dtau);
+    sa.zscale = (float)(-(double)bp::kPolyphaseRows * sa.inv_k);
     sa.halfwidth = kernel->halfwidth;
     sa.rdx = (float)(1.0 / (kernel->halfwidth / (kernel->n - 1.0)));
     sa.ntab = kernel->n;

@@ -28,24 +28,26 @@ struct alignas(16) PixGeom {
     int32_t pad_;
 };
 
-// Per-pixel record of the split-precision sum.  The pulse phase in cycles is
-// c_ref + sum_{n=1..4} b[n-1] s^n with s = (k - kc) * inv_h (FP64 Horner on
-// the FP64 fitted coefficients); the fractional range-bin position is
-// f_ref + (phase - c_ref) * inv_k evaluated in FP32, relative to bin n_ref.
+// Per-pixel record of the split-precision sum.  The pulse phase in cycles,
+// P(k), is a degree-4 polynomial in the pulse index fitted in FP64; the sum
+// kernel steps it pulse by pulse from the wavefront's first pulse kw0 by FP64
+// forward differences:  P += d1; d1 += d2; d2 += d3; d3 += d4.
+// The polyphase coordinate z = fma((float)P, zscale, zoff), with the global
+// zscale = -Q/(fc dtau), gives the first tap c0 = cbase - (floor(z) >> log2 Q)
+// and the weight row floor(z) & (Q - 1) + fract(z).
 struct alignas(16) PixSplit {
-    double b[4];
-    double c_ref;
-    double kc;
-    double inv_h;
-    float f_ref;
-    int32_t n_ref;
+    double p0;               // P(kw0)
+    double d1, d2, d3, d4;   // forward differences of P at kw0
+    float zoff;
+    int32_t cbase;
     int32_t kstart, kstop;
     int32_t status;
-    int32_t pad_;
+    int32_t edge;            // nonzero: some tap of the window may leave [0, nr)
 };
 
 constexpr int kFitDegree = 4;        // delay polynomial degree of the split path
-constexpr int kPolyphaseRows = 512;  // fractional-offset rows of the split weight table
+constexpr int kPolyphaseLog2 = 9;
+constexpr int kPolyphaseRows = 1 << kPolyphaseLog2;  // fractional-offset rows of the split weight table
 constexpr int kMaxTableLds = 8192;   // plain path keeps tables up to this size in LDS
 
 // Pixel tiling of one output grid: each 64-lane wavefront owns a tile of
@@ -81,6 +83,8 @@ struct GeomArgs {
     int64_t n_seq;              // total wavefront tiles
     int32_t n_blocks;           // 256-thread blocks (4 tiles each)
     int32_t in_side;
+    int32_t kwidth;             // interpolation kernel width W (taps)
+    int32_t nr;                 // range samples per pulse
     double in_t0, in_dt;
     int32_t na, tropo;
     double swst, dtau;
@@ -108,7 +112,7 @@ struct SumArgs {
     double inv_k;               // 1 / (fc * dtau): cycles -> range bins
     double halfwidth;           // kernel half width
     float rdx;                  // float(1/dx) of the tabulated kernel
-    int32_t pad_;
+    float zscale;               // -Q * inv_k (split polyphase coordinate per cycle)
     const float2* rc;           // range-compressed pulses [na x nr]
     const double* pulse_pv;     // per-pulse position/velocity [na x 6]
     const PixGeom* geom;
@@ -124,6 +128,5 @@ hipError_t launch_geometry(const GeomArgs& args, hipStream_t stream);
 hipError_t launch_sum_plain(const SumArgs& args, int width, hipStream_t stream);
 hipError_t launch_sum_split(const SumArgs& args, int width, int grid, hipStream_t stream);
 int sum_blocks_per_cu(int precision, int width);
-size_t split_lds_bytes(int width);
 
 }  // namespace bp

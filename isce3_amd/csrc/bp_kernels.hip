@@ -212,47 +212,55 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
     const int kw1 = wave_max_i(has_win ? g.kstop : INT_MIN);
     if (!valid) return;
     PixSplit sp;
-    sp.b[0] = sp.b[1] = sp.b[2] = sp.b[3] = 0.;
-    sp.c_ref = 0.;
-    sp.kc = 0.;
-    sp.inv_h = 0.;
-    sp.f_ref = 0.f;
-    sp.n_ref = 0;
+    sp.p0 = sp.d1 = sp.d2 = sp.d3 = sp.d4 = 0.;
+    sp.zoff = 0.f;
+    sp.cbase = 0;
     sp.kstart = has_win ? g.kstart : 0;
     sp.kstop = has_win ? g.kstop : 0;
     sp.status = g.status;
-    sp.pad_ = 0;
+    sp.edge = 0;
     if (has_win) {
         const double kc = 0.5 * ((double)kw0 + (double)(kw1 - 1));
         const double h = fmax(0.5 * (double)(kw1 - 1 - kw0), 0.5);
-        sp.kc = kc;
-        sp.inv_h = 1.0 / h;
         // delay at the reference pulse kc
         V3 pc, vc;
         orbit_interpolate(a.in_orbit, a.in_t0 + kc * a.in_dt, kBorderExtrapolate, &pc, &vc);
         const double tau_c = g.tau_atm + bistatic_delay(pc, vc, x);
         constexpr int M = kFitDegree + 1;
-        double gm[M], sm[M];
+        // Chebyshev nodes cos(pi (m + 1/2) / M) and T_j at the nodes
+        constexpr double kNode[M] = {0.95105651629515353, 0.58778525229247314, 0.0,
+                                     -0.58778525229247314, -0.95105651629515353};
+        double gm[M];
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-            sm[m] = cos(kPi * (m + 0.5) / M);
-            const double km = kc + h * sm[m];
+            const double km = kc + h * kNode[m];
             V3 pm, vm;
             orbit_interpolate(a.in_orbit, a.in_t0 + km * a.in_dt, kBorderExtrapolate, &pm, &vm);
             const double tau_m = g.tau_atm + bistatic_delay(pm, vm, x);
             gm[m] = a.fc * (tau_m - tau_c);  // cycles relative to the reference pulse
         }
-        // Chebyshev coefficients c_j = 2/M sum_m g_m T_j(s_m), c_0 halved.
+        // Chebyshev coefficients c_j = 2/M sum_m g_m T_j(s_m), c_0 halved,
+        // with T_j(s_m) = cos(j pi (m + 1/2) / M).
         double cc[M];
 #pragma unroll
         for (int j = 0; j < M; ++j) {
             double acc = 0.;
 #pragma unroll
-            for (int m = 0; m < M; ++m) acc += gm[m] * cos(j * kPi * (m + 0.5) / M);
+            for (int m = 0; m < M; ++m) {
+                // T_j(s_m) by the recurrence (exact to rounding at these nodes)
+                double t0 = 1., t1 = kNode[m], tj = j == 0 ? 1. : kNode[m];
+#pragma unroll
+                for (int i = 2; i <= j; ++i) {
+                    tj = 2. * kNode[m] * t1 - t0;
+                    t0 = t1;
+                    t1 = tj;
+                }
+                acc += gm[m] * tj;
+            }
             cc[j] = acc * (2.0 / M);
         }
         cc[0] *= 0.5;
-        // Chebyshev -> monomial: T_{j+1} = 2 s T_j - T_{j-1}
+        // Chebyshev -> monomial in s = (k - kc) / h: T_{j+1} = 2 s T_j - T_{j-1}
         double mono[M], tjm1[M], tj[M], tjp1[M];
 #pragma unroll
         for (int n = 0; n < M; ++n) {
@@ -275,17 +283,55 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
                 tj[n] = tjp1[n];
             }
         }
-        sp.b[0] = mono[1];
-        sp.b[1] = mono[2];
-        sp.b[2] = mono[3];
-        sp.b[3] = mono[4];
         // Reference phase (cycles, reduced) and range-bin position at s = 0.
         const double cyc_c = a.fc * tau_c;
-        sp.c_ref = (cyc_c - floor(cyc_c)) + mono[0];
+        const double c_ref = (cyc_c - floor(cyc_c)) + mono[0];
         const double u_c = (tau_c - a.swst) / a.dtau + mono[0] / (a.fc * a.dtau);
         const double n_ref = floor(u_c);
-        sp.n_ref = (int32_t)n_ref;
-        sp.f_ref = (float)(u_c - n_ref);
+        const double f_ref = u_c - n_ref;
+        // Re-expand dc(s) = sum_{n>=1} mono[n] s^n about the first pulse kw0:
+        // s = s0 + j / h, j = k - kw0; coefficients a_i of j^i.
+        const double s0 = ((double)kw0 - kc) / h;
+        const double ih = 1.0 / h;
+        double av[M];
+#pragma unroll
+        for (int i = 0; i < M; ++i) {
+            double acc = 0., binom = 1.;  // C(n, i) s0^(n - i), n = i..4
+            double sp0 = 1.;
+#pragma unroll
+            for (int n = i; n < M; ++n) {
+                if (n >= 1) acc += mono[n] * binom * sp0;
+                // next n: C(n+1, i) = C(n, i) (n + 1) / (n + 1 - i)
+                binom = binom * (double)(n + 1) / (double)(n + 1 - i);
+                sp0 *= s0;
+            }
+            av[i] = acc;
+        }
+        double ihp = 1.;
+#pragma unroll
+        for (int i = 1; i < M; ++i) {
+            ihp *= ih;
+            av[i] *= ihp;
+        }
+        // forward differences of P(j) = c_ref + sum a_i j^i at j = 0
+        sp.p0 = c_ref + av[0];
+        sp.d1 = av[1] + av[2] + av[3] + av[4];
+        sp.d2 = 2. * av[2] + 6. * av[3] + 14. * av[4];
+        sp.d3 = 6. * av[3] + 36. * av[4];
+        sp.d4 = 24. * av[4];
+        // polyphase coordinate z = -Q (u + shift), u = f_ref + (P - c_ref) / (fc dtau);
+        // shift 0 for even W (first tap ceil(u) - W/2), 1/2 for odd W (round(u) - W/2)
+        const double inv_k = 1.0 / (a.fc * a.dtau);
+        const bool odd = (a.kwidth & 1) != 0;
+        const double shift = odd ? 0.5 : 0.0;
+        sp.zoff = (float)(-(double)kPolyphaseRows * (f_ref - c_ref * inv_k + shift));
+        sp.cbase = (int32_t)n_ref - a.kwidth / 2 - (odd ? 1 : 0);
+        // Conservative range of the first tap over the window (|s| <= 1 on the
+        // wavefront's window, so |dc| <= sum |mono[n]|), one bin of margin.
+        const double bnd = inv_k * (fabs(mono[1]) + fabs(mono[2]) + fabs(mono[3]) + fabs(mono[4]));
+        const double c_lo = n_ref + ceil(f_ref - bnd + shift) - (a.kwidth / 2) - (odd ? 1 : 0) - 1;
+        const double c_hi = n_ref + ceil(f_ref + bnd + shift) - (a.kwidth / 2) - (odd ? 1 : 0) + 1;
+        sp.edge = (c_lo < 0. || c_hi + a.kwidth > (double)a.nr) ? 1 : 0;
     }
     a.split[rec] = sp;
 }
@@ -427,81 +473,84 @@ __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
 // The pulses every lane of the wavefront integrates, [max kstart, min kstop),
 // run in an unmasked loop whose tap loads are issued one pulse ahead; the
 // ragged ends of the lanes' windows run in a masked loop.
-template<int W>
+// FP64 phase of one lane in cycles, P(k), stepped by forward differences.
+struct PhaseState {
+    double p, d1, d2, d3, d4;
+    __device__ inline void advance()
+    {
+        p += d1;
+        d1 += d2;
+        d2 += d3;
+        d3 += d4;
+    }
+};
+
 struct SplitPulse {
     float ph;   // phase, revolutions in [0, 1)
-    float t0;   // fractional offset of the first tap
+    float qf;   // fraction between weight rows
+    int row;    // byte offset of the weight row in LDS
     int c0;     // first tap's range bin
 };
 
 template<int W>
-__device__ inline SplitPulse<W> split_pulse(double kd, double s_off, double inv_h, double b0,
-                                            double b1, double b2, double b3, double c_ref,
-                                            float inv_kf, float f_ref, int n_ref)
+__device__ inline SplitPulse split_pulse(const PhaseState& st, float zscale, float zoff, int cbase)
 {
-    const double s = fma(kd, inv_h, s_off);
-    const double dc = fma(fma(fma(b3, s, b2), s, b1), s, b0) * s;
-    SplitPulse<W> r;
-    r.ph = (float)__builtin_amdgcn_fract(c_ref + dc);
-    const float ul = fmaf((float)dc, inv_kf, f_ref);
-    const float fi = (W % 2 == 0) ? ceilf(ul) : roundf(ul);
-    r.t0 = fi - ul;
-    r.c0 = n_ref + (int)fi - W / 2;
+    constexpr int ROWB = 4 * 2 * ((W + 3) / 4 * 4);  // bytes of one weight row
+    SplitPulse r;
+    r.ph = (float)__builtin_amdgcn_fract(st.p);
+    const float z = fmaf((float)st.p, zscale, zoff);
+    const float zf = floorf(z);
+    const int m = (int)zf;
+    r.qf = z - zf;
+    r.row = (m & (kPolyphaseRows - 1)) * ROWB;
+    r.c0 = cbase - (m >> kPolyphaseLog2);
     return r;
 }
 
-// Requires nr >= W (split_tile routes narrower lines to the guarded loads).
-template<int W>
+// Tap loads.  Unguarded: the window is known to lie inside the line (the
+// clamp only keeps every load inside the line).  Guarded: taps outside
+// [0, nr) are zero (core/detail/Interp1d.h:62-83).
+template<int W, bool kGuard>
 __device__ inline void split_taps(const float2* line, int c0, int nr, float2 (&d)[W])
 {
-    // clamp the start so the load stays inside the line; lanes whose window
-    // reaches past an end are fixed up in split_update
-    const int cb = min(max(c0, 0), nr - W);
-    const float2* p = line + cb;
+    if (kGuard) {
 #pragma unroll
-    for (int m = 0; m < W; ++m) d[m] = p[m];
+        for (int m = 0; m < W; ++m) {
+            const int j = c0 + m;
+            d[m] = (j >= 0 && j < nr) ? line[j] : make_float2(0.f, 0.f);
+        }
+    } else {
+        // 32-bit byte offset from the wave-uniform line base
+        const uint32_t off = (uint32_t)min(max(c0, 0), nr - W) * (uint32_t)sizeof(float2);
+        const float2* p = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(line) + off);
+#pragma unroll
+        for (int m = 0; m < W; ++m) d[m] = p[m];
+    }
 }
 
-template<int W, bool kGuarded = false>
-__device__ inline void split_update(const float* s_poly, const float2* line, int nr,
-                                    const SplitPulse<W>& pu, const float2 (&d)[W], float& cre,
-                                    float& cim)
+template<int W>
+__device__ inline void split_update(const float* s_poly, const SplitPulse& pu,
+                                    const float2 (&d)[W], float& cre, float& cim)
 {
     constexpr int WP = (W + 3) / 4 * 4;
-    constexpr int ROW = 2 * WP;
-    constexpr float TMIN = (W % 2 == 0) ? 0.0f : -0.5f;
-    const float q = (pu.t0 - TMIN) * (float)kPolyphaseRows;
-    const int qi = (int)q;  // in [0, Q]: the table holds Q + 1 rows
-    const float qf = __builtin_amdgcn_fractf(q);
     // one polyphase row: WP weights then WP deltas, read as float4
-    const float4* row4 = reinterpret_cast<const float4*>(s_poly + qi * ROW);
+    const float4* row4 =
+        reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_poly) + pu.row);
     float w[WP];
 #pragma unroll
     for (int i = 0; i < WP / 4; ++i) {
         const float4 a = row4[i];
         const float4 b = row4[WP / 4 + i];
-        w[4 * i + 0] = fmaf(qf, b.x, a.x);
-        w[4 * i + 1] = fmaf(qf, b.y, a.y);
-        w[4 * i + 2] = fmaf(qf, b.z, a.z);
-        w[4 * i + 3] = fmaf(qf, b.w, a.w);
+        w[4 * i + 0] = fmaf(pu.qf, b.x, a.x);
+        w[4 * i + 1] = fmaf(pu.qf, b.y, a.y);
+        w[4 * i + 2] = fmaf(pu.qf, b.z, a.z);
+        w[4 * i + 3] = fmaf(pu.qf, b.w, a.w);
     }
     float sre = 0.f, sim = 0.f;
-    const int cb = min(max(pu.c0, 0), nr - W);
-    if (!kGuarded && cb == pu.c0) {
 #pragma unroll
-        for (int m = 0; m < W; ++m) {
-            sre = fmaf(w[m], d[m].x, sre);
-            sim = fmaf(w[m], d[m].y, sim);
-        }
-    } else {
-        // taps outside [0, nr) are zero (core/detail/Interp1d.h:62-83)
-#pragma unroll
-        for (int m = 0; m < W; ++m) {
-            const int j = pu.c0 + m;
-            const float2 v = (j >= 0 && j < nr) ? line[j] : make_float2(0.f, 0.f);
-            sre = fmaf(w[m], v.x, sre);
-            sim = fmaf(w[m], v.y, sim);
-        }
+    for (int m = 0; m < W; ++m) {
+        sre = fmaf(w[m], d[m].x, sre);
+        sim = fmaf(w[m], d[m].y, sim);
     }
     const float sn = __builtin_amdgcn_sinf(pu.ph);
     const float cs = __builtin_amdgcn_cosf(pu.ph);
@@ -518,6 +567,82 @@ __device__ inline void kahan_fold(float& acc, float& cmp, float& part)
     cmp = (t - acc) - y;
     acc = t;
     part = 0.f;
+}
+
+// The pulse loops of one wavefront over its union window [kw0, kw1): the
+// pulses every lane integrates, [kf0, kf1), run unmasked with the tap loads
+// of pulse k+1 issued before pulse k is accumulated; the ragged ends run
+// masked.  Every pulse advances every lane's phase state exactly once.
+template<int W, bool kGuard>
+__device__ inline void split_loops(const SumArgs& a, const float* s_poly, PhaseState st,
+                                   float zoff, int cbase, int ks, int ke, int kw0, int kw1,
+                                   int kf0, int kf1, float& acc_re, float& acc_im)
+{
+    const int nr = (int)a.nr;
+    const int chunk = a.chunk;
+    const float zscale = a.zscale;
+    const float2* rc = a.rc;
+    float cmp_re = 0.f, cmp_im = 0.f, cre = 0.f, cim = 0.f;
+
+    auto masked = [&](int k_lo, int k_hi) {
+        for (int k0 = k_lo; k0 < k_hi; k0 += chunk) {
+            const int k1 = min(k0 + chunk, k_hi);
+            for (int k = k0; k < k1; ++k) {
+                const SplitPulse pu = split_pulse<W>(st, zscale, zoff, cbase);
+                st.advance();
+                if (k >= ks && k < ke) {
+                    float2 d[W];
+                    split_taps<W, kGuard>(rc + (int64_t)k * nr, pu.c0, nr, d);
+                    split_update<W>(s_poly, pu, d, cre, cim);
+                }
+            }
+            kahan_fold(acc_re, cmp_re, cre);
+            kahan_fold(acc_im, cmp_im, cim);
+        }
+    };
+
+    if (kw0 < kw1) masked(kw0, kf0);
+    if (kf0 < kf1) {
+        // software pipeline over the whole range: the taps of the next pulse
+        // are in flight while the current one is accumulated; partial sums
+        // are folded every `chunk` pulses (rounded up to a pair)
+        const int pairs = (chunk + 1) / 2;
+        SplitPulse pa = split_pulse<W>(st, zscale, zoff, cbase), pb;
+        st.advance();
+        float2 da[W], db[W];
+        split_taps<W, kGuard>(rc + (int64_t)kf0 * nr, pa.c0, nr, da);
+        int k = kf0, it = 0;
+        for (; k + 2 < kf1; k += 2) {
+            pb = split_pulse<W>(st, zscale, zoff, cbase);
+            st.advance();
+            split_taps<W, kGuard>(rc + (int64_t)(k + 1) * nr, pb.c0, nr, db);
+            split_update<W>(s_poly, pa, da, cre, cim);
+            pa = split_pulse<W>(st, zscale, zoff, cbase);
+            st.advance();
+            split_taps<W, kGuard>(rc + (int64_t)(k + 2) * nr, pa.c0, nr, da);
+            split_update<W>(s_poly, pb, db, cre, cim);
+            if (++it == pairs) {
+                it = 0;
+                kahan_fold(acc_re, cmp_re, cre);
+                kahan_fold(acc_im, cmp_im, cim);
+            }
+        }
+        // the last one or two pulses (pulse k's taps are loaded)
+        if (k + 1 < kf1) {
+            pb = split_pulse<W>(st, zscale, zoff, cbase);
+            st.advance();
+            split_taps<W, kGuard>(rc + (int64_t)(k + 1) * nr, pb.c0, nr, db);
+            split_update<W>(s_poly, pa, da, cre, cim);
+            split_update<W>(s_poly, pb, db, cre, cim);
+        } else {
+            split_update<W>(s_poly, pa, da, cre, cim);
+        }
+        kahan_fold(acc_re, cmp_re, cre);
+        kahan_fold(acc_im, cmp_im, cim);
+    }
+    if (kw0 < kw1) masked(max(kf1, kw0), kw1);
+    acc_re -= cmp_re;
+    acc_im -= cmp_im;
 }
 
 template<int W>
@@ -547,95 +672,29 @@ __device__ inline void split_tile(const SumArgs& a, const float* s_poly, const O
         kf0 = kw1;
         kf1 = kw1;
     }
-    const double b0 = ok ? sp.b[0] : 0., b1 = ok ? sp.b[1] : 0.;
-    const double b2 = ok ? sp.b[2] : 0., b3 = ok ? sp.b[3] : 0.;
-    const double c_ref = ok ? sp.c_ref : 0.;
-    const double inv_h = ok ? sp.inv_h : 0.;
-    const double s_off = ok ? -sp.kc * sp.inv_h : 0.;
-    const float f_ref = ok ? sp.f_ref : 0.f;
-    const int n_ref = ok ? sp.n_ref : 0;
-    const float inv_kf = (float)a.inv_k;
-    const int nr = (int)a.nr;
-    const int chunk = a.chunk;
-    const float2* rc = a.rc;
+    PhaseState st;
+    st.p = win ? sp.p0 : 0.;
+    st.d1 = win ? sp.d1 : 0.;
+    st.d2 = win ? sp.d2 : 0.;
+    st.d3 = win ? sp.d3 : 0.;
+    st.d4 = win ? sp.d4 : 0.;
+    const float zoff = win ? sp.zoff : 0.f;
+    const int cbase = win ? sp.cbase : 0;
+    // guarded taps when some lane's window may reach past a line end, or the
+    // lines are narrower than the kernel (wavefront-uniform choice)
+    const bool guard = a.nr < W ||
+                       __builtin_amdgcn_readfirstlane(wave_max_i(win && sp.edge ? 1 : 0)) != 0;
 
-    float acc_re = 0.f, acc_im = 0.f, cmp_re = 0.f, cmp_im = 0.f;
-    float cre = 0.f, cim = 0.f;
-
-    if (nr < W) {
-        // lines narrower than the kernel: every tap through the guarded path
-        for (int k = kw0; k < kw1; ++k) {
-            if (k < ks || k >= ke) continue;
-            const SplitPulse<W> pu = split_pulse<W>((double)k, s_off, inv_h, b0, b1, b2, b3,
-                                                    c_ref, inv_kf, f_ref, n_ref);
-            float2 d[W];
-            split_update<W, true>(s_poly, rc + (int64_t)k * nr, nr, pu, d, cre, cim);
-            if ((k - kw0) % chunk == chunk - 1) {
-                kahan_fold(acc_re, cmp_re, cre);
-                kahan_fold(acc_im, cmp_im, cim);
-            }
-        }
-        kahan_fold(acc_re, cmp_re, cre);
-        kahan_fold(acc_im, cmp_im, cim);
-        if (valid) {
-            const float nanf_ = __builtin_nanf("");
-            od.out[pix] = ok ? make_float2(acc_re - cmp_re, acc_im - cmp_im) : make_float2(nanf_, nanf_);
-        }
-        return;
-    }
-
-    // masked loop over [k_lo, k_hi): ragged window ends
-    auto masked = [&](int k_lo, int k_hi) {
-        for (int k0 = k_lo; k0 < k_hi; k0 += chunk) {
-            const int k1 = min(k0 + chunk, k_hi);
-            for (int k = k0; k < k1; ++k) {
-                if (k < ks || k >= ke) continue;
-                const SplitPulse<W> pu = split_pulse<W>((double)k, s_off, inv_h, b0, b1, b2, b3,
-                                                        c_ref, inv_kf, f_ref, n_ref);
-                const float2* line = rc + (int64_t)k * nr;
-                float2 d[W];
-                split_taps<W>(line, pu.c0, nr, d);
-                split_update<W>(s_poly, line, nr, pu, d, cre, cim);
-            }
-            kahan_fold(acc_re, cmp_re, cre);
-            kahan_fold(acc_im, cmp_im, cim);
-        }
-    };
-
-    if (kw0 < kw1) masked(kw0, kf0);
-    // unmasked, software-pipelined loop over [kf0, kf1) (empty unless
-    // kw0 <= kf0 < kf1 <= kw1, so every pulse index is some lane's)
-    auto pulse = [&](int k) {
-        return split_pulse<W>((double)k, s_off, inv_h, b0, b1, b2, b3, c_ref, inv_kf, f_ref,
-                              n_ref);
-    };
-    for (int k0 = kf0; k0 < kf1; k0 += chunk) {
-        const int k1 = min(k0 + chunk, kf1);
-        // ping-pong buffers: the taps of pulse k+1 are in flight while pulse k
-        // is accumulated
-        SplitPulse<W> pa = pulse(k0), pb;
-        float2 da[W], db[W];
-        split_taps<W>(rc + (int64_t)k0 * nr, pa.c0, nr, da);
-        int k = k0;
-        for (; k + 1 < k1; k += 2) {
-            pb = pulse(k + 1);
-            split_taps<W>(rc + (int64_t)(k + 1) * nr, pb.c0, nr, db);
-            split_update<W>(s_poly, rc + (int64_t)k * nr, nr, pa, da, cre, cim);
-            if (k + 2 < k1) {
-                pa = pulse(k + 2);
-                split_taps<W>(rc + (int64_t)(k + 2) * nr, pa.c0, nr, da);
-            }
-            split_update<W>(s_poly, rc + (int64_t)(k + 1) * nr, nr, pb, db, cre, cim);
-        }
-        if (k < k1) split_update<W>(s_poly, rc + (int64_t)k * nr, nr, pa, da, cre, cim);
-        kahan_fold(acc_re, cmp_re, cre);
-        kahan_fold(acc_im, cmp_im, cim);
-    }
-    if (kw0 < kw1) masked(max(kf1, kw0), kw1);
-
+    float acc_re = 0.f, acc_im = 0.f;
+    if (guard)
+        split_loops<W, true>(a, s_poly, st, zoff, cbase, ks, ke, kw0, kw1, kf0, kf1, acc_re,
+                             acc_im);
+    else
+        split_loops<W, false>(a, s_poly, st, zoff, cbase, ks, ke, kw0, kw1, kf0, kf1, acc_re,
+                              acc_im);
     if (valid) {
         const float nanf_ = __builtin_nanf("");
-        od.out[pix] = ok ? make_float2(acc_re - cmp_re, acc_im - cmp_im) : make_float2(nanf_, nanf_);
+        od.out[pix] = ok ? make_float2(acc_re, acc_im) : make_float2(nanf_, nanf_);
     }
 }
 
@@ -643,7 +702,8 @@ template<int W>
 __global__ __launch_bounds__(512) void sum_split_kernel(SumArgs a)
 {
     constexpr int ROW = 2 * ((W + 3) / 4 * 4);
-    extern __shared__ float4 s_poly4[];
+    // static LDS array: row offsets fold into the ds_read offsets
+    __shared__ float4 s_poly4[(kPolyphaseRows + 1) * ROW / 4];
     float* s_poly = reinterpret_cast<float*>(s_poly4);
     for (int i = threadIdx.x; i < (kPolyphaseRows + 1) * ROW; i += blockDim.x) s_poly[i] = a.poly[i];
     __syncthreads();
@@ -699,9 +759,8 @@ int sum_blocks_per_cu(int precision, int width)
         e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_plain_kernel<8>, 256, lds)
                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_plain_kernel<9>, 256, lds);
     } else {
-        const size_t lds = split_lds_bytes(width);
-        e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<8>, 512, lds)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<9>, 512, lds);
+        e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<8>, 512, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<9>, 512, 0);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -710,18 +769,12 @@ int sum_blocks_per_cu(int precision, int width)
     return n < 1 ? 1 : n;
 }
 
-size_t split_lds_bytes(int width)
-{
-    const int wp = (width + 3) / 4 * 4;
-    return sizeof(float) * (size_t)(kPolyphaseRows + 1) * 2 * wp;
-}
-
 // Persistent grid: `grid` workgroups of 512 threads (a multiple of 8, one
 // share per XCD), sized by the caller from the device's CU count and occupancy.
 hipError_t launch_sum_split(const SumArgs& args, int width, int grid, hipStream_t stream)
 {
     if (args.n_seq <= 0) return hipSuccess;
-    const size_t lds = split_lds_bytes(width);
+    const size_t lds = 0;  // the weight table is a static LDS array
     switch (width) {
     case 8:
         hipLaunchKernelGGL(sum_split_kernel<8>, dim3(grid), dim3(512), lds, stream, args);
