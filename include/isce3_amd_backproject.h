@@ -160,6 +160,12 @@ int bp_context_destroy(bp_context* ctx);
 /* Thread-local message of the last failing call. */
 const char* bp_last_error_message(void);
 
+/* Device scratch bytes per output pixel that a context holds for a call of
+ * the given precision (per-pixel geometry records, plus the split-precision
+ * records); a context grows its scratch to pixels x this value and keeps it.
+ * No reference counterpart: used by callers that budget device memory. */
+int64_t bp_scratch_bytes_per_pixel(int precision);
+
 /*
  * Focus in azimuth by time-domain backprojection.
  *

@@ -94,6 +94,7 @@ _SIGNATURES = {
     "bp_context_create": (C.c_int, [C.c_int, _P(C.c_void_p)]),
     "bp_context_destroy": (C.c_int, [C.c_void_p]),
     "bp_last_error_message": (C.c_char_p, []),
+    "bp_scratch_bytes_per_pixel": (C.c_int64, [C.c_int]),
     "bp_backproject": (C.c_int, [C.c_void_p, _P(RadarGeometry_t), _P(RadarGeometry_t), _P(Dem_t),
                                  C.c_double, C.c_double, _P(Kernel_t), C.c_int,
                                  _P(Rdr2GeoParams_t), _P(Geo2RdrParams_t), C.c_void_p,

@@ -242,6 +242,12 @@ extern "C" {
 
 const char* bp_last_error_message(void) { return g_msg.c_str(); }
 
+int64_t bp_scratch_bytes_per_pixel(int precision)
+{
+    return (int64_t)sizeof(bp::PixGeom) +
+           (precision == BP_PRECISION_SPLIT ? (int64_t)sizeof(bp::PixSplit) : 0);
+}
+
 int bp_context_create(int device, bp_context** ctx)
 {
     if (ctx == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "ctx is null");

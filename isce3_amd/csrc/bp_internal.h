@@ -45,6 +45,9 @@ struct alignas(16) PixSplit {
     int32_t edge;            // nonzero: some tap of the window may leave [0, nr)
 };
 
+static_assert(sizeof(PixGeom) == 48, "PixGeom layout");
+static_assert(sizeof(PixSplit) == 64, "PixSplit layout");
+
 constexpr int kFitDegree = 4;        // delay polynomial degree of the split path
 constexpr int kPolyphaseLog2 = 9;
 constexpr int kPolyphaseRows = 1 << kPolyphaseLog2;  // fractional-offset rows of the split weight table

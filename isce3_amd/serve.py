@@ -37,7 +37,9 @@ from typing import Callable, List, Optional, Sequence
 from . import _native, errors
 
 OUT_BYTES_PER_PIXEL = 8                        # complex64 output
-RECORD_BYTES_PER_PIXEL = {"plain": 48, "split": 48 + 80}  # geometry (+ split) records
+# device scratch per pixel of a context: geometry (+ split) records; equal to
+# bp_scratch_bytes_per_pixel (checked by the tests against the library)
+RECORD_BYTES_PER_PIXEL = {"plain": 48, "split": 48 + 64}
 
 
 @dataclass

@@ -167,3 +167,13 @@ def test_orbit_rejects_nonuniform_state_vectors():
     svs = [core.StateVector(e + core.TimeDelta(t), [7e6, 0, t], [0, 0, 1]) for t in (0, 10, 21, 30)]
     with pytest.raises(ValueError):
         core.Orbit(svs)
+
+
+def test_scratch_bytes_match_the_server_accounting():
+    """The tile server charges each worker the library's per-pixel scratch."""
+    from isce3_amd import serve
+    lib = _native.lib()
+    assert lib.bp_scratch_bytes_per_pixel(_native.PRECISION_PLAIN) == \
+        serve.RECORD_BYTES_PER_PIXEL["plain"]
+    assert lib.bp_scratch_bytes_per_pixel(_native.PRECISION_SPLIT) == \
+        serve.RECORD_BYTES_PER_PIXEL["split"]
