@@ -139,7 +139,9 @@ typedef struct {
     int32_t fit_degree;    /* split: delay polynomial degree (0 = 4; only 4 is built) */
     int32_t order;         /* tile walk of the sum kernels: 0 auto, 1 row-major, 2 column-major */
     int32_t blocks_per_cu; /* persistent 512-thread workgroups per CU (0 = occupancy limit) */
-    int32_t reserved[3];
+    int32_t gather;        /* split: 0 stages each pulse's range window in LDS (default),
+                              1 gathers the taps from global memory */
+    int32_t reserved[2];
 } bp_options;
 
 /* Optional per-call measurements (filled when non-null). */
@@ -223,6 +225,38 @@ int bp_backproject_blocks(bp_context* ctx, const bp_orbit* out_orbit,
                           const bp_rdr2geo_params* rdr2geo, const bp_geo2rdr_params* geo2rdr,
                           const void* in_dev, int precision, const bp_options* options,
                           void* stream, int* any_failed, bp_stats* stats);
+
+/* ------------------------------------------------------------ scenes
+ * A scene keeps one range-compressed raw set resident together with
+ * everything its tile requests share: the input geometry, the output orbit
+ * and Doppler, DEM, kernel tables and parameters (uploaded once), per-pixel
+ * scratch for up to max_pixels output pixels in up to max_blocks blocks, and
+ * pinned host staging.  bp_scene_focus focuses a batch of output blocks by
+ * replaying a HIP graph of the whole step (descriptor upload, flag clear,
+ * geometry pass, coherent sum, flag read-back), captured on first use of each
+ * launch configuration.  The pulses at in_dev must stay valid and unchanged
+ * for the scene's lifetime.  A scene is used from one host thread at a time.
+ *
+ * No single reference counterpart: a scene replaces the per-block calls of
+ * isce3.focus.backproject that share one raw set in the RSLC workflow
+ * (python/packages/nisar/workflows/focus.py:1986-2004), with the same
+ * arguments and the same per-block results as bp_backproject_blocks.
+ */
+typedef struct bp_scene bp_scene;
+
+int bp_scene_create(bp_context* ctx, const bp_orbit* out_orbit, const bp_lut2d* out_doppler,
+                    const bp_radar_geometry* in_geometry, const bp_dem* dem, double fc,
+                    double ds, const bp_kernel* kernel, int dry_tropo_model,
+                    const bp_rdr2geo_params* rdr2geo, const bp_geo2rdr_params* geo2rdr,
+                    const void* in_dev, int precision, int64_t max_pixels, int max_blocks,
+                    bp_scene** scene);
+int bp_scene_destroy(bp_scene* scene);
+
+/* Focus blocks[n_blocks] (at most max_pixels pixels in at most max_blocks
+ * non-empty blocks) on `stream`; returns after the step has finished, like
+ * bp_backproject_blocks, whose return values and per-block flags it shares. */
+int bp_scene_focus(bp_scene* scene, bp_out_block* blocks, int n_blocks,
+                   const bp_options* options, void* stream, int* any_failed, bp_stats* stats);
 
 /* ---------------------------------------------------- host-side helpers
  * Host implementations of the same FP64 primitives the kernels use; they back

@@ -76,7 +76,8 @@ class Geo2RdrParams_t(C.Structure):
 
 class Options_t(C.Structure):
     _fields_ = [("tile_rows", C.c_int32), ("chunk", C.c_int32), ("fit_degree", C.c_int32),
-                ("order", C.c_int32), ("blocks_per_cu", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("order", C.c_int32), ("blocks_per_cu", C.c_int32), ("gather", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class OutBlock_t(C.Structure):
@@ -105,6 +106,13 @@ _SIGNATURES = {
                                         C.c_double, _P(Kernel_t), C.c_int, _P(Rdr2GeoParams_t),
                                         _P(Geo2RdrParams_t), C.c_void_p, C.c_int, _P(Options_t),
                                         C.c_void_p, _P(C.c_int), _P(Stats_t)]),
+    "bp_scene_create": (C.c_int, [C.c_void_p, _P(Orbit_t), _P(Lut2d_t), _P(RadarGeometry_t),
+                                  _P(Dem_t), C.c_double, C.c_double, _P(Kernel_t), C.c_int,
+                                  _P(Rdr2GeoParams_t), _P(Geo2RdrParams_t), C.c_void_p, C.c_int,
+                                  C.c_int64, C.c_int, _P(C.c_void_p)]),
+    "bp_scene_destroy": (C.c_int, [C.c_void_p]),
+    "bp_scene_focus": (C.c_int, [C.c_void_p, _P(OutBlock_t), C.c_int, _P(Options_t), C.c_void_p,
+                                 _P(C.c_int), _P(Stats_t)]),
     "bp_orbit_interpolate": (C.c_int, [_P(Orbit_t), C.c_double, C.c_int, _P(C.c_double),
                                        _P(C.c_double)]),
     "bp_lonlat_to_xyz": (None, [C.c_double, C.c_double, _P(C.c_double), _P(C.c_double)]),

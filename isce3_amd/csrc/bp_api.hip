@@ -32,14 +32,28 @@ struct DevBuf {
 
 }  // namespace
 
+// Device copies of everything a call needs besides the pulses and the output
+// blocks: orbits, Doppler tables, DEM, kernel tables and (plain precision)
+// the platform state at every pulse.
+struct StaticData {
+    DevBuf table, poly, in_pos, in_vel, out_pos, out_vel, in_dop, out_dop, dem, pulse_pv;
+    std::vector<float> poly_host;
+};
+
+// Per-pixel records, batch header + output descriptors, and the flags of one
+// call ([0] any failed, [1] out of range, [8..9] update count, [16 + g] grid g
+// failed).
+struct Scratch {
+    DevBuf geom, split, flags, desc;
+};
+
 struct bp_context {
     int device = 0;
     int n_cu = 0;
-    DevBuf table, poly, in_pos, in_vel, out_pos, out_vel, in_dop, out_dop, dem;
-    DevBuf pulse_pv, geom, split, flags, desc;
+    StaticData st;
+    Scratch sc;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    std::vector<float> poly_host;
-    std::vector<bp::OutDesc> desc_host;
+    std::vector<unsigned char> desc_host;
     std::vector<int> desc_block;
     std::vector<int32_t> flags_host;
 };
@@ -225,13 +239,14 @@ void build_polyphase(const bp_kernel& k, int width, std::vector<float>& out)
     }
     // Q + 1 rows so that offsets t0 = tmin + 1 (q = Q) index a valid row
     // (whose deltas are zero) without clamping in the kernel.
-    out.assign((size_t)(Q + 1) * 2 * wp, 0.f);
+    const int row = bp::poly_row_floats(width);
+    out.assign((size_t)(Q + 1) * row, 0.f);
     for (int q = 0; q <= Q; ++q) {
         for (int m = 0; m < width; ++m) {
             const float w0 = (float)w[(size_t)q * width + m];
             const float w1 = q < Q ? (float)w[(size_t)(q + 1) * width + m] : w0;
-            out[(size_t)q * 2 * wp + m] = w0;
-            out[(size_t)q * 2 * wp + wp + m] = w1 - w0;
+            out[(size_t)q * row + m] = w0;
+            out[(size_t)q * row + wp + m] = w1 - w0;
         }
     }
 }
@@ -272,52 +287,142 @@ int bp_context_create(int device, bp_context** ctx)
     return BP_SUCCESS;
 }
 
-int bp_context_destroy(bp_context* c)
+}  // extern "C"
+
+namespace {
+
+void free_static(StaticData& s)
+{
+    DevBuf* bufs[] = {&s.table, &s.poly, &s.in_pos, &s.in_vel, &s.out_pos, &s.out_vel,
+                      &s.in_dop, &s.out_dop, &s.dem, &s.pulse_pv};
+    for (DevBuf* b : bufs)
+        if (b->ptr) (void)hipFree(b->ptr);
+}
+
+void free_scratch(Scratch& s)
+{
+    DevBuf* bufs[] = {&s.geom, &s.split, &s.flags, &s.desc};
+    for (DevBuf* b : bufs)
+        if (b->ptr) (void)hipFree(b->ptr);
+}
+
+}  // namespace
+
+extern "C" int bp_context_destroy(bp_context* c)
 {
     if (c == nullptr) return BP_SUCCESS;
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
-    DevBuf* bufs[] = {&c->table, &c->poly, &c->in_pos, &c->in_vel, &c->out_pos, &c->out_vel,
-                      &c->in_dop, &c->out_dop, &c->dem, &c->pulse_pv, &c->geom, &c->split,
-                      &c->flags, &c->desc};
-    for (DevBuf* b : bufs)
-        if (b->ptr) (void)hipFree(b->ptr);
+    free_static(c->st);
+    free_scratch(c->sc);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     delete c;
     return BP_SUCCESS;
 }
 
-}  // extern "C"
-
 namespace {
 
-// Shared implementation of bp_backproject and bp_backproject_blocks: every
-// output block shares the output orbit and Doppler, the input geometry, DEM,
-// kernel and parameters; the pixels of all blocks are focused by one geometry
-// launch and one coherent-sum launch.
-int run_blocks(bp_context* c, const bp_orbit* oo_p, const bp_lut2d* odop_p, bp_out_block* blocks,
-               int n_blocks, const bp_radar_geometry* ig, const bp_dem* dem, double fc, double ds,
-               const bp_kernel* kernel, int dry_tropo_model, const bp_rdr2geo_params* r2g,
-               const bp_geo2rdr_params* g2r, const void* in_dev, int precision,
-               const bp_options* options, void* stream, int* any_failed, bp_stats* stats)
+constexpr int kFlagWords = 16;  // flag words in front of the per-grid flags
+
+// Arguments shared by every output block of a call (the raw set and its
+// geometry, the output orbit/Doppler, DEM, kernel and parameters).
+struct CallArgs {
+    const bp_orbit* out_orbit;
+    const bp_lut2d* out_doppler;
+    const bp_radar_geometry* in_geometry;
+    const bp_dem* dem;
+    double fc, ds;
+    const bp_kernel* kernel;
+    int tropo;
+    const bp_rdr2geo_params* r2g;
+    const bp_geo2rdr_params* g2r;
+    const void* in_dev;
+    int precision;
+};
+
+// Validated launch options.
+struct Opts {
+    int tile_rows, chunk, order, blocks_per_cu;
+    bool staged;
+};
+
+// Argument rules of focus/Backproject.cpp:77-92, the pybind binding
+// (focus/Backproject.cpp:97-135) and DryTroposphereModel.cpp:20-33.
+// Returns the kernel width on success through *width.
+int check_call(const CallArgs& a, int* width)
 {
-    if (any_failed) *any_failed = 0;
-    if (stats) std::memset(stats, 0, sizeof(*stats));
-    if (!c || !oo_p || !odop_p || !ig || !dem || !kernel || !r2g || !g2r)
+    if (!a.out_orbit || !a.out_doppler || !a.in_geometry || !a.dem || !a.kernel || !a.r2g || !a.g2r)
         return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
-    if (n_blocks < 0 || (n_blocks > 0 && blocks == nullptr))
-        return fail(BP_ERR_INVALID_ARGUMENT, "invalid output block list");
-    if (dry_tropo_model != BP_TROPO_NODELAY && dry_tropo_model != BP_TROPO_TSX)
+    if (a.tropo != BP_TROPO_NODELAY && a.tropo != BP_TROPO_TSX)
         return fail(BP_ERR_INVALID_ARGUMENT, "unexpected dry troposphere model");
-    if (precision != BP_PRECISION_PLAIN && precision != BP_PRECISION_SPLIT)
+    if (a.precision != BP_PRECISION_PLAIN && a.precision != BP_PRECISION_SPLIT)
         return fail(BP_ERR_INVALID_ARGUMENT, "unknown precision mode");
-    const bp_grid& gi = ig->grid;
+    const bp_grid& gi = a.in_geometry->grid;
     if (gi.length < 0 || gi.width < 0) return fail(BP_ERR_INVALID_ARGUMENT, "negative grid size");
     if (gi.length > INT32_MAX || gi.width > INT32_MAX)
         return fail(BP_ERR_INVALID_ARGUMENT, "grid size exceeds int range");
     if (!(gi.prf > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "PRF must be positive");
-    int64_t npix = 0;
+    if (!(a.fc > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "center frequency must be positive");
+    if (!(a.ds > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "azimuth resolution must be positive");
+    const bp_kernel* k = a.kernel;
+    if (k->n < 2 || k->table == nullptr || !(k->halfwidth > 0))
+        return fail(BP_ERR_INVALID_ARGUMENT, "kernel table needs at least two samples");
+    *width = (int)std::ceil(2.0 * k->halfwidth);
+    if (*width != 8 && *width != 9)
+        return fail(BP_ERR_INVALID_ARGUMENT, "kernel width must be 8 or 9 (got " + std::to_string(*width) + ")");
+    const bp_dem* dem = a.dem;
+    if (dem->have_raster) {
+        if (dem->epsg != 4326) return fail(BP_ERR_INVALID_ARGUMENT, "only EPSG:4326 DEM rasters are supported");
+        if (dem->method != BP_DEM_BILINEAR) return fail(BP_ERR_INVALID_ARGUMENT, "only bilinear DEM interpolation is supported");
+        if (dem->width < 4 || dem->length < 4 || dem->data == nullptr)
+            return fail(BP_ERR_INVALID_ARGUMENT, "DEM raster must be at least 4x4");
+    }
+    int rc;
+    if ((rc = check_orbit(*a.out_orbit, "output geometry")) != BP_SUCCESS) return rc;
+    if ((rc = check_orbit(a.in_geometry->orbit, "input geometry")) != BP_SUCCESS) return rc;
+    if ((rc = check_lut(*a.out_doppler, "output geometry")) != BP_SUCCESS) return rc;
+    if ((rc = check_lut(a.in_geometry->doppler, "input geometry")) != BP_SUCCESS) return rc;
+    // Pulse times must lie in the input orbit: Orbit::interpolate with the
+    // default border mode throws OutOfRange (Backproject.cpp:101-106).
+    const int64_t na = gi.length;
+    if (na > 0) {
+        const bp_orbit& io = a.in_geometry->orbit;
+        const double ostart = io.t0, oend = io.t0 + (io.n - 1) * io.dt;
+        const double in_dt = 1.0 / gi.prf;
+        const double tf = gi.sensing_start, tl = gi.sensing_start + (na - 1) * in_dt;
+        if (tf < ostart || tf > oend || tl < ostart || tl > oend)
+            return fail(BP_ERR_OUT_OF_RANGE, orbit_err_string(BP_ORBIT_INTERP_DOMAIN_ERROR));
+    }
+    return BP_SUCCESS;
+}
+
+int check_options(const bp_options* options, Opts* o)
+{
+    bp_options opt{};
+    if (options) opt = *options;
+    o->tile_rows = opt.tile_rows == 0 ? 1 : opt.tile_rows;
+    if (o->tile_rows != 1 && o->tile_rows != 2 && o->tile_rows != 4 && o->tile_rows != 8)
+        return fail(BP_ERR_INVALID_ARGUMENT, "tile_rows must be 1, 2, 4 or 8");
+    o->chunk = opt.chunk == 0 ? 32 : opt.chunk;
+    if (o->chunk < 1 || o->chunk > 4096) return fail(BP_ERR_INVALID_ARGUMENT, "chunk must be in [1, 4096]");
+    if (opt.fit_degree != 0 && opt.fit_degree != bp::kFitDegree)
+        return fail(BP_ERR_INVALID_ARGUMENT, "fit_degree must be 4");
+    if (opt.order < 0 || opt.order > 2) return fail(BP_ERR_INVALID_ARGUMENT, "order must be 0, 1 or 2");
+    o->order = opt.order == 1 ? 0 : 1;
+    if (opt.blocks_per_cu < 0 || opt.blocks_per_cu > 16)
+        return fail(BP_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [0, 16]");
+    o->blocks_per_cu = opt.blocks_per_cu;
+    if (opt.gather != 0 && opt.gather != 1) return fail(BP_ERR_INVALID_ARGUMENT, "gather must be 0 or 1");
+    o->staged = opt.gather == 0;
+    return BP_SUCCESS;
+}
+
+int check_blocks(bp_out_block* blocks, int n_blocks, int64_t* npix)
+{
+    if (n_blocks < 0 || (n_blocks > 0 && blocks == nullptr))
+        return fail(BP_ERR_INVALID_ARGUMENT, "invalid output block list");
+    *npix = 0;
     for (int b = 0; b < n_blocks; ++b) {
         const bp_grid& go = blocks[b].grid;
         blocks[b].failed = 0;
@@ -327,67 +432,28 @@ int run_blocks(bp_context* c, const bp_orbit* oo_p, const bp_lut2d* odop_p, bp_o
         if (!(go.prf > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "PRF must be positive");
         if (go.length * go.width > 0 && blocks[b].out_dev == nullptr)
             return fail(BP_ERR_INVALID_ARGUMENT, "out is null");
-        npix += go.length * go.width;
+        *npix += go.length * go.width;
     }
-    if (!(fc > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "center frequency must be positive");
-    if (!(ds > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "azimuth resolution must be positive");
-    if (kernel->n < 2 || kernel->table == nullptr || !(kernel->halfwidth > 0))
-        return fail(BP_ERR_INVALID_ARGUMENT, "kernel table needs at least two samples");
-    const int width = (int)std::ceil(2.0 * kernel->halfwidth);
-    if (width != 8 && width != 9)
-        return fail(BP_ERR_INVALID_ARGUMENT, "kernel width must be 8 or 9 (got " + std::to_string(width) + ")");
-    if (dem->have_raster) {
-        if (dem->epsg != 4326) return fail(BP_ERR_INVALID_ARGUMENT, "only EPSG:4326 DEM rasters are supported");
-        if (dem->method != BP_DEM_BILINEAR) return fail(BP_ERR_INVALID_ARGUMENT, "only bilinear DEM interpolation is supported");
-        if (dem->width < 4 || dem->length < 4 || dem->data == nullptr)
-            return fail(BP_ERR_INVALID_ARGUMENT, "DEM raster must be at least 4x4");
-    }
-    int rc;
-    if ((rc = check_orbit(*oo_p, "output geometry")) != BP_SUCCESS) return rc;
-    if ((rc = check_orbit(ig->orbit, "input geometry")) != BP_SUCCESS) return rc;
-    if ((rc = check_lut(*odop_p, "output geometry")) != BP_SUCCESS) return rc;
-    if ((rc = check_lut(ig->doppler, "input geometry")) != BP_SUCCESS) return rc;
-    bp_options opt{};
-    if (options) opt = *options;
-    int tile_rows = opt.tile_rows == 0 ? 1 : opt.tile_rows;
-    if (tile_rows != 1 && tile_rows != 2 && tile_rows != 4 && tile_rows != 8)
-        return fail(BP_ERR_INVALID_ARGUMENT, "tile_rows must be 1, 2, 4 or 8");
-    int chunk = opt.chunk == 0 ? 32 : opt.chunk;
-    if (chunk < 1 || chunk > 4096) return fail(BP_ERR_INVALID_ARGUMENT, "chunk must be in [1, 4096]");
-    if (opt.fit_degree != 0 && opt.fit_degree != bp::kFitDegree)
-        return fail(BP_ERR_INVALID_ARGUMENT, "fit_degree must be 4");
-    if (opt.order < 0 || opt.order > 2) return fail(BP_ERR_INVALID_ARGUMENT, "order must be 0, 1 or 2");
-    if (opt.blocks_per_cu < 0 || opt.blocks_per_cu > 16)
-        return fail(BP_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [0, 16]");
+    return BP_SUCCESS;
+}
 
-    const int64_t na = gi.length, nr = gi.width;
-    if (npix > 0 && na * nr > 0 && in_dev == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "in is null");
-
-    // Pulse times must lie in the input orbit: Orbit::interpolate with the
-    // default border mode throws OutOfRange (Backproject.cpp:101-106).
-    const double in_dt = 1.0 / gi.prf;
-    if (na > 0) {
-        const double ostart = ig->orbit.t0, oend = ig->orbit.t0 + (ig->orbit.n - 1) * ig->orbit.dt;
-        const double tf = gi.sensing_start, tl = gi.sensing_start + (na - 1) * in_dt;
-        if (tf < ostart || tf > oend || tl < ostart || tl > oend)
-            return fail(BP_ERR_OUT_OF_RANGE, orbit_err_string(BP_ORBIT_INTERP_DOMAIN_ERROR));
-    }
-    if (npix == 0) return BP_SUCCESS;
-
-    BP_HIP_CHECK(hipSetDevice(c->device));
-    hipStream_t s = static_cast<hipStream_t>(stream);
-
-    // Output descriptors: blocks with pixels, tiles numbered consecutively.
-    const int order = opt.order == 1 ? 0 : 1;
-    c->desc_host.clear();
-    c->desc_block.clear();
+// Header and descriptors of the non-empty blocks, written to `dst` (room for
+// sizeof(BatchHeader) + max_grids descriptors); block_of_grid maps grids back.
+int plan_batch(const bp_out_block* blocks, int n_blocks, const Opts& o, int32_t* d_failed,
+               unsigned char* dst, int max_grids, bp::BatchHeader* hdr,
+               std::vector<int>& block_of_grid)
+{
+    block_of_grid.clear();
+    auto* descs = reinterpret_cast<bp::OutDesc*>(dst + sizeof(bp::BatchHeader));
     int64_t n_seq = 0, n_rec = 0;
+    int g = 0;
     for (int b = 0; b < n_blocks; ++b) {
         const bp_grid& go = blocks[b].grid;
         if (go.length * go.width == 0) continue;
+        if (g >= max_grids) return fail(BP_ERR_INVALID_ARGUMENT, "too many output blocks");
         bp::OutDesc d;
         std::memset(&d, 0, sizeof(d));
-        d.map = make_map(go.length, go.width, tile_rows, order);
+        d.map = make_map(go.length, go.width, o.tile_rows, o.order);
         d.t0 = go.sensing_start;
         d.dt = 1.0 / go.prf;
         d.r0 = go.starting_range;
@@ -397,145 +463,182 @@ int run_blocks(bp_context* c, const bp_orbit* oo_p, const bp_lut2d* odop_p, bp_o
         d.rec_base = n_rec;
         d.out = static_cast<float2*>(blocks[b].out_dev);
         d.height = blocks[b].height_dev;
+        d.failed = d_failed + g;
         n_seq += d.map.n_tiles;
         n_rec += go.length * go.width;
-        c->desc_host.push_back(d);
-        c->desc_block.push_back(b);
+        std::memcpy(descs + g, &d, sizeof(d));
+        block_of_grid.push_back(b);
+        ++g;
     }
-    const int n_grids = (int)c->desc_host.size();
-    if ((n_seq + 3) / 4 > INT32_MAX) return fail(BP_ERR_INVALID_ARGUMENT, "too many output pixels");
-    if ((rc = ensure(c->flags, 64 + sizeof(int32_t) * n_grids))) return rc;
-    int32_t* d_flags = static_cast<int32_t*>(c->flags.ptr);
-    auto* d_updates = reinterpret_cast<unsigned long long*>(static_cast<char*>(c->flags.ptr) + 32);
-    int32_t* d_failed = reinterpret_cast<int32_t*>(static_cast<char*>(c->flags.ptr) + 64);
-    for (int g = 0; g < n_grids; ++g) c->desc_host[g].failed = d_failed + g;
+    std::memset(hdr, 0, sizeof(*hdr));
+    hdr->n_seq = n_seq;
+    hdr->n_groups = (n_seq + 3) / 4;
+    hdr->n_grids = g;
+    if (hdr->n_groups > INT32_MAX) return fail(BP_ERR_INVALID_ARGUMENT, "too many output pixels");
+    std::memcpy(dst, hdr, sizeof(*hdr));
+    return BP_SUCCESS;
+}
 
-    // Uploads.
-    const bp_orbit& oo = *oo_p;
-    const bp_orbit& io = ig->orbit;
-    if ((rc = upload(c->out_pos, oo.pos, sizeof(double) * 3 * oo.n, s))) return rc;
-    if ((rc = upload(c->out_vel, oo.vel, sizeof(double) * 3 * oo.n, s))) return rc;
-    if ((rc = upload(c->in_pos, io.pos, sizeof(double) * 3 * io.n, s))) return rc;
-    if ((rc = upload(c->in_vel, io.vel, sizeof(double) * 3 * io.n, s))) return rc;
-    if ((rc = upload(c->out_dop, odop_p->have_data ? odop_p->data : nullptr,
-                     odop_p->have_data ? sizeof(double) * odop_p->width * odop_p->length : 0, s)))
+// Upload the orbits, Doppler tables, DEM and kernel tables of a call.
+int upload_static(StaticData& st, const CallArgs& a, int width, hipStream_t s)
+{
+    int rc;
+    const bp_orbit& oo = *a.out_orbit;
+    const bp_orbit& io = a.in_geometry->orbit;
+    const bp_lut2d& od = *a.out_doppler;
+    const bp_lut2d& id = a.in_geometry->doppler;
+    if ((rc = upload(st.out_pos, oo.pos, sizeof(double) * 3 * oo.n, s))) return rc;
+    if ((rc = upload(st.out_vel, oo.vel, sizeof(double) * 3 * oo.n, s))) return rc;
+    if ((rc = upload(st.in_pos, io.pos, sizeof(double) * 3 * io.n, s))) return rc;
+    if ((rc = upload(st.in_vel, io.vel, sizeof(double) * 3 * io.n, s))) return rc;
+    if ((rc = upload(st.out_dop, od.have_data ? od.data : nullptr,
+                     od.have_data ? sizeof(double) * od.width * od.length : 0, s)))
         return rc;
-    if ((rc = upload(c->in_dop, ig->doppler.have_data ? ig->doppler.data : nullptr,
-                     ig->doppler.have_data ? sizeof(double) * ig->doppler.width * ig->doppler.length : 0, s)))
+    if ((rc = upload(st.in_dop, id.have_data ? id.data : nullptr,
+                     id.have_data ? sizeof(double) * id.width * id.length : 0, s)))
         return rc;
-    if ((rc = upload(c->dem, dem->have_raster ? dem->data : nullptr,
-                     dem->have_raster ? sizeof(float) * dem->width * dem->length : 0, s)))
+    if ((rc = upload(st.dem, a.dem->have_raster ? a.dem->data : nullptr,
+                     a.dem->have_raster ? sizeof(float) * a.dem->width * a.dem->length : 0, s)))
         return rc;
-    if ((rc = upload(c->table, kernel->table, sizeof(float) * kernel->n, s))) return rc;
-    if (precision == BP_PRECISION_SPLIT) {
-        build_polyphase(*kernel, width, c->poly_host);
-        if ((rc = upload(c->poly, c->poly_host.data(), sizeof(float) * c->poly_host.size(), s))) return rc;
+    if ((rc = upload(st.table, a.kernel->table, sizeof(float) * a.kernel->n, s))) return rc;
+    if (a.precision == BP_PRECISION_SPLIT) {
+        build_polyphase(*a.kernel, width, st.poly_host);
+        if ((rc = upload(st.poly, st.poly_host.data(), sizeof(float) * st.poly_host.size(), s)))
+            return rc;
     }
-    if ((rc = upload(c->desc, c->desc_host.data(), sizeof(bp::OutDesc) * n_grids, s))) return rc;
-    if ((rc = ensure(c->geom, sizeof(bp::PixGeom) * n_rec))) return rc;
-    if (precision == BP_PRECISION_SPLIT && (rc = ensure(c->split, sizeof(bp::PixSplit) * n_rec))) return rc;
-    if (precision == BP_PRECISION_PLAIN && (rc = ensure(c->pulse_pv, sizeof(double) * 6 * std::max<int64_t>(na, 1))))
-        return rc;
-    BP_HIP_CHECK(hipMemsetAsync(c->flags.ptr, 0, 64 + sizeof(int32_t) * n_grids, s));
+    return BP_SUCCESS;
+}
 
+// Platform position/velocity at every pulse (plain precision).
+int pulse_states(StaticData& st, const CallArgs& a, int32_t* d_flags, hipStream_t s)
+{
+    if (a.precision != BP_PRECISION_PLAIN) return BP_SUCCESS;
+    const bp_grid& gi = a.in_geometry->grid;
+    int rc;
+    if ((rc = ensure(st.pulse_pv, sizeof(double) * 6 * std::max<int64_t>(gi.length, 1)))) return rc;
+    if (gi.length > 0)
+        BP_HIP_CHECK(bp::launch_pulse_orbit(dev_orbit(a.in_geometry->orbit, st.in_pos, st.in_vel),
+                                            gi.sensing_start, 1.0 / gi.prf, (int)gi.length,
+                                            static_cast<double*>(st.pulse_pv.ptr), d_flags, s));
+    return BP_SUCCESS;
+}
+
+// Everything the launches of one step need, with device pointers.
+struct Step {
+    bp::GeomArgs ga;
+    bp::SumArgs sa;
+    int width, precision;
+    bool staged;
+    int geom_grid, sum_grid;
+    int32_t* flags;
+    int flag_words;
+    hipEvent_t ev[3];  // null: no timing
+};
+
+void build_step(Step& sp, const StaticData& st, const Scratch& sc, const CallArgs& a, int width,
+                const Opts& o, int n_cu)
+{
+    const bp_grid& gi = a.in_geometry->grid;
     const double swst = 2. * gi.starting_range / bp::kC;
     const double dtau = 2. * gi.range_pixel_spacing / bp::kC;
-    const int per_cu = opt.blocks_per_cu > 0 ? opt.blocks_per_cu
-                                             : bp::sum_blocks_per_cu(precision, width);
-    int grid = c->n_cu * per_cu;
-    grid = (grid + 7) / 8 * 8;
-    const auto* d_desc = static_cast<const bp::OutDesc*>(c->desc.ptr);
-    const int32_t wblocks = (int32_t)((n_seq + 3) / 4);
+    const auto* batch = static_cast<const bp::BatchHeader*>(sc.desc.ptr);
+    int32_t* d_flags = static_cast<int32_t*>(sc.flags.ptr);
 
-    bp::Dem ddem = host_dem(*dem);
-    ddem.data = static_cast<const float*>(c->dem.ptr);
+    bp::Dem ddem = host_dem(*a.dem);
+    ddem.data = static_cast<const float*>(st.dem.ptr);
 
-    if (precision == BP_PRECISION_PLAIN && na > 0) {
-        BP_HIP_CHECK(bp::launch_pulse_orbit(dev_orbit(io, c->in_pos, c->in_vel), gi.sensing_start,
-                                            in_dt, (int)na, static_cast<double*>(c->pulse_pv.ptr),
-                                            d_flags, s));
-    }
-
-    bp::GeomArgs ga;
+    bp::GeomArgs& ga = sp.ga;
     std::memset(&ga, 0, sizeof(ga));
-    ga.grids = d_desc;
-    ga.n_grids = n_grids;
-    ga.n_seq = n_seq;
-    ga.n_blocks = wblocks;
+    ga.batch = batch;
     ga.in_side = gi.look_side;
     ga.kwidth = width;
-    ga.nr = (int32_t)nr;
+    ga.nr = (int32_t)gi.width;
     ga.in_t0 = gi.sensing_start;
-    ga.in_dt = in_dt;
-    ga.na = (int32_t)na;
-    ga.tropo = dry_tropo_model;
+    ga.in_dt = 1.0 / gi.prf;
+    ga.na = (int32_t)gi.length;
+    ga.tropo = a.tropo;
     ga.swst = swst;
     ga.dtau = dtau;
-    ga.wvl = bp::kC / fc;
-    ga.fc = fc;
-    ga.ds = ds;
-    ga.out_orbit = dev_orbit(oo, c->out_pos, c->out_vel);
-    ga.in_orbit = dev_orbit(io, c->in_pos, c->in_vel);
-    ga.out_dop = dev_lut(*odop_p, c->out_dop);
-    ga.in_dop = dev_lut(ig->doppler, c->in_dop);
+    ga.wvl = bp::kC / a.fc;
+    ga.fc = a.fc;
+    ga.ds = a.ds;
+    ga.out_orbit = dev_orbit(*a.out_orbit, st.out_pos, st.out_vel);
+    ga.in_orbit = dev_orbit(a.in_geometry->orbit, st.in_pos, st.in_vel);
+    ga.out_dop = dev_lut(*a.out_doppler, st.out_dop);
+    ga.in_dop = dev_lut(a.in_geometry->doppler, st.in_dop);
     ga.dem = ddem;
-    ga.r2g = {r2g->tol_height, r2g->look_min, r2g->look_max};
-    ga.g2r.tol_aztime = g2r->tol_aztime;
-    ga.g2r.have_time_start = g2r->have_time_start;
-    ga.g2r.have_time_end = g2r->have_time_end;
-    ga.g2r.time_start = g2r->time_start;
-    ga.g2r.time_end = g2r->time_end;
-    ga.want_split = precision == BP_PRECISION_SPLIT;
-    ga.geom = static_cast<bp::PixGeom*>(c->geom.ptr);
-    ga.split = static_cast<bp::PixSplit*>(c->split.ptr);
+    ga.r2g = {a.r2g->tol_height, a.r2g->look_min, a.r2g->look_max};
+    ga.g2r.tol_aztime = a.g2r->tol_aztime;
+    ga.g2r.have_time_start = a.g2r->have_time_start;
+    ga.g2r.have_time_end = a.g2r->have_time_end;
+    ga.g2r.time_start = a.g2r->time_start;
+    ga.g2r.time_end = a.g2r->time_end;
+    ga.want_split = a.precision == BP_PRECISION_SPLIT;
+    ga.geom = static_cast<bp::PixGeom*>(sc.geom.ptr);
+    ga.split = static_cast<bp::PixSplit*>(sc.split.ptr);
     ga.flags = d_flags;
-    ga.updates = d_updates;
+    ga.updates = reinterpret_cast<unsigned long long*>(d_flags + 8);
 
-    if (stats) BP_HIP_CHECK(hipEventRecord(c->ev[0], s));
-    BP_HIP_CHECK(bp::launch_geometry(ga, s));
-    if (stats) BP_HIP_CHECK(hipEventRecord(c->ev[1], s));
-
-    bp::SumArgs sa;
+    bp::SumArgs& sa = sp.sa;
     std::memset(&sa, 0, sizeof(sa));
-    sa.grids = d_desc;
-    sa.n_grids = n_grids;
-    sa.n_seq = n_seq;
-    sa.n_blocks = wblocks;
-    sa.nr = nr;
+    sa.batch = batch;
+    sa.nr = gi.width;
     sa.swst = swst;
     sa.dtau = dtau;
-    sa.fc = fc;
-    sa.inv_k = 1.0 / (fc * dtau);
+    sa.fc = a.fc;
+    sa.inv_k = 1.0 / (a.fc * dtau);
     sa.zscale = (float)(-(double)bp::kPolyphaseRows * sa.inv_k);
-    sa.halfwidth = kernel->halfwidth;
-    sa.rdx = (float)(1.0 / (kernel->halfwidth / (kernel->n - 1.0)));
-    sa.ntab = kernel->n;
-    sa.chunk = chunk;
-    sa.rc = static_cast<const float2*>(in_dev);
-    sa.pulse_pv = static_cast<const double*>(c->pulse_pv.ptr);
-    sa.geom = static_cast<const bp::PixGeom*>(c->geom.ptr);
-    sa.split = static_cast<const bp::PixSplit*>(c->split.ptr);
-    sa.table = static_cast<const float*>(c->table.ptr);
-    sa.poly = static_cast<const float*>(c->poly.ptr);
-    if (precision == BP_PRECISION_PLAIN)
-        BP_HIP_CHECK(bp::launch_sum_plain(sa, width, s));
-    else
-        BP_HIP_CHECK(bp::launch_sum_split(sa, width, grid, s));
-    if (stats) BP_HIP_CHECK(hipEventRecord(c->ev[2], s));
+    sa.halfwidth = a.kernel->halfwidth;
+    sa.rdx = (float)(1.0 / (a.kernel->halfwidth / (a.kernel->n - 1.0)));
+    sa.ntab = a.kernel->n;
+    sa.chunk = o.chunk;
+    sa.rc = static_cast<const float2*>(a.in_dev);
+    sa.pulse_pv = static_cast<const double*>(st.pulse_pv.ptr);
+    sa.geom = static_cast<const bp::PixGeom*>(sc.geom.ptr);
+    sa.split = static_cast<const bp::PixSplit*>(sc.split.ptr);
+    sa.table = static_cast<const float*>(st.table.ptr);
+    sa.poly = static_cast<const float*>(st.poly.ptr);
 
-    c->flags_host.assign(16 + n_grids, 0);
-    BP_HIP_CHECK(hipMemcpyAsync(c->flags_host.data(), c->flags.ptr, 64 + sizeof(int32_t) * n_grids,
-                                hipMemcpyDeviceToHost, s));
-    BP_HIP_CHECK(hipStreamSynchronize(s));
-    const int32_t* hflags = c->flags_host.data();
+    sp.width = width;
+    sp.precision = a.precision;
+    sp.staged = o.staged;
+    const int per_cu = o.blocks_per_cu > 0 ? o.blocks_per_cu
+                                           : bp::sum_blocks_per_cu(a.precision, width, o.staged);
+    sp.sum_grid = (n_cu * per_cu + 7) / 8 * 8;  // persistent split grid
+    sp.geom_grid = 0;
+    sp.flags = d_flags;
+    sp.flag_words = 0;
+    sp.ev[0] = sp.ev[1] = sp.ev[2] = nullptr;
+}
+
+// Enqueue the launches of one step: clear the flags, per-pixel geometry,
+// coherent sum (with optional event records around the two passes).
+int enqueue_step(const Step& sp, int plain_grid, hipStream_t s)
+{
+    BP_HIP_CHECK(bp::launch_clear(sp.flags, sp.flag_words, s));
+    if (sp.ev[0]) BP_HIP_CHECK(hipEventRecord(sp.ev[0], s));
+    BP_HIP_CHECK(bp::launch_geometry(sp.ga, sp.geom_grid, s));
+    if (sp.ev[1]) BP_HIP_CHECK(hipEventRecord(sp.ev[1], s));
+    if (sp.precision == BP_PRECISION_PLAIN)
+        BP_HIP_CHECK(bp::launch_sum_plain(sp.sa, sp.width, plain_grid, s));
+    else
+        BP_HIP_CHECK(bp::launch_sum_split(sp.sa, sp.width, sp.sum_grid, sp.staged, s));
+    if (sp.ev[2]) BP_HIP_CHECK(hipEventRecord(sp.ev[2], s));
+    return BP_SUCCESS;
+}
+
+// Read the flags of a finished step into the caller's outputs.
+int finish_step(const int32_t* hflags, int n_grids, const std::vector<int>& block_of_grid,
+                bp_out_block* blocks, const hipEvent_t* ev, int* any_failed, bp_stats* stats)
+{
     unsigned long long hupd = 0;
     std::memcpy(&hupd, hflags + 8, sizeof(hupd));
-    for (int g = 0; g < n_grids; ++g) blocks[c->desc_block[g]].failed = hflags[16 + g] != 0;
+    for (int g = 0; g < n_grids; ++g) blocks[block_of_grid[g]].failed = hflags[kFlagWords + g] != 0;
     if (stats) {
         float ms = 0.f;
-        BP_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        BP_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
         stats->geometry_ms = ms;
-        BP_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+        BP_HIP_CHECK(hipEventElapsedTime(&ms, ev[1], ev[2]));
         stats->sum_ms = ms;
         stats->updates = (int64_t)hupd;
         stats->any_failed = hflags[0] != 0;
@@ -545,9 +648,287 @@ int run_blocks(bp_context* c, const bp_orbit* oo_p, const bp_lut2d* odop_p, bp_o
     return hflags[0] ? BP_FAILED_TO_CONVERGE : BP_SUCCESS;
 }
 
+// Shared implementation of bp_backproject and bp_backproject_blocks: every
+// output block shares the output orbit and Doppler, the input geometry, DEM,
+// kernel and parameters; the pixels of all blocks are focused by one geometry
+// launch and one coherent-sum launch.
+int run_blocks(bp_context* c, const CallArgs& a, bp_out_block* blocks, int n_blocks,
+               const bp_options* options, void* stream, int* any_failed, bp_stats* stats)
+{
+    if (any_failed) *any_failed = 0;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (!c) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    int width = 0, rc;
+    if ((rc = check_call(a, &width)) != BP_SUCCESS) return rc;
+    int64_t npix = 0;
+    if ((rc = check_blocks(blocks, n_blocks, &npix)) != BP_SUCCESS) return rc;
+    Opts o;
+    if ((rc = check_options(options, &o)) != BP_SUCCESS) return rc;
+    const bp_grid& gi = a.in_geometry->grid;
+    if (npix > 0 && gi.length * gi.width > 0 && a.in_dev == nullptr)
+        return fail(BP_ERR_INVALID_ARGUMENT, "in is null");
+    if (npix == 0) return BP_SUCCESS;
+
+    BP_HIP_CHECK(hipSetDevice(c->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+
+    const int n_flag = kFlagWords + n_blocks;
+    if ((rc = ensure(c->sc.flags, sizeof(int32_t) * n_flag))) return rc;
+    c->desc_host.assign(sizeof(bp::BatchHeader) + sizeof(bp::OutDesc) * std::max(n_blocks, 1), 0);
+    bp::BatchHeader hdr;
+    if ((rc = plan_batch(blocks, n_blocks, o, static_cast<int32_t*>(c->sc.flags.ptr) + kFlagWords,
+                         c->desc_host.data(), n_blocks, &hdr, c->desc_block)) != BP_SUCCESS)
+        return rc;
+    const int64_t n_rec = npix;
+
+    if ((rc = upload_static(c->st, a, width, s))) return rc;
+    if ((rc = upload(c->sc.desc, c->desc_host.data(), c->desc_host.size(), s))) return rc;
+    if ((rc = ensure(c->sc.geom, sizeof(bp::PixGeom) * n_rec))) return rc;
+    if (a.precision == BP_PRECISION_SPLIT && (rc = ensure(c->sc.split, sizeof(bp::PixSplit) * n_rec)))
+        return rc;
+    BP_HIP_CHECK(bp::launch_clear(static_cast<int32_t*>(c->sc.flags.ptr), n_flag, s));
+    if ((rc = pulse_states(c->st, a, static_cast<int32_t*>(c->sc.flags.ptr), s))) return rc;
+
+    Step sp;
+    build_step(sp, c->st, c->sc, a, width, o, c->n_cu);
+    sp.geom_grid = (int)hdr.n_groups;
+    sp.flag_words = 0;  // cleared above, before the pulse states
+    if (stats)
+        for (int i = 0; i < 3; ++i) sp.ev[i] = c->ev[i];
+    if ((rc = enqueue_step(sp, (int)hdr.n_groups, s))) return rc;
+
+    c->flags_host.assign(n_flag, 0);
+    BP_HIP_CHECK(hipMemcpyAsync(c->flags_host.data(), c->sc.flags.ptr, sizeof(int32_t) * n_flag,
+                                hipMemcpyDeviceToHost, s));
+    BP_HIP_CHECK(hipStreamSynchronize(s));
+    return finish_step(c->flags_host.data(), hdr.n_grids, c->desc_block, blocks, c->ev,
+                       any_failed, stats);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ scenes
+
+// One raw set kept resident with everything its tile requests share, scratch
+// for up to max_pixels output pixels in up to max_blocks blocks, and one HIP
+// graph per launch configuration.  The graph of a step copies the batch
+// header and descriptors from pinned host memory, clears the flags with a
+// kernel, runs the geometry and sum kernels (persistent grids that read the
+// batch counts from the device header) between event records, and copies the
+// flags back to pinned host memory; a request only rewrites the pinned
+// descriptors and replays the graph.
+struct bp_scene {
+    bp_context* ctx = nullptr;
+    CallArgs args{};
+    int width = 0;
+    int64_t max_pixels = 0;
+    int max_blocks = 0;
+    StaticData st;
+    Scratch sc;
+    // copies of the caller's structs that CallArgs points to
+    bp_orbit out_orbit{};
+    bp_lut2d out_doppler{};
+    bp_radar_geometry in_geometry{};
+    bp_dem dem{};
+    bp_kernel kernel{};
+    bp_rdr2geo_params r2g{};
+    bp_geo2rdr_params g2r{};
+    unsigned char* h_desc = nullptr;   // pinned: header + max_blocks descriptors
+    int32_t* h_flags = nullptr;        // pinned: kFlagWords + max_blocks words
+    hipStream_t cap = nullptr;         // capture stream (default priority)
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    std::vector<int> block_of_grid;
+    struct Graph {
+        int chunk, blocks_per_cu;
+        bool staged;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+};
+
+namespace {
+
+void destroy_scene(bp_scene* sc)
+{
+    for (auto& g : sc->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    free_static(sc->st);
+    free_scratch(sc->sc);
+    if (sc->h_desc) (void)hipHostFree(sc->h_desc);
+    if (sc->h_flags) (void)hipHostFree(sc->h_flags);
+    for (auto& ev : sc->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (sc->cap) (void)hipStreamDestroy(sc->cap);
+    delete sc;
+}
+
+// Capture the step of one launch configuration.
+int capture_graph(bp_scene* sc, const Opts& o, bp_scene::Graph* out)
+{
+    bp_context* c = sc->ctx;
+    Step sp;
+    build_step(sp, sc->st, sc->sc, sc->args, sc->width, o, c->n_cu);
+    const int64_t max_groups = (sc->max_pixels + 255) / 256 + sc->max_blocks;
+    const int cap_grid = (int)std::min<int64_t>(max_groups, (int64_t)c->n_cu * 8);
+    sp.geom_grid = std::max(cap_grid, 1);
+    sp.flag_words = kFlagWords + sc->max_blocks;
+    for (int i = 0; i < 3; ++i) sp.ev[i] = sc->ev[i];
+    const size_t desc_bytes = sizeof(bp::BatchHeader) + sizeof(bp::OutDesc) * sc->max_blocks;
+
+    hipStream_t s = sc->cap;
+    BP_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = BP_SUCCESS;
+    hipError_t e = hipMemcpyAsync(sc->sc.desc.ptr, sc->h_desc, desc_bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) rc = fail(BP_ERR_HIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+    if (rc == BP_SUCCESS) rc = enqueue_step(sp, std::max(cap_grid, 1), s);
+    if (rc == BP_SUCCESS) {
+        e = hipMemcpyAsync(sc->h_flags, sc->sc.flags.ptr, sizeof(int32_t) * (kFlagWords + sc->max_blocks),
+                           hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = fail(BP_ERR_HIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+    }
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s, &g);
+    if (rc != BP_SUCCESS) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(BP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    out->chunk = o.chunk;
+    out->blocks_per_cu = o.blocks_per_cu;
+    out->staged = o.staged;
+    out->graph = g;
+    out->exec = x;
+    return BP_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
+
+int bp_scene_create(bp_context* c, const bp_orbit* out_orbit, const bp_lut2d* out_doppler,
+                    const bp_radar_geometry* in_geometry, const bp_dem* dem, double fc,
+                    double ds, const bp_kernel* kernel, int dry_tropo_model,
+                    const bp_rdr2geo_params* rdr2geo, const bp_geo2rdr_params* geo2rdr,
+                    const void* in_dev, int precision, int64_t max_pixels, int max_blocks,
+                    bp_scene** scene)
+{
+    if (scene == nullptr || c == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    *scene = nullptr;
+    CallArgs a{out_orbit, out_doppler, in_geometry, dem, fc, ds, kernel, dry_tropo_model,
+               rdr2geo, geo2rdr, in_dev, precision};
+    int width = 0, rc;
+    if ((rc = check_call(a, &width)) != BP_SUCCESS) return rc;
+    if (max_pixels < 1 || max_blocks < 1 || max_blocks > (1 << 20))
+        return fail(BP_ERR_INVALID_ARGUMENT, "max_pixels and max_blocks must be positive");
+    const bp_grid& gi = in_geometry->grid;
+    if (gi.length * gi.width > 0 && in_dev == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "in is null");
+    BP_HIP_CHECK(hipSetDevice(c->device));
+
+    auto* sc = new bp_scene();
+    sc->ctx = c;
+    sc->width = width;
+    sc->max_pixels = max_pixels;
+    sc->max_blocks = max_blocks;
+    sc->out_orbit = *out_orbit;
+    sc->out_doppler = *out_doppler;
+    sc->in_geometry = *in_geometry;
+    sc->dem = *dem;
+    sc->kernel = *kernel;
+    sc->r2g = *rdr2geo;
+    sc->g2r = *geo2rdr;
+    sc->args = CallArgs{&sc->out_orbit, &sc->out_doppler, &sc->in_geometry, &sc->dem, fc, ds,
+                        &sc->kernel, dry_tropo_model, &sc->r2g, &sc->g2r, in_dev, precision};
+    auto bail = [&](int code) {
+        destroy_scene(sc);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&sc->cap, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(BP_ERR_HIP, "hipStreamCreate failed"));
+    for (auto& ev : sc->ev)
+        if (hipEventCreate(&ev) != hipSuccess) return bail(fail(BP_ERR_HIP, "hipEventCreate failed"));
+    const size_t desc_bytes = sizeof(bp::BatchHeader) + sizeof(bp::OutDesc) * max_blocks;
+    const size_t flag_bytes = sizeof(int32_t) * (kFlagWords + max_blocks);
+    if (hipHostMalloc(reinterpret_cast<void**>(&sc->h_desc), desc_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&sc->h_flags), flag_bytes, hipHostMallocDefault) != hipSuccess)
+        return bail(fail(BP_ERR_OUT_OF_MEMORY, "hipHostMalloc failed"));
+    std::memset(sc->h_desc, 0, desc_bytes);
+    std::memset(sc->h_flags, 0, flag_bytes);
+    if ((rc = ensure(sc->sc.desc, desc_bytes))) return bail(rc);
+    if ((rc = ensure(sc->sc.flags, flag_bytes))) return bail(rc);
+    if ((rc = ensure(sc->sc.geom, sizeof(bp::PixGeom) * max_pixels))) return bail(rc);
+    if (precision == BP_PRECISION_SPLIT && (rc = ensure(sc->sc.split, sizeof(bp::PixSplit) * max_pixels)))
+        return bail(rc);
+    hipStream_t s = sc->cap;
+    if ((rc = upload_static(sc->st, sc->args, width, s))) return bail(rc);
+    if (hipMemsetAsync(sc->sc.flags.ptr, 0, flag_bytes, s) != hipSuccess)
+        return bail(fail(BP_ERR_HIP, "hipMemsetAsync failed"));
+    if ((rc = pulse_states(sc->st, sc->args, static_cast<int32_t*>(sc->sc.flags.ptr), s))) return bail(rc);
+    int32_t oor = 0;
+    if (hipMemcpyAsync(&oor, static_cast<int32_t*>(sc->sc.flags.ptr) + 1, sizeof(oor),
+                       hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return bail(fail(BP_ERR_HIP, "scene upload failed"));
+    if (oor) return bail(fail(BP_ERR_OUT_OF_RANGE, orbit_err_string(BP_ORBIT_INTERP_DOMAIN_ERROR)));
+    *scene = sc;
+    return BP_SUCCESS;
+}
+
+int bp_scene_destroy(bp_scene* sc)
+{
+    if (sc == nullptr) return BP_SUCCESS;
+    (void)hipSetDevice(sc->ctx->device);
+    (void)hipStreamSynchronize(sc->cap);
+    destroy_scene(sc);
+    return BP_SUCCESS;
+}
+
+int bp_scene_focus(bp_scene* sc, bp_out_block* blocks, int n_blocks, const bp_options* options,
+                   void* stream, int* any_failed, bp_stats* stats)
+{
+    if (any_failed) *any_failed = 0;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (sc == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    int rc;
+    int64_t npix = 0;
+    if ((rc = check_blocks(blocks, n_blocks, &npix)) != BP_SUCCESS) return rc;
+    if (npix > sc->max_pixels)
+        return fail(BP_ERR_INVALID_ARGUMENT, "request of " + std::to_string(npix) +
+                                                 " pixels exceeds the scene's max_pixels");
+    Opts o;
+    if ((rc = check_options(options, &o)) != BP_SUCCESS) return rc;
+    if (npix == 0) return BP_SUCCESS;
+    BP_HIP_CHECK(hipSetDevice(sc->ctx->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+
+    bp_scene::Graph* g = nullptr;
+    for (auto& x : sc->graphs)
+        if (x.chunk == o.chunk && x.blocks_per_cu == o.blocks_per_cu && x.staged == o.staged) g = &x;
+    if (g == nullptr) {
+        bp_scene::Graph ng{};
+        if ((rc = capture_graph(sc, o, &ng)) != BP_SUCCESS) return rc;
+        sc->graphs.push_back(ng);
+        g = &sc->graphs.back();
+    }
+    // the previous replay has finished (every focus call waits for its step),
+    // so the pinned descriptors are free to rewrite
+    bp::BatchHeader hdr;
+    if ((rc = plan_batch(blocks, n_blocks, o, static_cast<int32_t*>(sc->sc.flags.ptr) + kFlagWords,
+                         sc->h_desc, sc->max_blocks, &hdr, sc->block_of_grid)) != BP_SUCCESS)
+        return rc;
+    BP_HIP_CHECK(hipGraphLaunch(g->exec, s));
+    BP_HIP_CHECK(hipStreamSynchronize(s));
+    return finish_step(sc->h_flags, hdr.n_grids, sc->block_of_grid, blocks, sc->ev, any_failed,
+                       stats);
+}
 
 int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_geometry* ig,
                    const bp_dem* dem, double fc, double ds, const bp_kernel* kernel,
@@ -563,9 +944,9 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
     blk.grid = og->grid;
     blk.out_dev = out_dev;
     blk.height_dev = height_dev;
-    return run_blocks(c, &og->orbit, &og->doppler, &blk, 1, ig, dem, fc, ds, kernel,
-                      dry_tropo_model, r2g, g2r, in_dev, precision, options, stream, any_failed,
-                      stats);
+    CallArgs a{&og->orbit, &og->doppler, ig, dem, fc, ds, kernel, dry_tropo_model, r2g, g2r,
+               in_dev, precision};
+    return run_blocks(c, a, &blk, 1, options, stream, any_failed, stats);
 }
 
 int bp_backproject_blocks(bp_context* c, const bp_orbit* out_orbit, const bp_lut2d* out_doppler,
@@ -576,9 +957,9 @@ int bp_backproject_blocks(bp_context* c, const bp_orbit* out_orbit, const bp_lut
                           const bp_options* options, void* stream, int* any_failed,
                           bp_stats* stats)
 {
-    return run_blocks(c, out_orbit, out_doppler, blocks, n_blocks, ig, dem, fc, ds, kernel,
-                      dry_tropo_model, r2g, g2r, in_dev, precision, options, stream, any_failed,
-                      stats);
+    CallArgs a{out_orbit, out_doppler, ig, dem, fc, ds, kernel, dry_tropo_model, r2g, g2r,
+               in_dev, precision};
+    return run_blocks(c, a, blocks, n_blocks, options, stream, any_failed, stats);
 }
 
 // ------------------------------------------------------------ host helpers

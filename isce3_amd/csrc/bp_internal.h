@@ -32,9 +32,9 @@ struct alignas(16) PixGeom {
 // P(k), is a degree-4 polynomial in the pulse index fitted in FP64; the sum
 // kernel steps it pulse by pulse from the wavefront's first pulse kw0 by FP64
 // forward differences:  P += d1; d1 += d2; d2 += d3; d3 += d4.
-// The polyphase coordinate z = fma((float)P, zscale, zoff), with the global
-// zscale = -Q/(fc dtau), gives the first tap c0 = cbase - (floor(z) >> log2 Q)
-// and the weight row floor(z) & (Q - 1) + fract(z).
+// The polyphase coordinate z = fma((float)P, zscale, zoff) >= 0, with the
+// global zscale = -Q/(fc dtau), gives the first tap c0 = cbase - (floor(z) >>
+// log2 Q) and the weight row floor(z) & (Q - 1) + fract(z).
 struct alignas(16) PixSplit {
     double p0;               // P(kw0)
     double d1, d2, d3, d4;   // forward differences of P at kw0
@@ -52,6 +52,15 @@ constexpr int kFitDegree = 4;        // delay polynomial degree of the split pat
 constexpr int kPolyphaseLog2 = 9;
 constexpr int kPolyphaseRows = 1 << kPolyphaseLog2;  // fractional-offset rows of the split weight table
 constexpr int kMaxTableLds = 8192;   // plain path keeps tables up to this size in LDS
+
+// One polyphase weight row of the split sum: the W tap weights (padded to a
+// multiple of 4), the same number of deltas to the next row, and for W <= 8
+// four floats of padding.  Rows are read as float4 at data-dependent row
+// offsets; an 80-byte stride spreads their starts over all LDS banks.
+__host__ __device__ constexpr int poly_row_floats(int w)
+{
+    return 2 * ((w + 3) / 4 * 4) + (((w + 3) / 4 * 4) % 8 == 0 ? 4 : 0);
+}
 
 // Pixel tiling of one output grid: each 64-lane wavefront owns a tile of
 // tile_rows x (64/tile_rows) output pixels.
@@ -79,12 +88,25 @@ struct OutDesc {
     void* pad2_;
 };
 
-struct GeomArgs {
-    const OutDesc* grids;       // device array
+// Counts of one (batched) call, stored on the device directly in front of its
+// output descriptors, so that the kernels read them at run time and a
+// captured step replays with new counts.
+struct BatchHeader {
+    int64_t n_seq;              // wavefront tiles of all grids
+    int64_t n_groups;           // groups of four tiles (geometry / plain-sum workgroups)
     int32_t n_grids;
+    int32_t pad_[3];
+};
+static_assert(sizeof(BatchHeader) == 32, "BatchHeader layout");
+
+__host__ __device__ inline const OutDesc* batch_grids(const BatchHeader* b)
+{
+    return reinterpret_cast<const OutDesc*>(b + 1);
+}
+
+struct GeomArgs {
+    const BatchHeader* batch;   // device: header then OutDesc[n_grids]
     int32_t want_split;
-    int64_t n_seq;              // total wavefront tiles
-    int32_t n_blocks;           // 256-thread blocks (4 tiles each)
     int32_t in_side;
     int32_t kwidth;             // interpolation kernel width W (taps)
     int32_t nr;                 // range samples per pulse
@@ -104,11 +126,8 @@ struct GeomArgs {
 };
 
 struct SumArgs {
-    const OutDesc* grids;       // device array
-    int32_t n_grids;
+    const BatchHeader* batch;   // device: header then OutDesc[n_grids]
     int32_t chunk;
-    int64_t n_seq;              // total wavefront tiles
-    int32_t n_blocks;           // 256-thread blocks of the plain kernel
     int32_t ntab;
     int64_t nr;                 // range samples per pulse
     double swst, dtau, fc;
@@ -121,15 +140,20 @@ struct SumArgs {
     const PixGeom* geom;
     const PixSplit* split;
     const float* table;         // tabulated kernel (plain)
-    const float* poly;          // polyphase weights (split): (rows+1) x (2W)
+    const float* poly;          // polyphase weights (split): (rows+1) x poly_row_floats(W)
 };
 
-// Host launchers (defined in bp_kernels.hip).
+// Host launchers (defined in bp_kernels.hip).  `grid` is the number of
+// workgroups; the geometry and plain-sum kernels loop over the groups of the
+// batch (one group per workgroup when grid >= n_groups), the split kernel
+// walks the wavefront tiles persistently.
 hipError_t launch_pulse_orbit(const Orbit& orbit, double t0, double dt, int na, double* pv,
                               int32_t* flags, hipStream_t stream);
-hipError_t launch_geometry(const GeomArgs& args, hipStream_t stream);
-hipError_t launch_sum_plain(const SumArgs& args, int width, hipStream_t stream);
-hipError_t launch_sum_split(const SumArgs& args, int width, int grid, hipStream_t stream);
-int sum_blocks_per_cu(int precision, int width);
+hipError_t launch_clear(int32_t* words, int n, hipStream_t stream);
+hipError_t launch_geometry(const GeomArgs& args, int grid, hipStream_t stream);
+hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_t stream);
+hipError_t launch_sum_split(const SumArgs& args, int width, int grid, bool staged,
+                            hipStream_t stream);
+int sum_blocks_per_cu(int precision, int width, bool staged);
 
 }  // namespace bp

@@ -33,17 +33,18 @@ __device__ inline long long wave_sum_ll(long long v)
     return v;
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs; remap so that each
-// XCD (own L2) walks a contiguous run of tiles, which share pulse data.
-// Returns the global wavefront-tile index of this wavefront (4 per block).
-__device__ inline int64_t wave_tile(int nb, int* lane)
+// Workgroups are dispatched round-robin over the 8 XCDs; remap group `it` of
+// `ng` (it = blockIdx.x, or blockIdx.x plus multiples of a grid that is a
+// multiple of 8, so that it % 8 is the XCD) so that each XCD (own L2) walks a
+// contiguous run of tiles, which share pulse data.  Returns the global
+// wavefront-tile index of this wavefront (4 per group).
+__device__ inline int64_t group_tile(int64_t it, int64_t ng, int* lane)
 {
-    const int b = blockIdx.x;
-    const int per = nb / 8, rem = nb % 8;
-    const int xcd = b % 8, idx = b / 8;
-    const int lb = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    const int64_t per = ng / 8, rem = ng % 8;
+    const int64_t xcd = it % 8, idx = it / 8;
+    const int64_t lb = xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
     *lane = threadIdx.x & 63;
-    return (int64_t)lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    return lb * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 }
 
 // Descriptor words read through vector loads are not known to be uniform;
@@ -125,13 +126,18 @@ __global__ void pulse_orbit_kernel(Orbit orbit, double t0, double dt, int na, do
     o[5] = v.z;
 }
 
-// Per-pixel geometry, following Backproject.cpp:127-199.
-__global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
+__global__ void clear_kernel(int32_t* words, int n)
 {
-    int lane;
-    const int64_t gseq = wave_tile(a.n_blocks, &lane);
-    const bool in_seq = gseq < a.n_seq;
-    const OutDesc d = load_desc(a.grids, in_seq ? find_grid(a.grids, a.n_grids, gseq) : 0);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        words[i] = 0;
+}
+
+// Per-pixel geometry of one wavefront tile, following Backproject.cpp:127-199.
+__device__ inline void geometry_tile(const GeomArgs& a, const OutDesc* grids, int n_grids,
+                                     int64_t n_seq, int64_t gseq, int lane)
+{
+    const bool in_seq = gseq < n_seq;
+    const OutDesc d = load_desc(grids, in_seq ? find_grid(grids, n_grids, gseq) : 0);
     int64_t row = 0, col = 0;
     const bool valid = in_seq && tile_pixel(d.map, gseq - d.seq_base, lane, &row, &col);
     const int64_t pix = row * d.map.width + col;
@@ -324,16 +330,32 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
         const double inv_k = 1.0 / (a.fc * a.dtau);
         const bool odd = (a.kwidth & 1) != 0;
         const double shift = odd ? 0.5 : 0.0;
-        sp.zoff = (float)(-(double)kPolyphaseRows * (f_ref - c_ref * inv_k + shift));
-        sp.cbase = (int32_t)n_ref - a.kwidth / 2 - (odd ? 1 : 0);
         // Conservative range of the first tap over the window (|s| <= 1 on the
         // wavefront's window, so |dc| <= sum |mono[n]|), one bin of margin.
         const double bnd = inv_k * (fabs(mono[1]) + fabs(mono[2]) + fabs(mono[3]) + fabs(mono[4]));
+        // koff whole bins of offset keep z >= 0 over the window, so that the
+        // sum kernel takes floor(z) and fract(z) by truncation and v_fract
+        const double koff = ceil(bnd) + 2.0;
+        sp.zoff = (float)(-(double)kPolyphaseRows * (f_ref - c_ref * inv_k + shift - koff));
+        sp.cbase = (int32_t)n_ref - a.kwidth / 2 - (odd ? 1 : 0) + (int32_t)koff;
         const double c_lo = n_ref + ceil(f_ref - bnd + shift) - (a.kwidth / 2) - (odd ? 1 : 0) - 1;
         const double c_hi = n_ref + ceil(f_ref + bnd + shift) - (a.kwidth / 2) - (odd ? 1 : 0) + 1;
         sp.edge = (c_lo < 0. || c_hi + a.kwidth > (double)a.nr) ? 1 : 0;
     }
     a.split[rec] = sp;
+}
+
+__global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
+{
+    const OutDesc* grids = batch_grids(a.batch);
+    const int n_grids = __builtin_amdgcn_readfirstlane(a.batch->n_grids);
+    const int64_t n_seq = uniform64(a.batch->n_seq);
+    const int64_t ng = uniform64(a.batch->n_groups);
+    for (int64_t it = blockIdx.x; it < ng; it += gridDim.x) {
+        int lane;
+        const int64_t gseq = group_tile(it, ng, &lane);
+        geometry_tile(a, grids, n_grids, n_seq, gseq, lane);
+    }
 }
 
 // Persistent tile schedule of the sum kernels.  A workgroup holds 8
@@ -355,8 +377,11 @@ __device__ inline int64_t seq_to_tile(const TileMap& m, int64_t s)
 // wavefront tile this wavefront owns; tiles past the last grid are passed as
 // tile == map.n_tiles (no valid pixel).
 template<typename Body>
-__device__ inline void for_each_tile(const OutDesc* grids, int n_grids, int64_t n_seq, Body&& body)
+__device__ inline void for_each_tile(const BatchHeader* batch, Body&& body)
 {
+    const OutDesc* grids = batch_grids(batch);
+    const int n_grids = __builtin_amdgcn_readfirstlane(batch->n_grids);
+    const int64_t n_seq = uniform64(batch->n_seq);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int xcd = blockIdx.x & 7;
@@ -439,8 +464,9 @@ __device__ inline void plain_tile(const SumArgs& a, const float* tab, const OutD
     }
 }
 
-// One 256-thread workgroup per four tiles (measured faster than the
-// persistent walk for this FP64-bound kernel, which needs ~100 VGPRs).
+// One 256-thread workgroup per four tiles when launched with one workgroup per
+// group (measured faster than the persistent walk of the split kernel for
+// this FP64-bound kernel, which needs ~100 VGPRs); smaller grids loop.
 template<int W>
 __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
 {
@@ -450,29 +476,39 @@ __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
         for (int i = threadIdx.x; i < a.ntab; i += blockDim.x) s_tab[i] = a.table[i];
     __syncthreads();
     const float* tab = tab_in_lds ? s_tab : a.table;
-    int lane;
-    const int64_t gseq = wave_tile(a.n_blocks, &lane);
-    if (gseq < a.n_seq) {
-        const OutDesc d = load_desc(a.grids, find_grid(a.grids, a.n_grids, gseq));
-        plain_tile<W>(a, tab, d, gseq - d.seq_base, lane);
+    const OutDesc* grids = batch_grids(a.batch);
+    const int n_grids = __builtin_amdgcn_readfirstlane(a.batch->n_grids);
+    const int64_t n_seq = uniform64(a.batch->n_seq);
+    const int64_t ng = uniform64(a.batch->n_groups);
+    for (int64_t it = blockIdx.x; it < ng; it += gridDim.x) {
+        int lane;
+        const int64_t gseq = group_tile(it, ng, &lane);
+        if (gseq < n_seq) {
+            const OutDesc d = load_desc(grids, find_grid(grids, n_grids, gseq));
+            plain_tile<W>(a, tab, d, gseq - d.seq_base, lane);
+        }
     }
 }
 
 // Split-precision coherent sum of one wavefront tile.
 //
-// Per pixel (FP64, from the geometry pass): phase polynomial coefficients
-// b0..b3 of s = (k - kc)/h, the reduced reference phase c_ref and the
-// reference range bin n_ref + f_ref.  Per pulse: s by one FP64 FMA, the phase
-// offset dc by FP64 Horner (full-rate FP64 FMA on CDNA), the phase fraction by
-// v_fract_f64, the fractional range-bin position f_ref + dc/(fc dtau) in FP32,
-// tap weights linearly interpolated between polyphase rows in LDS, FP32 complex
-// MACs, hardware sin/cos of the phase in revolutions, and a fused
-// rotate-accumulate into an FP32 partial sum that is folded every `chunk`
-// pulses into a Kahan-compensated accumulator.
+// Per pixel (FP64, from the geometry pass): the pulse phase in cycles P(k) as
+// a degree-4 polynomial stepped by forward differences from the wavefront's
+// first pulse, and the polyphase coordinate offset.  Per pulse: P advanced by
+// four FP64 adds (full rate on CDNA), its fraction for the phase, and one FP32
+// polyphase coordinate z that gives the first tap bin, the weight row and the
+// fraction between rows; weights linearly interpolated between polyphase rows
+// in LDS, FP32 complex MACs, hardware sin/cos of the phase in revolutions, and
+// a fused rotate-accumulate into an FP32 partial sum that is folded every
+// `chunk` pulses into a Kahan-compensated accumulator.
 //
 // The pulses every lane of the wavefront integrates, [max kstart, min kstop),
-// run in an unmasked loop whose tap loads are issued one pulse ahead; the
-// ragged ends of the lanes' windows run in a masked loop.
+// form the core loop; the ragged ends of the lanes' windows run in a masked
+// loop that gathers its taps from global memory.  In the core loop the taps
+// either come from the wavefront's LDS ring of staged pulse windows (one
+// coalesced 512-byte load per pulse; see split_core_staged) or are gathered
+// from global memory, one pulse ahead (split_core_gather).
+//
 // FP64 phase of one lane in cycles, P(k), stepped by forward differences.
 struct PhaseState {
     double p, d1, d2, d3, d4;
@@ -495,21 +531,22 @@ struct SplitPulse {
 template<int W>
 __device__ inline SplitPulse split_pulse(const PhaseState& st, float zscale, float zoff, int cbase)
 {
-    constexpr int ROWB = 4 * 2 * ((W + 3) / 4 * 4);  // bytes of one weight row
+    constexpr int ROWB = 4 * poly_row_floats(W);  // bytes of one weight row
     SplitPulse r;
     r.ph = (float)__builtin_amdgcn_fract(st.p);
+    // z >= 0 over the wavefront's window (the geometry pass offsets it), so
+    // truncation is floor
     const float z = fmaf((float)st.p, zscale, zoff);
-    const float zf = floorf(z);
-    const int m = (int)zf;
-    r.qf = z - zf;
-    r.row = (m & (kPolyphaseRows - 1)) * ROWB;
-    r.c0 = cbase - (m >> kPolyphaseLog2);
+    const uint32_t m = (uint32_t)z;
+    r.qf = __builtin_amdgcn_fractf(z);
+    r.row = (int)(m & (kPolyphaseRows - 1)) * ROWB;
+    r.c0 = cbase - (int)(m >> kPolyphaseLog2);
     return r;
 }
 
-// Tap loads.  Unguarded: the window is known to lie inside the line (the
-// clamp only keeps every load inside the line).  Guarded: taps outside
-// [0, nr) are zero (core/detail/Interp1d.h:62-83).
+// Tap loads from global memory.  Unguarded: the window is known to lie
+// inside the line (the clamp only keeps every load inside the line).
+// Guarded: taps outside [0, nr) are zero (core/detail/Interp1d.h:62-83).
 template<int W, bool kGuard>
 __device__ inline void split_taps(const float2* line, int c0, int nr, float2 (&d)[W])
 {
@@ -528,10 +565,11 @@ __device__ inline void split_taps(const float2* line, int c0, int nr, float2 (&d
     }
 }
 
-template<int W>
+template<int W, int N>
 __device__ inline void split_update(const float* s_poly, const SplitPulse& pu,
-                                    const float2 (&d)[W], float& cre, float& cim)
+                                    const float2 (&d)[N], float& cre, float& cim)
 {
+    static_assert(N >= W, "tap array");
     constexpr int WP = (W + 3) / 4 * 4;
     // one polyphase row: WP weights then WP deltas, read as float4
     const float4* row4 =
@@ -569,85 +607,199 @@ __device__ inline void kahan_fold(float& acc, float& cmp, float& part)
     part = 0.f;
 }
 
-// The pulse loops of one wavefront over its union window [kw0, kw1): the
-// pulses every lane integrates, [kf0, kf1), run unmasked with the tap loads
-// of pulse k+1 issued before pulse k is accumulated; the ragged ends run
-// masked.  Every pulse advances every lane's phase state exactly once.
-template<int W, bool kGuard>
-__device__ inline void split_loops(const SumArgs& a, const float* s_poly, PhaseState st,
-                                   float zoff, int cbase, int ks, int ke, int kw0, int kw1,
-                                   int kf0, int kf1, float& acc_re, float& acc_im)
-{
-    const int nr = (int)a.nr;
-    const int chunk = a.chunk;
-    const float zscale = a.zscale;
-    const float2* rc = a.rc;
-    float cmp_re = 0.f, cmp_im = 0.f, cre = 0.f, cim = 0.f;
-
-    auto masked = [&](int k_lo, int k_hi) {
-        for (int k0 = k_lo; k0 < k_hi; k0 += chunk) {
-            const int k1 = min(k0 + chunk, k_hi);
-            for (int k = k0; k < k1; ++k) {
-                const SplitPulse pu = split_pulse<W>(st, zscale, zoff, cbase);
-                st.advance();
-                if (k >= ks && k < ke) {
-                    float2 d[W];
-                    split_taps<W, kGuard>(rc + (int64_t)k * nr, pu.c0, nr, d);
-                    split_update<W>(s_poly, pu, d, cre, cim);
-                }
-            }
-            kahan_fold(acc_re, cmp_re, cre);
-            kahan_fold(acc_im, cmp_im, cim);
-        }
-    };
-
-    if (kw0 < kw1) masked(kw0, kf0);
-    if (kf0 < kf1) {
-        // software pipeline over the whole range: the taps of the next pulse
-        // are in flight while the current one is accumulated; partial sums
-        // are folded every `chunk` pulses (rounded up to a pair)
-        const int pairs = (chunk + 1) / 2;
-        SplitPulse pa = split_pulse<W>(st, zscale, zoff, cbase), pb;
-        st.advance();
-        float2 da[W], db[W];
-        split_taps<W, kGuard>(rc + (int64_t)kf0 * nr, pa.c0, nr, da);
-        int k = kf0, it = 0;
-        for (; k + 2 < kf1; k += 2) {
-            pb = split_pulse<W>(st, zscale, zoff, cbase);
-            st.advance();
-            split_taps<W, kGuard>(rc + (int64_t)(k + 1) * nr, pb.c0, nr, db);
-            split_update<W>(s_poly, pa, da, cre, cim);
-            pa = split_pulse<W>(st, zscale, zoff, cbase);
-            st.advance();
-            split_taps<W, kGuard>(rc + (int64_t)(k + 2) * nr, pa.c0, nr, da);
-            split_update<W>(s_poly, pb, db, cre, cim);
-            if (++it == pairs) {
-                it = 0;
-                kahan_fold(acc_re, cmp_re, cre);
-                kahan_fold(acc_im, cmp_im, cim);
-            }
-        }
-        // the last one or two pulses (pulse k's taps are loaded)
-        if (k + 1 < kf1) {
-            pb = split_pulse<W>(st, zscale, zoff, cbase);
-            st.advance();
-            split_taps<W, kGuard>(rc + (int64_t)(k + 1) * nr, pb.c0, nr, db);
-            split_update<W>(s_poly, pa, da, cre, cim);
-            split_update<W>(s_poly, pb, db, cre, cim);
-        } else {
-            split_update<W>(s_poly, pa, da, cre, cim);
-        }
+// State of one wavefront's split sum shared by its pulse loops.
+struct SplitAcc {
+    PhaseState st;
+    float zoff;
+    int cbase;
+    float cre, cim;          // partial sums since the last fold
+    float acc_re, acc_im;    // compensated sums
+    float cmp_re, cmp_im;    // compensation terms
+    int count;               // pulses since the last fold
+    __device__ inline void fold()
+    {
         kahan_fold(acc_re, cmp_re, cre);
         kahan_fold(acc_im, cmp_im, cim);
+        count = 0;
     }
-    if (kw0 < kw1) masked(max(kf1, kw0), kw1);
-    acc_re -= cmp_re;
-    acc_im -= cmp_im;
+    __device__ inline void tick(int chunk)
+    {
+        if (++count >= chunk) fold();
+    }
+};
+
+// Pulses [k_lo, k_hi) of which each lane integrates [ks, ke): masked taps
+// gathered from global memory.  Every pulse advances every lane's phase.
+template<int W, bool kGuard>
+__device__ inline void split_masked(const SumArgs& a, const float* s_poly, SplitAcc& s, int ks,
+                                    int ke, int k_lo, int k_hi)
+{
+    const int nr = (int)a.nr;
+    for (int k = k_lo; k < k_hi; ++k) {
+        const SplitPulse pu = split_pulse<W>(s.st, a.zscale, s.zoff, s.cbase);
+        s.st.advance();
+        if (k >= ks && k < ke) {
+            float2 d[W];
+            split_taps<W, kGuard>(a.rc + (int64_t)k * nr, pu.c0, nr, d);
+            split_update<W>(s_poly, pu, d, s.cre, s.cim);
+        }
+        s.tick(a.chunk);
+    }
 }
 
+// Core pulses [k0, k1), taps gathered from global memory with the loads of
+// pulse k+1 issued before pulse k is accumulated.
+template<int W, bool kGuard>
+__device__ inline void split_core_gather(const SumArgs& a, const float* s_poly, SplitAcc& s,
+                                         int k0, int k1)
+{
+    const int nr = (int)a.nr;
+    const float2* rc = a.rc;
+    SplitPulse pa = split_pulse<W>(s.st, a.zscale, s.zoff, s.cbase), pb;
+    s.st.advance();
+    float2 da[W], db[W];
+    split_taps<W, kGuard>(rc + (int64_t)k0 * nr, pa.c0, nr, da);
+    int k = k0;
+    for (; k + 2 < k1; k += 2) {
+        pb = split_pulse<W>(s.st, a.zscale, s.zoff, s.cbase);
+        s.st.advance();
+        split_taps<W, kGuard>(rc + (int64_t)(k + 1) * nr, pb.c0, nr, db);
+        split_update<W>(s_poly, pa, da, s.cre, s.cim);
+        s.tick(a.chunk);
+        pa = split_pulse<W>(s.st, a.zscale, s.zoff, s.cbase);
+        s.st.advance();
+        split_taps<W, kGuard>(rc + (int64_t)(k + 2) * nr, pa.c0, nr, da);
+        split_update<W>(s_poly, pb, db, s.cre, s.cim);
+        s.tick(a.chunk);
+    }
+    // the last one or two pulses (pulse k's taps are loaded)
+    if (k + 1 < k1) {
+        pb = split_pulse<W>(s.st, a.zscale, s.zoff, s.cbase);
+        s.st.advance();
+        split_taps<W, kGuard>(rc + (int64_t)(k + 1) * nr, pb.c0, nr, db);
+        split_update<W>(s_poly, pa, da, s.cre, s.cim);
+        s.tick(a.chunk);
+        split_update<W>(s_poly, pb, db, s.cre, s.cim);
+        s.tick(a.chunk);
+    } else {
+        split_update<W>(s_poly, pa, da, s.cre, s.cim);
+        s.tick(a.chunk);
+    }
+}
+
+// LDS-staged pulse windows.  Each wavefront owns a ring of kWinSlots slots;
+// a slot holds the kWinBins range bins [b, b + kWinBins) of one pulse twice:
+// copy A with bin b + i at byte 8 i, and copy B with bin b + i at byte
+// kWinCopyB + 8 i, kWinCopyB = 8 (mod 16).  The N = 2 ceil(W / 2) bins from
+// bin b + r on then start on a 16-byte boundary in copy A for even r and in
+// copy B for odd r, and are read with N / 2 ds_read_b128.  The window base b
+// is the first tap of a reference lane less kWinMargin bins, clamped into the
+// line; lanes whose taps leave the window (never for tiles narrower than
+// about 40 range bins) gather them from global memory instead.
+constexpr int kWinBins = 64;
+constexpr int kWinSlots = 3;
+constexpr int kWinCopyB = 520;
+constexpr int kWinSlotBytes = 1040;
+constexpr int kWinWaveBytes = kWinSlots * kWinSlotBytes;
+constexpr int kWinMargin = 4;
+static_assert(kWinCopyB % 16 == 8 && kWinCopyB >= 8 * kWinBins, "copy B placement");
+static_assert(kWinWaveBytes % 16 == 0, "rings must start on 16-byte boundaries");
+static_assert(kWinCopyB + 8 * kWinBins <= kWinSlotBytes && kWinSlotBytes % 16 == 0, "slot size");
+
 template<int W>
-__device__ inline void split_tile(const SumArgs& a, const float* s_poly, const OutDesc& od,
-                                  int64_t tile, int lane)
+struct StagedPulse {
+    SplitPulse pu;
+    int b;      // window base bin (wave-uniform)
+    float2 g;   // this lane's bin b + lane of the window, in flight
+};
+
+// Core pulses [k0, k1) with taps from the wavefront's LDS ring.  Pulse k's
+// window is loaded at iteration k - 2, written to its slot at iteration k - 1
+// and read at iteration k.  `ring` is the byte offset of the wavefront's ring
+// in the shared array `s_win`; `active` marks lanes whose output is used.
+template<int W>
+__device__ inline void split_core_staged(const SumArgs& a, const float* s_poly, char* s_win,
+                                         int ring, SplitAcc& s, int k0, int k1, int ref,
+                                         bool active)
+{
+    constexpr int N = 2 * ((W + 1) / 2);          // taps read per pulse
+    constexpr int kMaxRel = kWinBins - N;         // largest window-relative first tap
+    const int nr = (int)a.nr;
+    const int lane = threadIdx.x & 63;
+    const uint32_t lane8 = 8u * (uint32_t)lane;
+    const float2* rc = a.rc;
+    const uint64_t act = __builtin_amdgcn_ballot_w64(active);
+
+    auto start = [&](StagedPulse<W>& x, int k) {
+        x.pu = split_pulse<W>(s.st, a.zscale, s.zoff, s.cbase);
+        s.st.advance();
+        const int c = __builtin_amdgcn_readlane(x.pu.c0, ref) - kWinMargin;
+        x.b = min(max(c, 0), nr - kWinBins);
+        // wave-uniform window start plus a 32-bit lane offset (the empty asm
+        // keeps the offset from being folded into a 64-bit per-lane address)
+        const char* w = reinterpret_cast<const char*>(rc + (int64_t)k * nr + x.b);
+        uint32_t lo = lane8;
+        asm volatile("" : "+v"(lo));
+        x.g = *reinterpret_cast<const float2*>(w + lo);
+    };
+    auto stage = [&](const StagedPulse<W>& x, int slot) {
+        char* base = s_win + ring + slot * kWinSlotBytes + lane8;
+        *reinterpret_cast<float2*>(base) = x.g;
+        *reinterpret_cast<float2*>(base + kWinCopyB) = x.g;
+    };
+    auto finish = [&](const StagedPulse<W>& x, int slot, int k) {
+        const uint32_t rel = (uint32_t)(x.pu.c0 - x.b);
+        const uint32_t r = min(rel, (uint32_t)kMaxRel);
+        // byte offset 8 r in copy A (even r) or kWinCopyB + 8 r in copy B
+        // (odd r): y + (y & 8) * (kWinCopyB / 8) with y = 8 r + ring, ring a
+        // multiple of 16
+        const uint32_t y = 8u * r + (uint32_t)ring;
+        uint32_t off;
+        asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(off) : "v"(y & 8u), "s"(kWinCopyB / 8), "v"(y));
+        const float4* p = reinterpret_cast<const float4*>(s_win + slot * kWinSlotBytes + off);
+        float2 d[N];
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) {
+            const float4 v = p[i];
+            d[2 * i] = make_float2(v.x, v.y);
+            d[2 * i + 1] = make_float2(v.z, v.w);
+        }
+        // some active lane's taps leave the window: all lanes take this
+        // pulse's taps from global memory (same values as the window holds)
+        if ((__builtin_amdgcn_ballot_w64(rel > (uint32_t)kMaxRel) & act) != 0) {
+            float2 e[W];
+            split_taps<W, false>(rc + (int64_t)k * nr, x.pu.c0, nr, e);
+#pragma unroll
+            for (int m = 0; m < W; ++m) d[m] = e[m];
+        }
+        split_update<W>(s_poly, x.pu, d, s.cre, s.cim);
+        s.tick(a.chunk);
+    };
+
+    StagedPulse<W> A, B, C;
+    start(A, k0);
+    if (k0 + 1 < k1) start(B, k0 + 1);
+    stage(A, 0);
+    // three pulses per trip: the ring slot of pulse k is (k - k0) % 3
+    for (int k = k0;;) {
+        if (k + 2 < k1) start(C, k + 2);
+        if (k + 1 < k1) stage(B, 1);
+        finish(A, 0, k);
+        if (++k >= k1) break;
+        if (k + 2 < k1) start(A, k + 2);
+        if (k + 1 < k1) stage(C, 2);
+        finish(B, 1, k);
+        if (++k >= k1) break;
+        if (k + 2 < k1) start(B, k + 2);
+        if (k + 1 < k1) stage(A, 0);
+        finish(C, 2, k);
+        if (++k >= k1) break;
+    }
+}
+
+template<int W, bool kStaged>
+__device__ inline void split_tile(const SumArgs& a, const float* s_poly, char* s_win,
+                                  const OutDesc& od, int64_t tile, int lane)
 {
     int64_t row = 0, col = 0;
     const bool valid = tile_pixel(od.map, tile, lane, &row, &col);
@@ -672,43 +824,63 @@ __device__ inline void split_tile(const SumArgs& a, const float* s_poly, const O
         kf0 = kw1;
         kf1 = kw1;
     }
-    PhaseState st;
-    st.p = win ? sp.p0 : 0.;
-    st.d1 = win ? sp.d1 : 0.;
-    st.d2 = win ? sp.d2 : 0.;
-    st.d3 = win ? sp.d3 : 0.;
-    st.d4 = win ? sp.d4 : 0.;
-    const float zoff = win ? sp.zoff : 0.f;
-    const int cbase = win ? sp.cbase : 0;
+    SplitAcc s;
+    s.st.p = win ? sp.p0 : 0.;
+    s.st.d1 = win ? sp.d1 : 0.;
+    s.st.d2 = win ? sp.d2 : 0.;
+    s.st.d3 = win ? sp.d3 : 0.;
+    s.st.d4 = win ? sp.d4 : 0.;
+    s.zoff = win ? sp.zoff : 0.f;
+    s.cbase = win ? sp.cbase : 0;
+    s.cre = s.cim = s.acc_re = s.acc_im = s.cmp_re = s.cmp_im = 0.f;
+    s.count = 0;
     // guarded taps when some lane's window may reach past a line end, or the
     // lines are narrower than the kernel (wavefront-uniform choice)
     const bool guard = a.nr < W ||
                        __builtin_amdgcn_readfirstlane(wave_max_i(win && sp.edge ? 1 : 0)) != 0;
 
-    float acc_re = 0.f, acc_im = 0.f;
-    if (guard)
-        split_loops<W, true>(a, s_poly, st, zoff, cbase, ks, ke, kw0, kw1, kf0, kf1, acc_re,
-                             acc_im);
-    else
-        split_loops<W, false>(a, s_poly, st, zoff, cbase, ks, ke, kw0, kw1, kf0, kf1, acc_re,
-                              acc_im);
+    if (kw0 < kw1) {
+        if (guard) {
+            split_masked<W, true>(a, s_poly, s, ks, ke, kw0, kf0);
+            if (kf0 < kf1) split_core_gather<W, true>(a, s_poly, s, kf0, kf1);
+            split_masked<W, true>(a, s_poly, s, ks, ke, max(kf1, kw0), kw1);
+        } else {
+            split_masked<W, false>(a, s_poly, s, ks, ke, kw0, kf0);
+            if (kf0 < kf1) {
+                if (kStaged && a.nr >= kWinBins) {
+                    // reference lane: the first lane with pulses
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(win);
+                    const int ref = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(m));
+                    const int ring = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * kWinWaveBytes;
+                    split_core_staged<W>(a, s_poly, s_win, ring, s, kf0, kf1, ref, win);
+                } else {
+                    split_core_gather<W, false>(a, s_poly, s, kf0, kf1);
+                }
+            }
+            split_masked<W, false>(a, s_poly, s, ks, ke, max(kf1, kw0), kw1);
+        }
+        s.fold();
+    }
     if (valid) {
         const float nanf_ = __builtin_nanf("");
-        od.out[pix] = ok ? make_float2(acc_re, acc_im) : make_float2(nanf_, nanf_);
+        od.out[pix] = ok ? make_float2(s.acc_re - s.cmp_re, s.acc_im - s.cmp_im)
+                         : make_float2(nanf_, nanf_);
     }
 }
 
-template<int W>
+template<int W, bool kStaged>
 __global__ __launch_bounds__(512) void sum_split_kernel(SumArgs a)
 {
-    constexpr int ROW = 2 * ((W + 3) / 4 * 4);
-    // static LDS array: row offsets fold into the ds_read offsets
+    constexpr int ROW = poly_row_floats(W);
+    // static LDS arrays: row and slot offsets fold into the ds_read offsets
     __shared__ float4 s_poly4[(kPolyphaseRows + 1) * ROW / 4];
+    __shared__ float4 s_win4[kStaged ? kSumWaves * kWinWaveBytes / 16 : 1];
     float* s_poly = reinterpret_cast<float*>(s_poly4);
     for (int i = threadIdx.x; i < (kPolyphaseRows + 1) * ROW; i += blockDim.x) s_poly[i] = a.poly[i];
     __syncthreads();
-    for_each_tile(a.grids, a.n_grids, a.n_seq, [&](const OutDesc& d, int64_t tile, int lane) {
-        split_tile<W>(a, s_poly, d, tile, lane);
+    char* s_win = reinterpret_cast<char*>(s_win4);
+    for_each_tile(a.batch, [&](const OutDesc& d, int64_t tile, int lane) {
+        split_tile<W, kStaged>(a, s_poly, s_win, d, tile, lane);
     });
 }
 
@@ -725,23 +897,30 @@ hipError_t launch_pulse_orbit(const Orbit& orbit, double t0, double dt, int na, 
     return hipGetLastError();
 }
 
-hipError_t launch_geometry(const GeomArgs& args, hipStream_t stream)
+hipError_t launch_clear(int32_t* words, int n, hipStream_t stream)
 {
-    if (args.n_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(geometry_kernel, dim3(args.n_blocks), dim3(256), 0, stream, args);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(clear_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, words, n);
     return hipGetLastError();
 }
 
-hipError_t launch_sum_plain(const SumArgs& args, int width, hipStream_t stream)
+hipError_t launch_geometry(const GeomArgs& args, int grid, hipStream_t stream)
 {
-    if (args.n_seq <= 0) return hipSuccess;
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL(geometry_kernel, dim3(grid), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_t stream)
+{
+    if (grid <= 0) return hipSuccess;
     const size_t lds = args.ntab <= kMaxTableLds ? sizeof(float) * args.ntab : 0;
     switch (width) {
     case 8:
-        hipLaunchKernelGGL(sum_plain_kernel<8>, dim3(args.n_blocks), dim3(256), lds, stream, args);
+        hipLaunchKernelGGL(sum_plain_kernel<8>, dim3(grid), dim3(256), lds, stream, args);
         break;
     case 9:
-        hipLaunchKernelGGL(sum_plain_kernel<9>, dim3(args.n_blocks), dim3(256), lds, stream, args);
+        hipLaunchKernelGGL(sum_plain_kernel<9>, dim3(grid), dim3(256), lds, stream, args);
         break;
     default:
         return hipErrorInvalidValue;
@@ -749,8 +928,9 @@ hipError_t launch_sum_plain(const SumArgs& args, int width, hipStream_t stream)
     return hipGetLastError();
 }
 
-// Workgroups of 512 threads one CU can hold for the chosen sum kernel.
-int sum_blocks_per_cu(int precision, int width)
+// Workgroups one CU can hold for the chosen sum kernel (256 threads for the
+// plain kernel, 512 for the split kernel).
+int sum_blocks_per_cu(int precision, int width, bool staged)
 {
     int n = 0;
     hipError_t e = hipSuccess;
@@ -758,9 +938,12 @@ int sum_blocks_per_cu(int precision, int width)
         const size_t lds = sizeof(float) * kMaxTableLds;
         e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_plain_kernel<8>, 256, lds)
                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_plain_kernel<9>, 256, lds);
+    } else if (staged) {
+        e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<8, true>, 512, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<9, true>, 512, 0);
     } else {
-        e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<8>, 512, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<9>, 512, 0);
+        e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<8, false>, 512, 0)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<9, false>, 512, 0);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -771,20 +954,21 @@ int sum_blocks_per_cu(int precision, int width)
 
 // Persistent grid: `grid` workgroups of 512 threads (a multiple of 8, one
 // share per XCD), sized by the caller from the device's CU count and occupancy.
-hipError_t launch_sum_split(const SumArgs& args, int width, int grid, hipStream_t stream)
+hipError_t launch_sum_split(const SumArgs& args, int width, int grid, bool staged,
+                            hipStream_t stream)
 {
-    if (args.n_seq <= 0) return hipSuccess;
-    const size_t lds = 0;  // the weight table is a static LDS array
-    switch (width) {
-    case 8:
-        hipLaunchKernelGGL(sum_split_kernel<8>, dim3(grid), dim3(512), lds, stream, args);
-        break;
-    case 9:
-        hipLaunchKernelGGL(sum_split_kernel<9>, dim3(grid), dim3(512), lds, stream, args);
-        break;
-    default:
+    if (grid <= 0) return hipSuccess;
+    const size_t lds = 0;  // the weight table and window rings are static LDS arrays
+    if (width == 8 && staged)
+        hipLaunchKernelGGL((sum_split_kernel<8, true>), dim3(grid), dim3(512), lds, stream, args);
+    else if (width == 8)
+        hipLaunchKernelGGL((sum_split_kernel<8, false>), dim3(grid), dim3(512), lds, stream, args);
+    else if (width == 9 && staged)
+        hipLaunchKernelGGL((sum_split_kernel<9, true>), dim3(grid), dim3(512), lds, stream, args);
+    else if (width == 9)
+        hipLaunchKernelGGL((sum_split_kernel<9, false>), dim3(grid), dim3(512), lds, stream, args);
+    else
         return hipErrorInvalidValue;
-    }
     return hipGetLastError();
 }
 

@@ -138,7 +138,7 @@ def backproject(out, out_geometry: RadarGeometry, signal_data, in_geometry: Rada
 
     Extensions: ``precision`` "plain" (reference semantics, FP64 inner loop)
     or "split" (FP64 per-pixel reference + FP32 deltas), ``options`` tiling
-    dict {tile_rows, chunk, order, blocks_per_cu}, ``stream`` (torch stream or raw hipStream_t int)
+    dict {tile_rows, chunk, order, blocks_per_cu, gather}, ``stream`` (torch stream or raw hipStream_t int)
     and ``stats`` (dict filled with updates, geometry_ms, sum_ms).
 
     Returns True if any pixel failed to converge (those pixels are NaN).
@@ -244,7 +244,7 @@ def _copy_back(torch, a, d):
 def _options_struct(options) -> _native.Options_t:
     opt = _native.Options_t()
     for k, v in (options or {}).items():
-        if k not in ("tile_rows", "chunk", "fit_degree", "order", "blocks_per_cu"):
+        if k not in ("tile_rows", "chunk", "fit_degree", "order", "blocks_per_cu", "gather"):
             raise errors.InvalidArgument(f"unknown option {k!r}")
         setattr(opt, k, int(v))
     return opt
@@ -393,3 +393,124 @@ def backproject_blocks(outs, out_geometries, signal_data, in_geometry: RadarGeom
             _copy_back(torch, h, d)
         _fill_stats(stats, st)
         return [bool(blocks[i].failed) for i in range(len(outs))]
+
+
+class Scene:
+    """One raw data set resident on the GPU, focused tile by tile through a
+    captured HIP-graph step (bp_scene_* in include/isce3_amd_backproject.h).
+
+    Holds the arguments that the per-block isce3.focus.backproject calls of
+    the RSLC workflow share (python/packages/nisar/workflows/focus.py:1986-2004):
+    the range-compressed pulses ``signal_data`` (a ROCm complex64 tensor that
+    must stay alive and unchanged), ``in_geometry``, the output ``out_orbit``
+    and ``out_doppler``, ``dem``, ``fc``, ``ds``, ``kernel``, the troposphere
+    model and solver parameters.  Orbits, Doppler tables, DEM and kernel
+    tables are uploaded once; per-pixel scratch is reserved for up to
+    ``max_pixels`` pixels in up to ``max_blocks`` output blocks per call.
+    :meth:`focus` gives, block for block, the result of
+    :func:`backproject_blocks` with the same arguments.  One host thread at a
+    time may use a scene.
+    """
+
+    def __init__(self, signal_data, in_geometry: RadarGeometry, out_orbit, out_doppler,
+                 dem: DEMInterpolator, fc: float, ds: float, kernel: KernelF32,
+                 dry_tropo_model: str = "tsx", rdr2geo_params: Optional[dict] = None,
+                 geo2rdr_params: Optional[dict] = None, *, precision: str = "split",
+                 max_pixels: int = 1 << 22, max_blocks: int = 64):
+        self._h = None
+        if precision not in _PRECISION:
+            raise errors.InvalidArgument(f"precision must be 'plain' or 'split', got {precision!r}")
+        if not isinstance(kernel, TabulatedKernelF32):
+            raise errors.InvalidArgument("kernel must be a TabulatedKernelF32")
+        if not _is_torch(signal_data) or not signal_data.is_cuda:
+            raise errors.InvalidArgument("a scene's signal data must be a device tensor")
+        if signal_data.ndim != 2 or \
+                _shape_of(signal_data) != (in_geometry.grid_length, in_geometry.grid_width):
+            raise errors.InvalidArgument("input signal data shape must match input radar grid shape")
+        if out_orbit.reference_epoch != in_geometry.reference_epoch:
+            raise errors.RuntimeError_("input reference epoch must match output reference epoch")
+        torch = _torch()
+        if signal_data.dtype != torch.complex64 or not signal_data.is_contiguous():
+            raise errors.InvalidArgument("signal data must be a C-contiguous complex64 tensor")
+        self.device = signal_data.device.index or 0
+        self.precision = precision
+        self.max_pixels = int(max_pixels)
+        self.max_blocks = int(max_blocks)
+        self.epoch = out_orbit.reference_epoch
+        self._signal = signal_data
+        oorb = out_orbit._struct()
+        odop = out_doppler._struct()
+        ig = geometry_struct(in_geometry)
+        dm = dem._struct()
+        kr = kernel._struct()
+        r2g = parse_rdr2geo_params(rdr2geo_params or {})
+        g2r = parse_geo2rdr_params(geo2rdr_params or {})
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            ctx = _contexts.get(self.device)
+            _keep = (in_geometry, out_orbit, out_doppler, dem, kernel)
+            rc = _native.lib().bp_scene_create(
+                ctx, C.byref(oorb), C.byref(odop), C.byref(ig), C.byref(dm), float(fc),
+                float(ds), C.byref(kr), parse_dry_tropo_model(dry_tropo_model), C.byref(r2g),
+                C.byref(g2r), C.c_void_p(signal_data.data_ptr()), _PRECISION[precision],
+                self.max_pixels, self.max_blocks, C.byref(h))
+            del _keep
+        _native.check(rc)
+        self._h = h
+
+    def _grid_struct(self, grid) -> _native.Grid_t:
+        t0 = grid.sensing_start
+        if grid.ref_epoch != self.epoch:
+            t0 = t0 + (grid.ref_epoch - self.epoch).total_seconds()
+        return _native.Grid_t(t0, grid.prf, grid.starting_range, grid.range_pixel_spacing,
+                              grid.length, grid.width, int(grid.lookside), 0)
+
+    def focus(self, outs, grids, heights=None, *, options: Optional[dict] = None, stream=None,
+              stats: Optional[dict] = None):
+        """Focus output blocks: ``outs`` are complex64 device tensors of the
+        shapes of ``grids`` (RadarGridParameters), ``heights`` None or a list
+        of float32 device tensors / None.  Returns per-block failure flags."""
+        if self._h is None:
+            raise errors.RuntimeError_("scene is closed")
+        outs, grids = list(outs), list(grids)
+        if len(outs) != len(grids):
+            raise errors.InvalidArgument("need one output grid per output array")
+        heights = [None] * len(outs) if heights is None else list(heights)
+        if len(heights) != len(outs):
+            raise errors.InvalidArgument("need one height entry per output array")
+        torch = _torch()
+        blocks = (_native.OutBlock_t * max(len(outs), 1))()
+        for i, (o, g, hh) in enumerate(zip(outs, grids, heights)):
+            if not _is_torch(o) or not o.is_cuda or o.dtype != torch.complex64 \
+                    or not o.is_contiguous() or _shape_of(o) != (g.length, g.width):
+                raise errors.InvalidArgument(
+                    "outputs must be C-contiguous complex64 device tensors of the grid shape")
+            if hh is not None and (not _is_torch(hh) or hh.dtype != torch.float32
+                                   or not hh.is_contiguous() or _shape_of(hh) != (g.length, g.width)):
+                raise errors.InvalidArgument(
+                    "heights must be C-contiguous float32 device tensors of the grid shape")
+            blocks[i].grid = self._grid_struct(g)
+            blocks[i].out_dev = o.data_ptr()
+            blocks[i].height_dev = hh.data_ptr() if hh is not None else None
+        opt = _options_struct(options)
+        st = _native.Stats_t()
+        failed = C.c_int(0)
+        with torch.cuda.device(self.device):
+            hstream = _stream_handle(torch, stream)
+            rc = _native.lib().bp_scene_focus(self._h, blocks, len(outs), C.byref(opt),
+                                              C.c_void_p(hstream), C.byref(failed), C.byref(st))
+        _native.check(rc)
+        _fill_stats(stats, st)
+        return [bool(blocks[i].failed) for i in range(len(outs))]
+
+    def close(self):
+        if self._h is not None:
+            _native.lib().bp_scene_destroy(self._h)
+            self._h = None
+            self._signal = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

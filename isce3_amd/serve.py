@@ -21,9 +21,10 @@ workers overlap on the device.  The ``fcfs`` policy (one worker, one request
 per batch) is the first-come-first-served baseline.
 
 Memory accounting: the resident raw sets and each worker's per-pixel scratch
-(sized for ``max_batch_pixels``) are charged once at construction; every
-admitted request is charged its output bytes until it is released (at once
-when ``keep_outputs`` is false, else by ``Request.release``).
+(sized for ``max_batch_pixels``, one copy per raw set when the executor keeps
+a resident scene per set) are charged once at construction; every admitted
+request is charged its output bytes until it is released (at once when
+``keep_outputs`` is false, else by ``Request.release``).
 """
 from __future__ import annotations
 
@@ -113,15 +114,30 @@ class Request:
 
 
 class GpuExecutor:
-    """Runs batches on the GPU: one HIP stream and one native context per worker."""
+    """Runs batches on the GPU: one HIP stream and one native context per
+    worker.  With ``graphs`` (the default) each worker keeps one resident
+    scene per raw set (focus.Scene: shared data uploaded once, the step
+    replayed from a captured HIP graph); otherwise every batch is one
+    bp_backproject_blocks call that uploads the shared data again."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, graphs: bool = True):
         import torch  # deferred: the server logic itself needs no torch
         if not torch.cuda.is_available():
             raise errors.HipError("the tile server needs a ROCm GPU (none visible)")
         self.torch = torch
         self.device = int(device)
+        self.graphs = bool(graphs)
         self._workers = {}
+        self._scenes = {}
+        self._limits = (1 << 24, 256)
+
+    def scratch_copies(self, n_sets: int) -> int:
+        """Per-pixel scratch buffers each worker holds (one per raw set when
+        every set has its own scene)."""
+        return n_sets if self.graphs else 1
+
+    def configure(self, max_batch_pixels: int, max_batch_requests: int):
+        self._limits = (int(max_batch_pixels), int(max_batch_requests))
 
     def start_worker(self, index: int):
         torch = self.torch
@@ -132,8 +148,22 @@ class GpuExecutor:
         self._workers[index] = (stream, h)
 
     def stop_worker(self, index: int):
+        for key in [k for k in self._scenes if k[0] == index]:
+            self._scenes.pop(key).close()
         stream, h = self._workers.pop(index)
         _native.lib().bp_context_destroy(h)
+
+    def _scene(self, index: int, raw: RawSet, precision: str):
+        from .focus import Scene
+        key = (index, id(raw), precision)
+        sc = self._scenes.get(key)
+        if sc is None:
+            sc = Scene(raw.signal, raw.in_geometry, raw.out_orbit, raw.out_doppler, raw.dem,
+                       raw.fc, raw.ds, raw.kernel, raw.dry_tropo_model, raw.rdr2geo_params,
+                       raw.geo2rdr_params, precision=precision, max_pixels=self._limits[0],
+                       max_blocks=self._limits[1])
+            self._scenes[key] = sc
+        return sc
 
     def run(self, index: int, raw: RawSet, requests: Sequence[Request], precision: str,
             options: dict) -> dict:
@@ -144,6 +174,15 @@ class GpuExecutor:
         with torch.cuda.device(self.device), torch.cuda.stream(stream):
             outs = [torch.empty((r.grid.length, r.grid.width), dtype=torch.complex64,
                                 device=f"cuda:{self.device}") for r in requests]
+        if self.graphs:
+            st = {}
+            failed = self._scene(index, raw, precision).focus(
+                outs, [r.grid for r in requests], options=options, stream=stream, stats=st)
+            for r, o, f in zip(requests, outs, failed):
+                r.out = o
+                r.failed = bool(f)
+            return {"updates": int(st["updates"]), "geometry_ms": float(st["geometry_ms"]),
+                    "sum_ms": float(st["sum_ms"])}
         blocks = (_native.OutBlock_t * len(requests))()
         for i, r in enumerate(requests):
             gs = _native.Grid_t(r.grid.sensing_start, r.grid.prf, r.grid.starting_range,
@@ -199,7 +238,10 @@ class Server:
         # options: a dict, or a callable (set_id, requests) -> dict
         self._options = options if options is not None else {}
         self.budget_bytes = int(budget_bytes)
-        self.base_bytes = sum(s.nbytes for s in self.sets) + self.workers * (
+        self.executor = executor if executor is not None else GpuExecutor(device)
+        copies = self.executor.scratch_copies(len(self.sets)) \
+            if hasattr(self.executor, "scratch_copies") else 1
+        self.base_bytes = sum(s.nbytes for s in self.sets) + self.workers * copies * (
             self.max_batch_pixels * RECORD_BYTES_PER_PIXEL[precision])
         if self.base_bytes > self.budget_bytes:
             raise errors.InvalidArgument(
@@ -214,7 +256,8 @@ class Server:
         self._closed = False
         self._batch_seq = 0
         self.batches: List[dict] = []
-        self.executor = executor if executor is not None else GpuExecutor(device)
+        if hasattr(self.executor, "configure"):
+            self.executor.configure(self.max_batch_pixels, self.max_batch_requests)
         self._threads = []
         for i in range(self.workers):
             self.executor.start_worker(i)
