@@ -309,7 +309,7 @@ def run_serve(args, torch, np, dev, tuner):
                 fresh = srv.submit_many([serve.Request(0, r.grid) for r in proto])
                 for r in fresh:
                     r.wait()
-                return sum(b["geometry_ms"] + b["sum_ms"] for b in srv.batches[n0:])
+                return sum(b["step_ms"] for b in srv.batches[n0:])
 
             tuner.tune(key_of(proto), "split", run, force=True)
     log(f"serve: tuned 3 batch classes in {time.time() - t0:.1f}s")

@@ -139,18 +139,20 @@ typedef struct {
     int32_t fit_degree;    /* split: delay polynomial degree (0 = 4; only 4 is built) */
     int32_t order;         /* tile walk of the sum kernels: 0 auto, 1 row-major, 2 column-major */
     int32_t blocks_per_cu; /* persistent 512-thread workgroups per CU (0 = occupancy limit) */
-    int32_t gather;        /* split: 0 stages each pulse's range window in LDS (default),
-                              1 gathers the taps from global memory */
+    int32_t staged;        /* split: 0 gathers the taps from global memory (default),
+                              1 stages each pulse's range window in LDS first */
     int32_t reserved[2];
 } bp_options;
 
 /* Optional per-call measurements (filled when non-null). */
 typedef struct {
     int64_t updates;       /* sum over pixels of (kstop - kstart) */
-    double geometry_ms;    /* device time of the geometry pass (HIP events) */
-    double sum_ms;         /* device time of the coherent-sum kernel */
+    double geometry_ms;    /* device time of the geometry pass (HIP events; 0 in a
+                              graph-replayed scene step, which is timed as a whole) */
+    double sum_ms;         /* device time of the coherent-sum kernel (likewise) */
     int32_t any_failed;    /* some pixel failed to converge */
     int32_t pad_;
+    double step_ms;        /* device time of the whole step */
 } bp_stats;
 
 typedef struct bp_context bp_context;

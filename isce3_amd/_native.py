@@ -76,7 +76,7 @@ class Geo2RdrParams_t(C.Structure):
 
 class Options_t(C.Structure):
     _fields_ = [("tile_rows", C.c_int32), ("chunk", C.c_int32), ("fit_degree", C.c_int32),
-                ("order", C.c_int32), ("blocks_per_cu", C.c_int32), ("gather", C.c_int32),
+                ("order", C.c_int32), ("blocks_per_cu", C.c_int32), ("staged", C.c_int32),
                 ("reserved", C.c_int32 * 2)]
 
 
@@ -87,7 +87,7 @@ class OutBlock_t(C.Structure):
 
 class Stats_t(C.Structure):
     _fields_ = [("updates", C.c_int64), ("geometry_ms", C.c_double), ("sum_ms", C.c_double),
-                ("any_failed", C.c_int32), ("pad_", C.c_int32)]
+                ("any_failed", C.c_int32), ("pad_", C.c_int32), ("step_ms", C.c_double)]
 
 
 _P = C.POINTER

@@ -7,8 +7,8 @@ The tunable options are those of ``bp_options`` (include/isce3_amd_backproject.h
                    2 column-major)
     chunk          pulses per compensated FP32 partial sum (split)
     blocks_per_cu  resident 512-thread workgroups per CU (0 = occupancy limit)
-    gather         split taps from LDS-staged pulse windows (0) or gathered
-                   from global memory (1)
+    staged         split taps gathered from global memory (0) or read from
+                   pulse windows staged in LDS (1)
 
 The plain kernel only has ``tile_rows``.  A tuning run times candidates with
 the device-side HIP-event times that every call reports (geometry + sum),
@@ -24,10 +24,10 @@ import os
 import threading
 from typing import Callable, Dict, Iterable, Optional
 
-DEFAULTS = {"tile_rows": 4, "order": 2, "chunk": 32, "blocks_per_cu": 0, "gather": 0}
+DEFAULTS = {"tile_rows": 8, "order": 2, "chunk": 32, "blocks_per_cu": 0, "staged": 0}
 SPACE = {
     "split": {"tile_rows": (1, 2, 4, 8), "order": (1, 2), "chunk": (16, 32, 64),
-              "blocks_per_cu": (0, 1, 2), "gather": (0, 1)},
+              "blocks_per_cu": (0, 1, 2), "staged": (0, 1)},
     "plain": {"tile_rows": (1, 2, 4, 8)},
 }
 

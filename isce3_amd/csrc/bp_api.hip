@@ -413,8 +413,8 @@ int check_options(const bp_options* options, Opts* o)
     if (opt.blocks_per_cu < 0 || opt.blocks_per_cu > 16)
         return fail(BP_ERR_INVALID_ARGUMENT, "blocks_per_cu must be in [0, 16]");
     o->blocks_per_cu = opt.blocks_per_cu;
-    if (opt.gather != 0 && opt.gather != 1) return fail(BP_ERR_INVALID_ARGUMENT, "gather must be 0 or 1");
-    o->staged = opt.gather == 0;
+    if (opt.staged != 0 && opt.staged != 1) return fail(BP_ERR_INVALID_ARGUMENT, "staged must be 0 or 1");
+    o->staged = opt.staged == 1;
     return BP_SUCCESS;
 }
 
@@ -636,10 +636,14 @@ int finish_step(const int32_t* hflags, int n_grids, const std::vector<int>& bloc
     for (int g = 0; g < n_grids; ++g) blocks[block_of_grid[g]].failed = hflags[kFlagWords + g] != 0;
     if (stats) {
         float ms = 0.f;
-        BP_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-        stats->geometry_ms = ms;
-        BP_HIP_CHECK(hipEventElapsedTime(&ms, ev[1], ev[2]));
-        stats->sum_ms = ms;
+        if (ev[1]) {
+            BP_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            stats->geometry_ms = ms;
+            BP_HIP_CHECK(hipEventElapsedTime(&ms, ev[1], ev[2]));
+            stats->sum_ms = ms;
+        }
+        BP_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[2]));
+        stats->step_ms = ms;
         stats->updates = (int64_t)hupd;
         stats->any_failed = hflags[0] != 0;
     }
@@ -743,6 +747,11 @@ struct bp_scene {
         bool staged;
         hipGraph_t graph;
         hipGraphExec_t exec;
+        // kernel nodes whose grid follows the batch's group count (geometry,
+        // and the plain sum), with their captured launch parameters
+        hipGraphNode_t node[2];
+        hipKernelNodeParams params[2];
+        int n_nodes;
     };
     std::vector<Graph> graphs;
 };
@@ -771,11 +780,13 @@ int capture_graph(bp_scene* sc, const Opts& o, bp_scene::Graph* out)
     bp_context* c = sc->ctx;
     Step sp;
     build_step(sp, sc->st, sc->sc, sc->args, sc->width, o, c->n_cu);
-    const int64_t max_groups = (sc->max_pixels + 255) / 256 + sc->max_blocks;
-    const int cap_grid = (int)std::min<int64_t>(max_groups, (int64_t)c->n_cu * 8);
-    sp.geom_grid = std::max(cap_grid, 1);
+    // the geometry and plain-sum grids are set per replay to the batch's group
+    // count (bp_scene_focus); one workgroup at capture
+    const int cap_grid = 1;
+    sp.geom_grid = cap_grid;
     sp.flag_words = kFlagWords + sc->max_blocks;
-    for (int i = 0; i < 3; ++i) sp.ev[i] = sc->ev[i];
+    // no event records inside the graph: a replay is timed as a whole by
+    // events recorded around the launch
     const size_t desc_bytes = sizeof(bp::BatchHeader) + sizeof(bp::OutDesc) * sc->max_blocks;
 
     hipStream_t s = sc->cap;
@@ -783,7 +794,7 @@ int capture_graph(bp_scene* sc, const Opts& o, bp_scene::Graph* out)
     int rc = BP_SUCCESS;
     hipError_t e = hipMemcpyAsync(sc->sc.desc.ptr, sc->h_desc, desc_bytes, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) rc = fail(BP_ERR_HIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
-    if (rc == BP_SUCCESS) rc = enqueue_step(sp, std::max(cap_grid, 1), s);
+    if (rc == BP_SUCCESS) rc = enqueue_step(sp, cap_grid, s);
     if (rc == BP_SUCCESS) {
         e = hipMemcpyAsync(sc->h_flags, sc->sc.flags.ptr, sizeof(int32_t) * (kFlagWords + sc->max_blocks),
                            hipMemcpyDeviceToHost, s);
@@ -796,6 +807,34 @@ int capture_graph(bp_scene* sc, const Opts& o, bp_scene::Graph* out)
         return rc;
     }
     if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    // find the kernel nodes whose grid follows the batch
+    out->n_nodes = 0;
+    size_t n_nodes = 0;
+    e = hipGraphGetNodes(g, nullptr, &n_nodes);
+    std::vector<hipGraphNode_t> nodes(n_nodes);
+    if (e == hipSuccess && n_nodes) e = hipGraphGetNodes(g, nodes.data(), &n_nodes);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(BP_ERR_HIP, std::string("hipGraphGetNodes: ") + hipGetErrorString(e));
+    }
+    const void* geo_fn = bp::kernel_fn(0, sc->width, o.staged);
+    const void* plain_fn = bp::kernel_fn(1, sc->width, o.staged);
+    for (hipGraphNode_t nd : nodes) {
+        hipGraphNodeType type;
+        if (hipGraphNodeGetType(nd, &type) != hipSuccess || type != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams p{};
+        if (hipGraphKernelNodeGetParams(nd, &p) != hipSuccess) continue;
+        if ((p.func == geo_fn || p.func == plain_fn) && out->n_nodes < 2) {
+            out->node[out->n_nodes] = nd;
+            out->params[out->n_nodes] = p;
+            ++out->n_nodes;
+        }
+    }
+    const int want = sc->args.precision == BP_PRECISION_PLAIN ? 2 : 1;
+    if (out->n_nodes != want) {
+        (void)hipGraphDestroy(g);
+        return fail(BP_ERR_HIP, "captured step is missing its kernel nodes");
+    }
     hipGraphExec_t x = nullptr;
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
@@ -924,10 +963,18 @@ int bp_scene_focus(bp_scene* sc, bp_out_block* blocks, int n_blocks, const bp_op
     if ((rc = plan_batch(blocks, n_blocks, o, static_cast<int32_t*>(sc->sc.flags.ptr) + kFlagWords,
                          sc->h_desc, sc->max_blocks, &hdr, sc->block_of_grid)) != BP_SUCCESS)
         return rc;
+    if (hdr.n_groups > INT32_MAX / 2) return fail(BP_ERR_INVALID_ARGUMENT, "too many output tiles");
+    for (int i = 0; i < g->n_nodes; ++i) {
+        hipKernelNodeParams p = g->params[i];
+        p.gridDim = dim3((unsigned)hdr.n_groups);
+        BP_HIP_CHECK(hipGraphExecKernelNodeSetParams(g->exec, g->node[i], &p));
+    }
+    if (stats) BP_HIP_CHECK(hipEventRecord(sc->ev[0], s));
     BP_HIP_CHECK(hipGraphLaunch(g->exec, s));
+    if (stats) BP_HIP_CHECK(hipEventRecord(sc->ev[2], s));
     BP_HIP_CHECK(hipStreamSynchronize(s));
-    return finish_step(sc->h_flags, hdr.n_grids, sc->block_of_grid, blocks, sc->ev, any_failed,
-                       stats);
+    const hipEvent_t ev[3] = {sc->ev[0], nullptr, sc->ev[2]};
+    return finish_step(sc->h_flags, hdr.n_grids, sc->block_of_grid, blocks, ev, any_failed, stats);
 }
 
 int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_geometry* ig,

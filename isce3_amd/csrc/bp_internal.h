@@ -144,9 +144,9 @@ struct SumArgs {
 };
 
 // Host launchers (defined in bp_kernels.hip).  `grid` is the number of
-// workgroups; the geometry and plain-sum kernels loop over the groups of the
-// batch (one group per workgroup when grid >= n_groups), the split kernel
-// walks the wavefront tiles persistently.
+// workgroups: the geometry and plain-sum kernels take one group of four tiles
+// per workgroup (grid >= n_groups; extra workgroups return at once), the
+// split kernel walks the wavefront tiles persistently.
 hipError_t launch_pulse_orbit(const Orbit& orbit, double t0, double dt, int na, double* pv,
                               int32_t* flags, hipStream_t stream);
 hipError_t launch_clear(int32_t* words, int n, hipStream_t stream);
@@ -155,5 +155,8 @@ hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_
 hipError_t launch_sum_split(const SumArgs& args, int width, int grid, bool staged,
                             hipStream_t stream);
 int sum_blocks_per_cu(int precision, int width, bool staged);
+// Host-side handle of a kernel (0 geometry, 1 plain sum, 2 split sum), as
+// graph kernel nodes report it.
+const void* kernel_fn(int which, int width, bool staged);
 
 }  // namespace bp

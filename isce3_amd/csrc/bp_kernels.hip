@@ -345,17 +345,20 @@ __device__ inline void geometry_tile(const GeomArgs& a, const OutDesc* grids, in
     a.split[rec] = sp;
 }
 
+// One workgroup per group of four tiles (workgroups past the batch's groups
+// return at once).  No persistent loop: this kernel needs ~220 VGPRs, and a
+// loop around its body makes the compiler keep the arguments live across
+// iterations and spill.
 __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
 {
+    const int64_t ng = uniform64(a.batch->n_groups);
+    if ((int64_t)blockIdx.x >= ng) return;
     const OutDesc* grids = batch_grids(a.batch);
     const int n_grids = __builtin_amdgcn_readfirstlane(a.batch->n_grids);
     const int64_t n_seq = uniform64(a.batch->n_seq);
-    const int64_t ng = uniform64(a.batch->n_groups);
-    for (int64_t it = blockIdx.x; it < ng; it += gridDim.x) {
-        int lane;
-        const int64_t gseq = group_tile(it, ng, &lane);
-        geometry_tile(a, grids, n_grids, n_seq, gseq, lane);
-    }
+    int lane;
+    const int64_t gseq = group_tile(blockIdx.x, ng, &lane);
+    geometry_tile(a, grids, n_grids, n_seq, gseq, lane);
 }
 
 // Persistent tile schedule of the sum kernels.  A workgroup holds 8
@@ -464,12 +467,14 @@ __device__ inline void plain_tile(const SumArgs& a, const float* tab, const OutD
     }
 }
 
-// One 256-thread workgroup per four tiles when launched with one workgroup per
-// group (measured faster than the persistent walk of the split kernel for
-// this FP64-bound kernel, which needs ~100 VGPRs); smaller grids loop.
+// One 256-thread workgroup per group of four tiles (measured faster than the
+// persistent walk of the split kernel for this FP64-bound kernel, which needs
+// ~90 VGPRs); workgroups past the batch's groups return at once.
 template<int W>
 __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
 {
+    const int64_t ng = uniform64(a.batch->n_groups);
+    if ((int64_t)blockIdx.x >= ng) return;
     extern __shared__ float s_tab[];
     const bool tab_in_lds = a.ntab <= kMaxTableLds;
     if (tab_in_lds)
@@ -479,14 +484,11 @@ __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
     const OutDesc* grids = batch_grids(a.batch);
     const int n_grids = __builtin_amdgcn_readfirstlane(a.batch->n_grids);
     const int64_t n_seq = uniform64(a.batch->n_seq);
-    const int64_t ng = uniform64(a.batch->n_groups);
-    for (int64_t it = blockIdx.x; it < ng; it += gridDim.x) {
-        int lane;
-        const int64_t gseq = group_tile(it, ng, &lane);
-        if (gseq < n_seq) {
-            const OutDesc d = load_desc(grids, find_grid(grids, n_grids, gseq));
-            plain_tile<W>(a, tab, d, gseq - d.seq_base, lane);
-        }
+    int lane;
+    const int64_t gseq = group_tile(blockIdx.x, ng, &lane);
+    if (gseq < n_seq) {
+        const OutDesc d = load_desc(grids, find_grid(grids, n_grids, gseq));
+        plain_tile<W>(a, tab, d, gseq - d.seq_base, lane);
     }
 }
 
@@ -954,6 +956,21 @@ int sum_blocks_per_cu(int precision, int width, bool staged)
 
 // Persistent grid: `grid` workgroups of 512 threads (a multiple of 8, one
 // share per XCD), sized by the caller from the device's CU count and occupancy.
+const void* kernel_fn(int which, int width, bool staged)
+{
+    switch (which) {
+    case 0: return reinterpret_cast<const void*>(&geometry_kernel);
+    case 1: return width == 8 ? reinterpret_cast<const void*>(&sum_plain_kernel<8>)
+                              : reinterpret_cast<const void*>(&sum_plain_kernel<9>);
+    default:
+        if (width == 8)
+            return staged ? reinterpret_cast<const void*>(&sum_split_kernel<8, true>)
+                          : reinterpret_cast<const void*>(&sum_split_kernel<8, false>);
+        return staged ? reinterpret_cast<const void*>(&sum_split_kernel<9, true>)
+                      : reinterpret_cast<const void*>(&sum_split_kernel<9, false>);
+    }
+}
+
 hipError_t launch_sum_split(const SumArgs& args, int width, int grid, bool staged,
                             hipStream_t stream)
 {

@@ -181,8 +181,7 @@ class GpuExecutor:
             for r, o, f in zip(requests, outs, failed):
                 r.out = o
                 r.failed = bool(f)
-            return {"updates": int(st["updates"]), "geometry_ms": float(st["geometry_ms"]),
-                    "sum_ms": float(st["sum_ms"])}
+            return {"updates": int(st["updates"]), "step_ms": float(st["step_ms"])}
         blocks = (_native.OutBlock_t * len(requests))()
         for i, r in enumerate(requests):
             gs = _native.Grid_t(r.grid.sensing_start, r.grid.prf, r.grid.starting_range,
@@ -210,8 +209,7 @@ class GpuExecutor:
         for i, r in enumerate(requests):
             r.out = outs[i]
             r.failed = bool(blocks[i].failed)
-        return {"updates": int(st.updates), "geometry_ms": float(st.geometry_ms),
-                "sum_ms": float(st.sum_ms)}
+        return {"updates": int(st.updates), "step_ms": float(st.step_ms)}
 
 
 class Server:

@@ -79,6 +79,6 @@ def test_staged_windows_equal_global_gathers(torch_gpu, config, tile_rows):
     64-column tiles (tile_rows 1) span more range bins than one staged window
     holds and exercise the per-pulse fallback to global gathers."""
     print(f"[staged vs gather] config={config} tile_rows={tile_rows}", flush=True)
-    staged, _, _, _ = run_gpu(config, "split", options={"tile_rows": tile_rows, "gather": 0})
-    gathered, _, _, _ = run_gpu(config, "split", options={"tile_rows": tile_rows, "gather": 1})
+    staged, _, _, _ = run_gpu(config, "split", options={"tile_rows": tile_rows, "staged": 1})
+    gathered, _, _, _ = run_gpu(config, "split", options={"tile_rows": tile_rows, "staged": 0})
     np.testing.assert_array_equal(staged, gathered)

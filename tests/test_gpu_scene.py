@@ -78,7 +78,7 @@ def test_new_launch_options_capture_a_new_graph(torch_gpu):
     with Scene(raw_dev, sc.in_geometry(), sc.orbit, sc.doppler, sc.dem, sc.fc, sc.ds,
                sc.kernel, sc.dry_tropo_model, precision="split", max_pixels=1 << 14,
                max_blocks=4) as scene:
-        for opts in ({"tile_rows": 1}, {"tile_rows": 8, "chunk": 16, "gather": 1},
+        for opts in ({"tile_rows": 1}, {"tile_rows": 8, "chunk": 16, "staged": 1},
                      {"tile_rows": 4, "blocks_per_cu": 1}, {"tile_rows": 1}):
             print(f"[scene] options={opts}", flush=True)
             ref = eager(sc, raw_dev, grids, "split", opts)

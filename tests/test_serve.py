@@ -52,7 +52,7 @@ class FakeExecutor:
         for r in requests:
             r.out = ("tile", r.tag)
             r.failed = False
-        return {"updates": sum(r.pixels for r in requests), "geometry_ms": 0.0, "sum_ms": 0.0}
+        return {"updates": sum(r.pixels for r in requests), "step_ms": 0.0}
 
 
 SCRATCH = 4096 * serve.RECORD_BYTES_PER_PIXEL["split"]  # one worker, 4096-pixel batches
