@@ -141,7 +141,10 @@ typedef struct {
     int32_t blocks_per_cu; /* persistent 512-thread workgroups per CU (0 = occupancy limit) */
     int32_t staged;        /* split: 0 gathers the taps from global memory (default),
                               1 stages each pulse's range window in LDS first */
-    int32_t reserved[2];
+    int32_t lane_rows;     /* output lines per lane of the split sum: 2 (adjacent lines
+                              share their taps; kernel width 8, staged 0 only), 1, or 0 =
+                              2 where offered, else 1 (default) */
+    int32_t reserved[1];
 } bp_options;
 
 /* Optional per-call measurements (filled when non-null). */

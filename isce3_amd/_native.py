@@ -77,7 +77,7 @@ class Geo2RdrParams_t(C.Structure):
 class Options_t(C.Structure):
     _fields_ = [("tile_rows", C.c_int32), ("chunk", C.c_int32), ("fit_degree", C.c_int32),
                 ("order", C.c_int32), ("blocks_per_cu", C.c_int32), ("staged", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("lane_rows", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class OutBlock_t(C.Structure):

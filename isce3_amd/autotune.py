@@ -9,6 +9,10 @@ The tunable options are those of ``bp_options`` (include/isce3_amd_backproject.h
     blocks_per_cu  resident 512-thread workgroups per CU (0 = occupancy limit)
     staged         split taps gathered from global memory (0) or read from
                    pulse windows staged in LDS (1)
+    lane_rows      output lines per lane of the split sum (2: adjacent lines
+                   share one gather; kernel width 8 with staged 0 only --
+                   other combinations raise InvalidArgument and score as
+                   infinitely slow)
 
 The plain kernel only has ``tile_rows``.  A tuning run times candidates with
 the device-side HIP-event times that every call reports (geometry + sum),
@@ -24,10 +28,11 @@ import os
 import threading
 from typing import Callable, Dict, Iterable, Optional
 
-DEFAULTS = {"tile_rows": 8, "order": 2, "chunk": 32, "blocks_per_cu": 0, "staged": 0}
+DEFAULTS = {"tile_rows": 8, "order": 2, "chunk": 32, "blocks_per_cu": 0, "staged": 0,
+            "lane_rows": 2}
 SPACE = {
     "split": {"tile_rows": (1, 2, 4, 8), "order": (1, 2), "chunk": (16, 32, 64),
-              "blocks_per_cu": (0, 1, 2), "staged": (0, 1)},
+              "blocks_per_cu": (0, 1, 2), "staged": (0, 1), "lane_rows": (1, 2)},
     "plain": {"tile_rows": (1, 2, 4, 8)},
 }
 
@@ -140,7 +145,10 @@ class Autotuner:
         def score(opts):
             sig = tuple(sorted(opts.items()))
             if sig not in seen:
-                seen[sig] = min(float(run(dict(opts))) for _ in range(self.repeats))
+                try:
+                    seen[sig] = min(float(run(dict(opts))) for _ in range(self.repeats))
+                except ValueError:  # InvalidArgument: a combination the kernels do not offer
+                    seen[sig] = math.inf
                 log.append((dict(opts), seen[sig]))
             return seen[sig]
 

@@ -36,8 +36,8 @@ struct DevBuf {
 // blocks: orbits, Doppler tables, DEM, kernel tables and (plain precision)
 // the platform state at every pulse.
 struct StaticData {
-    DevBuf table, poly, in_pos, in_vel, out_pos, out_vel, in_dop, out_dop, dem, pulse_pv;
-    std::vector<float> poly_host;
+    DevBuf table, poly, poly2, in_pos, in_vel, out_pos, out_vel, in_dop, out_dop, dem, pulse_pv;
+    std::vector<float> poly_host, poly2_host;
 };
 
 // Per-pixel records, batch header + output descriptors, and the flags of one
@@ -199,18 +199,19 @@ bp::Dem host_dem(const bp_dem& m)
     return d;
 }
 
-bp::TileMap make_map(int64_t length, int64_t width, int tile_rows, int order)
+bp::TileMap make_map(int64_t length, int64_t width, int tile_rows, int order, int lane_rows)
 {
     bp::TileMap m;
     m.tile_rows = tile_rows;
     m.tile_cols = 64 / tile_rows;
+    m.lane_rows = lane_rows;
+    const int64_t rows = (int64_t)tile_rows * lane_rows;
     m.tiles_x = (int32_t)((width + m.tile_cols - 1) / m.tile_cols);
-    m.tiles_y = (int32_t)((length + m.tile_rows - 1) / m.tile_rows);
+    m.tiles_y = (int32_t)((length + rows - 1) / rows);
     m.length = length;
     m.width = width;
     m.n_tiles = (int64_t)m.tiles_x * m.tiles_y;
     m.order = order;
-    m.pad_ = 0;
     return m;
 }
 
@@ -247,6 +248,44 @@ void build_polyphase(const bp_kernel& k, int width, std::vector<float>& out)
             const float w1 = q < Q ? (float)w[(size_t)(q + 1) * width + m] : w0;
             out[(size_t)q * row + m] = w0;
             out[(size_t)q * row + wp + m] = w1 - w0;
+        }
+    }
+}
+
+// Two-line weight table (W = 8): kPairPolyRows + 1 rows of 9 entries at
+// fractional offsets q / kPairPolyRows, in two copies: copy 0 holds the 8
+// weights in entries 0-7 and a zero in entry 8, copy 1 a zero in entry 0 and
+// the weights in entries 1-8.  A row is its 9 entries, the 9 differences to
+// the next row of the same copy, and 2 floats of padding.
+void build_pair_table(const bp_kernel& k, std::vector<float>& out)
+{
+    constexpr int W = 8;
+    const int Q = bp::kPairPolyRows;
+    const double hw = k.halfwidth;
+    const double dx = hw / (k.n - 1.0);
+    const float rdx = (float)(1.0 / dx);
+    auto weight = [&](double x) {
+        double ax = std::fabs(x);
+        if (ax > hw) return 0.0;
+        double axn = ax * (double)rdx;
+        int i = std::min((int)std::floor(axn), k.n - 2);
+        return (double)k.table[i] + (axn - i) * (double)(k.table[i + 1] - k.table[i]);
+    };
+    std::vector<float> rows((size_t)(Q + 1) * W);
+    for (int q = 0; q <= Q; ++q)
+        for (int m = 0; m < W; ++m) rows[(size_t)q * W + m] = (float)weight((m - W / 2) + (double)q / Q);
+    out.assign((size_t)bp::kPairTableFloats, 0.f);
+    for (int o = 0; o < 2; ++o) {
+        for (int q = 0; q <= Q; ++q) {
+            float* r = &out[((size_t)o * (Q + 1) + q) * bp::kPairRowFloats];
+            for (int e = 0; e < 9; ++e) {
+                const int m = e - o;
+                if (m < 0 || m >= W) continue;
+                const float w0 = rows[(size_t)q * W + m];
+                const float w1 = q < Q ? rows[(size_t)(q + 1) * W + m] : w0;
+                r[e] = w0;
+                r[9 + e] = w1 - w0;
+            }
         }
     }
 }
@@ -293,8 +332,8 @@ namespace {
 
 void free_static(StaticData& s)
 {
-    DevBuf* bufs[] = {&s.table, &s.poly, &s.in_pos, &s.in_vel, &s.out_pos, &s.out_vel,
-                      &s.in_dop, &s.out_dop, &s.dem, &s.pulse_pv};
+    DevBuf* bufs[] = {&s.table, &s.poly, &s.poly2, &s.in_pos, &s.in_vel, &s.out_pos,
+                      &s.out_vel, &s.in_dop, &s.out_dop, &s.dem, &s.pulse_pv};
     for (DevBuf* b : bufs)
         if (b->ptr) (void)hipFree(b->ptr);
 }
@@ -343,7 +382,7 @@ struct CallArgs {
 
 // Validated launch options.
 struct Opts {
-    int tile_rows, chunk, order, blocks_per_cu;
+    int tile_rows, chunk, order, blocks_per_cu, lane_rows;
     bool staged;
 };
 
@@ -415,6 +454,20 @@ int check_options(const bp_options* options, Opts* o)
     o->blocks_per_cu = opt.blocks_per_cu;
     if (opt.staged != 0 && opt.staged != 1) return fail(BP_ERR_INVALID_ARGUMENT, "staged must be 0 or 1");
     o->staged = opt.staged == 1;
+    if (opt.lane_rows < 0 || opt.lane_rows > 2) return fail(BP_ERR_INVALID_ARGUMENT, "lane_rows must be 0, 1 or 2");
+    o->lane_rows = opt.lane_rows;  // 0: resolved by check_lane_rows
+    return BP_SUCCESS;
+}
+
+// Two lines per lane need the split kernel of width 8 with gathered taps;
+// lane_rows 0 takes them where offered.
+int check_lane_rows(Opts& o, int precision, int width)
+{
+    const bool offered = precision == BP_PRECISION_SPLIT && width == 8 && !o.staged;
+    if (o.lane_rows == 0) o.lane_rows = offered ? 2 : 1;
+    if (o.lane_rows == 2 && !offered)
+        return fail(BP_ERR_INVALID_ARGUMENT,
+                    "lane_rows 2 needs split precision, kernel width 8 and staged 0");
     return BP_SUCCESS;
 }
 
@@ -453,7 +506,7 @@ int plan_batch(const bp_out_block* blocks, int n_blocks, const Opts& o, int32_t*
         if (g >= max_grids) return fail(BP_ERR_INVALID_ARGUMENT, "too many output blocks");
         bp::OutDesc d;
         std::memset(&d, 0, sizeof(d));
-        d.map = make_map(go.length, go.width, o.tile_rows, o.order);
+        d.map = make_map(go.length, go.width, o.tile_rows, o.order, o.lane_rows);
         d.t0 = go.sensing_start;
         d.dt = 1.0 / go.prf;
         d.r0 = go.starting_range;
@@ -473,8 +526,10 @@ int plan_batch(const bp_out_block* blocks, int n_blocks, const Opts& o, int32_t*
     std::memset(hdr, 0, sizeof(*hdr));
     hdr->n_seq = n_seq;
     hdr->n_groups = (n_seq + 3) / 4;
+    hdr->n_geom_groups = (n_seq * o.lane_rows + 3) / 4;
     hdr->n_grids = g;
-    if (hdr->n_groups > INT32_MAX) return fail(BP_ERR_INVALID_ARGUMENT, "too many output pixels");
+    hdr->lane_rows = o.lane_rows;
+    if (hdr->n_geom_groups > INT32_MAX / 2) return fail(BP_ERR_INVALID_ARGUMENT, "too many output pixels");
     std::memcpy(dst, hdr, sizeof(*hdr));
     return BP_SUCCESS;
 }
@@ -505,6 +560,11 @@ int upload_static(StaticData& st, const CallArgs& a, int width, hipStream_t s)
         build_polyphase(*a.kernel, width, st.poly_host);
         if ((rc = upload(st.poly, st.poly_host.data(), sizeof(float) * st.poly_host.size(), s)))
             return rc;
+        if (width == 8) {
+            build_pair_table(*a.kernel, st.poly2_host);
+            if ((rc = upload(st.poly2, st.poly2_host.data(), sizeof(float) * st.poly2_host.size(), s)))
+                return rc;
+        }
     }
     return BP_SUCCESS;
 }
@@ -528,8 +588,8 @@ struct Step {
     bp::GeomArgs ga;
     bp::SumArgs sa;
     int width, precision;
-    bool staged;
-    int geom_grid, sum_grid;
+    bool staged, pair;
+    int geom_grid, fit_grid, sum_grid;
     int32_t* flags;
     int flag_words;
     hipEvent_t ev[3];  // null: no timing
@@ -598,14 +658,18 @@ void build_step(Step& sp, const StaticData& st, const Scratch& sc, const CallArg
     sa.split = static_cast<const bp::PixSplit*>(sc.split.ptr);
     sa.table = static_cast<const float*>(st.table.ptr);
     sa.poly = static_cast<const float*>(st.poly.ptr);
+    sa.poly2 = static_cast<const float*>(st.poly2.ptr);
+    sa.zscale2 = (float)(-(double)bp::kPairPolyRows * sa.inv_k);
 
     sp.width = width;
     sp.precision = a.precision;
     sp.staged = o.staged;
+    sp.pair = o.lane_rows == 2;
     const int per_cu = o.blocks_per_cu > 0 ? o.blocks_per_cu
-                                           : bp::sum_blocks_per_cu(a.precision, width, o.staged);
+                                           : bp::sum_blocks_per_cu(a.precision, width, o.staged, sp.pair);
     sp.sum_grid = (n_cu * per_cu + 7) / 8 * 8;  // persistent split grid
     sp.geom_grid = 0;
+    sp.fit_grid = 0;
     sp.flags = d_flags;
     sp.flag_words = 0;
     sp.ev[0] = sp.ev[1] = sp.ev[2] = nullptr;
@@ -618,11 +682,12 @@ int enqueue_step(const Step& sp, int plain_grid, hipStream_t s)
     BP_HIP_CHECK(bp::launch_clear(sp.flags, sp.flag_words, s));
     if (sp.ev[0]) BP_HIP_CHECK(hipEventRecord(sp.ev[0], s));
     BP_HIP_CHECK(bp::launch_geometry(sp.ga, sp.geom_grid, s));
+    if (sp.precision == BP_PRECISION_SPLIT) BP_HIP_CHECK(bp::launch_fit(sp.ga, sp.fit_grid, s));
     if (sp.ev[1]) BP_HIP_CHECK(hipEventRecord(sp.ev[1], s));
     if (sp.precision == BP_PRECISION_PLAIN)
         BP_HIP_CHECK(bp::launch_sum_plain(sp.sa, sp.width, plain_grid, s));
     else
-        BP_HIP_CHECK(bp::launch_sum_split(sp.sa, sp.width, sp.sum_grid, sp.staged, s));
+        BP_HIP_CHECK(bp::launch_sum_split(sp.sa, sp.width, sp.sum_grid, sp.staged, sp.pair, s));
     if (sp.ev[2]) BP_HIP_CHECK(hipEventRecord(sp.ev[2], s));
     return BP_SUCCESS;
 }
@@ -668,6 +733,7 @@ int run_blocks(bp_context* c, const CallArgs& a, bp_out_block* blocks, int n_blo
     if ((rc = check_blocks(blocks, n_blocks, &npix)) != BP_SUCCESS) return rc;
     Opts o;
     if ((rc = check_options(options, &o)) != BP_SUCCESS) return rc;
+    if ((rc = check_lane_rows(o, a.precision, width)) != BP_SUCCESS) return rc;
     const bp_grid& gi = a.in_geometry->grid;
     if (npix > 0 && gi.length * gi.width > 0 && a.in_dev == nullptr)
         return fail(BP_ERR_INVALID_ARGUMENT, "in is null");
@@ -695,7 +761,8 @@ int run_blocks(bp_context* c, const CallArgs& a, bp_out_block* blocks, int n_blo
 
     Step sp;
     build_step(sp, c->st, c->sc, a, width, o, c->n_cu);
-    sp.geom_grid = (int)hdr.n_groups;
+    sp.geom_grid = (int)hdr.n_geom_groups;
+    sp.fit_grid = (int)hdr.n_groups;
     sp.flag_words = 0;  // cleared above, before the pulse states
     if (stats)
         for (int i = 0; i < 3; ++i) sp.ev[i] = c->ev[i];
@@ -743,14 +810,16 @@ struct bp_scene {
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     std::vector<int> block_of_grid;
     struct Graph {
-        int chunk, blocks_per_cu;
+        int chunk, blocks_per_cu, lane_rows;
         bool staged;
         hipGraph_t graph;
         hipGraphExec_t exec;
-        // kernel nodes whose grid follows the batch's group count (geometry,
-        // and the plain sum), with their captured launch parameters
+        // kernel nodes whose grid follows the batch (geometry: geometry
+        // groups; fit and plain sum: tile groups), with their captured
+        // launch parameters
         hipGraphNode_t node[2];
         hipKernelNodeParams params[2];
+        bool geom_groups[2];
         int n_nodes;
     };
     std::vector<Graph> graphs;
@@ -780,10 +849,11 @@ int capture_graph(bp_scene* sc, const Opts& o, bp_scene::Graph* out)
     bp_context* c = sc->ctx;
     Step sp;
     build_step(sp, sc->st, sc->sc, sc->args, sc->width, o, c->n_cu);
-    // the geometry and plain-sum grids are set per replay to the batch's group
-    // count (bp_scene_focus); one workgroup at capture
+    // the geometry, fit and plain-sum grids are set per replay to the batch's
+    // group counts (bp_scene_focus); one workgroup at capture
     const int cap_grid = 1;
     sp.geom_grid = cap_grid;
+    sp.fit_grid = cap_grid;
     sp.flag_words = kFlagWords + sc->max_blocks;
     // no event records inside the graph: a replay is timed as a whole by
     // events recorded around the launch
@@ -817,20 +887,24 @@ int capture_graph(bp_scene* sc, const Opts& o, bp_scene::Graph* out)
         (void)hipGraphDestroy(g);
         return fail(BP_ERR_HIP, std::string("hipGraphGetNodes: ") + hipGetErrorString(e));
     }
-    const void* geo_fn = bp::kernel_fn(0, sc->width, o.staged);
-    const void* plain_fn = bp::kernel_fn(1, sc->width, o.staged);
+    const bool pair = o.lane_rows == 2;
+    const void* geo_fn = bp::kernel_fn(0, sc->width, o.staged, pair);
+    const void* tile_fn = sc->args.precision == BP_PRECISION_PLAIN
+                              ? bp::kernel_fn(1, sc->width, o.staged, pair)
+                              : bp::kernel_fn(3, sc->width, o.staged, pair);
     for (hipGraphNode_t nd : nodes) {
         hipGraphNodeType type;
         if (hipGraphNodeGetType(nd, &type) != hipSuccess || type != hipGraphNodeTypeKernel) continue;
         hipKernelNodeParams p{};
         if (hipGraphKernelNodeGetParams(nd, &p) != hipSuccess) continue;
-        if ((p.func == geo_fn || p.func == plain_fn) && out->n_nodes < 2) {
+        if ((p.func == geo_fn || p.func == tile_fn) && out->n_nodes < 2) {
             out->node[out->n_nodes] = nd;
             out->params[out->n_nodes] = p;
+            out->geom_groups[out->n_nodes] = p.func == geo_fn;
             ++out->n_nodes;
         }
     }
-    const int want = sc->args.precision == BP_PRECISION_PLAIN ? 2 : 1;
+    const int want = 2;
     if (out->n_nodes != want) {
         (void)hipGraphDestroy(g);
         return fail(BP_ERR_HIP, "captured step is missing its kernel nodes");
@@ -843,6 +917,7 @@ int capture_graph(bp_scene* sc, const Opts& o, bp_scene::Graph* out)
     }
     out->chunk = o.chunk;
     out->blocks_per_cu = o.blocks_per_cu;
+    out->lane_rows = o.lane_rows;
     out->staged = o.staged;
     out->graph = g;
     out->exec = x;
@@ -944,13 +1019,16 @@ int bp_scene_focus(bp_scene* sc, bp_out_block* blocks, int n_blocks, const bp_op
                                                  " pixels exceeds the scene's max_pixels");
     Opts o;
     if ((rc = check_options(options, &o)) != BP_SUCCESS) return rc;
+    if ((rc = check_lane_rows(o, sc->args.precision, sc->width)) != BP_SUCCESS) return rc;
     if (npix == 0) return BP_SUCCESS;
     BP_HIP_CHECK(hipSetDevice(sc->ctx->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
 
     bp_scene::Graph* g = nullptr;
     for (auto& x : sc->graphs)
-        if (x.chunk == o.chunk && x.blocks_per_cu == o.blocks_per_cu && x.staged == o.staged) g = &x;
+        if (x.chunk == o.chunk && x.blocks_per_cu == o.blocks_per_cu && x.staged == o.staged &&
+            x.lane_rows == o.lane_rows)
+            g = &x;
     if (g == nullptr) {
         bp_scene::Graph ng{};
         if ((rc = capture_graph(sc, o, &ng)) != BP_SUCCESS) return rc;
@@ -963,10 +1041,9 @@ int bp_scene_focus(bp_scene* sc, bp_out_block* blocks, int n_blocks, const bp_op
     if ((rc = plan_batch(blocks, n_blocks, o, static_cast<int32_t*>(sc->sc.flags.ptr) + kFlagWords,
                          sc->h_desc, sc->max_blocks, &hdr, sc->block_of_grid)) != BP_SUCCESS)
         return rc;
-    if (hdr.n_groups > INT32_MAX / 2) return fail(BP_ERR_INVALID_ARGUMENT, "too many output tiles");
     for (int i = 0; i < g->n_nodes; ++i) {
         hipKernelNodeParams p = g->params[i];
-        p.gridDim = dim3((unsigned)hdr.n_groups);
+        p.gridDim = dim3((unsigned)(g->geom_groups[i] ? hdr.n_geom_groups : hdr.n_groups));
         BP_HIP_CHECK(hipGraphExecKernelNodeSetParams(g->exec, g->node[i], &p));
     }
     if (stats) BP_HIP_CHECK(hipEventRecord(sc->ev[0], s));

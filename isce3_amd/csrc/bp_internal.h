@@ -62,16 +62,25 @@ __host__ __device__ constexpr int poly_row_floats(int w)
     return 2 * ((w + 3) / 4 * 4) + (((w + 3) / 4 * 4) % 8 == 0 ? 4 : 0);
 }
 
-// Pixel tiling of one output grid: each 64-lane wavefront owns a tile of
-// tile_rows x (64/tile_rows) output pixels.
+// Pixel tiling of one output grid: each 64-lane wavefront of the sum owns a
+// tile of (tile_rows * lane_rows) x (64 / tile_rows) output pixels; lane l
+// holds the lane_rows vertically adjacent pixels of lane row l / tile_cols and
+// column l % tile_cols.
 struct TileMap {
     int32_t tile_rows, tile_cols;
     int32_t tiles_x, tiles_y;
     int64_t length, width;      // output grid
     int64_t n_tiles;            // wavefront tiles of this grid
     int32_t order;              // sum-kernel tile sequence: 0 row-major, 1 column-major
-    int32_t pad_;
+    int32_t lane_rows;          // output lines per lane: 1, or 2 (split sum, W = 8)
 };
+
+// Rows of the two-line weight table (split sum with lane_rows = 2): 9 taps
+// per row at Q = 2^8 fractional offsets, in two copies shifted by one tap.
+constexpr int kPairPolyLog2 = 8;
+constexpr int kPairPolyRows = 1 << kPairPolyLog2;
+constexpr int kPairRowFloats = 20;   // 9 weights, 9 deltas, 2 padding (80 bytes)
+constexpr int kPairTableFloats = 2 * (kPairPolyRows + 1) * kPairRowFloats;
 
 // One output grid of a (batched) call.  Wavefront tiles of all grids are
 // numbered consecutively: grid g owns [seq_base, seq_base + map.n_tiles).
@@ -93,9 +102,10 @@ struct OutDesc {
 // captured step replays with new counts.
 struct BatchHeader {
     int64_t n_seq;              // wavefront tiles of all grids
-    int64_t n_groups;           // groups of four tiles (geometry / plain-sum workgroups)
+    int64_t n_groups;           // groups of four tiles (plain-sum and fit workgroups)
+    int64_t n_geom_groups;      // groups of four geometry wavefronts (lane_rows per tile)
     int32_t n_grids;
-    int32_t pad_[3];
+    int32_t lane_rows;
 };
 static_assert(sizeof(BatchHeader) == 32, "BatchHeader layout");
 
@@ -141,6 +151,9 @@ struct SumArgs {
     const PixSplit* split;
     const float* table;         // tabulated kernel (plain)
     const float* poly;          // polyphase weights (split): (rows+1) x poly_row_floats(W)
+    const float* poly2;         // two-line weights (split, lane_rows 2): kPairTableFloats
+    float zscale2;              // -2^kPairPolyLog2 * inv_k
+    int32_t pad2_;
 };
 
 // Host launchers (defined in bp_kernels.hip).  `grid` is the number of
@@ -151,12 +164,13 @@ hipError_t launch_pulse_orbit(const Orbit& orbit, double t0, double dt, int na, 
                               int32_t* flags, hipStream_t stream);
 hipError_t launch_clear(int32_t* words, int n, hipStream_t stream);
 hipError_t launch_geometry(const GeomArgs& args, int grid, hipStream_t stream);
+hipError_t launch_fit(const GeomArgs& args, int grid, hipStream_t stream);
 hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_t stream);
-hipError_t launch_sum_split(const SumArgs& args, int width, int grid, bool staged,
+hipError_t launch_sum_split(const SumArgs& args, int width, int grid, bool staged, bool pair,
                             hipStream_t stream);
-int sum_blocks_per_cu(int precision, int width, bool staged);
-// Host-side handle of a kernel (0 geometry, 1 plain sum, 2 split sum), as
-// graph kernel nodes report it.
-const void* kernel_fn(int which, int width, bool staged);
+int sum_blocks_per_cu(int precision, int width, bool staged, bool pair);
+// Host-side handle of a kernel (0 geometry, 1 plain sum, 2 split sum, 3 fit),
+// as graph kernel nodes report it.
+const void* kernel_fn(int which, int width, bool staged, bool pair);
 
 }  // namespace bp

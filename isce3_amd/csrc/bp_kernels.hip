@@ -83,12 +83,13 @@ __device__ inline OutDesc load_desc(const OutDesc* g, int i)
     return d;
 }
 
-__device__ inline bool tile_pixel(const TileMap& m, int64_t tile, int lane, int64_t* row,
+// Pixel j (0 <= j < lane_rows) of `lane` in wavefront tile `tile`.
+__device__ inline bool tile_pixel(const TileMap& m, int64_t tile, int lane, int j, int64_t* row,
                                   int64_t* col)
 {
     if (tile >= m.n_tiles) return false;
     const int64_t ty = tile / m.tiles_x, tx = tile % m.tiles_x;
-    *row = ty * m.tile_rows + lane / m.tile_cols;
+    *row = (ty * m.tile_rows + lane / m.tile_cols) * m.lane_rows + j;
     *col = tx * m.tile_cols + lane % m.tile_cols;
     return *row < m.length && *col < m.width;
 }
@@ -132,14 +133,17 @@ __global__ void clear_kernel(int32_t* words, int n)
         words[i] = 0;
 }
 
-// Per-pixel geometry of one wavefront tile, following Backproject.cpp:127-199.
+// Per-pixel geometry of geometry wavefront `gw`: pixel gw % lane_rows of
+// every lane of sum tile gw / lane_rows, following Backproject.cpp:127-199.
 __device__ inline void geometry_tile(const GeomArgs& a, const OutDesc* grids, int n_grids,
-                                     int64_t n_seq, int64_t gseq, int lane)
+                                     int64_t n_geom, int lane_rows, int64_t gw, int lane)
 {
-    const bool in_seq = gseq < n_seq;
+    const bool in_seq = gw < n_geom;
+    const int64_t gseq = gw / lane_rows;
+    const int j = (int)(gw - gseq * lane_rows);
     const OutDesc d = load_desc(grids, in_seq ? find_grid(grids, n_grids, gseq) : 0);
     int64_t row = 0, col = 0;
-    const bool valid = in_seq && tile_pixel(d.map, gseq - d.seq_base, lane, &row, &col);
+    const bool valid = in_seq && tile_pixel(d.map, gseq - d.seq_base, lane, j, &row, &col);
     const int64_t pix = row * d.map.width + col;
     const int64_t rec = d.rec_base + pix;
 
@@ -210,13 +214,32 @@ __device__ inline void geometry_tile(const GeomArgs& a, const OutDesc* grids, in
     const long long wsum = wave_sum_ll(nupd);
     if (lane == 0 && wsum > 0) atomicAdd(a.updates, (unsigned long long)wsum);
 
-    if (!a.want_split) return;
+}
 
-    // Split mode: fit the pulse phase (cycles) over the wavefront's union
-    // window with a degree-4 Chebyshev interpolant evaluated in FP64.
-    const int kw0 = wave_min_i(has_win ? g.kstart : INT_MAX);
-    const int kw1 = wave_max_i(has_win ? g.kstop : INT_MIN);
-    if (!valid) return;
+// One workgroup per group of four tiles (workgroups past the batch's groups
+// return at once).  No persistent loop: this kernel needs ~220 VGPRs, and a
+// loop around its body makes the compiler keep the arguments live across
+// iterations and spill.
+__global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
+{
+    const int64_t ng = uniform64(a.batch->n_geom_groups);
+    if ((int64_t)blockIdx.x >= ng) return;
+    const OutDesc* grids = batch_grids(a.batch);
+    const int n_grids = __builtin_amdgcn_readfirstlane(a.batch->n_grids);
+    const int lane_rows = __builtin_amdgcn_readfirstlane(a.batch->lane_rows);
+    const int64_t n_geom = uniform64(a.batch->n_seq) * lane_rows;
+    int lane;
+    const int64_t gw = group_tile(blockIdx.x, ng, &lane);
+    geometry_tile(a, grids, n_grids, n_geom, lane_rows, gw, lane);
+}
+
+// Split-precision record of one pixel: the pulse phase in cycles fitted
+// over the sum wavefront's union window [kw0, kw1) with a degree-4 Chebyshev
+// interpolant evaluated in FP64, re-expanded as forward differences at kw0.
+__device__ inline PixSplit fit_pixel(const GeomArgs& a, const PixGeom& g, bool has_win, int kw0,
+                                     int kw1, double q_rows)
+{
+    const V3 x{g.x[0], g.x[1], g.x[2]};
     PixSplit sp;
     sp.p0 = sp.d1 = sp.d2 = sp.d3 = sp.d4 = 0.;
     sp.zoff = 0.f;
@@ -336,20 +359,53 @@ __device__ inline void geometry_tile(const GeomArgs& a, const OutDesc* grids, in
         // koff whole bins of offset keep z >= 0 over the window, so that the
         // sum kernel takes floor(z) and fract(z) by truncation and v_fract
         const double koff = ceil(bnd) + 2.0;
-        sp.zoff = (float)(-(double)kPolyphaseRows * (f_ref - c_ref * inv_k + shift - koff));
+        sp.zoff = (float)(-q_rows * (f_ref - c_ref * inv_k + shift - koff));
         sp.cbase = (int32_t)n_ref - a.kwidth / 2 - (odd ? 1 : 0) + (int32_t)koff;
         const double c_lo = n_ref + ceil(f_ref - bnd + shift) - (a.kwidth / 2) - (odd ? 1 : 0) - 1;
         const double c_hi = n_ref + ceil(f_ref + bnd + shift) - (a.kwidth / 2) - (odd ? 1 : 0) + 1;
         sp.edge = (c_lo < 0. || c_hi + a.kwidth > (double)a.nr) ? 1 : 0;
     }
-    a.split[rec] = sp;
+    return sp;
 }
 
-// One workgroup per group of four tiles (workgroups past the batch's groups
-// return at once).  No persistent loop: this kernel needs ~220 VGPRs, and a
-// loop around its body makes the compiler keep the arguments live across
-// iterations and spill.
-__global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
+// One sum wavefront tile: read the geometry records of its pixels (lane_rows
+// per lane), take the union window of the tile and fit every pixel.
+__device__ inline void fit_tile(const GeomArgs& a, const OutDesc* grids, int n_grids,
+                                int64_t n_seq, int64_t gseq, int lane)
+{
+    const bool in_seq = gseq < n_seq;
+    const OutDesc d = load_desc(grids, in_seq ? find_grid(grids, n_grids, gseq) : 0);
+    const int lr = d.map.lane_rows;
+    int ks = INT_MAX, ke = INT_MIN;
+    for (int j = 0; j < lr; ++j) {
+        int64_t row = 0, col = 0;
+        if (in_seq && tile_pixel(d.map, gseq - d.seq_base, lane, j, &row, &col)) {
+            const PixGeom& g = a.geom[d.rec_base + row * d.map.width + col];
+            if (g.status == kPixOk && g.kstop > g.kstart) {
+                ks = min(ks, g.kstart);
+                ke = max(ke, g.kstop);
+            }
+        }
+    }
+    const int kw0 = wave_min_i(ks);
+    const int kw1 = wave_max_i(ke);
+    // polyphase rows of the weight table the sum reads for this tiling
+    const double q_rows = lr == 2 ? (double)kPairPolyRows : (double)kPolyphaseRows;
+#pragma unroll 1
+    for (int j = 0; j < lr; ++j) {
+        int64_t row = 0, col = 0;
+        if (in_seq && tile_pixel(d.map, gseq - d.seq_base, lane, j, &row, &col)) {
+            const int64_t rec = d.rec_base + row * d.map.width + col;
+            const PixGeom g = a.geom[rec];
+            const bool has_win = g.status == kPixOk && g.kstop > g.kstart;
+            a.split[rec] = fit_pixel(a, g, has_win, kw0, kw1, q_rows);
+        }
+    }
+}
+
+// Split-precision records of a batch, one wavefront per sum tile.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void fit_kernel(GeomArgs a)
 {
     const int64_t ng = uniform64(a.batch->n_groups);
     if ((int64_t)blockIdx.x >= ng) return;
@@ -358,7 +414,7 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
     const int64_t n_seq = uniform64(a.batch->n_seq);
     int lane;
     const int64_t gseq = group_tile(blockIdx.x, ng, &lane);
-    geometry_tile(a, grids, n_grids, n_seq, gseq, lane);
+    fit_tile(a, grids, n_grids, n_seq, gseq, lane);
 }
 
 // Persistent tile schedule of the sum kernels.  A workgroup holds 8
@@ -413,7 +469,7 @@ __device__ inline void plain_tile(const SumArgs& a, const float* tab, const OutD
                                   int64_t tile, int lane)
 {
     int64_t row = 0, col = 0;
-    const bool valid = tile_pixel(od.map, tile, lane, &row, &col);
+    const bool valid = tile_pixel(od.map, tile, lane, 0, &row, &col);
     const int64_t pix = row * od.map.width + col;
 
     PixGeom g;
@@ -804,7 +860,7 @@ __device__ inline void split_tile(const SumArgs& a, const float* s_poly, char* s
                                   const OutDesc& od, int64_t tile, int lane)
 {
     int64_t row = 0, col = 0;
-    const bool valid = tile_pixel(od.map, tile, lane, &row, &col);
+    const bool valid = tile_pixel(od.map, tile, lane, 0, &row, &col);
     const int64_t pix = row * od.map.width + col;
 
     PixSplit sp;
@@ -870,19 +926,266 @@ __device__ inline void split_tile(const SumArgs& a, const float* s_poly, char* s
     }
 }
 
-template<int W, bool kStaged>
-__global__ __launch_bounds__(512) void sum_split_kernel(SumArgs a)
+// Two output lines per lane (lane_rows = 2, W = 8).  The lane's pixels A and
+// B are vertically adjacent: at any pulse their first taps differ by at most
+// one range bin (adjacent lines are one azimuth sample apart, so their ranges
+// to a pulse differ by a few centimetres), and one 9-bin gather from the
+// smaller first tap serves both.  The weights come from a table of 9-tap rows
+// in two copies, the 8 weights preceded (copy 1) or followed (copy 0) by a
+// zero, so that each pixel picks its alignment inside the shared 9 bins by
+// its row offset alone.  This halves the gather traffic per update, the
+// bound of the one-line kernel.  Where the first taps of some lane's pixels
+// are further apart, that pulse is gathered per pixel.
+struct PairPulse {
+    float ph;   // phase, revolutions in [0, 1)
+    float qf;   // fraction between weight rows
+    int row;    // byte offset of the weight row (copy 0)
+    int c0;     // first tap's range bin
+};
+
+constexpr int kPairRowBytes = 4 * kPairRowFloats;
+constexpr int kPairCopyBytes = (kPairPolyRows + 1) * kPairRowBytes;
+
+__device__ inline PairPulse pair_pulse(const PhaseState& st, float zscale, float zoff, int cbase)
 {
+    PairPulse r;
+    r.ph = (float)__builtin_amdgcn_fract(st.p);
+    const float z = fmaf((float)st.p, zscale, zoff);  // >= 0 over the window
+    const uint32_t m = (uint32_t)z;
+    r.qf = __builtin_amdgcn_fractf(z);
+    r.row = (int)(m & (kPairPolyRows - 1)) * kPairRowBytes;
+    r.c0 = cbase - (int)(m >> kPairPolyLog2);
+    return r;
+}
+
+// Interpolate the 9 bins `d` with the weight row at byte offset `row`
+// (9 weights, 9 deltas, 2 padding), rotate by the phase and accumulate.
+__device__ inline void pair_update(const float* s_pair, int row, float qf, float ph,
+                                   const float2 (&d)[9], float& cre, float& cim)
+{
+    const float4* r4 = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_pair) + row);
+    const float4 A = r4[0], B = r4[1], C = r4[2], D = r4[3], E = r4[4];
+    float w[9];
+    w[0] = fmaf(qf, C.y, A.x);
+    w[1] = fmaf(qf, C.z, A.y);
+    w[2] = fmaf(qf, C.w, A.z);
+    w[3] = fmaf(qf, D.x, A.w);
+    w[4] = fmaf(qf, D.y, B.x);
+    w[5] = fmaf(qf, D.z, B.y);
+    w[6] = fmaf(qf, D.w, B.z);
+    w[7] = fmaf(qf, E.x, B.w);
+    w[8] = fmaf(qf, E.y, C.x);
+    float sre = 0.f, sim = 0.f;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+        sre = fmaf(w[m], d[m].x, sre);
+        sim = fmaf(w[m], d[m].y, sim);
+    }
+    const float sn = __builtin_amdgcn_sinf(ph);
+    const float cs = __builtin_amdgcn_cosf(ph);
+    cre = fmaf(sre, cs, cre);
+    cre = fmaf(-sim, sn, cre);
+    cim = fmaf(sre, sn, cim);
+    cim = fmaf(sim, cs, cim);
+}
+
+// The 8 taps from bin c0 (guarded: zero outside [0, nr)), and a ninth zero.
+template<bool kGuard>
+__device__ inline void pair_taps_one(const float2* line, int c0, int nr, float2 (&d)[9])
+{
+    float2 e[8];
+    split_taps<8, kGuard>(line, c0, nr, e);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) d[m] = e[m];
+    d[8] = make_float2(0.f, 0.f);
+}
+
+// The 9 bins from `base` (base + 9 <= nr).
+__device__ inline void pair_taps_shared(const float2* line, int base, float2 (&d)[9])
+{
+    const float2* p = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(line) +
+                                                      (uint32_t)base * (uint32_t)sizeof(float2));
+#pragma unroll
+    for (int m = 0; m < 9; ++m) d[m] = p[m];
+}
+
+struct PairLine {
+    SplitAcc s;
+    int ks, ke;     // this pixel's pulses (empty: ks = INT_MAX)
+};
+
+// Pulses [k_lo, k_hi): each pixel gathers its own taps where it integrates.
+template<bool kGuard>
+__device__ inline void pair_masked(const SumArgs& a, const float* s_pair, PairLine& A,
+                                   PairLine& B, int k_lo, int k_hi)
+{
+    const int nr = (int)a.nr;
+    for (int k = k_lo; k < k_hi; ++k) {
+        const float2* line = a.rc + (int64_t)k * nr;
+        const PairPulse pa = pair_pulse(A.s.st, a.zscale2, A.s.zoff, A.s.cbase);
+        const PairPulse pb = pair_pulse(B.s.st, a.zscale2, B.s.zoff, B.s.cbase);
+        A.s.st.advance();
+        B.s.st.advance();
+        if (k >= A.ks && k < A.ke) {
+            float2 d[9];
+            pair_taps_one<kGuard>(line, pa.c0, nr, d);
+            pair_update(s_pair, pa.row, pa.qf, pa.ph, d, A.s.cre, A.s.cim);
+        }
+        if (k >= B.ks && k < B.ke) {
+            float2 d[9];
+            pair_taps_one<kGuard>(line, pb.c0, nr, d);
+            pair_update(s_pair, pb.row, pb.qf, pb.ph, d, B.s.cre, B.s.cim);
+        }
+        if (++A.s.count >= a.chunk) {
+            A.s.fold();
+            B.s.fold();
+        }
+    }
+}
+
+// Core pulses [k0, k1), every pixel of every lane integrating: one shared
+// 9-bin gather per lane and pulse (no software prefetch: the two lines' state
+// leaves no registers for a second pulse in flight).
+__device__ inline void pair_core(const SumArgs& a, const float* s_pair, PairLine& A, PairLine& B,
+                                 int k0, int k1, uint64_t act_a, uint64_t act_b)
+{
+    const int nr = (int)a.nr;
+    auto next = [&](PairPulse& pa, PairPulse& pb, int& base, float2 (&d)[9], int k) {
+        pa = pair_pulse(A.s.st, a.zscale2, A.s.zoff, A.s.cbase);
+        pb = pair_pulse(B.s.st, a.zscale2, B.s.zoff, B.s.cbase);
+        A.s.st.advance();
+        B.s.st.advance();
+        base = min(min(pa.c0, pb.c0), nr - 9);
+        pair_taps_shared(a.rc + (int64_t)k * nr, base, d);
+    };
+    auto consume = [&](const PairPulse& pa, const PairPulse& pb, int base, const float2 (&d)[9],
+                       int k) {
+        const uint32_t oa = (uint32_t)(pa.c0 - base), ob = (uint32_t)(pb.c0 - base);
+        const uint64_t bad = (__builtin_amdgcn_ballot_w64(oa > 1u) & act_a) |
+                             (__builtin_amdgcn_ballot_w64(ob > 1u) & act_b);
+        if (bad == 0) {
+            pair_update(s_pair, pa.row + (int)min(oa, 1u) * kPairCopyBytes, pa.qf, pa.ph, d,
+                        A.s.cre, A.s.cim);
+            pair_update(s_pair, pb.row + (int)min(ob, 1u) * kPairCopyBytes, pb.qf, pb.ph, d,
+                        B.s.cre, B.s.cim);
+        } else {
+            const float2* line = a.rc + (int64_t)k * nr;
+            float2 e[9];
+            pair_taps_one<false>(line, pa.c0, nr, e);
+            pair_update(s_pair, pa.row, pa.qf, pa.ph, e, A.s.cre, A.s.cim);
+            pair_taps_one<false>(line, pb.c0, nr, e);
+            pair_update(s_pair, pb.row, pb.qf, pb.ph, e, B.s.cre, B.s.cim);
+        }
+        if (++A.s.count >= a.chunk) {
+            A.s.fold();
+            B.s.fold();
+        }
+    };
+    for (int k = k0; k < k1; ++k) {
+        PairPulse pa, pb;
+        int base;
+        float2 d[9];
+        next(pa, pb, base, d, k);
+        consume(pa, pb, base, d, k);
+    }
+}
+
+__device__ inline void pair_tile(const SumArgs& a, const float* s_pair, const OutDesc& od,
+                                 int64_t tile, int lane)
+{
+    int64_t row[2] = {0, 0}, col[2] = {0, 0};
+    bool valid[2], ok[2], win[2];
+    PixSplit sp[2];
+    PairLine L[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        valid[j] = tile_pixel(od.map, tile, lane, j, &row[j], &col[j]);
+        if (valid[j]) sp[j] = a.split[od.rec_base + row[j] * od.map.width + col[j]];
+        ok[j] = valid[j] && sp[j].status == kPixOk;
+        win[j] = ok[j] && sp[j].kstop > sp[j].kstart;
+        L[j].ks = win[j] ? sp[j].kstart : INT_MAX;
+        L[j].ke = win[j] ? sp[j].kstop : INT_MIN;
+    }
+    // a pixel without pulses follows its partner's phase and taps (its sum
+    // is discarded), so that the shared gather always serves the lane
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int src = win[j] ? j : (win[1 - j] ? 1 - j : -1);
+        SplitAcc& s = L[j].s;
+        if (src >= 0) {
+            const PixSplit& r = sp[src];
+            s.st.p = r.p0;
+            s.st.d1 = r.d1;
+            s.st.d2 = r.d2;
+            s.st.d3 = r.d3;
+            s.st.d4 = r.d4;
+            s.zoff = r.zoff;
+            s.cbase = r.cbase;
+        } else {
+            s.st.p = s.st.d1 = s.st.d2 = s.st.d3 = s.st.d4 = 0.;
+            s.zoff = 0.f;
+            s.cbase = 0;
+        }
+        s.cre = s.cim = s.acc_re = s.acc_im = s.cmp_re = s.cmp_im = 0.f;
+        s.count = 0;
+    }
+    const int kw0 = __builtin_amdgcn_readfirstlane(wave_min_i(min(L[0].ks, L[1].ks)));
+    const int kw1 = __builtin_amdgcn_readfirstlane(wave_max_i(max(L[0].ke, L[1].ke)));
+    int kf0 = __builtin_amdgcn_readfirstlane(
+        wave_max_i(max(win[0] ? L[0].ks : INT_MIN, win[1] ? L[1].ks : INT_MIN)));
+    int kf1 = __builtin_amdgcn_readfirstlane(
+        wave_min_i(min(win[0] ? L[0].ke : INT_MAX, win[1] ? L[1].ke : INT_MAX)));
+    const bool all_win = __builtin_amdgcn_readfirstlane(
+        wave_min_i((!valid[0] || win[0]) && (!valid[1] || win[1]) ? 1 : 0)) != 0;
+    if (kw0 >= kw1 || !all_win || kf0 < kw0 || kf1 > kw1 || kf0 >= kf1) {
+        kf0 = kw1;
+        kf1 = kw1;
+    }
+    const bool edge = (win[0] && sp[0].edge) || (win[1] && sp[1].edge);
+    const bool guard = a.nr < 9 || __builtin_amdgcn_readfirstlane(wave_max_i(edge ? 1 : 0)) != 0;
+    if (kw0 < kw1) {
+        if (guard) {
+            pair_masked<true>(a, s_pair, L[0], L[1], kw0, kw1);
+        } else {
+            pair_masked<false>(a, s_pair, L[0], L[1], kw0, kf0);
+            if (kf0 < kf1)
+                pair_core(a, s_pair, L[0], L[1], kf0, kf1, __builtin_amdgcn_ballot_w64(win[0]),
+                          __builtin_amdgcn_ballot_w64(win[1]));
+            pair_masked<false>(a, s_pair, L[0], L[1], max(kf1, kw0), kw1);
+        }
+        L[0].s.fold();
+        L[1].s.fold();
+    }
+    const float nanf_ = __builtin_nanf("");
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+        if (valid[j])
+            od.out[row[j] * od.map.width + col[j]] =
+                ok[j] ? make_float2(L[j].s.acc_re - L[j].s.cmp_re, L[j].s.acc_im - L[j].s.cmp_im)
+                      : make_float2(nanf_, nanf_);
+}
+
+// Two workgroups per CU need at most 128 VGPRs (4 waves per SIMD).
+template<int W, bool kStaged, bool kPair>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+void sum_split_kernel(SumArgs a)
+{
+    static_assert(!kPair || (W == 8 && !kStaged), "two-line lanes: W = 8, gathered taps");
     constexpr int ROW = poly_row_floats(W);
+    constexpr int kTable = kPair ? kPairTableFloats : (kPolyphaseRows + 1) * ROW;
     // static LDS arrays: row and slot offsets fold into the ds_read offsets
-    __shared__ float4 s_poly4[(kPolyphaseRows + 1) * ROW / 4];
+    __shared__ float4 s_poly4[kTable / 4];
     __shared__ float4 s_win4[kStaged ? kSumWaves * kWinWaveBytes / 16 : 1];
     float* s_poly = reinterpret_cast<float*>(s_poly4);
-    for (int i = threadIdx.x; i < (kPolyphaseRows + 1) * ROW; i += blockDim.x) s_poly[i] = a.poly[i];
+    const float* src = kPair ? a.poly2 : a.poly;
+    for (int i = threadIdx.x; i < kTable; i += blockDim.x) s_poly[i] = src[i];
     __syncthreads();
     char* s_win = reinterpret_cast<char*>(s_win4);
     for_each_tile(a.batch, [&](const OutDesc& d, int64_t tile, int lane) {
-        split_tile<W, kStaged>(a, s_poly, s_win, d, tile, lane);
+        if (kPair)
+            pair_tile(a, s_poly, d, tile, lane);
+        else
+            split_tile<W, kStaged>(a, s_poly, s_win, d, tile, lane);
     });
 }
 
@@ -913,6 +1216,13 @@ hipError_t launch_geometry(const GeomArgs& args, int grid, hipStream_t stream)
     return hipGetLastError();
 }
 
+hipError_t launch_fit(const GeomArgs& args, int grid, hipStream_t stream)
+{
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fit_kernel, dim3(grid), dim3(256), 0, stream, args);
+    return hipGetLastError();
+}
+
 hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_t stream)
 {
     if (grid <= 0) return hipSuccess;
@@ -932,7 +1242,7 @@ hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_
 
 // Workgroups one CU can hold for the chosen sum kernel (256 threads for the
 // plain kernel, 512 for the split kernel).
-int sum_blocks_per_cu(int precision, int width, bool staged)
+int sum_blocks_per_cu(int precision, int width, bool staged, bool pair)
 {
     int n = 0;
     hipError_t e = hipSuccess;
@@ -940,12 +1250,8 @@ int sum_blocks_per_cu(int precision, int width, bool staged)
         const size_t lds = sizeof(float) * kMaxTableLds;
         e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_plain_kernel<8>, 256, lds)
                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_plain_kernel<9>, 256, lds);
-    } else if (staged) {
-        e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<8, true>, 512, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<9, true>, 512, 0);
     } else {
-        e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<8, false>, 512, 0)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_split_kernel<9, false>, 512, 0);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel_fn(2, width, staged, pair), 512, 0);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -956,36 +1262,39 @@ int sum_blocks_per_cu(int precision, int width, bool staged)
 
 // Persistent grid: `grid` workgroups of 512 threads (a multiple of 8, one
 // share per XCD), sized by the caller from the device's CU count and occupancy.
-const void* kernel_fn(int which, int width, bool staged)
+const void* kernel_fn(int which, int width, bool staged, bool pair)
 {
     switch (which) {
     case 0: return reinterpret_cast<const void*>(&geometry_kernel);
+    case 3: return reinterpret_cast<const void*>(&fit_kernel);
     case 1: return width == 8 ? reinterpret_cast<const void*>(&sum_plain_kernel<8>)
                               : reinterpret_cast<const void*>(&sum_plain_kernel<9>);
     default:
+        if (width == 8 && pair) return reinterpret_cast<const void*>(&sum_split_kernel<8, false, true>);
         if (width == 8)
-            return staged ? reinterpret_cast<const void*>(&sum_split_kernel<8, true>)
-                          : reinterpret_cast<const void*>(&sum_split_kernel<8, false>);
-        return staged ? reinterpret_cast<const void*>(&sum_split_kernel<9, true>)
-                      : reinterpret_cast<const void*>(&sum_split_kernel<9, false>);
+            return staged ? reinterpret_cast<const void*>(&sum_split_kernel<8, true, false>)
+                          : reinterpret_cast<const void*>(&sum_split_kernel<8, false, false>);
+        return staged ? reinterpret_cast<const void*>(&sum_split_kernel<9, true, false>)
+                      : reinterpret_cast<const void*>(&sum_split_kernel<9, false, false>);
     }
 }
 
-hipError_t launch_sum_split(const SumArgs& args, int width, int grid, bool staged,
+hipError_t launch_sum_split(const SumArgs& args, int width, int grid, bool staged, bool pair,
                             hipStream_t stream)
 {
     if (grid <= 0) return hipSuccess;
+    if ((width != 8 && width != 9) || (pair && (width != 8 || staged))) return hipErrorInvalidValue;
     const size_t lds = 0;  // the weight table and window rings are static LDS arrays
-    if (width == 8 && staged)
-        hipLaunchKernelGGL((sum_split_kernel<8, true>), dim3(grid), dim3(512), lds, stream, args);
+    if (pair)
+        hipLaunchKernelGGL((sum_split_kernel<8, false, true>), dim3(grid), dim3(512), lds, stream, args);
+    else if (width == 8 && staged)
+        hipLaunchKernelGGL((sum_split_kernel<8, true, false>), dim3(grid), dim3(512), lds, stream, args);
     else if (width == 8)
-        hipLaunchKernelGGL((sum_split_kernel<8, false>), dim3(grid), dim3(512), lds, stream, args);
-    else if (width == 9 && staged)
-        hipLaunchKernelGGL((sum_split_kernel<9, true>), dim3(grid), dim3(512), lds, stream, args);
-    else if (width == 9)
-        hipLaunchKernelGGL((sum_split_kernel<9, false>), dim3(grid), dim3(512), lds, stream, args);
+        hipLaunchKernelGGL((sum_split_kernel<8, false, false>), dim3(grid), dim3(512), lds, stream, args);
+    else if (staged)
+        hipLaunchKernelGGL((sum_split_kernel<9, true, false>), dim3(grid), dim3(512), lds, stream, args);
     else
-        return hipErrorInvalidValue;
+        hipLaunchKernelGGL((sum_split_kernel<9, false, false>), dim3(grid), dim3(512), lds, stream, args);
     return hipGetLastError();
 }
 

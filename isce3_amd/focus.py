@@ -138,7 +138,7 @@ def backproject(out, out_geometry: RadarGeometry, signal_data, in_geometry: Rada
 
     Extensions: ``precision`` "plain" (reference semantics, FP64 inner loop)
     or "split" (FP64 per-pixel reference + FP32 deltas), ``options`` tiling
-    dict {tile_rows, chunk, order, blocks_per_cu, staged}, ``stream`` (torch stream or raw hipStream_t int)
+    dict {tile_rows, chunk, order, blocks_per_cu, staged, lane_rows}, ``stream`` (torch stream or raw hipStream_t int)
     and ``stats`` (dict filled with updates, geometry_ms, sum_ms).
 
     Returns True if any pixel failed to converge (those pixels are NaN).
@@ -244,7 +244,8 @@ def _copy_back(torch, a, d):
 def _options_struct(options) -> _native.Options_t:
     opt = _native.Options_t()
     for k, v in (options or {}).items():
-        if k not in ("tile_rows", "chunk", "fit_degree", "order", "blocks_per_cu", "staged"):
+        if k not in ("tile_rows", "chunk", "fit_degree", "order", "blocks_per_cu", "staged",
+                     "lane_rows"):
             raise errors.InvalidArgument(f"unknown option {k!r}")
         setattr(opt, k, int(v))
     return opt

@@ -39,12 +39,12 @@ def run_blocks(sc, raw, grids, precision, **kw):
     return outs, hs, flags, stats
 
 
-@pytest.mark.parametrize("precision", ["plain", "split"])
+@pytest.mark.parametrize("precision,lane_rows", [("plain", 1), ("split", 1), ("split", 2)])
 @pytest.mark.parametrize("tile_rows", [1, 2, 8])
-def test_blocks_equal_single_calls(torch_gpu, precision, tile_rows):
+def test_blocks_equal_single_calls(torch_gpu, precision, lane_rows, tile_rows):
     sc, raw = scene_and_raw("small")
     grids = uneven_blocks(sc)
-    opts = {"tile_rows": tile_rows}
+    opts = {"tile_rows": tile_rows, "lane_rows": lane_rows}
     outs, hs, flags, stats = run_blocks(sc, raw, grids, precision, options=opts)
     total = 0
     for b, z, h, f in zip(grids, outs, hs, flags):

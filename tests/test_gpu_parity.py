@@ -80,5 +80,36 @@ def test_staged_windows_equal_global_gathers(torch_gpu, config, tile_rows):
     holds and exercise the per-pulse fallback to global gathers."""
     print(f"[staged vs gather] config={config} tile_rows={tile_rows}", flush=True)
     staged, _, _, _ = run_gpu(config, "split", options={"tile_rows": tile_rows, "staged": 1})
-    gathered, _, _, _ = run_gpu(config, "split", options={"tile_rows": tile_rows, "staged": 0})
+    gathered, _, _, _ = run_gpu(config, "split", options={"tile_rows": tile_rows, "staged": 0,
+                                                          "lane_rows": 1})
     np.testing.assert_array_equal(staged, gathered)
+
+
+@pytest.mark.parametrize("config", ["tiny", "small"])
+@pytest.mark.parametrize("tile_rows", [1, 2, 4, 8])
+def test_two_lines_per_lane_match_fp64_oracle(torch_gpu, config, tile_rows):
+    """Two adjacent output lines per lane sharing one 9-bin gather: within the
+    split tolerance of the all-FP64 oracle and of the one-line kernel, for
+    every tiling (odd line counts leave the second line of the last lane row
+    outside the grid)."""
+    print(f"[two lines] config={config} tile_rows={tile_rows}", flush=True)
+    opts = {"tile_rows": tile_rows, "lane_rows": 2}
+    out, h, err, _ = run_gpu(config, "split", options=opts)
+    ref, href, _, rc = oracle_run(config, 1)
+    assert err == (rc == 6)
+    assert complex_rel_error(out, ref) <= SPLIT_REL
+    assert rms(phase_errors(out, ref)) <= SPLIT_PHS
+    np.testing.assert_allclose(h, href, atol=1e-3)
+    one, _, _, _ = run_gpu(config, "split", options={"tile_rows": tile_rows, "lane_rows": 1})
+    assert complex_rel_error(out, one) <= 1e-4
+    assert np.array_equal(np.isnan(out), np.isnan(one))
+
+
+def test_two_lines_per_lane_rejected_where_unsupported(torch_gpu):
+    from isce3_amd import errors
+    with pytest.raises(errors.InvalidArgument):
+        run_gpu("tiny", "plain", options={"lane_rows": 2})
+    with pytest.raises(errors.InvalidArgument):
+        run_gpu("pt", "split", options={"lane_rows": 2})       # kernel width 9
+    with pytest.raises(errors.InvalidArgument):
+        run_gpu("tiny", "split", options={"lane_rows": 2, "staged": 1})

@@ -79,7 +79,8 @@ def test_new_launch_options_capture_a_new_graph(torch_gpu):
                sc.kernel, sc.dry_tropo_model, precision="split", max_pixels=1 << 14,
                max_blocks=4) as scene:
         for opts in ({"tile_rows": 1}, {"tile_rows": 8, "chunk": 16, "staged": 1},
-                     {"tile_rows": 4, "blocks_per_cu": 1}, {"tile_rows": 1}):
+                     {"tile_rows": 4, "blocks_per_cu": 1}, {"tile_rows": 2, "lane_rows": 2},
+                     {"tile_rows": 1}):
             print(f"[scene] options={opts}", flush=True)
             ref = eager(sc, raw_dev, grids, "split", opts)
             outs, _, _, _ = replay(scene, grids, opts, heights=False)
