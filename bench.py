@@ -40,7 +40,7 @@ sys.path.insert(0, ROOT)
 FLOPS_PER_UPDATE = 101  # algorithmic flops per pixel-pulse update (FMA = 2)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sum_split_kernel_full_4096x4096_pmc.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sum_split_kernel_two_lines_pmc.md")
 
 
 def log(msg):
@@ -343,6 +343,70 @@ def run_serve(args, torch, np, dev, tuner):
             "rates": rows}
 
 
+# ------------------------------------------------------- range compression
+def run_rangecomp(args, torch, np, dev):
+    """Range compression of 16384 pulses x 15000 samples (complex64) with a
+    961-sample linear chirp, "same" mode, inputs resident in HBM: one fused
+    FFT-convolution launch (2^14-point transforms).  HBM-bound: the roofline
+    counts the pulses read and the compressed pulses written."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import rangecomp as rco
+    from isce3_amd.focus import RangeComp, form_linear_chirp
+    na, nr = 16384, 15000
+    chirp = form_linear_chirp(0.6e12, 40e-6, 24e6)
+    rc = RangeComp(chirp, nr, maxbatch=na, mode=RangeComp.Mode.Same)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(11)
+    x = torch.randn((na, nr), dtype=torch.complex64, device=dev, generator=gen)
+    y = torch.empty((na, rc.output_size), dtype=torch.complex64, device=dev)
+    stream = torch.cuda.current_stream()
+    for _ in range(max(args.warmup, 1)):
+        rc.rangecompress(y, x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        rc.rangecompress(y, x)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    nbytes = (na * nr + na * rc.output_size) * 8
+    # accuracy on a sample of pulses against the FP64 oracle
+    rows = [0, 1, na // 2, na - 1]
+    xs = x[rows].cpu().numpy()
+    ref = rco.rangecompress(chirp, xs, rco.SAME, "fp64")
+    rel = float(np.linalg.norm(y[rows].cpu().numpy() - ref) / np.linalg.norm(ref))
+    # CPU baseline: the oracle in complex64 (numpy's single-threaded FFTs)
+    nb = 256
+    xb = xs = x[:nb].cpu().numpy()
+    t0 = time.perf_counter()
+    rco.rangecompress(chirp, xb, rco.SAME, "fp32")
+    cpu_s = time.perf_counter() - t0
+    res = {"workload": f"{na} pulses x {nr} samples c64, {len(chirp)}-sample chirp, same mode",
+           "fft_size": rc.fft_size, "ms": ms, "pulses_per_s": na / (ms * 1e-3),
+           "kernel": f"rc::conv_kernel<{int(np.log2(rc.fft_size))}>",
+           "roofline": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": 8000.0,
+                        "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 8e12,
+                        "traffic": None},
+           "rel_error_vs_fp64_oracle": rel,
+           "cpu_baseline": {"value": nb / cpu_s, "unit": "pulses/s", "cores": 1, "kind": "port",
+                            "sample": f"{nb} pulses, numpy complex64 FFTs"}}
+    log(f"rangecomp: {ms:.3f} ms for {na} pulses, {res['roofline']['achieved']:.0f} GB/s, "
+        f"rel err {rel:.2e}, CPU {nb / cpu_s:.0f} pulses/s")
+    rc.close()
+    return res
+
+
+def split_kernel_name(opts) -> str:
+    """The coherent-sum kernel the split step ran with these launch options
+    (kernel width 8)."""
+    staged = bool((opts or {}).get("staged", 0))
+    lanes = (opts or {}).get("lane_rows", 0)
+    pair = not staged and lanes in (0, 2)
+    return (f"sum_split_kernel<8, {'true' if staged else 'false'}, {'true' if pair else 'false'}>"
+            + (" (two output lines per lane)" if pair else ""))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -362,6 +426,9 @@ def main():
                     help="time one precision of the full block only (profiling runs)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-rangecomp", action="store_true")
+    ap.add_argument("--only-rangecomp", action="store_true",
+                    help="time the range compressor only (profiling runs)")
     ap.add_argument("--rates", type=lambda s: [float(x) for x in s.split(",")],
                     default=[20.0, 50.0, 100.0])
     ap.add_argument("--serve-seconds", type=float, default=4.0,
@@ -393,6 +460,10 @@ def main():
         del src, dst
         log(f"calibration copy of {args.calibrate_copy} MiB done")
 
+    if args.only_rangecomp:
+        print(json.dumps({"rangecomp": run_rangecomp(args, torch, np, dev)}))
+        return
+
     out_dir = os.path.join(ROOT, "gpurun_out")
     tuner = autotune.Autotuner(os.path.join(out_dir, "autotune.json")
                                if os.path.isdir(out_dir) else None)
@@ -411,6 +482,8 @@ def main():
     batch = None if args.no_batch else run_batch(args, torch, np, dev, tuner)
     torch.cuda.empty_cache()
     serve_res = None if args.no_serve else run_serve(args, torch, np, dev, tuner)
+    torch.cuda.empty_cache()
+    rc_res = None if args.no_rangecomp else run_rangecomp(args, torch, np, dev)
 
     sp, pl = results["split"], results["plain"]
     achieved = sp["updates"] * FLOPS_PER_UPDATE / (sp["sum_ms"] * 1e-3) / 1e12
@@ -442,12 +515,13 @@ def main():
                   "sum_ms": sp["sum_ms"], "geometry_ms": sp["geometry_ms"],
                   "options": sp["options"], "accuracy": acc.get("split")},
         "split_over_plain": sp["updates_per_s"] / pl["updates_per_s"],
-        "roofline": {"bound": "valu", "kernel": "sum_split_kernel<8>",
+        "roofline": {"bound": "valu", "kernel": split_kernel_name(sp["options"]),
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": hbm_bytes_per_launch()},
         "latency": latency,
         "batch": batch,
         "serve": serve_res,
+        "rangecomp": rc_res,
         "cpu_baseline": cpu,
     }
     if cpu:

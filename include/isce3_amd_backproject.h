@@ -47,7 +47,8 @@ enum {
     BP_ERR_DOMAIN = -3,           /* isce3::except::DomainError */
     BP_ERR_HIP = -4,              /* HIP runtime failure */
     BP_ERR_RUNTIME = -5,          /* isce3::except::RuntimeError */
-    BP_ERR_OUT_OF_MEMORY = -6
+    BP_ERR_OUT_OF_MEMORY = -6,
+    BP_ERR_LENGTH = -7            /* isce3::except::LengthError */
 };
 
 enum { BP_INTERP_HERMITE = 0, BP_INTERP_LEGENDRE = 1 };   /* OrbitInterpMethod */

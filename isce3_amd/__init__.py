@@ -8,8 +8,9 @@ The look-alike host types mirror isce3's module layout:
     isce3_amd.geometry   DEMInterpolator
     isce3_amd.product    RadarGridParameters
     isce3_amd.container  RadarGeometry
-    isce3_amd.focus      backproject
+    isce3_amd.focus      backproject, backproject_blocks, Scene, RangeComp,
+                         form_linear_chirp
 """
-from . import container, core, errors, focus, geometry, product  # noqa: F401
+from . import container, core, errors, focus, geometry, product, rangecomp  # noqa: F401
 
 __version__ = "0.1.0"

@@ -24,6 +24,7 @@ BP_ERR_DOMAIN = -3
 BP_ERR_HIP = -4
 BP_ERR_RUNTIME = -5
 BP_ERR_OUT_OF_MEMORY = -6
+BP_ERR_LENGTH = -7
 
 PRECISION_PLAIN = 0
 PRECISION_SPLIT = 1
@@ -113,6 +114,11 @@ _SIGNATURES = {
     "bp_scene_destroy": (C.c_int, [C.c_void_p]),
     "bp_scene_focus": (C.c_int, [C.c_void_p, _P(OutBlock_t), C.c_int, _P(Options_t), C.c_void_p,
                                  _P(C.c_int), _P(Stats_t)]),
+    "bp_rangecomp_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      _P(C.c_void_p)]),
+    "bp_rangecomp_destroy": (C.c_int, [C.c_void_p]),
+    "bp_rangecomp_sizes": (C.c_int, [C.c_void_p, _P(C.c_int32)]),
+    "bp_rangecomp_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "bp_orbit_interpolate": (C.c_int, [_P(Orbit_t), C.c_double, C.c_int, _P(C.c_double),
                                        _P(C.c_double)]),
     "bp_lonlat_to_xyz": (None, [C.c_double, C.c_double, _P(C.c_double), _P(C.c_double)]),
@@ -168,6 +174,8 @@ def check(rc: int, what: str = "") -> int:
         raise errors.DomainError(msg)
     if rc == BP_ERR_OUT_OF_MEMORY:
         raise MemoryError(msg)
+    if rc == BP_ERR_LENGTH:
+        raise errors.LengthError(msg)
     if rc == BP_ERR_HIP:
         raise errors.HipError(msg)
     raise errors.RuntimeError_(msg)

@@ -25,6 +25,13 @@ int fail(int code, const std::string& msg)
     return code;
 }
 
+}  // namespace
+
+// Shared with the range-compression entry points (rc_api.hip).
+void bp_set_last_error(const std::string& msg) { g_msg = msg; }
+
+namespace {
+
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;

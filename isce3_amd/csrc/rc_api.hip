@@ -1,0 +1,193 @@
+// C-ABI of range compression (include/isce3_amd_rangecomp.h): argument rules
+// of cxx/isce3/focus/RangeComp.cpp:39-79 and :93-99, the matched filter's
+// spectrum formed on the host in FP64 (RangeComp.cpp:22-37), and the launch
+// of the fused FFT-convolution kernel (rc_kernels.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/isce3_amd_backproject.h"
+#include "../../include/isce3_amd_rangecomp.h"
+#include "rc_internal.h"
+
+// Message of the last failing call of this thread (shared with bp_api.hip).
+void bp_set_last_error(const std::string& msg);
+
+namespace {
+
+int fail(int code, const std::string& msg)
+{
+    bp_set_last_error(msg);
+    return code;
+}
+
+int output_size(int m, int n, int mode)
+{
+    switch (mode) {
+    case BP_RC_FULL: return m + n - 1;
+    case BP_RC_VALID: return std::max(m, n) - std::min(m, n) + 1;
+    default: return n;
+    }
+}
+
+int first_valid(int m, int mode)
+{
+    switch (mode) {
+    case BP_RC_FULL: return m - 1;
+    case BP_RC_VALID: return 0;
+    default: return m / 2;
+    }
+}
+
+// In-place radix-2 FFT (sign -1) of a power-of-two length, FP64.
+void fft_host(std::vector<std::complex<double>>& x)
+{
+    const size_t n = x.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(x[i], x[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const double ang = -2.0 * M_PI / (double)len;
+        for (size_t i = 0; i < n; i += len) {
+            for (size_t k = 0; k < len / 2; ++k) {
+                const std::complex<double> w(std::cos(ang * (double)k), std::sin(ang * (double)k));
+                const std::complex<double> u = x[i + k], v = x[i + k + len / 2] * w;
+                x[i + k] = u + v;
+                x[i + k + len / 2] = u - v;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+struct bp_rangecomp {
+    int device = 0;
+    int chirp_size = 0, input_size = 0, max_batch = 0, mode = 0;
+    int fft_size = 0, log2l = 0, out_size = 0, offset = 0;
+    float2* H = nullptr;    // device [fft_size]
+    float2* tw = nullptr;   // device [fft_size]
+};
+
+extern "C" {
+
+int bp_rangecomp_create(int device, const float* chirp, int chirp_size, int input_size,
+                        int max_batch, int mode, bp_rangecomp** rc)
+{
+    if (rc == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    *rc = nullptr;
+    if (chirp_size < 1 || chirp == nullptr)
+        return fail(BP_ERR_INVALID_ARGUMENT, "chirp must have at least one sample");
+    if (input_size < 1) return fail(BP_ERR_DOMAIN, "number of samples must be > 0");
+    if (max_batch < 1) return fail(BP_ERR_DOMAIN, "max batch size must be > 0");
+    if (mode != BP_RC_FULL && mode != BP_RC_VALID && mode != BP_RC_SAME)
+        return fail(BP_ERR_RUNTIME, "unexpected range compression mode");
+    const int64_t full = (int64_t)chirp_size + input_size - 1;
+    int log2l = rc::kMinLog2;
+    while (log2l <= rc::kMaxLog2 && ((int64_t)1 << log2l) < full) ++log2l;
+    if (log2l > rc::kMaxLog2)
+        return fail(BP_ERR_INVALID_ARGUMENT,
+                    "range compression of " + std::to_string(full) + " convolution samples exceeds "
+                    "the supported FFT length " + std::to_string(1 << rc::kMaxLog2));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(BP_ERR_HIP, "no HIP device available (isce3_amd needs an MI355X GPU)");
+    if (device < 0 || device >= ndev) return fail(BP_ERR_INVALID_ARGUMENT, "invalid device index");
+    if (hipSetDevice(device) != hipSuccess) return fail(BP_ERR_HIP, "hipSetDevice failed");
+
+    const int L = 1 << log2l;
+    // matched filter: time-reversed conjugate of the chirp, zero padded to L,
+    // transformed, scaled by 1/L (RangeComp.cpp:22-37, :108-115)
+    std::vector<std::complex<double>> h(L, 0.0);
+    for (int i = 0; i < chirp_size; ++i) {
+        const int j = chirp_size - 1 - i;
+        h[i] = std::complex<double>(chirp[2 * j], -chirp[2 * j + 1]);
+    }
+    fft_host(h);
+    std::vector<float2> Hh(L), twh(L);
+    for (int i = 0; i < L; ++i) {
+        Hh[i] = make_float2((float)(h[i].real() / L), (float)(h[i].imag() / L));
+        const double ang = -2.0 * M_PI * (double)i / (double)L;
+        twh[i] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    auto* r = new bp_rangecomp();
+    r->device = device;
+    r->chirp_size = chirp_size;
+    r->input_size = input_size;
+    r->max_batch = max_batch;
+    r->mode = mode;
+    r->fft_size = L;
+    r->log2l = log2l;
+    r->out_size = output_size(chirp_size, input_size, mode);
+    r->offset = mode == BP_RC_FULL ? 0 : (mode == BP_RC_VALID ? chirp_size - 1 : chirp_size / 2);
+    if (hipMalloc(&r->H, sizeof(float2) * L) != hipSuccess ||
+        hipMalloc(&r->tw, sizeof(float2) * L) != hipSuccess) {
+        bp_rangecomp_destroy(r);
+        return fail(BP_ERR_OUT_OF_MEMORY, "hipMalloc failed");
+    }
+    if (hipMemcpy(r->H, Hh.data(), sizeof(float2) * L, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(r->tw, twh.data(), sizeof(float2) * L, hipMemcpyHostToDevice) != hipSuccess) {
+        bp_rangecomp_destroy(r);
+        return fail(BP_ERR_HIP, "hipMemcpy failed");
+    }
+    *rc = r;
+    return BP_SUCCESS;
+}
+
+int bp_rangecomp_destroy(bp_rangecomp* rc)
+{
+    if (rc == nullptr) return BP_SUCCESS;
+    (void)hipSetDevice(rc->device);
+    if (rc->H) (void)hipFree(rc->H);
+    if (rc->tw) (void)hipFree(rc->tw);
+    delete rc;
+    return BP_SUCCESS;
+}
+
+int bp_rangecomp_sizes(const bp_rangecomp* rc, int32_t* sizes)
+{
+    if (rc == nullptr || sizes == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    sizes[0] = rc->chirp_size;
+    sizes[1] = rc->input_size;
+    sizes[2] = rc->fft_size;
+    sizes[3] = rc->max_batch;
+    sizes[4] = rc->mode;
+    sizes[5] = rc->out_size;
+    sizes[6] = first_valid(rc->chirp_size, rc->mode);
+    return BP_SUCCESS;
+}
+
+int bp_rangecomp_run(bp_rangecomp* rc, void* out_dev, const void* in_dev, int batch,
+                     void* stream)
+{
+    if (rc == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    if (batch > rc->max_batch) return fail(BP_ERR_LENGTH, "batch size exceeds max batch");
+    if (batch < 0) return fail(BP_ERR_INVALID_ARGUMENT, "negative batch size");
+    if (batch == 0) return BP_SUCCESS;
+    if (out_dev == nullptr || in_dev == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null buffer");
+    if (hipSetDevice(rc->device) != hipSuccess) return fail(BP_ERR_HIP, "hipSetDevice failed");
+    rc::ConvArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = static_cast<const float2*>(in_dev);
+    a.out = static_cast<float2*>(out_dev);
+    a.H = rc->H;
+    a.tw = rc->tw;
+    a.n_in = rc->input_size;
+    a.n_out = rc->out_size;
+    a.offset = rc->offset;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = rc::launch_conv(a, rc->log2l, batch, s);
+    if (e == hipSuccess && s == nullptr) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("range compression launch: ") + hipGetErrorString(e));
+    return BP_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,247 @@
+// Range compression on gfx950: frequency-domain matched filtering of a batch
+// of complex64 pulses (the operation of cxx/isce3/focus/RangeComp.cpp:93-131),
+// one workgroup per pulse and one pass over HBM.
+//
+// A workgroup of L/16 threads loads its pulse zero-padded to L = 2^p samples
+// into LDS, transforms it with a Stockham FFT (radix-16 passes and one
+// radix-2/4/8 pass, each thread holding 16 values per pass), multiplies by
+// the matched filter's spectrum (1/L folded in), conjugates, transforms again
+// (conj(FFT(conj(X))) = L IFFT(X)) and writes the conjugated crop of the
+// mode.  Twiddles W_L^k come from a table in global memory (L2 resident);
+// radix-16/8 butterflies read W^t and W^(4t) and form the other powers by at
+// most two complex products.  LDS addresses are padded by one slot every 16
+// to spread the strided stores of the early passes over the banks.
+#include <hip/hip_runtime.h>
+
+#include "rc_internal.h"
+
+namespace rc {
+namespace {
+
+__device__ inline float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a * (-i)
+__device__ inline float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+__device__ inline float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+__device__ inline int pad(int i) { return i + (i >> 4); }
+
+// Forward DFTs (sign -1) of R values in registers.
+__device__ inline void dft2(float2& a, float2& b)
+{
+    const float2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+__device__ inline void dft4(float2& x0, float2& x1, float2& x2, float2& x3)
+{
+    const float2 a0 = cadd(x0, x2), a1 = csub(x0, x2);
+    const float2 a2 = cadd(x1, x3), a3 = mul_mi(csub(x1, x3));
+    x0 = cadd(a0, a2);
+    x2 = csub(a0, a2);
+    x1 = cadd(a1, a3);
+    x3 = csub(a1, a3);
+}
+
+template<int R>
+__device__ inline void dft(float2 (&v)[R]);
+
+template<>
+__device__ inline void dft<2>(float2 (&v)[2])
+{
+    dft2(v[0], v[1]);
+}
+
+template<>
+__device__ inline void dft<4>(float2 (&v)[4])
+{
+    dft4(v[0], v[1], v[2], v[3]);
+}
+
+template<>
+__device__ inline void dft<8>(float2 (&v)[8])
+{
+    constexpr float c = 0.70710678118654752f;
+    // even and odd halves
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4(e0, e1, e2, e3);
+    dft4(o0, o1, o2, o3);
+    // W8^k o_k
+    o1 = make_float2(c * (o1.x + o1.y), c * (o1.y - o1.x));
+    o2 = mul_mi(o2);
+    o3 = make_float2(c * (o3.y - o3.x), -c * (o3.x + o3.y));
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+}
+
+template<>
+__device__ inline void dft<16>(float2 (&v)[16])
+{
+    // n = 4 n1 + n2: DFT4 over n1, twiddle W16^(k1 n2), DFT4 over n2;
+    // output k = k1 + 4 k2
+    constexpr float c = 0.70710678118654752f;   // cos(pi/4)
+    constexpr float c1 = 0.92387953251128676f;  // cos(pi/8)
+    constexpr float s1 = 0.38268343236508977f;  // sin(pi/8)
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) dft4(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    // v[4 k1 + n2] now holds A[k1][n2]; multiply by W16^(k1 n2) = exp(-i pi k1 n2 / 8)
+    auto rot = [](float2 a, float cs, float sn) {  // a * (cs - i sn)
+        return make_float2(a.x * cs + a.y * sn, a.y * cs - a.x * sn);
+    };
+    v[5] = rot(v[5], c1, s1);     // k1=1, n2=1: W^1
+    v[6] = rot(v[6], c, c);       // W^2
+    v[7] = rot(v[7], s1, c1);     // W^3
+    v[9] = rot(v[9], c, c);       // k1=2, n2=1: W^2
+    v[10] = mul_mi(v[10]);        // W^4
+    v[11] = rot(v[11], -c, c);    // W^6
+    v[13] = rot(v[13], s1, c1);   // k1=3, n2=1: W^3
+    v[14] = rot(v[14], -c, c);    // W^6
+    v[15] = rot(v[15], -c1, -s1); // W^9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) dft4(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+    // v[4 k1 + k2] holds y[k1 + 4 k2]: transpose the 4 x 4 block
+    float2 t[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) t[i] = v[i];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[4 * k1 + k2];
+}
+
+// v[r] *= W^(r t) with W = exp(-2 pi i / L), from the table tw[k] = W^k.
+template<int R>
+__device__ inline void twiddle(float2 (&v)[R], const float2* __restrict__ tw, int t)
+{
+    const float2 w1 = tw[t];
+    if (R == 2) {
+        v[1] = cmul(v[1], w1);
+        return;
+    }
+    const float2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+    if (R == 4) {
+        v[1] = cmul(v[1], w1);
+        v[2] = cmul(v[2], w2);
+        v[3] = cmul(v[3], w3);
+        return;
+    }
+    const float2 w4 = tw[4 * t];
+    float2 w[R];
+    w[1] = w1;
+    w[2] = w2;
+    w[3] = w3;
+    w[4] = w4;
+#pragma unroll
+    for (int r = 5; r < 8 && r < R; ++r) w[r] = cmul(w4, w[r - 4]);
+    if (R == 16) {
+        const float2 w8 = cmul(w4, w4), w12 = cmul(w8, w4);
+        w[8] = w8;
+        w[12] = w12;
+#pragma unroll
+        for (int r = 1; r < 4; ++r) {
+            w[8 + r] = cmul(w8, w[r]);
+            w[12 + r] = cmul(w12, w[r]);
+        }
+    }
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
+}
+
+// One Stockham pass of radix R at sub-transform size Ns over L values in
+// padded LDS, T = L / 16 threads, each taking 16 / R butterflies.
+template<int R, int L>
+__device__ inline void stockham_pass(float2* s, int Ns, const float2* __restrict__ tw)
+{
+    constexpr int T = L / 16;
+    constexpr int NB = L / R;
+    constexpr int Q = NB / T;
+    float2 v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = threadIdx.x + q * T;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q][r] = s[pad(j + r * NB)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int j = threadIdx.x + q * T;
+        const int k = j & (Ns - 1);
+        if (Ns > 1) twiddle<R>(v[q], tw, k * (L / (Ns * R)));
+        dft<R>(v[q]);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[pad(base + r * Ns)] = v[q][r];
+    }
+    __syncthreads();
+}
+
+// Forward FFT of L = 2^LOG2L (10 <= LOG2L <= 14) values in padded LDS.
+template<int LOG2L>
+__device__ inline void fft(float2* s, const float2* __restrict__ tw)
+{
+    constexpr int L = 1 << LOG2L;
+    constexpr int NR16 = LOG2L / 4;
+    constexpr int REM = LOG2L % 4;
+    int Ns = 1;
+#pragma unroll
+    for (int p = 0; p < NR16; ++p) {
+        stockham_pass<16, L>(s, Ns, tw);
+        Ns *= 16;
+    }
+    if (REM == 1) stockham_pass<2, L>(s, Ns, tw);
+    if (REM == 2) stockham_pass<4, L>(s, Ns, tw);
+    if (REM == 3) stockham_pass<8, L>(s, Ns, tw);
+}
+
+template<int LOG2L>
+__global__ __launch_bounds__((1 << LOG2L) / 16) void conv_kernel(ConvArgs a)
+{
+    constexpr int L = 1 << LOG2L;
+    constexpr int T = L / 16;
+    __shared__ float2 s[L + L / 16];
+    const int64_t b = blockIdx.x;
+    const float2* x = a.in + b * a.n_in;
+    for (int i = threadIdx.x; i < L; i += T) s[pad(i)] = i < a.n_in ? x[i] : make_float2(0.f, 0.f);
+    __syncthreads();
+    fft<LOG2L>(s, a.tw);
+    for (int i = threadIdx.x; i < L; i += T) s[pad(i)] = conjf2(cmul(s[pad(i)], a.H[i]));
+    __syncthreads();
+    fft<LOG2L>(s, a.tw);
+    float2* y = a.out + b * a.n_out;
+    for (int i = threadIdx.x; i < a.n_out; i += T) {
+        const int p = a.offset + i;
+        y[i] = p < L ? conjf2(s[pad(p)]) : make_float2(0.f, 0.f);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_conv(const ConvArgs& a, int log2l, int batch, hipStream_t stream)
+{
+    if (batch <= 0) return hipSuccess;
+    const dim3 grid((unsigned)batch);
+    switch (log2l) {
+    case 10: hipLaunchKernelGGL(conv_kernel<10>, grid, dim3(1 << 6), 0, stream, a); break;
+    case 11: hipLaunchKernelGGL(conv_kernel<11>, grid, dim3(1 << 7), 0, stream, a); break;
+    case 12: hipLaunchKernelGGL(conv_kernel<12>, grid, dim3(1 << 8), 0, stream, a); break;
+    case 13: hipLaunchKernelGGL(conv_kernel<13>, grid, dim3(1 << 9), 0, stream, a); break;
+    case 14: hipLaunchKernelGGL(conv_kernel<14>, grid, dim3(1 << 10), 0, stream, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace rc

@@ -516,3 +516,8 @@ class Scene:
 
     def __exit__(self, *exc):
         self.close()
+
+
+# range compression, the step before backprojection (isce3.focus.RangeComp,
+# isce3.focus.form_linear_chirp)
+from .rangecomp import RangeComp, form_linear_chirp  # noqa: E402,F401
