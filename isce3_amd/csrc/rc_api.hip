@@ -1,7 +1,8 @@
 // C-ABI of range compression (include/isce3_amd_rangecomp.h): argument rules
 // of cxx/isce3/focus/RangeComp.cpp:39-79 and :93-99, the matched filter's
 // spectrum formed on the host in FP64 (RangeComp.cpp:22-37), and the launch
-// of the fused FFT-convolution kernel (rc_kernels.hip).
+// of the FFT-convolution kernels (rc_kernels.hip): one fused kernel for FFT
+// lengths up to 2^14, three four-step passes through a work buffer above.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -73,9 +74,19 @@ struct bp_rangecomp {
     int device = 0;
     int chirp_size = 0, input_size = 0, max_batch = 0, mode = 0;
     int fft_size = 0, log2l = 0, out_size = 0, offset = 0;
-    float2* H = nullptr;    // device [fft_size]
+    float2* H = nullptr;    // device [fft_size] (four steps: permuted)
     float2* tw = nullptr;   // device [fft_size]
+    float2* tw2 = nullptr;  // device [fft_size / 32] (four steps)
+    float2* work = nullptr; // device [work_batch x fft_size] (four steps)
+    int work_batch = 0;     // pulses per four-step launch
 };
+
+namespace {
+
+// Bytes of the four-step work buffer at most (it holds whole padded pulses).
+constexpr size_t kWorkBytes = size_t(512) << 20;
+
+}  // namespace
 
 extern "C" {
 
@@ -112,11 +123,19 @@ int bp_rangecomp_create(int device, const float* chirp, int chirp_size, int inpu
         h[i] = std::complex<double>(chirp[2 * j], -chirp[2 * j + 1]);
     }
     fft_host(h);
-    std::vector<float2> Hh(L), twh(L);
+    const bool four_step = log2l > rc::kMaxFusedLog2;
+    const int L2 = L / 32;
+    std::vector<float2> Hh(L), twh(L), tw2h(four_step ? L2 : 0);
     for (int i = 0; i < L; ++i) {
-        Hh[i] = make_float2((float)(h[i].real() / L), (float)(h[i].imag() / L));
+        // four steps read bin k1 + 32 k2 at k1 L2 + k2
+        const int k = four_step ? (i / L2) + 32 * (i % L2) : i;
+        Hh[i] = make_float2((float)(h[k].real() / L), (float)(h[k].imag() / L));
         const double ang = -2.0 * M_PI * (double)i / (double)L;
         twh[i] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    for (int i = 0; i < (int)tw2h.size(); ++i) {
+        const double ang = -2.0 * M_PI * (double)i / (double)L2;
+        tw2h[i] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
     auto* r = new bp_rangecomp();
     r->device = device;
@@ -128,13 +147,21 @@ int bp_rangecomp_create(int device, const float* chirp, int chirp_size, int inpu
     r->log2l = log2l;
     r->out_size = output_size(chirp_size, input_size, mode);
     r->offset = mode == BP_RC_FULL ? 0 : (mode == BP_RC_VALID ? chirp_size - 1 : chirp_size / 2);
+    if (four_step)
+        r->work_batch = (int)std::max<size_t>(
+            1, std::min<size_t>((size_t)max_batch, kWorkBytes / (sizeof(float2) * (size_t)L)));
     if (hipMalloc(&r->H, sizeof(float2) * L) != hipSuccess ||
-        hipMalloc(&r->tw, sizeof(float2) * L) != hipSuccess) {
+        hipMalloc(&r->tw, sizeof(float2) * L) != hipSuccess ||
+        (four_step && (hipMalloc(&r->tw2, sizeof(float2) * L2) != hipSuccess ||
+                       hipMalloc(&r->work, sizeof(float2) * (size_t)L * r->work_batch) !=
+                           hipSuccess))) {
         bp_rangecomp_destroy(r);
         return fail(BP_ERR_OUT_OF_MEMORY, "hipMalloc failed");
     }
     if (hipMemcpy(r->H, Hh.data(), sizeof(float2) * L, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(r->tw, twh.data(), sizeof(float2) * L, hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(r->tw, twh.data(), sizeof(float2) * L, hipMemcpyHostToDevice) != hipSuccess ||
+        (four_step && hipMemcpy(r->tw2, tw2h.data(), sizeof(float2) * L2,
+                                hipMemcpyHostToDevice) != hipSuccess)) {
         bp_rangecomp_destroy(r);
         return fail(BP_ERR_HIP, "hipMemcpy failed");
     }
@@ -148,6 +175,8 @@ int bp_rangecomp_destroy(bp_rangecomp* rc)
     (void)hipSetDevice(rc->device);
     if (rc->H) (void)hipFree(rc->H);
     if (rc->tw) (void)hipFree(rc->tw);
+    if (rc->tw2) (void)hipFree(rc->tw2);
+    if (rc->work) (void)hipFree(rc->work);
     delete rc;
     return BP_SUCCESS;
 }
@@ -183,8 +212,21 @@ int bp_rangecomp_run(bp_rangecomp* rc, void* out_dev, const void* in_dev, int ba
     a.n_in = rc->input_size;
     a.n_out = rc->out_size;
     a.offset = rc->offset;
+    a.tw2 = rc->tw2;
+    a.work = rc->work;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipError_t e = rc::launch_conv(a, rc->log2l, batch, s);
+    hipError_t e = hipSuccess;
+    if (rc->log2l <= rc::kMaxFusedLog2) {
+        e = rc::launch_conv(a, rc->log2l, batch, s);
+    } else {
+        // sub-batches of the work buffer, launched in order on one stream
+        for (int b0 = 0; b0 < batch && e == hipSuccess; b0 += rc->work_batch) {
+            rc::ConvArgs sub = a;
+            sub.in = a.in + (int64_t)b0 * a.n_in;
+            sub.out = a.out + (int64_t)b0 * a.n_out;
+            e = rc::launch_four_step(sub, rc->log2l, std::min(rc->work_batch, batch - b0), s);
+        }
+    }
     if (e == hipSuccess && s == nullptr) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("range compression launch: ") + hipGetErrorString(e));
     return BP_SUCCESS;

@@ -227,7 +227,156 @@ __global__ __launch_bounds__((1 << LOG2L) / 16) void conv_kernel(ConvArgs a)
     }
 }
 
+// ------------------------------------------------------------------------
+// Lengths L = 32 L2 above 2^14 (four steps; L2 = 2^10 .. 2^14).  Sample n of a
+// pulse is element (n1, n2) = (n / L2, n % L2) of a 32 x L2 matrix and
+// spectrum bin k = k1 + 32 k2 is element (k1, k2):
+//   col_fwd   per column n2: 32-point DFT over n1, times W_L^(n2 k1)   -> Y[k1][n2]
+//   row       per row k1: L2-point DFT over n2 -> bins (k1, k2), times the
+//             permuted filter spectrum, conjugate, L2-point DFT        -> Z[k1][n2]
+//   col_inv   per column n2: Z times W_L^(n2 k1), 32-point DFT over k1,
+//             conjugate: sample L2 n1 + n2 of the convolution, cropped
+// (the inverse transform is conj(DFT(conj(.))) with 1/L folded into the
+// spectrum; the conjugations of the two inverse steps cancel except the last).
+
+template<>
+__device__ inline void dft<32>(float2 (&v)[32])
+{
+    float2 e[16], o[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        e[i] = v[2 * i];
+        o[i] = v[2 * i + 1];
+    }
+    dft<16>(e);
+    dft<16>(o);
+    // W32^k = exp(-2 pi i k / 32) = cos(pi k / 16) - i sin(pi k / 16)
+    constexpr float kc[16] = {1.0f, 0.98078528040323043f, 0.92387953251128674f,
+                              0.83146961230254524f, 0.70710678118654752f, 0.55557023301960218f,
+                              0.38268343236508977f, 0.19509032201612827f, 0.0f,
+                              -0.19509032201612827f, -0.38268343236508977f, -0.55557023301960218f,
+                              -0.70710678118654752f, -0.83146961230254524f, -0.92387953251128674f,
+                              -0.98078528040323043f};
+    constexpr float ks[16] = {0.0f, 0.19509032201612827f, 0.38268343236508977f,
+                              0.55557023301960218f, 0.70710678118654752f, 0.83146961230254524f,
+                              0.92387953251128674f, 0.98078528040323043f, 1.0f,
+                              0.98078528040323043f, 0.92387953251128674f, 0.83146961230254524f,
+                              0.70710678118654752f, 0.55557023301960218f, 0.38268343236508977f,
+                              0.19509032201612827f};
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const float cs = kc[k], sn = ks[k];
+        const float2 t = make_float2(o[k].x * cs + o[k].y * sn, o[k].y * cs - o[k].x * sn);
+        v[k] = cadd(e[k], t);
+        v[k + 16] = csub(e[k], t);
+    }
+}
+
+// v[k] *= W_L^(n k) for k < 32 from the table tw[j] = W_L^j, j < L: three
+// reads (W^n, W^4n, W^16n) and at most four products per factor.
+__device__ inline void twiddle32(float2 (&v)[32], const float2* __restrict__ tw, int n)
+{
+    const float2 w1 = tw[n], w4 = tw[4 * n], w16 = tw[16 * n];
+    const float2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+    const float2 w8 = cmul(w4, w4), w12 = cmul(w8, w4);
+    const float2 lo[4] = {make_float2(1.f, 0.f), w1, w2, w3};
+    const float2 mid[4] = {make_float2(1.f, 0.f), w4, w8, w12};
+#pragma unroll
+    for (int k = 1; k < 32; ++k) {
+        float2 w = mid[(k >> 2) & 3];
+        if (k & 3) w = (k >> 2) & 3 ? cmul(w, lo[k & 3]) : lo[k & 3];
+        if (k & 16) w = cmul(w, w16);
+        v[k] = cmul(v[k], w);
+    }
+}
+
+template<int LOG2L2>
+__global__ __launch_bounds__(256) void col_fwd_kernel(ConvArgs a)
+{
+    constexpr int L2 = 1 << LOG2L2;
+    const int n2 = blockIdx.x * 256 + threadIdx.x;
+    const int64_t b = blockIdx.y;
+    const float2* x = a.in + b * a.n_in;
+    float2 v[32];
+#pragma unroll
+    for (int n1 = 0; n1 < 32; ++n1) {
+        const int idx = n1 * L2 + n2;
+        v[n1] = idx < a.n_in ? x[idx] : make_float2(0.f, 0.f);
+    }
+    dft<32>(v);
+    twiddle32(v, a.tw, n2);
+    float2* y = a.work + b * (int64_t)(32 * L2);
+#pragma unroll
+    for (int k1 = 0; k1 < 32; ++k1) y[(int64_t)k1 * L2 + n2] = v[k1];
+}
+
+template<int LOG2L2>
+__global__ __launch_bounds__((1 << LOG2L2) / 16) void row_kernel(ConvArgs a)
+{
+    constexpr int L2 = 1 << LOG2L2;
+    constexpr int T = L2 / 16;
+    __shared__ float2 s[L2 + L2 / 16];
+    const int k1 = blockIdx.x;
+    float2* row = a.work + (int64_t)blockIdx.y * (32 * L2) + (int64_t)k1 * L2;
+    for (int i = threadIdx.x; i < L2; i += T) s[pad(i)] = row[i];
+    __syncthreads();
+    fft<LOG2L2>(s, a.tw2);
+    const float2* h = a.H + (int64_t)k1 * L2;   // permuted spectrum: bin k1 + 32 k2 at [k1][k2]
+    for (int i = threadIdx.x; i < L2; i += T) s[pad(i)] = conjf2(cmul(s[pad(i)], h[i]));
+    __syncthreads();
+    fft<LOG2L2>(s, a.tw2);
+    for (int i = threadIdx.x; i < L2; i += T) row[i] = s[pad(i)];
+}
+
+template<int LOG2L2>
+__global__ __launch_bounds__(256) void col_inv_kernel(ConvArgs a)
+{
+    constexpr int L2 = 1 << LOG2L2;
+    const int n2 = blockIdx.x * 256 + threadIdx.x;
+    const int64_t b = blockIdx.y;
+    const float2* z = a.work + b * (int64_t)(32 * L2);
+    float2 v[32];
+#pragma unroll
+    for (int k1 = 0; k1 < 32; ++k1) v[k1] = z[(int64_t)k1 * L2 + n2];
+    twiddle32(v, a.tw, n2);
+    dft<32>(v);
+    float2* y = a.out + b * a.n_out;
+#pragma unroll
+    for (int n1 = 0; n1 < 32; ++n1) {
+        const int i = n1 * L2 + n2 - a.offset;
+        if (i >= 0 && i < a.n_out) y[i] = conjf2(v[n1]);
+    }
+    // outputs past the transform (valid mode with a chirp over twice the
+    // pulse length) are zero, as in the fused kernel
+    for (int i = max(0, 32 * L2 - a.offset) + n2; i < a.n_out; i += L2)
+        y[i] = make_float2(0.f, 0.f);
+}
+
 }  // namespace
+
+hipError_t launch_four_step(const ConvArgs& a, int log2l, int batch, hipStream_t stream)
+{
+    if (batch <= 0) return hipSuccess;
+    const int log2l2 = log2l - 5;
+    const dim3 cols((unsigned)((1 << log2l2) / 256), (unsigned)batch);
+    const dim3 rows(32u, (unsigned)batch);
+    switch (log2l2) {
+#define RC_FOUR_STEP(P)                                                                      \
+    case P:                                                                                  \
+        hipLaunchKernelGGL(col_fwd_kernel<P>, cols, dim3(256), 0, stream, a);                \
+        hipLaunchKernelGGL(row_kernel<P>, rows, dim3((1 << P) / 16), 0, stream, a);          \
+        hipLaunchKernelGGL(col_inv_kernel<P>, cols, dim3(256), 0, stream, a);                \
+        break;
+        RC_FOUR_STEP(10)
+        RC_FOUR_STEP(11)
+        RC_FOUR_STEP(12)
+        RC_FOUR_STEP(13)
+        RC_FOUR_STEP(14)
+#undef RC_FOUR_STEP
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_conv(const ConvArgs& a, int log2l, int batch, hipStream_t stream)
 {

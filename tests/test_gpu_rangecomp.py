@@ -2,7 +2,7 @@
 cxx/isce3/focus/RangeComp.cpp) and the reference's own known answers
 (tests/cxx/isce3/focus/rangecomp.cpp, tests/python/extensions/pybind/focus/
 rangecomp.py).  Tolerance: complex relative error <= 1e-5 against the FP64
-oracle (FP32 FFTs of up to 2^14 points)."""
+oracle (FP32 FFTs of up to 2^19 points)."""
 import numpy as np
 import pytest
 
@@ -107,3 +107,47 @@ def test_device_tensors_in_place_and_batch_limit(torch_gpu):
                          np.zeros((5, 2000), np.complex64))
     with pytest.raises(errors.LengthError):
         rc.rangecompress(np.zeros((4, rc.output_size + 1), np.complex64), x)
+
+
+@pytest.mark.parametrize("mode", ["full", "valid", "same"])
+@pytest.mark.parametrize("nin,batch", [(20000, 3), (40000, 2), (100000, 1), (250000, 2),
+                                       (500000, 1)])
+def test_long_pulses_four_step_match_fp64_oracle(torch_gpu, mode, nin, batch):
+    """FFT lengths 2^15 .. 2^19 (the four-step path)."""
+    from isce3_amd.focus import RangeComp, form_linear_chirp
+    rng = np.random.default_rng(nin + 7 * batch)
+    chirp = form_linear_chirp(0.6e12, 40e-6, 24e6)               # 961 samples
+    x = (rng.standard_normal((batch, nin)) + 1j * rng.standard_normal((batch, nin))).astype(np.complex64)
+    rc = RangeComp(chirp, nin, maxbatch=batch, mode=RangeComp.Mode[mode.capitalize()])
+    assert rc.fft_size >= nin + len(chirp) - 1 > 16384
+    y = np.zeros((batch, rc.output_size), np.complex64)
+    rc.rangecompress(y, x)
+    ref = rco.rangecompress(chirp, x, MODES[mode], "fp64")
+    e = rel_err(y, ref)
+    print(f"[rangecomp long] mode={mode} nin={nin} batch={batch} fft={rc.fft_size} "
+          f"rel={e:.2e}", flush=True)
+    assert y.shape == ref.shape
+    assert e <= REL
+
+
+def test_long_pulses_more_than_one_work_batch(torch_gpu):
+    """2^19-point FFTs: the 512 MiB work buffer holds 128 pulses, so a batch of
+    130 runs as two sub-batches; pulses either side of the seam are checked."""
+    from isce3_amd.focus import RangeComp, form_linear_chirp
+    torch = torch_gpu
+    nin, batch = 300000, 130
+    chirp = form_linear_chirp(0.6e12, 40e-6, 24e6)
+    rc = RangeComp(chirp, nin, maxbatch=batch, mode=RangeComp.Mode.Same)
+    assert rc.fft_size == 1 << 19
+    g = torch.Generator(device="cuda:0").manual_seed(3)
+    d_in = torch.randn((batch, nin), dtype=torch.complex64, device="cuda:0", generator=g)
+    d_out = torch.empty((batch, rc.output_size), dtype=torch.complex64, device="cuda:0")
+    rc.rangecompress(d_out, d_in)
+    rows = [0, 1, 126, 127, 128, 129]
+    x = d_in[rows].cpu().numpy()
+    y = d_out[rows].cpu().numpy()
+    ref = rco.rangecompress(chirp, x, rco.SAME, "fp64")
+    for i, r in enumerate(rows):
+        e = rel_err(y[i], ref[i])
+        print(f"[rangecomp seam] pulse {r} rel={e:.2e}", flush=True)
+        assert e <= REL

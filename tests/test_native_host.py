@@ -98,7 +98,10 @@ def test_ellipsoid_matches_oracle():
         back = e.xyz_to_lon_lat(xyz)
         lo = D3()
         oracle.lib().orc_xyz_to_lonlat(e.a, e.e2, D3(*xyz), lo)
-        np.testing.assert_allclose(back, list(lo), rtol=0, atol=1e-9)
+        # the product code takes a cube root where the reference (and the
+        # oracle) call pow(x, 1/3): heights agree to ~1e-9 m
+        np.testing.assert_allclose(back[:2], list(lo)[:2], rtol=0, atol=1e-12)
+        assert abs(back[2] - lo[2]) < 1e-8
 
 
 def test_dem_and_lut_match_oracle():

@@ -18,7 +18,7 @@ def test_constructor_argument_rules():
     with pytest.raises(RuntimeError):
         RangeComp(chirp, 10, mode=7)
     with pytest.raises(errors.InvalidArgument):
-        RangeComp(chirp, 20000)          # 20004-sample convolution > 2^14-point FFTs
+        RangeComp(chirp, 524285)         # 524289-sample convolution > 2^19-point FFTs
     with pytest.raises(errors.InvalidArgument):
         RangeComp(np.zeros(0, np.complex64), 10)
 
