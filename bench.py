@@ -397,6 +397,66 @@ def run_rangecomp(args, torch, np, dev):
     return res
 
 
+# ------------------------------------------------------- RSLC block finishing
+def run_rslc(args, torch, np, dev):
+    """Finishing of a 16384 x 16384 complex64 focused block resident in HBM:
+    range-correction product, block statistics and complex32 encoding in one
+    pass (the per-block work of the RSLC writer).  HBM-bound: the roofline
+    counts 8 B read + 8 B written back + 4 B encoded per sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import rslc as ro
+    from isce3_amd.rslc import BlockFinisher
+    n = 16384
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(13)
+    z = torch.randn((n, n), dtype=torch.complex64, device=dev, generator=gen)
+    ph = torch.rand((n,), device=dev, generator=gen, dtype=torch.float64) * 6.283185307179586
+    rc = torch.polar(torch.ones_like(ph), ph).to(torch.complex64)
+    f = BlockFinisher(n * n)
+    stream = torch.cuda.current_stream()
+    for _ in range(max(args.warmup, 1)):
+        enc, st = f.finish(z, range_cor=rc, data_type="complex32")
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        enc, st = f.finish(z, range_cor=rc, data_type="complex32")
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = sum(times) / len(times)
+    nbytes = n * n * (8 + 8 + 4)
+    # parity on a band of rows: one more pass from a saved copy, against the oracle
+    rows = slice(n // 2, n // 2 + 64)
+    zb = z[rows].cpu().numpy()
+    zo, enc_o, ex = ro.finish_block(zb, rc.cpu().numpy(), None, "complex32")
+    enc_b, stb = BlockFinisher(zb.size).finish(zb.copy(), range_cor=rc.cpu().numpy(),
+                                               data_type="complex32")
+    bit_exact = bool(np.array_equal(enc_b.view(np.uint16), enc_o.view(np.uint16)))
+    # CPU baseline: the oracle (numpy, one thread) on 1024 rows
+    nb = 1024
+    zc = z[:nb].cpu().numpy()
+    rcc = rc.cpu().numpy()
+    t0 = time.perf_counter()
+    ro.finish_block(zc, rcc, None, "complex32")
+    cpu_s = time.perf_counter() - t0
+    res = {"workload": f"{n}x{n} c64 focused block: range correction, StatsRealImag, complex32",
+           "ms": ms, "samples_per_s": n * n / (ms * 1e-3),
+           "kernel": "rslc::finish_kernel (+ one-workgroup merge_kernel)",
+           "roofline": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": 8000.0,
+                        "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 8e12, "traffic": None},
+           "bit_exact_vs_oracle": bit_exact, "n_valid": st.n_valid,
+           "cpu_baseline": {"value": nb * n / cpu_s, "unit": "samples/s", "cores": 1,
+                            "kind": "port", "sample": f"{nb} x {n} samples, numpy oracle"}}
+    log(f"rslc: {ms:.3f} ms per {n}x{n} block, {res['roofline']['achieved']:.0f} GB/s, "
+        f"bit-exact {bit_exact}, CPU {nb * n / cpu_s:.3g} samples/s")
+    f.close()
+    del z, enc
+    torch.cuda.empty_cache()
+    return res
+
+
 def split_kernel_name(opts) -> str:
     """The coherent-sum kernel the split step ran with these launch options
     (kernel width 8)."""
@@ -429,6 +489,9 @@ def main():
     ap.add_argument("--no-rangecomp", action="store_true")
     ap.add_argument("--only-rangecomp", action="store_true",
                     help="time the range compressor only (profiling runs)")
+    ap.add_argument("--no-rslc", action="store_true")
+    ap.add_argument("--only-rslc", action="store_true",
+                    help="time the RSLC block finisher only (profiling runs)")
     ap.add_argument("--rates", type=lambda s: [float(x) for x in s.split(",")],
                     default=[20.0, 50.0, 100.0])
     ap.add_argument("--serve-seconds", type=float, default=4.0,
@@ -463,6 +526,9 @@ def main():
     if args.only_rangecomp:
         print(json.dumps({"rangecomp": run_rangecomp(args, torch, np, dev)}))
         return
+    if args.only_rslc:
+        print(json.dumps({"rslc": run_rslc(args, torch, np, dev)}))
+        return
 
     out_dir = os.path.join(ROOT, "gpurun_out")
     tuner = autotune.Autotuner(os.path.join(out_dir, "autotune.json")
@@ -484,6 +550,8 @@ def main():
     serve_res = None if args.no_serve else run_serve(args, torch, np, dev, tuner)
     torch.cuda.empty_cache()
     rc_res = None if args.no_rangecomp else run_rangecomp(args, torch, np, dev)
+    torch.cuda.empty_cache()
+    rslc_res = None if args.no_rslc else run_rslc(args, torch, np, dev)
 
     sp, pl = results["split"], results["plain"]
     achieved = sp["updates"] * FLOPS_PER_UPDATE / (sp["sum_ms"] * 1e-3) / 1e12
@@ -522,6 +590,7 @@ def main():
         "batch": batch,
         "serve": serve_res,
         "rangecomp": rc_res,
+        "rslc": rslc_res,
         "cpu_baseline": cpu,
     }
     if cpu:

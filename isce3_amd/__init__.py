@@ -9,8 +9,12 @@ The look-alike host types mirror isce3's module layout:
     isce3_amd.product    RadarGridParameters
     isce3_amd.container  RadarGeometry
     isce3_amd.focus      backproject, backproject_blocks, Scene, RangeComp,
-                         form_linear_chirp
+                         form_linear_chirp, get_sar_duration, get_radar_velocities
+    isce3_amd.rslc       BackgroundWriter, StatsRealImagFloat32, to_complex32,
+                         truncate_mantissa (block finishing on the GPU)
+    isce3_amd.planning   plan_processing_blocks and the workflow's parameter helpers
 """
-from . import container, core, errors, focus, geometry, product, rangecomp  # noqa: F401
+from . import (container, core, errors, focus, geometry, planning, product,  # noqa: F401
+               rangecomp, rslc)
 
 __version__ = "0.1.0"

@@ -53,6 +53,16 @@ class RadarGeometry_t(C.Structure):
     _fields_ = [("grid", Grid_t), ("orbit", Orbit_t), ("doppler", Lut2d_t)]
 
 
+class StatsF32_t(C.Structure):
+    _fields_ = [("min", C.c_float), ("max", C.c_float), ("mean", C.c_float),
+                ("pad_", C.c_int32), ("n_valid", C.c_int64), ("real_valued_mean", C.c_double),
+                ("square_diff_sum", C.c_double)]
+
+
+class StatsRealImagF32_t(C.Structure):
+    _fields_ = [("real", StatsF32_t), ("imag", StatsF32_t), ("n_valid", C.c_int64)]
+
+
 class Dem_t(C.Structure):
     _fields_ = [("have_raster", C.c_int32), ("epsg", C.c_int32), ("method", C.c_int32),
                 ("pad_", C.c_int32), ("ref_height", C.c_double), ("xstart", C.c_double),
@@ -119,6 +129,11 @@ _SIGNATURES = {
     "bp_rangecomp_destroy": (C.c_int, [C.c_void_p]),
     "bp_rangecomp_sizes": (C.c_int, [C.c_void_p, _P(C.c_int32)]),
     "bp_rangecomp_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "bp_rslc_finisher_create": (C.c_int, [C.c_int, C.c_int64, _P(C.c_void_p)]),
+    "bp_rslc_finisher_destroy": (C.c_int, [C.c_void_p]),
+    "bp_rslc_finish_block": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
+                                       C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                       _P(StatsRealImagF32_t), C.c_void_p]),
     "bp_orbit_interpolate": (C.c_int, [_P(Orbit_t), C.c_double, C.c_int, _P(C.c_double),
                                        _P(C.c_double)]),
     "bp_lonlat_to_xyz": (None, [C.c_double, C.c_double, _P(C.c_double), _P(C.c_double)]),

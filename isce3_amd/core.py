@@ -555,3 +555,49 @@ class Ellipsoid:
         out = np.empty(3)
         _native.lib().bp_xyz_to_lonlat(self.a, self.e2, _native.dptr(xyz), _native.dptr(out))
         return out
+
+    def r_east(self, lat: float) -> float:
+        """Prime-vertical radius (core/Ellipsoid.h:117-120)."""
+        return self.a / math.sqrt(1.0 - self.e2 * math.sin(lat) ** 2)
+
+    def r_north(self, lat: float) -> float:
+        """Meridional radius (core/Ellipsoid.h:127-130)."""
+        return (self.a * (1.0 - self.e2)) / (1.0 - self.e2 * math.sin(lat) ** 2) ** 1.5
+
+    def r_dir(self, hdg: float, lat: float) -> float:
+        """Directional radius at heading `hdg` (clockwise from north,
+        core/Ellipsoid.h:138-143)."""
+        re, rn = self.r_east(lat), self.r_north(lat)
+        return (re * rn) / (re * math.cos(hdg) ** 2 + rn * math.sin(hdg) ** 2)
+
+    def n_vector(self, lon: float, lat: float) -> np.ndarray:
+        """Outward unit normal at (lon, lat) (core/Ellipsoid.h:152-158)."""
+        clat = math.cos(lat)
+        return np.array([clat * math.cos(lon), clat * math.sin(lon), math.sin(lat)])
+
+
+class Basis:
+    """Orthonormal basis x0, x1, x2 (core/Basis.h).  ``Basis(p, v)`` is the
+    geocentric TCN frame: x2 = -p/|p|, x1 = x2 x v normalised,
+    x0 = x1 x x2 normalised (core/Basis.h:50-58)."""
+
+    def __init__(self, p=None, v=None):
+        if p is None:
+            self.x0, self.x1, self.x2 = np.eye(3)
+            return
+        p = np.asarray(p, dtype=np.float64)
+        v = np.asarray(v, dtype=np.float64)
+        n = -p / np.linalg.norm(p)
+        c = np.cross(n, v)
+        c = c / np.linalg.norm(c)
+        t = np.cross(c, n)
+        t = t / np.linalg.norm(t)
+        self.x0, self.x1, self.x2 = t, c, n
+
+
+def geodetic_tcn(x, v, ellipsoid: "Ellipsoid | None" = None) -> Basis:
+    """TCN frame about the geodetic normal at x (core/Basis.cpp:21-26): the
+    basis of (n_vector(lon, lat), v), so x2 points along the geodetic nadir."""
+    ellipsoid = ellipsoid if ellipsoid is not None else Ellipsoid()
+    lon, lat, _ = ellipsoid.xyz_to_lon_lat(x)
+    return Basis(ellipsoid.n_vector(lon, lat), v)

@@ -1,0 +1,354 @@
+// Finishing focused RSLC blocks on gfx950 (include/isce3_amd_rslc.h): the
+// per-block work of BackgroundWriter.write (nisar/workflows/focus.py:883-914)
+// in one pass over HBM -- range-correction product, accumulation of the block
+// already written, statistics of isce3::math::StatsRealImag<float>
+// (cxx/isce3/math/Stats.cpp:33-165) and the storage encoding -- plus a
+// one-workgroup merge of the per-workgroup statistics.
+//
+// Each thread takes kPerThread samples strided by the workgroup size (every
+// load and store a coalesced 8-byte-per-lane wave access), keeps them in
+// registers, and summarises its valid samples exactly in FP64 by two passes
+// (mean, then the sum of squared deviations).  Summaries are combined with
+// Chan's pairwise update -- across the wave by lane shuffles, across the
+// workgroup through LDS, across workgroups in a second kernel that merges the
+// partials in a fixed order, so results do not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+
+#include "../../include/isce3_amd_backproject.h"
+#include "../../include/isce3_amd_rslc.h"
+
+void bp_set_last_error(const std::string& msg);
+
+namespace rslc {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPerThread = 16;
+constexpr int64_t kPerBlock = (int64_t)kThreads * kPerThread;
+
+// Summary of the valid samples of one component: count, mean, sum of squared
+// deviations, extremes.
+struct Moments {
+    double mean, m2;
+    float lo, hi;
+};
+
+struct Partial {
+    int64_t n;
+    Moments re, im;
+};
+
+struct FinishArgs {
+    float2* z;
+    const float2* range_cor;   // [cols] or null
+    const float2* prior;       // [n] or null
+    uint32_t* encoded;         // [n] pairs of half (complex32) or null
+    Partial* partials;         // [gridDim.x]
+    int64_t n, cols;
+    int32_t col_step;          // kThreads % cols
+    int32_t data_type;
+    uint32_t mantissa_mask;
+    int32_t store_z;
+};
+
+__device__ inline void merge(int64_t& na, Moments& a, int64_t nb, const Moments& b, int64_t n)
+{
+    // Chan et al.: combine (na, a) and (nb, b) into n = na + nb samples
+    const double delta = b.mean - a.mean;
+    const double fb = (double)nb / (double)n;
+    a.mean += delta * fb;
+    a.m2 += b.m2 + delta * delta * (double)na * fb;
+    a.lo = fminf(a.lo, b.lo);
+    a.hi = fmaxf(a.hi, b.hi);
+}
+
+__device__ inline void merge(Partial& a, const Partial& b)
+{
+    if (b.n == 0) return;
+    if (a.n == 0) {
+        a = b;
+        return;
+    }
+    const int64_t n = a.n + b.n;
+    merge(a.n, a.re, b.n, b.re, n);
+    merge(a.n, a.im, b.n, b.im, n);
+    a.n = n;
+}
+
+__device__ inline Partial shfl_xor(const Partial& p, int mask)
+{
+    Partial q;
+    q.n = __shfl_xor(p.n, mask);
+    q.re.mean = __shfl_xor(p.re.mean, mask);
+    q.re.m2 = __shfl_xor(p.re.m2, mask);
+    q.re.lo = __shfl_xor(p.re.lo, mask);
+    q.re.hi = __shfl_xor(p.re.hi, mask);
+    q.im.mean = __shfl_xor(p.im.mean, mask);
+    q.im.m2 = __shfl_xor(p.im.m2, mask);
+    q.im.lo = __shfl_xor(p.im.lo, mask);
+    q.im.hi = __shfl_xor(p.im.hi, mask);
+    return q;
+}
+
+__device__ inline Partial empty_partial()
+{
+    Partial p;
+    p.n = 0;
+    p.re = {0.0, 0.0, INFINITY, -INFINITY};
+    p.im = p.re;
+    return p;
+}
+
+// complex64 product rounded term by term, (ac - bd) + i (ad + bc), no FMA
+__device__ inline float2 cmul_rounded(float2 a, float2 b)
+{
+#pragma clang fp contract(off)
+    const float rr = a.x * b.x, ii = a.y * b.y, ri = a.x * b.y, ir = a.y * b.x;
+    return make_float2(rr - ii, ri + ir);
+}
+
+__device__ inline uint32_t to_half_pair(float2 v)
+{
+    const _Float16 r = (_Float16)v.x, i = (_Float16)v.y;
+    return (uint32_t)__builtin_bit_cast(uint16_t, r) |
+           ((uint32_t)__builtin_bit_cast(uint16_t, i) << 16);
+}
+
+// Reduce the workgroup's summaries into one (wave shuffles, then LDS across
+// the four waves, in a fixed order).
+__device__ inline Partial block_reduce(Partial p)
+{
+    __shared__ Partial waves[kThreads / 64];
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) merge(p, shfl_xor(p, m));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) waves[wave] = p;
+    __syncthreads();
+    p = waves[0];
+#pragma unroll
+    for (int w = 1; w < kThreads / 64; ++w) merge(p, waves[w]);
+    return p;
+}
+
+__global__ __launch_bounds__(kThreads) void finish_kernel(FinishArgs a)
+{
+    const int64_t e0 = (int64_t)blockIdx.x * kPerBlock + threadIdx.x;
+    int64_t col = e0 % a.cols;
+    float2 v[kPerThread];
+    bool ok[kPerThread];
+    int64_t cnt = 0;
+    double sre = 0.0, sim = 0.0;
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        const int64_t e = e0 + (int64_t)j * kThreads;
+        ok[j] = false;
+        v[j] = make_float2(0.f, 0.f);
+        if (e < a.n) {
+            float2 x = a.z[e];
+            if (a.range_cor) x = cmul_rounded(x, a.range_cor[col]);
+            if (a.prior) {
+                const float2 q = a.prior[e];
+                x = make_float2(x.x + q.x, x.y + q.y);
+            }
+            v[j] = x;
+            ok[j] = !(isnan(x.x) || isnan(x.y));
+            if (ok[j]) {
+                ++cnt;
+                sre += (double)x.x;
+                sim += (double)x.y;
+            }
+        }
+        col += a.col_step;
+        if (col >= a.cols) col -= a.cols;
+    }
+    Partial p = empty_partial();
+    p.n = cnt;
+    if (cnt > 0) {
+        p.re.mean = sre / (double)cnt;
+        p.im.mean = sim / (double)cnt;
+#pragma unroll
+        for (int j = 0; j < kPerThread; ++j) {
+            if (!ok[j]) continue;
+            const double dr = (double)v[j].x - p.re.mean, di = (double)v[j].y - p.im.mean;
+            p.re.m2 += dr * dr;
+            p.im.m2 += di * di;
+            p.re.lo = fminf(p.re.lo, v[j].x);
+            p.re.hi = fmaxf(p.re.hi, v[j].x);
+            p.im.lo = fminf(p.im.lo, v[j].y);
+            p.im.hi = fmaxf(p.im.hi, v[j].y);
+        }
+    }
+    // stores: statistics are of the values before truncation
+#pragma unroll
+    for (int j = 0; j < kPerThread; ++j) {
+        const int64_t e = e0 + (int64_t)j * kThreads;
+        if (e >= a.n) continue;
+        float2 x = v[j];
+        if (a.data_type == BP_RSLC_COMPLEX32) {
+            a.encoded[e] = to_half_pair(x);
+        } else if (a.data_type == BP_RSLC_COMPLEX64_ZERO_MANTISSA) {
+            x.x = __uint_as_float(__float_as_uint(x.x) & a.mantissa_mask);
+            x.y = __uint_as_float(__float_as_uint(x.y) & a.mantissa_mask);
+        }
+        if (a.store_z) a.z[e] = x;
+    }
+    p = block_reduce(p);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = p;
+}
+
+__device__ inline void to_stats(const Moments& m, int64_t n, bp_stats_f32& s)
+{
+    const float qnan = __builtin_nanf("");
+    s.min = n > 0 ? m.lo : qnan;
+    s.max = n > 0 ? m.hi : qnan;
+    s.mean = n > 0 ? (float)m.mean : 0.f;
+    s.pad_ = 0;
+    s.n_valid = n;
+    s.real_valued_mean = n > 0 ? m.mean : 0.0;
+    s.square_diff_sum = n > 0 ? m.m2 : 0.0;
+}
+
+__global__ __launch_bounds__(kThreads) void merge_kernel(const Partial* partials, int count,
+                                                         bp_stats_real_imag_f32* out)
+{
+    Partial p = empty_partial();
+    for (int i = threadIdx.x; i < count; i += kThreads) merge(p, partials[i]);
+    p = block_reduce(p);
+    if (threadIdx.x == 0) {
+        to_stats(p.re, p.n, out->real);
+        to_stats(p.im, p.n, out->imag);
+        out->n_valid = p.n;
+    }
+}
+
+int fail(int code, const std::string& msg)
+{
+    bp_set_last_error(msg);
+    return code;
+}
+
+}  // namespace
+}  // namespace rslc
+
+struct bp_rslc_finisher {
+    int device = 0;
+    int64_t max_pixels = 0;
+    int max_partials = 0;
+    rslc::Partial* partials = nullptr;       // device
+    bp_stats_real_imag_f32* d_stats = nullptr;
+    bp_stats_real_imag_f32* h_stats = nullptr; // pinned
+};
+
+extern "C" {
+
+int bp_rslc_finisher_create(int device, int64_t max_pixels, bp_rslc_finisher** f)
+{
+    using rslc::fail;
+    if (f == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    *f = nullptr;
+    if (max_pixels < 1) return fail(BP_ERR_INVALID_ARGUMENT, "max_pixels must be > 0");
+    const int64_t blocks = (max_pixels + rslc::kPerBlock - 1) / rslc::kPerBlock;
+    if (blocks > (int64_t)1 << 30) return fail(BP_ERR_INVALID_ARGUMENT, "max_pixels too large");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(BP_ERR_HIP, "no HIP device available (isce3_amd needs an MI355X GPU)");
+    if (device < 0 || device >= ndev) return fail(BP_ERR_INVALID_ARGUMENT, "invalid device index");
+    if (hipSetDevice(device) != hipSuccess) return fail(BP_ERR_HIP, "hipSetDevice failed");
+    auto* r = new bp_rslc_finisher();
+    r->device = device;
+    r->max_pixels = max_pixels;
+    r->max_partials = (int)blocks;
+    if (hipMalloc(&r->partials, sizeof(rslc::Partial) * blocks) != hipSuccess ||
+        hipMalloc(&r->d_stats, sizeof(bp_stats_real_imag_f32)) != hipSuccess ||
+        hipHostMalloc(&r->h_stats, sizeof(bp_stats_real_imag_f32), hipHostMallocDefault) !=
+            hipSuccess) {
+        bp_rslc_finisher_destroy(r);
+        return fail(BP_ERR_OUT_OF_MEMORY, "device or pinned allocation failed");
+    }
+    *f = r;
+    return BP_SUCCESS;
+}
+
+int bp_rslc_finisher_destroy(bp_rslc_finisher* f)
+{
+    if (f == nullptr) return BP_SUCCESS;
+    (void)hipSetDevice(f->device);
+    if (f->partials) (void)hipFree(f->partials);
+    if (f->d_stats) (void)hipFree(f->d_stats);
+    if (f->h_stats) (void)hipHostFree(f->h_stats);
+    delete f;
+    return BP_SUCCESS;
+}
+
+int bp_rslc_finish_block(bp_rslc_finisher* f, void* z, int64_t rows, int64_t cols,
+                         const void* range_cor, const void* prior, int data_type,
+                         int mantissa_nbits, void* encoded, bp_stats_real_imag_f32* stats,
+                         void* stream)
+{
+    using rslc::fail;
+    if (f == nullptr || stats == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    if (rows < 0 || cols < 0) return fail(BP_ERR_INVALID_ARGUMENT, "negative block shape");
+    if (data_type != BP_RSLC_COMPLEX64 && data_type != BP_RSLC_COMPLEX32 &&
+        data_type != BP_RSLC_COMPLEX64_ZERO_MANTISSA)
+        return fail(BP_ERR_INVALID_ARGUMENT, "Requested invalid data_type");
+    uint32_t mask = 0xffffffffu;
+    if (data_type == BP_RSLC_COMPLEX64_ZERO_MANTISSA) {
+        if (!(mantissa_nbits > 0 && mantissa_nbits <= 23))
+            return fail(BP_ERR_INVALID_ARGUMENT, "Require 0 < significant_bits=" +
+                                                     std::to_string(mantissa_nbits) + " <= 23");
+        mask = 0xffffffffu << (23 - mantissa_nbits);
+    }
+    if (cols > 0 && rows > INT64_MAX / cols) return fail(BP_ERR_LENGTH, "block too large");
+    const int64_t n = rows * cols;
+    if (n > f->max_pixels)
+        return fail(BP_ERR_LENGTH, "block of " + std::to_string(n) + " samples exceeds the "
+                                   "finisher's maximum of " + std::to_string(f->max_pixels));
+    if (n > 0 && z == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null block");
+    if (n > 0 && data_type == BP_RSLC_COMPLEX32 && encoded == nullptr)
+        return fail(BP_ERR_INVALID_ARGUMENT, "complex32 encoding needs an output buffer");
+    if (n == 0) {
+        bp_stats_real_imag_f32 s;
+        std::memset(&s, 0, sizeof(s));
+        s.real.min = s.real.max = s.imag.min = s.imag.max =
+            std::numeric_limits<float>::quiet_NaN();
+        *stats = s;
+        return BP_SUCCESS;
+    }
+    if (hipSetDevice(f->device) != hipSuccess) return fail(BP_ERR_HIP, "hipSetDevice failed");
+    rslc::FinishArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.z = static_cast<float2*>(z);
+    a.range_cor = static_cast<const float2*>(range_cor);
+    a.prior = static_cast<const float2*>(prior);
+    a.encoded = static_cast<uint32_t*>(encoded);
+    a.partials = f->partials;
+    a.n = n;
+    a.cols = cols;
+    a.col_step = (int32_t)(rslc::kThreads % cols);
+    a.data_type = data_type;
+    a.mantissa_mask = mask;
+    a.store_z = range_cor != nullptr || prior != nullptr ||
+                (data_type == BP_RSLC_COMPLEX64_ZERO_MANTISSA && mask != 0xffffffffu);
+    const int blocks = (int)((n + rslc::kPerBlock - 1) / rslc::kPerBlock);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(rslc::finish_kernel, dim3(blocks), dim3(rslc::kThreads), 0, s, a);
+    hipLaunchKernelGGL(rslc::merge_kernel, dim3(1), dim3(rslc::kThreads), 0, s,
+                       (const rslc::Partial*)f->partials, blocks, f->d_stats);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(f->h_stats, f->d_stats, sizeof(bp_stats_real_imag_f32),
+                           hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return fail(BP_ERR_HIP, std::string("RSLC block finishing: ") + hipGetErrorString(e));
+    *stats = *f->h_stats;
+    return BP_SUCCESS;
+}
+
+}  // extern "C"

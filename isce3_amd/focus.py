@@ -521,3 +521,4 @@ class Scene:
 # range compression, the step before backprojection (isce3.focus.RangeComp,
 # isce3.focus.form_linear_chirp)
 from .rangecomp import RangeComp, form_linear_chirp  # noqa: E402,F401
+from .planning import get_radar_velocities, get_sar_duration  # noqa: E402,F401

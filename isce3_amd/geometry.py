@@ -183,3 +183,33 @@ def geo2rdr_bracket(xyz, orbit, doppler, wavelength: float, side, tol_aztime: fl
     if rc != 0:
         raise RuntimeError("failed to converge")
     return t.value, r.value
+
+
+def heading(lon: float, lat: float, vel) -> float:
+    """Heading of velocity `vel` (ECEF) at (lon, lat), clockwise from north
+    (geometry/geometry.cpp:399-408)."""
+    import math
+    v = np.asarray(vel, dtype=np.float64)
+    coslat, sinlat = math.cos(lat), math.sin(lat)
+    coslon, sinlon = math.cos(lon), math.sin(lon)
+    return math.atan2(-sinlon * v[0] + coslon * v[1],
+                      -sinlat * coslon * v[0] - sinlat * sinlon * v[1] + coslat * v[2])
+
+
+def rdr2rdr(t: float, r: float, orbit, side, doppler, wavelength: float,
+            dem: "DEMInterpolator | None" = None, ellipsoid=None, doppler_out=None,
+            orbit_out=None, rdr2geo_params=None, geo2rdr_params=None):
+    """Radar coordinates in one geometry -> another with a different Doppler
+    or orbit: rdr2geo_bracket at the input Doppler, then geo2rdr_bracket
+    against the output orbit and Doppler
+    (python/packages/isce3/geometry/rdr2rdr.py:6-72)."""
+    if orbit_out is None and doppler_out is None:
+        return t, r
+    orbit_out = orbit if orbit_out is None else orbit_out
+    doppler_out = doppler if doppler_out is None else doppler_out
+    dem = dem if dem is not None else DEMInterpolator()
+    doppler_in = doppler.eval(t, r)
+    xyz = rdr2geo_bracket(t, r, orbit, side, doppler_in, wavelength, dem,
+                          **(rdr2geo_params or {}))
+    return geo2rdr_bracket(xyz, orbit_out, doppler_out, wavelength, side,
+                           **(geo2rdr_params or {}))

@@ -70,6 +70,29 @@ class RadarGridParameters:
                                    self.slant_range(xoff), self.range_pixel_spacing,
                                    self.lookside, ysize, xsize, self.ref_epoch)
 
+    def __getitem__(self, key) -> "RadarGridParameters":
+        """Sub-grid of a pair of slices (pybind product/RadarGridParameters.cpp:126-151):
+        a row step scales the PRF down, a column step scales the spacing up."""
+        if not isinstance(key, tuple) or len(key) != 2:
+            raise ValueError("require 2 slices")
+        rows, cols = key
+        if not isinstance(rows, slice) or not isinstance(cols, slice):
+            raise TypeError("require 2 slices")
+        start, stop, step = rows.indices(self.length)
+        if step <= 0:
+            raise IndexError("cannot reverse grid")
+        nt = len(range(start, stop, step))
+        prf = self.prf / step
+        t0 = self.sensing_start + start / self.prf
+        start, stop, step = cols.indices(self.width)
+        if step <= 0:
+            raise IndexError("cannot reverse grid")
+        nr = len(range(start, stop, step))
+        dr = self.range_pixel_spacing * step
+        r0 = self.starting_range + start * self.range_pixel_spacing
+        return RadarGridParameters(t0, self.wavelength, prf, r0, dr, self.lookside, nt, nr,
+                                   self.ref_epoch)
+
     def copy(self) -> "RadarGridParameters":
         return RadarGridParameters(self.sensing_start, self.wavelength, self.prf,
                                    self.starting_range, self.range_pixel_spacing,

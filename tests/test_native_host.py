@@ -27,10 +27,11 @@ def declared_functions(header):
 
 @pytest.mark.parametrize("header,lib", [("isce3_amd_backproject.h", "libisce3_amd.so"),
                                         ("isce3_amd_rangecomp.h", "libisce3_amd.so"),
+                                        ("isce3_amd_rslc.h", "libisce3_amd.so"),
                                         ("isce3_amd_sim.h", "libisce3_amd_sim.so")])
 def test_library_exports_every_declared_symbol(header, lib):
     names = declared_functions(header)
-    assert len(names) >= {"isce3_amd_sim.h": 2, "isce3_amd_rangecomp.h": 4}.get(header, 15)
+    assert len(names) >= {"isce3_amd_sim.h": 2, "isce3_amd_rangecomp.h": 4, "isce3_amd_rslc.h": 3}.get(header, 15)
     h = C.CDLL(os.path.join(ROOT, "isce3_amd", "lib", lib))
     missing = [n for n in names if not hasattr(h, n)]
     assert not missing, missing

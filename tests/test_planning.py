@@ -1,0 +1,109 @@
+"""Block planning on the host (isce3_amd.planning): synthetic-aperture
+duration against closed forms on the straight-line orbit, the tiling and
+raw-time bounds of plan_processing_blocks (focus.py:726-783), the grid slice
+rule of pybind RadarGridParameters.__getitem__, and the TCN/heading helpers."""
+import math
+from types import SimpleNamespace as NS
+
+import numpy as np
+import pytest
+
+from isce3_amd import core, geometry, planning
+from isce3_amd.sim import ALTITUDE, SPEED, WAVELENGTH, make_scene, straight_orbit
+
+
+def cfg(na=64, nr=100, azres=6.0):
+    return NS(processing=NS(
+        azcomp=NS(block_size=NS(azimuth=na, range=nr), azimuth_resolution=azres),
+        rdr2geo=NS(tol_height=1e-5, look_min_deg=0.0, look_max_deg=90.0),
+        geo2rdr=NS(tol_aztime=1e-7, time_start=None, time_end=None)))
+
+
+def test_radar_velocities_closed_form_at_orbit_midpoint():
+    """At t = 0 the platform sits over the equator moving north: nadir is
+    perpendicular to v, heading 0, and the directional radius is the
+    meridional radius a (1 - e^2)."""
+    orbit = straight_orbit()
+    ell = core.Ellipsoid()
+    vs, vg = planning.get_radar_velocities(orbit, 0.0, ell)
+    assert vs == pytest.approx(SPEED, rel=1e-12)
+    a_n = ell.a * (1 - ell.e2)
+    assert vg == pytest.approx(SPEED / (1 + ALTITUDE / a_n), rel=1e-12)
+    # default time is the orbit midpoint (t = 0 here)
+    assert planning.get_radar_velocities(orbit) == pytest.approx((vs, vg), rel=1e-15)
+    cpi = planning.get_sar_duration(0.0, 850e3, orbit, ell, 6.0, WAVELENGTH)
+    assert cpi == pytest.approx(WAVELENGTH / 12.0 * 850e3 / vg, rel=1e-12)
+
+
+def test_ellipsoid_radii_and_heading():
+    ell = core.Ellipsoid()
+    assert ell.r_dir(0.0, 0.0) == pytest.approx(ell.a * (1 - ell.e2), rel=1e-15)
+    assert ell.r_dir(math.pi / 2, 0.0) == pytest.approx(ell.a, rel=1e-15)
+    # at the pole east and north radii agree: a / sqrt(1 - e^2)
+    assert ell.r_east(math.pi / 2) == pytest.approx(ell.r_north(math.pi / 2), rel=1e-12)
+    # heading: due north 0, due east +pi/2 at (lon, lat) = (0, 0)
+    assert geometry.heading(0.0, 0.0, [0, 0, 1.0]) == pytest.approx(0.0)
+    assert geometry.heading(0.0, 0.0, [0, 1.0, 0]) == pytest.approx(math.pi / 2)
+    b = core.geodetic_tcn([ell.a + 1e5, 0, 0], [0, 0, 7e3], ell)
+    assert np.allclose(b.x2, [-1, 0, 0]) and np.allclose(b.x0, [0, 0, 1])
+    assert np.allclose(np.cross(b.x0, b.x1), b.x2)
+
+
+def test_grid_slicing_rule():
+    sc = make_scene("small")
+    g = sc.out_grid
+    s = g[10:20, 3:40:2]
+    assert (s.length, s.width) == (10, 19)
+    assert s.sensing_start == pytest.approx(g.sensing_start + 10 / g.prf)
+    assert s.range_pixel_spacing == pytest.approx(2 * g.range_pixel_spacing)
+    assert s.starting_range == pytest.approx(g.starting_range + 3 * g.range_pixel_spacing)
+    assert g[::3, :].prf == pytest.approx(g.prf / 3)
+    with pytest.raises(IndexError):
+        g[::-1, :]
+    with pytest.raises(ValueError):
+        g[1:2]
+
+
+def test_plan_processing_blocks_zero_doppler():
+    """Zero Doppler in and out: each block's raw span is its own azimuth
+    span widened by half the padded aperture on each side."""
+    sc = make_scene("small")
+    grid, orbit = sc.out_grid, sc.orbit
+    c = cfg(na=100, nr=90)
+    plan = planning.plan_processing_blocks(c, grid, core.LUT2d(), sc.dem, orbit, pad=0.1)
+    nb_az, nb_rg = math.ceil(grid.length / 100), math.ceil(grid.width / 90)
+    assert len(plan) == nb_az * nb_rg
+    covered = np.zeros(grid.shape, int)
+    for (rows, cols), (t0, t1) in plan:
+        covered[rows, cols] += 1
+        sub = grid[rows, cols]
+        cpi = planning.get_sar_duration(sub.sensing_mid, sub.end_range, orbit,
+                                        core.Ellipsoid(), 6.0, grid.wavelength) * 1.1
+        ta = grid.sensing_start + rows.start / grid.prf
+        tb = grid.sensing_start + (rows.stop - 1) / grid.prf
+        assert t0 == pytest.approx(ta - cpi / 2, abs=1e-6)
+        assert t1 == pytest.approx(tb + cpi / 2, abs=1e-6)
+    assert np.all(covered == 1)
+    t0, t1 = planning.total_bounds(plan)
+    assert t0 == min(b[1][0] for b in plan) and t1 == max(b[1][1] for b in plan)
+    # range < 1 means one block across the whole width
+    plan = planning.plan_processing_blocks(cfg(na=128, nr=0), grid, core.LUT2d(), sc.dem, orbit)
+    assert all(cols == slice(0, grid.width) for (_, cols), _ in plan)
+    assert planning.is_overlapping(0, 1, 1, 2) and not planning.is_overlapping(0, 1, 1.5, 2)
+
+
+def test_plan_processing_blocks_native_doppler_shifts_times():
+    """A constant native Doppler fD moves the raw times by the squint:
+    for a straight track, t_native - t_zero = -fD lambda r / (2 v^2) to first
+    order (positive Doppler looks forward)."""
+    sc = make_scene("small")
+    grid, orbit = sc.out_grid, sc.orbit
+    fd = 200.0
+    lut = core.LUT2d(np.array([0.0, 2e6]), np.array([-100.0, 100.0]), np.full((2, 2), fd))
+    p0 = planning.plan_processing_blocks(cfg(na=256, nr=256), grid, core.LUT2d(), sc.dem, orbit)
+    p1 = planning.plan_processing_blocks(cfg(na=256, nr=256), grid, lut, sc.dem, orbit)
+    (_, (a0, b0)), (_, (a1, b1)) = p0[0], p1[0]
+    r = grid.starting_range
+    dt_expected = -fd * WAVELENGTH * r / (2 * SPEED ** 2)
+    assert (a1 - a0) == pytest.approx(dt_expected, rel=0.05)
+    assert (b1 - b0) == pytest.approx(dt_expected, rel=0.05)
