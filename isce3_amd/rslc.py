@@ -203,7 +203,7 @@ class BlockFinisher:
             dprior = to_dev(prior, (rows, cols))
             denc = None
             if data_type == "complex32":
-                denc = torch.empty((rows, cols), dtype=torch.complex32, device=dev)
+                denc = torch.empty((rows, cols), dtype=torch.int32, device=dev)
             if stream is None:
                 hstream = torch.cuda.current_stream().cuda_stream
             else:
@@ -227,8 +227,8 @@ class BlockFinisher:
             if denc is None:
                 return z, stats
             if _is_torch(z):
-                return denc, stats
-            return denc.view(torch.int32).cpu().numpy().view(complex32), stats
+                return denc.view(torch.complex32), stats
+            return denc.cpu().numpy().view(complex32), stats
 
     def close(self):
         if self._h is not None:
