@@ -54,7 +54,9 @@ enum {
 enum { BP_INTERP_HERMITE = 0, BP_INTERP_LEGENDRE = 1 };   /* OrbitInterpMethod */
 enum { BP_LOOK_LEFT = 1, BP_LOOK_RIGHT = -1 };            /* isce3 LookSide */
 enum { BP_TROPO_NODELAY = 0, BP_TROPO_TSX = 1 };          /* DryTroposphereModel */
-enum { BP_DEM_BILINEAR = 0 };                             /* dataInterpMethod */
+/* DEM interpolation: isce3::core::dataInterpMethod values (core/Constants.h:23-29);
+ * sinc is not offered. */
+enum { BP_DEM_BILINEAR = 1, BP_DEM_BICUBIC = 2, BP_DEM_NEAREST = 3, BP_DEM_BIQUINTIC = 4 };
 enum {
     BP_PRECISION_PLAIN = 0, /* FP64 delay/phase/taps index, complex<double> sum */
     BP_PRECISION_SPLIT = 1  /* FP64 per-pixel reference + FP32 deltas, Kahan sum */
@@ -99,15 +101,17 @@ typedef struct {
     bp_lut2d doppler;
 } bp_radar_geometry;
 
-/* DEMInterpolator (geometry/DEMInterpolator.h): constant height or an
- * EPSG:4326 float raster with bilinear interpolation. */
+/* DEMInterpolator (geometry/DEMInterpolator.h): constant height or a float
+ * raster in the coordinates of `epsg` -- 4326 (lon/lat degrees), 4978
+ * (geocentric), 32601-32660 / 32701-32760 (UTM north/south), 3031 / 3413
+ * (polar stereographic), 6933 (EASE-Grid 2.0) -- interpolated by `method`. */
 typedef struct {
     int32_t have_raster;
     int32_t epsg;
-    int32_t method;      /* BP_DEM_BILINEAR */
+    int32_t method;      /* BP_DEM_* */
     int32_t pad_;
     double ref_height;
-    double xstart, ystart, dx, dy;  /* degrees */
+    double xstart, ystart, dx, dy;  /* raster coordinates of the projection */
     int64_t width, length;
     const float* data;   /* host, length x width */
 } bp_dem;
@@ -278,6 +282,11 @@ void bp_xyz_to_lonlat(double a, double e2, const double* xyz, double* llh);
 /* DEMInterpolator::interpolateLonLat / interpolateXY (DEMInterpolator.cpp:593-662). */
 double bp_dem_interpolate_lonlat(const bp_dem* dem, double lon, double lat);
 double bp_dem_interpolate_xy(const bp_dem* dem, double x, double y);
+/* ProjectionBase::forward / inverse of the EPSG code (core/Projections.cpp):
+ * (lon, lat, h) in radians <-> projected (x, y, h).  Return 0, 1 where the
+ * projection reports failure, BP_ERR_INVALID_ARGUMENT for an unknown code. */
+int bp_proj_forward(int epsg, const double* llh, double* xyz);
+int bp_proj_inverse(int epsg, const double* xyz, double* llh);
 /* LUT2d<double>::eval (core/LUT2d.cpp:127-159). */
 double bp_lut2d_eval(const bp_lut2d* lut, double y, double x);
 /* KnabKernel<double>::operator() (core/Kernels.icc:29-52). */

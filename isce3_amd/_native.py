@@ -140,6 +140,8 @@ _SIGNATURES = {
     "bp_xyz_to_lonlat": (None, [C.c_double, C.c_double, _P(C.c_double), _P(C.c_double)]),
     "bp_dem_interpolate_lonlat": (C.c_double, [_P(Dem_t), C.c_double, C.c_double]),
     "bp_dem_interpolate_xy": (C.c_double, [_P(Dem_t), C.c_double, C.c_double]),
+    "bp_proj_forward": (C.c_int, [C.c_int, _P(C.c_double), _P(C.c_double)]),
+    "bp_proj_inverse": (C.c_int, [C.c_int, _P(C.c_double), _P(C.c_double)]),
     "bp_lut2d_eval": (C.c_double, [_P(Lut2d_t), C.c_double, C.c_double]),
     "bp_knab_eval": (C.c_double, [C.c_double, C.c_double, C.c_double]),
     "bp_tabulate_knab": (C.c_int, [C.c_double, C.c_double, C.c_int, _P(C.c_float)]),

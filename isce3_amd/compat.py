@@ -57,7 +57,8 @@ def geometry_from(geom) -> RadarGeometry:
 
 
 def dem_from(dem) -> DEMInterpolator:
-    out = DEMInterpolator(dem.ref_height, epsg=int(getattr(dem, "epsg_code", 4326)))
+    out = DEMInterpolator(dem.ref_height, method=getattr(dem, "interp_method", "bilinear"),
+                          epsg=int(getattr(dem, "epsg_code", 4326)))
     if getattr(dem, "have_raster", False):
         out.load_dem_array(np.asarray(dem.data, dtype=np.float32), dem.x_start, dem.y_start,
                            dem.delta_x, dem.delta_y, int(dem.epsg_code))

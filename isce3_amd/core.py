@@ -576,6 +576,110 @@ class Ellipsoid:
         return np.array([clat * math.cos(lon), clat * math.sin(lon), math.sin(lat)])
 
 
+class ProjectionBase:
+    """Map projection of an EPSG code (core/Projections.h, pybind
+    core/Projections.cpp): ``forward(llh)`` from (lon, lat, h) in radians to
+    projection coordinates, ``inverse(xyz)`` back; RuntimeError where the
+    projection reports failure.  WGS84."""
+
+    _name = "ProjectionBase"
+
+    def __init__(self, code: int):
+        self._code = int(code)
+        probe = np.zeros(3)
+        rc = _native.lib().bp_proj_forward(self._code, _native.dptr(probe), _native.dptr(probe.copy()))
+        if rc < 0:
+            raise errors.RuntimeError_(f"Unknown EPSG code (in factory): {self._code}")
+
+    @property
+    def code(self) -> int:
+        return self._code
+
+    @property
+    def ellipsoid(self) -> "Ellipsoid":
+        return Ellipsoid()
+
+    def _call(self, fn, v, what):
+        src = np.ascontiguousarray(v, dtype=np.float64).reshape(3)
+        out = np.empty(3)
+        rc = fn(self._code, _native.dptr(src), _native.dptr(out))
+        if rc != 0:
+            raise errors.RuntimeError_(f"{what} projection transformation failed")
+        return out
+
+    def forward(self, llh) -> np.ndarray:
+        return self._call(_native.lib().bp_proj_forward, llh, "Forward")
+
+    def inverse(self, xyz) -> np.ndarray:
+        return self._call(_native.lib().bp_proj_inverse, xyz, "Inverse")
+
+    def __repr__(self):
+        return f"{self._name}(epsg={self._code})"
+
+
+class LonLat(ProjectionBase):
+    _name = "LonLat"
+
+    def __init__(self):
+        super().__init__(4326)
+
+
+class Geocent(ProjectionBase):
+    _name = "Geocent"
+
+    def __init__(self):
+        super().__init__(4978)
+
+
+class UTM(ProjectionBase):
+    """EPSG 32601-32660 (north zones 1-60), 32701-32760 (south)."""
+    _name = "UTM"
+
+    def __init__(self, code: int):
+        code = int(code)
+        if not (32600 < code <= 32660 or 32700 < code <= 32760):
+            raise ValueError("In UTM::UTM - Invalid EPSG Code for UTM Projection. Received "
+                             f"{code}, expected in ranges (32600,32660] or (32700,32760].")
+        super().__init__(code)
+
+
+class PolarStereo(ProjectionBase):
+    """EPSG 3413 (north, Greenland) and 3031 (south, Antarctic)."""
+    _name = "PolarStereo"
+
+    def __init__(self, code: int):
+        code = int(code)
+        if code not in (3031, 3413):
+            raise ValueError("In PolarStereo::PolarStereo - Invalid EPSG Code for Polar "
+                             f"Stereographic projection. Received {code}, expected either "
+                             "3031 (Antarctic) or 3413 (Greenland).")
+        super().__init__(code)
+
+
+class CEA(ProjectionBase):
+    """Cylindrical equal area of EASE-Grid 2.0 (EPSG 6933)."""
+    _name = "CEA"
+
+    def __init__(self):
+        super().__init__(6933)
+
+
+def make_projection(epsg: int) -> ProjectionBase:
+    """core/Projections.cpp:413-437 (createProj)."""
+    epsg = int(epsg)
+    if epsg == 4326:
+        return LonLat()
+    if epsg == 4978:
+        return Geocent()
+    if 32600 < epsg < 32800:
+        return UTM(epsg)
+    if epsg in (3031, 3413):
+        return PolarStereo(epsg)
+    if epsg == 6933:
+        return CEA()
+    raise errors.RuntimeError_(f"Unknown EPSG code (in factory): {epsg}")
+
+
 class Basis:
     """Orthonormal basis x0, x1, x2 (core/Basis.h).  ``Basis(p, v)`` is the
     geocentric TCN frame: x2 = -p/|p|, x1 = x2 x v normalised,

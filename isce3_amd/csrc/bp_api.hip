@@ -203,7 +203,14 @@ bp::Dem host_dem(const bp_dem& m)
     d.width = m.width;
     d.length = m.length;
     d.data = m.data;
+    if (!bp::proj_init(m.epsg, &d.proj)) bp::proj_init(4326, &d.proj);
     return d;
+}
+
+bool dem_method_ok(int m)
+{
+    return m == BP_DEM_BILINEAR || m == BP_DEM_BICUBIC || m == BP_DEM_NEAREST ||
+           m == BP_DEM_BIQUINTIC;
 }
 
 bp::TileMap make_map(int64_t length, int64_t width, int tile_rows, int order, int lane_rows)
@@ -419,8 +426,11 @@ int check_call(const CallArgs& a, int* width)
         return fail(BP_ERR_INVALID_ARGUMENT, "kernel width must be 8 or 9 (got " + std::to_string(*width) + ")");
     const bp_dem* dem = a.dem;
     if (dem->have_raster) {
-        if (dem->epsg != 4326) return fail(BP_ERR_INVALID_ARGUMENT, "only EPSG:4326 DEM rasters are supported");
-        if (dem->method != BP_DEM_BILINEAR) return fail(BP_ERR_INVALID_ARGUMENT, "only bilinear DEM interpolation is supported");
+        bp::Proj pj;
+        if (!bp::proj_init(dem->epsg, &pj))
+            return fail(BP_ERR_RUNTIME, "Unknown EPSG code (in factory): " + std::to_string(dem->epsg));
+        if (!dem_method_ok(dem->method))
+            return fail(BP_ERR_INVALID_ARGUMENT, "unsupported DEM interpolation method " + std::to_string(dem->method));
         if (dem->width < 4 || dem->length < 4 || dem->data == nullptr)
             return fail(BP_ERR_INVALID_ARGUMENT, "DEM raster must be at least 4x4");
     }
@@ -895,7 +905,8 @@ int capture_graph(bp_scene* sc, const Opts& o, bp_scene::Graph* out)
         return fail(BP_ERR_HIP, std::string("hipGraphGetNodes: ") + hipGetErrorString(e));
     }
     const bool pair = o.lane_rows == 2;
-    const void* geo_fn = bp::kernel_fn(0, sc->width, o.staged, pair);
+    const void* geo_fn = bp::kernel_fn(bp::dem_generic(host_dem(sc->dem)) ? 4 : 0, sc->width,
+                                       o.staged, pair);
     const void* tile_fn = sc->args.precision == BP_PRECISION_PLAIN
                               ? bp::kernel_fn(1, sc->width, o.staged, pair)
                               : bp::kernel_fn(3, sc->width, o.staged, pair);
@@ -1125,6 +1136,30 @@ double bp_dem_interpolate_lonlat(const bp_dem* dem, double lon, double lat)
 double bp_dem_interpolate_xy(const bp_dem* dem, double x, double y)
 {
     return bp::dem_interpolate_xy(host_dem(*dem), x, y);
+}
+
+int bp_proj_forward(int epsg, const double* llh, double* xyz)
+{
+    bp::Proj p;
+    if (llh == nullptr || xyz == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    if (!bp::proj_init(epsg, &p))
+        return fail(BP_ERR_INVALID_ARGUMENT, "Unknown EPSG code (in factory): " + std::to_string(epsg));
+    bp::V3 out;
+    if (!bp::proj_forward(p, bp::ld3(llh), &out)) return 1;
+    xyz[0] = out.x; xyz[1] = out.y; xyz[2] = out.z;
+    return 0;
+}
+
+int bp_proj_inverse(int epsg, const double* xyz, double* llh)
+{
+    bp::Proj p;
+    if (llh == nullptr || xyz == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    if (!bp::proj_init(epsg, &p))
+        return fail(BP_ERR_INVALID_ARGUMENT, "Unknown EPSG code (in factory): " + std::to_string(epsg));
+    bp::V3 out;
+    if (!bp::proj_inverse(p, bp::ld3(xyz), &out)) return 1;
+    llh[0] = out.x; llh[1] = out.y; llh[2] = out.z;
+    return 0;
 }
 
 double bp_lut2d_eval(const bp_lut2d* lut, double y, double x)

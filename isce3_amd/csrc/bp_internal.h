@@ -169,7 +169,8 @@ hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_
 hipError_t launch_sum_split(const SumArgs& args, int width, int grid, bool staged, bool pair,
                             hipStream_t stream);
 int sum_blocks_per_cu(int precision, int width, bool staged, bool pair);
-// Host-side handle of a kernel (0 geometry, 1 plain sum, 2 split sum, 3 fit),
+// Host-side handle of a kernel (0 geometry, 4 geometry for generic DEMs,
+// 1 plain sum, 2 split sum, 3 fit),
 // as graph kernel nodes report it.
 const void* kernel_fn(int which, int width, bool staged, bool pair);
 

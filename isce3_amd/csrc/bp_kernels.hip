@@ -135,6 +135,7 @@ __global__ void clear_kernel(int32_t* words, int n)
 
 // Per-pixel geometry of geometry wavefront `gw`: pixel gw % lane_rows of
 // every lane of sum tile gw / lane_rows, following Backproject.cpp:127-199.
+template<bool kGenericDem>
 __device__ inline void geometry_tile(const GeomArgs& a, const OutDesc* grids, int n_grids,
                                      int64_t n_geom, int lane_rows, int64_t gw, int lane)
 {
@@ -162,7 +163,8 @@ __device__ inline void geometry_tile(const GeomArgs& a, const OutDesc* grids, in
         const double ta = d.t0 + row * d.dt;
         const double ra = d.r0 + col * d.dr;
         const double fd = lut2d_eval(a.out_dop, ta, ra);
-        int st = rdr2geo_bracket(ta, ra, fd, a.out_orbit, a.dem, a.wvl, d.side, a.r2g, &x);
+        int st = rdr2geo_bracket<kGenericDem>(ta, ra, fd, a.out_orbit, a.dem, a.wvl, d.side,
+                                              a.r2g, &x);
         if (st == kOrbitInterpDomainError || st == kOrbitInterpSizeError ||
             st == kOrbitInterpUnknownMethod) {
             atomicOr(&a.flags[1], 1);
@@ -220,6 +222,9 @@ __device__ inline void geometry_tile(const GeomArgs& a, const OutDesc* grids, in
 // return at once).  No persistent loop: this kernel needs ~220 VGPRs, and a
 // loop around its body makes the compiler keep the arguments live across
 // iterations and spill.
+// kGenericDem: DEM rasters in other projections or interpolated other than
+// bilinearly; the EPSG:4326 bilinear variant keeps the lookup compact.
+template<bool kGenericDem>
 __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
 {
     const int64_t ng = uniform64(a.batch->n_geom_groups);
@@ -230,7 +235,7 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
     const int64_t n_geom = uniform64(a.batch->n_seq) * lane_rows;
     int lane;
     const int64_t gw = group_tile(blockIdx.x, ng, &lane);
-    geometry_tile(a, grids, n_grids, n_geom, lane_rows, gw, lane);
+    geometry_tile<kGenericDem>(a, grids, n_grids, n_geom, lane_rows, gw, lane);
 }
 
 // Split-precision record of one pixel: the pulse phase in cycles fitted
@@ -1212,7 +1217,10 @@ hipError_t launch_clear(int32_t* words, int n, hipStream_t stream)
 hipError_t launch_geometry(const GeomArgs& args, int grid, hipStream_t stream)
 {
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL(geometry_kernel, dim3(grid), dim3(256), 0, stream, args);
+    if (dem_generic(args.dem))
+        hipLaunchKernelGGL(geometry_kernel<true>, dim3(grid), dim3(256), 0, stream, args);
+    else
+        hipLaunchKernelGGL(geometry_kernel<false>, dim3(grid), dim3(256), 0, stream, args);
     return hipGetLastError();
 }
 
@@ -1265,7 +1273,8 @@ int sum_blocks_per_cu(int precision, int width, bool staged, bool pair)
 const void* kernel_fn(int which, int width, bool staged, bool pair)
 {
     switch (which) {
-    case 0: return reinterpret_cast<const void*>(&geometry_kernel);
+    case 0: return reinterpret_cast<const void*>(&geometry_kernel<false>);
+    case 4: return reinterpret_cast<const void*>(&geometry_kernel<true>);
     case 3: return reinterpret_cast<const void*>(&fit_kernel);
     case 1: return width == 8 ? reinterpret_cast<const void*>(&sum_plain_kernel<8>)
                               : reinterpret_cast<const void*>(&sum_plain_kernel<9>);

@@ -76,13 +76,36 @@ struct Lut2d {
     const double* data;
 };
 
-// DEM: constant ref_height, or an EPSG:4326 float raster (bilinear).
+// Map projection of a DEM raster (core/Projections.h, Projections.cpp), WGS84;
+// every parameter follows from the EPSG code (proj_init).
+enum ProjKind : int { kProjLonLat = 0, kProjGeocent = 1, kProjUtm = 2, kProjPolar = 3, kProjCea = 4 };
+struct Proj {
+    int32_t kind, north;
+    double lon0;
+    // UTM: transverse Mercator by Krueger series (Poder/Engsager)
+    double qn, zb;
+    double cgb[6], cbg[6], utg[6], gtu[6];
+    // polar stereographic
+    double akm1, e;
+    // cylindrical equal area (EASE-Grid 2.0)
+    double k0, one_es, qp;
+    double apa[3];
+};
+
+// DEM interpolation methods: isce3::core::dataInterpMethod (core/Constants.h:23-29)
+enum DemMethod : int {
+    kDemSinc = 0, kDemBilinear = 1, kDemBicubic = 2, kDemNearest = 3, kDemBiquintic = 4
+};
+
+// DEM: constant ref_height, or a float raster in the projection of `epsg`
+// interpolated by `method`.
 struct Dem {
     int32_t have_raster, epsg, method, pad_;
     double ref_height;
     double xstart, ystart, dx, dy;
     int64_t width, length;
     const float* data;
+    Proj proj;
 };
 
 // core/Linspace.icc:77-89
@@ -313,7 +336,290 @@ BP_HD double bilinear_f64(double x, double y, const double* z, int64_t width)
            ((q12 * ((x2 - x) * (y - y1))) / den) + ((q22 * ((x - x1) * (y - y1))) / den);
 }
 
-// DEM lookup in native (lon/lat degree) coordinates
+// ------------------------------------------------------------ projections
+// Clenshaw summations of core/Projections.cpp:23-77: real (sin series at
+// `arg`) and complex (at arg_r + i arg_i, real part returned, imaginary in *im).
+BP_HD double clenshaw_sin(const double* a, int n, double arg)
+{
+    const double c2 = 2. * cos(arg);
+    double h = 0., h1 = a[n - 1], h2 = 0.;
+    for (int k = n - 2; k >= 0; --k) {
+        h = -h2 + c2 * h1 + a[k];
+        h2 = h1;
+        h1 = h;
+    }
+    if (n == 1) h = h1;
+    return sin(arg) * h;
+}
+
+BP_HD double clenshaw_sin_complex(const double* a, int n, double ar, double ai, double* im)
+{
+    const double sr = sin(ar), cr = cos(ar), sh = sinh(ai), ch = cosh(ai);
+    const double r = 2. * cr * ch, i = -2. * sr * sh;
+    double hr = 0., hi = 0., hr1 = a[n - 1], hi1 = 0., hr2 = 0., hi2 = 0.;
+    for (int k = n - 2; k >= 0; --k) {
+        hr = -hr2 + (r * hr1 - i * hi1) + a[k];
+        hi = -hi2 + (i * hr1 + r * hi1);
+        hr2 = hr1;
+        hi2 = hi1;
+        hr1 = hr;
+        hi1 = hi;
+    }
+    if (n == 1) {
+        hr = hr1;
+        hi = hi1;
+    }
+    // (sin(ar) cosh(ai) + i cos(ar) sinh(ai)) * (hr + i hi)
+    const double zr = sr * ch, zi = cr * sh;
+    *im = zr * hi + zi * hr;
+    return zr * hr - zi * hi;
+}
+
+// Projection parameters of an EPSG code (Projections.cpp:79-168, 182-214,
+// 262-281, 368-393); false for a code the reference's factory rejects.
+inline bool proj_init(int epsg, Proj* p)
+{
+    *p = Proj{};
+    const double e2 = kWgsE2;
+    if (epsg == 4326) {
+        p->kind = kProjLonLat;
+        return true;
+    }
+    if (epsg == 4978) {
+        p->kind = kProjGeocent;
+        return true;
+    }
+    if ((epsg > 32600 && epsg <= 32660) || (epsg > 32700 && epsg <= 32760)) {
+        p->kind = kProjUtm;
+        p->north = epsg < 32700;
+        const int zone = p->north ? epsg - 32600 : epsg - 32700;
+        p->lon0 = ((zone - 0.5) * (kPi / 30.)) - kPi;
+        const double f = e2 / (1. + sqrt(1. - e2));  // flattening
+        const double n = f / (2. - f);                // third flattening
+        const double n2 = n * n, n3 = n2 * n, n4 = n3 * n, n5 = n4 * n, n6 = n5 * n;
+        // Gauss <-> geodetic latitude (Krueger, to n^6)
+        p->cgb[0] = n * (2 + n * (-2. / 3. + n * (-2 + n * (116. / 45. + n * (26. / 45. + n * (-2854. / 675.))))));
+        p->cbg[0] = n * (-2 + n * (2. / 3. + n * (4. / 3. + n * (-82. / 45. + n * (32. / 45. + n * (4642. / 4725.))))));
+        p->cgb[1] = n2 * (7. / 3. + n * (-8. / 5. + n * (-227. / 45. + n * (2704. / 315. + n * (2323. / 945.)))));
+        p->cbg[1] = n2 * (5. / 3. + n * (-16. / 15. + n * (-13. / 9. + n * (904. / 315. + n * (-1522. / 945.)))));
+        p->cgb[2] = n3 * (56. / 15. + n * (-136. / 35. + n * (-1262. / 105. + n * (73814. / 2835.))));
+        p->cbg[2] = n3 * (-26. / 15. + n * (34. / 21. + n * (8. / 5. + n * (-12686. / 2835.))));
+        p->cgb[3] = n4 * (4279. / 630. + n * (-332. / 35. + n * (-399572. / 14175.)));
+        p->cbg[3] = n4 * (1237. / 630. + n * (-12. / 5. + n * (-24832. / 14175.)));
+        p->cgb[4] = n5 * (4174. / 315. + n * (-144838. / 6237.));
+        p->cbg[4] = n5 * (-734. / 315. + n * (109598. / 31185.));
+        p->cgb[5] = n6 * (601676. / 22275.);
+        p->cbg[5] = n6 * (444337. / 155925.);
+        // meridian quadrant scaled by k0 = 0.9996
+        p->qn = (0.9996 / (1. + n)) * (1. + n2 * (1. / 4. + n2 * (1. / 64. + n2 / 256.)));
+        // ellipsoidal <-> spherical northing/easting
+        p->utg[0] = n * (-.5 + n * (2. / 3. + n * (-37. / 96. + n * (1. / 360. + n * (81. / 512. + n * (-96199. / 604800.))))));
+        p->gtu[0] = n * (.5 + n * (-2. / 3. + n * (5. / 16. + n * (41. / 180. + n * (-127. / 288. + n * (7891. / 37800.))))));
+        p->utg[1] = n2 * (-1. / 48. + n * (-1. / 15. + n * (437. / 1440. + n * (-46. / 105. + n * (1118711. / 3870720.)))));
+        p->gtu[1] = n2 * (13. / 48. + n * (-3. / 5. + n * (557. / 1440. + n * (281. / 630. + n * (-1983433. / 1935360.)))));
+        p->utg[2] = n3 * (-17. / 480. + n * (37. / 840. + n * (209. / 4480. + n * (-5569. / 90720.))));
+        p->gtu[2] = n3 * (61. / 240. + n * (-103. / 140. + n * (15061. / 26880. + n * (167603. / 181440.))));
+        p->utg[3] = n4 * (-4397. / 161280. + n * (11. / 504. + n * (830251. / 7257600.)));
+        p->gtu[3] = n4 * (49561. / 161280. + n * (-179. / 168. + n * (6601661. / 7257600.)));
+        p->utg[4] = n5 * (-4583. / 161280. + n * (108847. / 3991680.));
+        p->gtu[4] = n5 * (34729. / 80640. + n * (-3418889. / 1995840.));
+        p->utg[5] = n6 * (-20648693. / 638668800.);
+        p->gtu[5] = n6 * (212378941. / 319334400.);
+        // northing of the equator: Gauss latitude of 0 is 0
+        const double z = clenshaw_sin(p->cbg, 6, 0.);
+        p->zb = -p->qn * (z + clenshaw_sin(p->gtu, 6, 2 * z));
+        return true;
+    }
+    if (epsg == 3031 || epsg == 3413) {
+        p->kind = kProjPolar;
+        p->north = epsg == 3413;
+        const double lat_ts = p->north ? 70. * (kPi / 180.) : (71. * kPi) / 180.;
+        p->lon0 = p->north ? -45. * (kPi / 180.) : 0.;
+        p->e = sqrt(e2);
+        const double es = p->e * sin(lat_ts);
+        const double ts = tan(.5 * ((.5 * kPi) - lat_ts)) / pow((1. - es) / (1. + es), .5 * p->e);
+        p->akm1 = cos(lat_ts) / ts;
+        p->akm1 *= kWgsA / sqrt(1. - ((p->e * p->e) * (sin(lat_ts) * sin(lat_ts))));
+        return true;
+    }
+    if (epsg == 6933) {
+        p->kind = kProjCea;
+        const double lat_ts = kPi / 6.;
+        p->k0 = cos(lat_ts) / sqrt(1. - (e2 * (sin(lat_ts) * sin(lat_ts))));
+        p->e = sqrt(e2);
+        p->one_es = 1. - e2;
+        p->apa[0] = e2 * ((1. / 3.) + (e2 * ((31. / 180.) + (e2 * (517. / 5040.)))));
+        p->apa[1] = e2 * e2 * ((23. / 360.) + (e2 * (251. / 3780.)));
+        p->apa[2] = e2 * e2 * e2 * (761. / 45360.);
+        const double con = p->e;  // authalic q at the pole (sin(phi) = 1)
+        p->qp = p->one_es * ((1. / (1. - con * con)) - ((.5 / p->e) * log((1. - con) / (1. + con))));
+        return true;
+    }
+    return false;
+}
+
+// PROJ's t(phi) of the polar stereographic (Projections.cpp:252-257)
+BP_HD double polar_tsfn(double phi, double sinphi, double e)
+{
+    sinphi *= e;
+    return tan(.5 * ((.5 * kPi) - phi)) / pow((1. - sinphi) / (1. + sinphi), .5 * e);
+}
+
+// authalic q(phi) of the equal-area projection (Projections.cpp:353-358)
+BP_HD double cea_qsfn(double sinphi, double e, double one_es)
+{
+    const double con = e * sinphi;
+    return one_es * ((sinphi / (1. - con * con)) - ((.5 / e) * log((1. - con) / (1. + con))));
+}
+
+// (lon, lat, h) in radians -> projected (x, y, h); false where the reference's
+// forward reports failure (UTM beyond its easting limit).
+BP_HD bool proj_forward(const Proj& p, V3 llh, V3* out)
+{
+    switch (p.kind) {
+    case kProjLonLat:  // Projections.h:127-133
+        *out = {llh.x * 180.0 / kPi, llh.y * 180.0 / kPi, llh.z};
+        return true;
+    case kProjGeocent:  // Projections.cpp:10-16
+        *out = lonlat_to_xyz(kWgsA, kWgsE2, llh);
+        return true;
+    case kProjUtm: {  // Projections.cpp:170-197
+        const double gauss = clenshaw_sin(p.cbg, 6, 2. * llh.y) + llh.y;
+        const double lam = llh.x - p.lon0;
+        double cn = atan2(sin(gauss), cos(lam) * cos(gauss));
+        double ce = atan2(sin(lam) * cos(gauss), hypot(sin(gauss), cos(gauss) * cos(lam)));
+        ce = asinh(tan(ce));
+        double dce;
+        cn += clenshaw_sin_complex(p.gtu, 6, 2 * cn, 2 * ce, &dce);
+        ce += dce;
+        if (!(fabs(ce) <= 2.623395162778)) return false;
+        *out = {(p.qn * ce * kWgsA) + 500000.,
+                (((p.qn * cn) + p.zb) * kWgsA) + (p.north ? 0. : 10000000.), llh.z};
+        return true;
+    }
+    case kProjPolar: {  // Projections.cpp:283-297
+        const double lam = llh.x - p.lon0;
+        const double phi = llh.y * (p.north ? 1. : -1.);
+        const double t = p.akm1 * polar_tsfn(phi, sin(phi), p.e);
+        *out = {t * sin(lam), -t * cos(lam) * (p.north ? 1. : -1.), llh.z};
+        return true;
+    }
+    default:  // kProjCea, Projections.cpp:395-401
+        *out = {p.k0 * llh.x * kWgsA, (.5 * kWgsA * cea_qsfn(sin(llh.y), p.e, p.one_es)) / p.k0,
+                llh.z};
+        return true;
+    }
+}
+
+// Projected (x, y, h) -> (lon, lat, h) in radians; false where the reference's
+// inverse reports failure.
+inline bool proj_inverse(const Proj& p, V3 xyz, V3* llh);
+
+// ------------------------------------------------------------ DEM interpolators
+BP_HD int64_t clamp64(int64_t v, int64_t lo, int64_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Catmull-Rom value between p1 and p2 (core/BicubicInterpolator.cpp:9-38, float)
+BP_HD float catmull_rom(float p0, float p1, float p2, float p3, double t)
+{
+    const double u = 1. - t;
+    return (float(t) * (p2 - p0 * float(u * u) + (p2 * float(u * 3. + 1.) - p3 * float(u)) * float(t)) +
+            p1 * float(t * t * (t * 3. - 5.) + 2.)) / 2.f;
+}
+
+BP_HD float bicubic_f32(double x, double y, const float* z, int64_t width, int64_t length)
+{
+    // core/BicubicInterpolator.cpp:43-61; rows/columns past the raster (the
+    // reference reads beyond it there) are clamped to its last row/column
+    const int x0 = static_cast<int>(floor(x)), y0 = static_cast<int>(floor(y));
+    float col[4];
+#pragma unroll
+    for (int i = -1; i < 3; ++i) {
+        const float* row = z + clamp64(y0 + i, 0, length - 1) * width;
+        col[i + 1] = catmull_rom(row[x0 - 1], row[x0], row[clamp64(x0 + 1, 0, width - 1)],
+                                 row[clamp64(x0 + 2, 0, width - 1)], x - x0);
+    }
+    return catmull_rom(col[0], col[1], col[2], col[3], y - y0);
+}
+
+// Natural cubic spline through n samples (second derivatives R by the
+// tridiagonal sweep of core/Spline2dInterpolator.cpp:109-126), evaluated at
+// 1-based position t inside (1, n) (:80-106, middle branch).
+// The sweep's pivots p_i = 1 / (Q_{i-1} / 2 + 2) and Q_i = -p_i / 2 do not
+// depend on the samples: they are evaluated once, in float, at compile time.
+template<int N>
+struct SplinePivots {
+    float p[N], q[N];
+    constexpr SplinePivots() : p{}, q{}
+    {
+        for (int i = 1; i < N - 1; ++i) {
+            p[i] = 1.f / (0.5f * q[i - 1] + 2.f);
+            q[i] = -0.5f * p[i];
+        }
+    }
+};
+
+template<int N>
+BP_HD void spline_init(const float (&Y)[N], float (&R)[N])
+{
+    constexpr SplinePivots<N> piv;
+    float Q[N];
+    Q[0] = 0.f;
+    R[0] = 0.f;
+#pragma unroll
+    for (int i = 1; i < N - 1; ++i) {
+        const float p = piv.p[i];
+        Q[i] = piv.q[i];
+        R[i] = (3.f * (Y[i + 1] - 2.f * Y[i] + Y[i - 1]) - 0.5f * R[i - 1]) * p;
+    }
+    R[N - 1] = 0.f;
+#pragma unroll
+    for (int i = N - 2; i > 0; --i) R[i] = Q[i] * R[i + 1] + R[i];
+}
+
+template<int N>
+BP_HD float spline_eval(double t, const float (&Y)[N], const float (&R)[N])
+{
+    if (t < 1.0) return Y[0] + static_cast<float>(t - 1.0) * (Y[1] - Y[0] - (R[1] / 6.f));
+    if (t > N) return Y[N - 1] + static_cast<float>(t - N) * (Y[N - 1] - Y[N - 2] + (R[N - 2] / 6.f));
+    const int j = static_cast<int>(floor(t));
+    const float u = static_cast<float>(t - j);
+    // Y[j - 1], Y[j], R[j - 1], R[j] by unrolled selection (no indexed private arrays)
+    float y0 = Y[0], y1 = Y[1], r0 = R[0], r1 = R[1];
+#pragma unroll
+    for (int k = 2; k < N; ++k) {
+        if (j == k) {
+            y0 = Y[k - 1];
+            y1 = Y[k];
+            r0 = R[k - 1];
+            r1 = R[k];
+        }
+    }
+    const float t0 = y1 - y0 - (r0 / 3.f) - (r1 / 6.f);
+    const float t1 = u * ((r0 / 2.f) + (u * ((r1 - r0) / 6.f)));
+    return y0 + (u * (t0 + t1));
+}
+
+// Order-6 spline of core/Spline2dInterpolator.cpp:31-64 (the biquintic
+// method): rows and columns clamp(start + k, 0, n - 2) + 1, k < 6, with
+// start = trunc(coordinate) - 2.
+BP_HD float biquintic_f32(double x, double y, const float* z, int64_t width, int64_t length)
+{
+    constexpr int N = 6;
+    const int i0 = static_cast<int>(y) - 2, j0 = static_cast<int>(x) - 2;
+    float hc[N], a[N], r[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const float* row = z + (clamp64(i0 + i, 0, length - 2) + 1) * width;
+#pragma unroll
+        for (int j = 0; j < N; ++j) a[j] = row[clamp64(j0 + j, 0, width - 2) + 1];
+        spline_init(a, r);
+        hc[i] = spline_eval(x - j0, a, r);
+    }
+    spline_init(hc, r);
+    return spline_eval(y - i0, hc, r);
+}
+
+// DEM lookup in native raster coordinates
 // (geometry/DEMInterpolator.cpp:615-662).
 BP_HD double dem_interpolate_xy(const Dem& dem, double x, double y)
 {
@@ -333,15 +639,110 @@ BP_HD double dem_interpolate_xy(const Dem& dem, double x, double y)
     const int icol = static_cast<int>(floor(col));
     if (irow < 2 || irow >= int(dem.length - 1)) return dem.ref_height;
     if (icol < 2 || icol >= int(dem.width - 1)) return dem.ref_height;
-    return bilinear_f32(col, row, dem.data, dem.width);
+    switch (dem.method) {
+    case kDemBicubic: return bicubic_f32(col, row, dem.data, dem.width, dem.length);
+    case kDemNearest:  // core/NearestNeighborInterpolator.cpp:11-22
+        return dem.data[(int64_t)llround(row) * dem.width + llround(col)];
+    case kDemBiquintic: return biquintic_f32(col, row, dem.data, dem.width, dem.length);
+    default: return bilinear_f32(col, row, dem.data, dem.width);
+    }
 }
 
-// geometry/DEMInterpolator.cpp:593-610 through the LonLat projection
-// (core/Projections.h:127-133: radians -> degrees).
+// geometry/DEMInterpolator.cpp:593-610: (lon, lat) through the raster's
+// projection, then the native-coordinate lookup.
 BP_HD double dem_interpolate_lonlat(const Dem& dem, double lon, double lat)
 {
     if (!dem.have_raster) return dem.ref_height;
-    return dem_interpolate_xy(dem, lon * 180.0 / kPi, lat * 180.0 / kPi);
+    V3 xy;
+    if (!proj_forward(dem.proj, V3{lon, lat, 0.0}, &xy)) return dem.ref_height;
+    return dem_interpolate_xy(dem, xy.x, xy.y);
+}
+
+// True when a DEM needs more than the lon/lat-degree bilinear lookup (the
+// geometry kernel has a compact variant for that common case).
+BP_HD bool dem_generic(const Dem& dem)
+{
+    return dem.have_raster && !(dem.proj.kind == kProjLonLat && dem.method == kDemBilinear);
+}
+
+// The lookup of dem_interpolate_lonlat, specialised: kGeneric false is the
+// EPSG:4326 bilinear case only (callers check dem_generic first).
+template<bool kGeneric>
+BP_HD double dem_lookup(const Dem& dem, double lon, double lat)
+{
+    if (kGeneric) return dem_interpolate_lonlat(dem, lon, lat);
+    if (!dem.have_raster) return dem.ref_height;
+    double x = lon * 180.0 / kPi, y = lat * 180.0 / kPi;
+    if (x > 360 || x < -360) x = fmod(x, 360.0);
+    if (x < -180) x += 360;
+    if (x - 360 >= dem.xstart) {
+        x -= 360;
+    } else if (x < dem.xstart && x + 360 >= dem.xstart) {
+        x += 360;
+    } else if (x < dem.xstart) {
+        return dem.ref_height;
+    }
+    const double row = (y - dem.ystart) / dem.dy;
+    const double col = (x - dem.xstart) / dem.dx;
+    const int irow = static_cast<int>(floor(row));
+    const int icol = static_cast<int>(floor(col));
+    if (irow < 2 || irow >= int(dem.length - 1)) return dem.ref_height;
+    if (icol < 2 || icol >= int(dem.width - 1)) return dem.ref_height;
+    return bilinear_f32(col, row, dem.data, dem.width);
+}
+
+inline bool proj_inverse(const Proj& p, V3 xyz, V3* llh)
+{
+    switch (p.kind) {
+    case kProjLonLat:  // Projections.h:135-141
+        *llh = {xyz.x * kPi / 180.0, xyz.y * kPi / 180.0, xyz.z};
+        return true;
+    case kProjGeocent:  // Projections.cpp:18-24
+        *llh = xyz_to_lonlat(kWgsA, kWgsE2, xyz);
+        return true;
+    case kProjUtm: {  // Projections.cpp:199-232
+        double cn = (xyz.y - (p.north ? 0. : 10000000.)) / kWgsA;
+        double ce = (xyz.x - 500000.) / kWgsA;
+        cn = (cn - p.zb) / p.qn;
+        ce /= p.qn;
+        if (!(fabs(ce) <= 2.623395162778)) return false;
+        double dce;
+        cn += clenshaw_sin_complex(p.utg, 6, 2 * cn, 2 * ce, &dce);
+        ce = atan(sinh(ce + dce));
+        const double sce = sin(ce), cce = cos(ce);
+        ce = atan2(sce, cce * cos(cn));
+        cn = atan2(sin(cn) * cce, hypot(sce, cce * cos(cn)));
+        *llh = {ce + p.lon0, clenshaw_sin(p.cgb, 6, 2 * cn) + cn, xyz.z};
+        return true;
+    }
+    case kProjPolar: {  // Projections.cpp:299-322
+        const double tp = -hypot(xyz.x, xyz.y) / p.akm1;
+        const double fact = p.north ? 1. : -1.;
+        double phi_l = (.5 * kPi) - (2. * atan(tp));
+        for (int i = 0; i < 8; ++i) {
+            const double sinphi = p.e * sin(phi_l);
+            const double phi = 0.5 * kPi +
+                               2. * atan(tp * pow((1. + sinphi) / (1. - sinphi), -0.5 * p.e));
+            if (fabs(phi_l - phi) < 1.e-10) {
+                const double lon = (xyz.x == 0. && xyz.y == 0.)
+                                       ? 0.
+                                       : atan2(xyz.x, -fact * xyz.y) + p.lon0;
+                *llh = {lon, phi * fact, xyz.z};
+                return true;
+            }
+            phi_l = phi;
+        }
+        return false;
+    }
+    default: {  // kProjCea, Projections.cpp:403-411
+        const double beta = asin((2. * xyz.y * p.k0) / (kWgsA * p.qp));
+        *llh = {xyz.x / (p.k0 * kWgsA),
+                beta + (p.apa[0] * sin(2. * beta)) + (p.apa[1] * sin(4. * beta)) +
+                    (p.apa[2] * sin(6. * beta)),
+                xyz.z};
+        return true;
+    }
+    }
 }
 
 // core/LUT2d.cpp:127-159 (bilinear, clamped to the table).
@@ -470,6 +871,7 @@ struct G2RParams {
 // rdr2geo by Brent on the pseudo-look angle against the DEM
 // (geometry/detail/Rdr2Geo.icc:172-242).  Returns kOrbitInterpDomainError
 // when aztime is outside the orbit (the CPU raises OutOfRange there).
+template<bool kGenericDem = true>
 BP_HD int rdr2geo_bracket(double aztime, double slant_range, double doppler, const Orbit& orbit,
                           const Dem& dem, double wavelength, int side, const R2GParams& params,
                           V3* xyz)
@@ -497,7 +899,7 @@ BP_HD int rdr2geo_bracket(double aztime, double slant_range, double doppler, con
     };
     auto dh = [&](double look) {
         V3 llh = xyz_to_lonlat(kWgsA, kWgsE2, get_xyz(look));
-        return llh.z - dem_interpolate_lonlat(dem, llh.x, llh.y);
+        return llh.z - dem_lookup<kGenericDem>(dem, llh.x, llh.y);
     };
     const double tol_look = params.tol_height / radius;
     double look = 0.0;

@@ -3,8 +3,10 @@
 The reference loads rasters through GDAL (geometry/DEMInterpolator.cpp:442-494),
 which is outside this package; rasters are handed over as in-memory arrays
 with :meth:`DEMInterpolator.load_dem_array` instead.  Lookup semantics follow
-geometry/DEMInterpolator.cpp:593-662 (EPSG:4326, bilinear, reference height
-outside the raster's [2, n-1) interior).
+geometry/DEMInterpolator.cpp:593-662: (lon, lat) through the raster's map
+projection (any EPSG code of core/Projections.cpp's factory), interpolation by
+the nearest / bilinear / bicubic / biquintic method, reference height outside
+the raster's [2, n-1) interior.
 """
 from __future__ import annotations
 
@@ -13,6 +15,9 @@ import ctypes as C
 import numpy as np
 
 from . import _native, errors
+
+# isce3::core::dataInterpMethod values (core/Constants.h:23-29)
+_METHODS = {"bilinear": 1, "bicubic": 2, "nearest": 3, "biquintic": 4}
 
 
 class DEMInterpolator:
@@ -30,10 +35,12 @@ class DEMInterpolator:
 
     @staticmethod
     def _parse_method(method) -> str:
-        m = str(method).lower()
-        if m not in ("bilinear",):
-            raise errors.InvalidArgument(
-                f"DEM interpolation method {method!r} is not supported (bilinear only)")
+        # core/Constants.cpp:10-24 (parseDataInterpMethod), sinc not offered
+        m = str(getattr(method, "name", method)).lower()
+        if m == "sinc":
+            raise errors.InvalidArgument("sinc DEM interpolation is not supported")
+        if m not in _METHODS:
+            raise errors.InvalidArgument("Unknown interp method")
         return m
 
     def load_dem_array(self, data, x_start: float, y_start: float, delta_x: float,
@@ -43,8 +50,8 @@ class DEMInterpolator:
         data = np.ascontiguousarray(data, dtype=np.float32)
         if data.ndim != 2:
             raise errors.InvalidArgument("DEM raster must be 2-D")
-        if int(epsg) != 4326:
-            raise errors.InvalidArgument("only EPSG:4326 DEM rasters are supported")
+        from .core import make_projection
+        make_projection(int(epsg))       # RuntimeError for a code the factory rejects
         self._data = data
         self._data.setflags(write=False)
         self._x_start, self._y_start = float(x_start), float(y_start)
@@ -124,7 +131,8 @@ class DEMInterpolator:
 
     def _struct(self) -> _native.Dem_t:
         have = self._data is not None
-        return _native.Dem_t(int(have), self._epsg, 0, 0, self._ref_height, self._x_start,
+        return _native.Dem_t(int(have), self._epsg, _METHODS[self._method], 0, self._ref_height,
+                             self._x_start,
                              self._y_start, self._dx, self._dy, self.width, self.length,
                              self._data.ctypes.data if have else None)
 

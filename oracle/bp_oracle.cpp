@@ -274,6 +274,354 @@ double bilinear_f64(double x, double y, const double* z, int64_t width)
            ((q22 * ((x - x1) * (y - y1))) / den);
 }
 
+// ---------------------------------------------------------------- 2-D interpolation
+// Value type T of the raster; weights and intermediate sums in T, as the
+// reference's Interpolator<U> templates compute them.
+template<class T>
+T raster_at(const T* z, int64_t width, int64_t r, int64_t c) { return z[r * width + c]; }
+
+// core/NearestNeighborInterpolator.cpp:11-22
+template<class T>
+T nearest2d(double x, double y, const T* z, int64_t width)
+{
+    return raster_at(z, width, (int64_t)std::round(y), (int64_t)std::round(x));
+}
+
+// Catmull-Rom segment between p1 and p2 at fraction t
+// (core/BicubicInterpolator.cpp:9-38).
+template<class T>
+T catmull_rom(T p0, T p1, T p2, T p3, double t)
+{
+    const double s = 1. - t;
+    const T a = p2 - p0 * T(s * s);
+    const T b = (p2 * T(s * 3. + 1.) - p3 * T(s)) * T(t);
+    const T c = p1 * T(t * t * (t * 3. - 5.) + 2.);
+    return (T(t) * (a + b) + c) / T(2.);
+}
+
+// core/BicubicInterpolator.cpp:43-61 over rows/columns floor-1 .. floor+2;
+// indices past the raster (which the reference reads beyond it) are clamped
+// to its last row/column.
+template<class T>
+T bicubic2d(double x, double y, const T* z, int64_t width, int64_t length)
+{
+    const int64_t xf = (int64_t)std::floor(x), yf = (int64_t)std::floor(y);
+    auto cl = [](int64_t v, int64_t hi) { return v > hi ? hi : v; };
+    T v[4];
+    for (int k = 0; k < 4; ++k) {
+        const int64_t r = cl(yf - 1 + k, length - 1);
+        v[k] = catmull_rom<T>(raster_at(z, width, r, xf - 1), raster_at(z, width, r, xf),
+                              raster_at(z, width, r, cl(xf + 1, width - 1)),
+                              raster_at(z, width, r, cl(xf + 2, width - 1)), x - (double)xf);
+    }
+    return catmull_rom<T>(v[0], v[1], v[2], v[3], y - (double)yf);
+}
+
+// Natural cubic spline coefficients (second derivatives) of samples y by a
+// forward elimination / back substitution on the tridiagonal system
+// (core/Spline2dInterpolator.cpp:109-126).
+template<class T>
+std::vector<T> spline_second_derivs(const std::vector<T>& y)
+{
+    const size_t n = y.size();
+    std::vector<T> r(n, T(0)), q(n, T(0));
+    for (size_t i = 1; i + 1 < n; ++i) {
+        const T piv = T(1.0) / (T(0.5) * q[i - 1] + T(2.0));
+        q[i] = T(-0.5) * piv;
+        r[i] = (T(3.0) * (y[i + 1] - T(2.0) * y[i] + y[i - 1]) - T(0.5) * r[i - 1]) * piv;
+    }
+    r[n - 1] = T(0);
+    for (size_t i = n - 2; i > 0; --i) r[i] = q[i] * r[i + 1] + r[i];
+    return r;
+}
+
+// Spline value at 1-based position t (core/Spline2dInterpolator.cpp:80-106).
+template<class T>
+T spline_value(double t, const std::vector<T>& y, const std::vector<T>& r)
+{
+    const int n = (int)y.size();
+    if (t < 1.0) return y[0] + T(t - 1.0) * (y[1] - y[0] - (r[1] / T(6.0)));
+    if (t > n) return y[n - 1] + T(t - n) * (y[n - 1] - y[n - 2] + (r[n - 2] / T(6.0)));
+    const int j = (int)std::floor(t);
+    const T u = T(t - j);
+    const T lin = y[j] - y[j - 1] - (r[j - 1] / T(3.0)) - (r[j] / T(6.0));
+    const T cur = u * ((r[j - 1] / T(2.0)) + (u * ((r[j] - r[j - 1]) / T(6.0))));
+    return y[j - 1] + (u * (lin + cur));
+}
+
+// Separable spline of `order` points (core/Spline2dInterpolator.cpp:31-64;
+// the biquintic method is order 6): window start trunc(coordinate)
+// (minus 0.5 for odd orders) - order/2 + 1, indices clamped to [0, n-2], +1.
+template<class T>
+T spline2d(double x, double y, const T* z, int64_t width, int64_t length, int order = 6)
+{
+    int i0 = (order % 2) ? (int)(y - 0.5) : (int)y;
+    int j0 = (order % 2) ? (int)(x - 0.5) : (int)x;
+    i0 = i0 - order / 2 + 1;
+    j0 = j0 - order / 2 + 1;
+    std::vector<T> cols(order), line(order);
+    for (int i = 0; i < order; ++i) {
+        const int64_t r = std::min<int64_t>(std::max<int64_t>(i0 + i, 0), length - 2) + 1;
+        for (int j = 0; j < order; ++j) {
+            const int64_t c = std::min<int64_t>(std::max<int64_t>(j0 + j, 0), width - 2) + 1;
+            line[j] = raster_at(z, width, r, c);
+        }
+        cols[i] = spline_value(x - j0, line, spline_second_derivs(line));
+    }
+    return spline_value(y - i0, cols, spline_second_derivs(cols));
+}
+
+template<class T>
+T interp2d(int method, double x, double y, const T* z, int64_t width, int64_t length);
+
+template<>
+float interp2d<float>(int method, double x, double y, const float* z, int64_t width,
+                      int64_t length)
+{
+    switch (method) {
+    case ORC_DEM_BICUBIC: return bicubic2d(x, y, z, width, length);
+    case ORC_DEM_NEAREST: return nearest2d(x, y, z, width);
+    case ORC_DEM_BIQUINTIC: return spline2d(x, y, z, width, length);
+    default: return bilinear_f32(x, y, z, width);
+    }
+}
+
+template<>
+double interp2d<double>(int method, double x, double y, const double* z, int64_t width,
+                        int64_t length)
+{
+    switch (method) {
+    case ORC_DEM_BICUBIC: return bicubic2d(x, y, z, width, length);
+    case ORC_DEM_NEAREST: return nearest2d(x, y, z, width);
+    case ORC_DEM_BIQUINTIC: return spline2d(x, y, z, width, length);
+    default: return bilinear_f64(x, y, z, width);
+    }
+}
+
+// ---------------------------------------------------------------- projections
+// core/Projections.cpp, WGS84.  Series and constants as there (the
+// Poder/Engsager transverse Mercator of PROJ, PROJ's polar stereographic and
+// equal-area cylindrical formulas).
+struct OrcProj {
+    int code = 4326;
+    bool north = true;
+    double lon0 = 0.;
+    double Qn = 0., Zb = 0., cgb[6] = {}, cbg[6] = {}, utg[6] = {}, gtu[6] = {};
+    double akm1 = 0., e = 0.;
+    double k0 = 0., one_es = 0., qp = 0., apa[3] = {};
+};
+
+// sin(arg) * sum_k a_k U_k(cos arg) by Clenshaw's recurrence (real argument)
+double clenshaw_real(const double* a, int n, double arg)
+{
+    double b1 = 0., b2 = 0., b = 0.;
+    const double c = std::cos(arg);
+    b1 = a[n - 1];
+    for (int k = n - 2; k >= 0; --k) {
+        b = -b2 + (2. * b1 * c) + a[k];
+        b2 = b1;
+        b1 = b;
+    }
+    return std::sin(arg) * b;
+}
+
+// Complex Clenshaw at (re, im): returns the real part, im part in *out_im
+double clenshaw_cplx(const double* a, int n, double re, double im, double* out_im)
+{
+    const double s = std::sin(re), c = std::cos(re), sh = std::sinh(im), ch = std::cosh(im);
+    double br = 0., bi = 0., br1 = a[n - 1], bi1 = 0., br2 = 0., bi2 = 0.;
+    for (int k = n - 2; k >= 0; --k) {
+        br = -br2 + (2. * br1 * c * ch) - (-2. * bi1 * s * sh) + a[k];
+        bi = -bi2 + (-2. * br1 * s * sh) + (2. * bi1 * c * ch);
+        br2 = br1;
+        bi2 = bi1;
+        br1 = br;
+        bi1 = bi;
+    }
+    *out_im = (s * ch * bi) + (c * sh * br);
+    return (s * ch * br) - (c * sh * bi);
+}
+
+double tsfn(double phi, double sinphi, double e)
+{
+    const double es = sinphi * e;
+    return std::tan(.5 * ((.5 * M_PI) - phi)) / std::pow((1. - es) / (1. + es), .5 * e);
+}
+
+double qsfn(double sinphi, double e, double one_es)
+{
+    const double con = e * sinphi;
+    return one_es * ((sinphi / (1. - std::pow(con, 2))) - ((.5 / e) * std::log((1. - con) / (1. + con))));
+}
+
+bool make_proj(int code, OrcProj& p)
+{
+    p = OrcProj{};
+    p.code = code;
+    const double e2 = kWgsE2;
+    if (code == 4326 || code == 4978) return true;
+    if ((code > 32600 && code <= 32660) || (code > 32700 && code <= 32760)) {
+        p.north = code < 32700;
+        const int zone = p.north ? code - 32600 : code - 32700;
+        p.lon0 = ((zone - 0.5) * (M_PI / 30.)) - M_PI;
+        const double f = e2 / (1. + std::sqrt(1 - e2));
+        const double n = f / (2. - f);
+        const double np[7] = {1., n, std::pow(n, 2), std::pow(n, 3), std::pow(n, 4),
+                              std::pow(n, 5), std::pow(n, 6)};
+        // polynomials in n, highest power last (Horner from the top)
+        auto horner = [n](std::initializer_list<double> c) {
+            double acc = 0.;
+            const double* b = c.begin();
+            for (int i = (int)c.size() - 1; i >= 0; --i) acc = (i == (int)c.size() - 1) ? b[i] : b[i] + n * acc;
+            return acc;
+        };
+        p.cgb[0] = n * horner({2., -2. / 3., -2., 116. / 45., 26. / 45., -2854. / 675.});
+        p.cbg[0] = n * horner({-2., 2. / 3., 4. / 3., -82. / 45., 32. / 45., 4642. / 4725.});
+        p.cgb[1] = np[2] * horner({7. / 3., -8. / 5., -227. / 45., 2704. / 315., 2323. / 945.});
+        p.cbg[1] = np[2] * horner({5. / 3., -16. / 15., -13. / 9., 904. / 315., -1522. / 945.});
+        p.cgb[2] = np[3] * horner({56. / 15., -136. / 35., -1262. / 105., 73814. / 2835.});
+        p.cbg[2] = np[3] * horner({-26. / 15., 34. / 21., 8. / 5., -12686. / 2835.});
+        p.cgb[3] = np[4] * horner({4279. / 630., -332. / 35., -399572. / 14175.});
+        p.cbg[3] = np[4] * horner({1237. / 630., -12. / 5., -24832. / 14175.});
+        p.cgb[4] = np[5] * horner({4174. / 315., -144838. / 6237.});
+        p.cbg[4] = np[5] * horner({-734. / 315., 109598. / 31185.});
+        p.cgb[5] = np[6] * (601676. / 22275.);
+        p.cbg[5] = np[6] * (444337. / 155925.);
+        p.Qn = (0.9996 / (1. + n)) * (1. + n * n * ((1. / 4.) + n * n * ((1. / 64.) + ((n * n) / 256.))));
+        p.utg[0] = n * horner({-.5, 2. / 3., -37. / 96., 1. / 360., 81. / 512., -96199. / 604800.});
+        p.gtu[0] = n * horner({.5, -2. / 3., 5. / 16., 41. / 180., -127. / 288., 7891. / 37800.});
+        p.utg[1] = np[2] * horner({-1. / 48., -1. / 15., 437. / 1440., -46. / 105., 1118711. / 3870720.});
+        p.gtu[1] = np[2] * horner({13. / 48., -3. / 5., 557. / 1440., 281. / 630., -1983433. / 1935360.});
+        p.utg[2] = np[3] * horner({-17. / 480., 37. / 840., 209. / 4480., -5569. / 90720.});
+        p.gtu[2] = np[3] * horner({61. / 240., -103. / 140., 15061. / 26880., 167603. / 181440.});
+        p.utg[3] = np[4] * horner({-4397. / 161280., 11. / 504., 830251. / 7257600.});
+        p.gtu[3] = np[4] * horner({49561. / 161280., -179. / 168., 6601661. / 7257600.});
+        p.utg[4] = np[5] * horner({-4583. / 161280., 108847. / 3991680.});
+        p.gtu[4] = np[5] * horner({34729. / 80640., -3418889. / 1995840.});
+        p.utg[5] = np[6] * (-20648693. / 638668800.);
+        p.gtu[5] = np[6] * (212378941. / 319334400.);
+        const double z = clenshaw_real(p.cbg, 6, 0.);
+        p.Zb = -p.Qn * (z + clenshaw_real(p.gtu, 6, 2 * z));
+        return true;
+    }
+    if (code == 3031 || code == 3413) {
+        p.north = code == 3413;
+        const double lat_ts = p.north ? 70. * (M_PI / 180.) : (71. * M_PI) / 180.;
+        p.lon0 = p.north ? -45. * (M_PI / 180.) : 0.;
+        p.e = std::sqrt(e2);
+        p.akm1 = std::cos(lat_ts) / tsfn(lat_ts, std::sin(lat_ts), p.e);
+        p.akm1 *= kWgsA / std::sqrt(1. - (std::pow(p.e, 2) * std::pow(std::sin(lat_ts), 2)));
+        return true;
+    }
+    if (code == 6933) {
+        const double lat_ts = M_PI / 6.;
+        p.k0 = std::cos(lat_ts) / std::sqrt(1. - (e2 * std::pow(std::sin(lat_ts), 2)));
+        p.e = std::sqrt(e2);
+        p.one_es = 1. - e2;
+        p.apa[0] = e2 * ((1. / 3.) + (e2 * ((31. / 180.) + (e2 * (517. / 5040.)))));
+        p.apa[1] = std::pow(e2, 2) * ((23. / 360.) + (e2 * (251. / 3780.)));
+        p.apa[2] = std::pow(e2, 3) * (761. / 45360.);
+        p.qp = qsfn(1., p.e, p.one_es);
+        return true;
+    }
+    return false;
+}
+
+int proj_fwd(const OrcProj& p, V3 llh, V3& out)
+{
+    if (p.code == 4326) {
+        out = {llh.x * 180.0 / M_PI, llh.y * 180.0 / M_PI, llh.z};
+        return 0;
+    }
+    if (p.code == 4978) {
+        out = lonlat_to_xyz(kWgsA, kWgsE2, llh);
+        return 0;
+    }
+    if (p.code > 32600 && p.code <= 32760) {
+        const double gauss = clenshaw_real(p.cbg, 6, 2. * llh.y) + llh.y;
+        const double lam = llh.x - p.lon0;
+        double cn = std::atan2(std::sin(gauss), std::cos(lam) * std::cos(gauss));
+        double ce = std::atan2(std::sin(lam) * std::cos(gauss),
+                               std::hypot(std::sin(gauss), std::cos(gauss) * std::cos(lam)));
+        ce = std::asinh(std::tan(ce));
+        double dce = 0.;
+        cn += clenshaw_cplx(p.gtu, 6, 2 * cn, 2 * ce, &dce);
+        ce += dce;
+        if (!(std::fabs(ce) <= 2.623395162778)) return 1;
+        out = {(p.Qn * ce * kWgsA) + 500000.,
+               (((p.Qn * cn) + p.Zb) * kWgsA) + (p.north ? 0. : 10000000.), llh.z};
+        return 0;
+    }
+    if (p.code == 3031 || p.code == 3413) {
+        const double sgn = p.north ? 1. : -1.;
+        const double lam = llh.x - p.lon0, phi = llh.y * sgn;
+        const double t = p.akm1 * tsfn(phi, std::sin(phi), p.e);
+        out = {t * std::sin(lam), -t * std::cos(lam) * sgn, llh.z};
+        return 0;
+    }
+    out = {p.k0 * llh.x * kWgsA, (.5 * kWgsA * qsfn(std::sin(llh.y), p.e, p.one_es)) / p.k0, llh.z};
+    return 0;
+}
+
+int proj_inv(const OrcProj& p, V3 xyz, V3& llh)
+{
+    if (p.code == 4326) {
+        llh = {xyz.x * M_PI / 180.0, xyz.y * M_PI / 180.0, xyz.z};
+        return 0;
+    }
+    if (p.code == 4978) {
+        llh = xyz_to_lonlat(kWgsA, kWgsE2, xyz);
+        return 0;
+    }
+    if (p.code > 32600 && p.code <= 32760) {
+        double cn = ((xyz.y - (p.north ? 0. : 10000000.)) / kWgsA - p.Zb) / p.Qn;
+        double ce = ((xyz.x - 500000.) / kWgsA) / p.Qn;
+        if (!(std::fabs(ce) <= 2.623395162778)) return 1;
+        double dce = 0.;
+        cn += clenshaw_cplx(p.utg, 6, 2 * cn, 2 * ce, &dce);
+        ce = std::atan(std::sinh(ce + dce));
+        const double sce = std::sin(ce), cce = std::cos(ce);
+        const double lon_s = std::atan2(sce, cce * std::cos(cn));
+        const double lat_g = std::atan2(std::sin(cn) * cce, std::hypot(sce, cce * std::cos(cn)));
+        llh = {lon_s + p.lon0, clenshaw_real(p.cgb, 6, 2 * lat_g) + lat_g, xyz.z};
+        return 0;
+    }
+    if (p.code == 3031 || p.code == 3413) {
+        const double sgn = p.north ? 1. : -1.;
+        const double tp = -std::hypot(xyz.x, xyz.y) / p.akm1;
+        double prev = (.5 * M_PI) - (2. * std::atan(tp));
+        for (int it = 0; it < 8; ++it) {
+            const double es = p.e * std::sin(prev);
+            const double phi = 0.5 * M_PI + 2. * std::atan(tp * std::pow((1. + es) / (1. - es), -0.5 * p.e));
+            if (std::fabs(prev - phi) < 1.e-10) {
+                const double lon = (xyz.x == 0. && xyz.y == 0.) ? 0.
+                                   : std::atan2(xyz.x, -sgn * xyz.y) + p.lon0;
+                llh = {lon, phi * sgn, xyz.z};
+                return 0;
+            }
+            prev = phi;
+        }
+        return 1;
+    }
+    const double beta = std::asin((2. * xyz.y * p.k0) / (kWgsA * p.qp));
+    llh = {xyz.x / (p.k0 * kWgsA),
+           beta + (p.apa[0] * std::sin(2. * beta)) + (p.apa[1] * std::sin(4. * beta)) +
+               (p.apa[2] * std::sin(6. * beta)),
+           xyz.z};
+    return 0;
+}
+
+const OrcProj& cached_proj(int code)
+{
+    thread_local OrcProj p;
+    thread_local int have = -1;
+    if (have != code) {
+        if (!make_proj(code, p)) make_proj(4326, p);
+        have = code;
+    }
+    return p;
+}
+
 // geometry/DEMInterpolator.cpp:615-662
 double dem_interp_xy(const orc_dem& dem, double x, double y)
 {
@@ -294,17 +642,16 @@ double dem_interp_xy(const orc_dem& dem, double x, double y)
     const int icol = static_cast<int>(std::floor(col));
     if (irow < 2 || irow >= int(dem.length - 1)) return dem.ref_height;
     if (icol < 2 || icol >= int(dem.width - 1)) return dem.ref_height;
-    return bilinear_f32(col, row, dem.data, dem.width);
+    return interp2d<float>(dem.method, col, row, dem.data, dem.width, dem.length);
 }
 
-// geometry/DEMInterpolator.cpp:593-610 with LonLat::forward
-// (core/Projections.h:127-133)
+// geometry/DEMInterpolator.cpp:593-610: through the raster's projection
 double dem_interp_lonlat(const orc_dem& dem, double lon, double lat)
 {
     if (!dem.have_raster) return dem.ref_height;
-    double x = lon * 180.0 / M_PI;
-    double y = lat * 180.0 / M_PI;
-    return dem_interp_xy(dem, x, y);
+    V3 xy;
+    if (proj_fwd(cached_proj(dem.epsg), V3{lon, lat, 0.0}, xy) != 0) return dem.ref_height;
+    return dem_interp_xy(dem, xy.x, xy.y);
 }
 
 // ---------------------------------------------------------------- LUT2d
@@ -643,6 +990,38 @@ double orc_dem_interpolate_xy(const orc_dem* dem, double x, double y)
 
 double orc_lut2d_eval(const orc_lut2d* lut, double y, double x) { return lut_eval(*lut, y, x); }
 
+int orc_proj_forward(int epsg, const double* llh, double* xyz)
+{
+    OrcProj p;
+    if (!make_proj(epsg, p)) return -1;
+    V3 out{};
+    const int rc = proj_fwd(p, load3(llh), out);
+    if (rc == 0) store3(out, xyz);
+    return rc;
+}
+
+int orc_proj_inverse(int epsg, const double* xyz, double* llh)
+{
+    OrcProj p;
+    if (!make_proj(epsg, p)) return -1;
+    V3 out{};
+    const int rc = proj_inv(p, load3(xyz), out);
+    if (rc == 0) store3(out, llh);
+    return rc;
+}
+
+float orc_interp2d_f32(int method, double x, double y, const float* z, int64_t width,
+                       int64_t length)
+{
+    return interp2d<float>(method, x, y, z, width, length);
+}
+
+double orc_interp2d_f64(int method, double x, double y, const double* z, int64_t width,
+                        int64_t length)
+{
+    return interp2d<double>(method, x, y, z, width, length);
+}
+
 double orc_bistatic_delay(const double* p, const double* v, const double* x)
 {
     return bistatic_delay(load3(p), load3(v), load3(x));
@@ -711,9 +1090,16 @@ int orc_backproject(const orc_grid* out_grid, const orc_orbit* out_orbit,
         g_err = "unexpected dry troposphere model";
         return ORC_ARG_ERROR;
     }
-    if (dem->have_raster && dem->epsg != 4326) {
-        g_err = "oracle supports EPSG:4326 DEMs only";
-        return ORC_ARG_ERROR;
+    if (dem->have_raster) {
+        OrcProj probe;
+        if (!make_proj(dem->epsg, probe)) {
+            g_err = "Unknown EPSG code (in factory): " + std::to_string(dem->epsg);
+            return ORC_ARG_ERROR;
+        }
+        if (dem->method < ORC_DEM_BILINEAR || dem->method > ORC_DEM_BIQUINTIC) {
+            g_err = "unsupported DEM interpolation method";
+            return ORC_ARG_ERROR;
+        }
     }
     if (mode == ORC_MODE_FP64 && kernel->table64 == nullptr) {
         g_err = "FP64 mode needs a double kernel table";

@@ -78,9 +78,14 @@ typedef struct {
     int32_t pad_;
 } orc_grid;
 
+/* DEM interpolation methods: isce3::core::dataInterpMethod values */
+enum { ORC_DEM_BILINEAR = 1, ORC_DEM_BICUBIC = 2, ORC_DEM_NEAREST = 3, ORC_DEM_BIQUINTIC = 4 };
+
 typedef struct {
     int32_t have_raster;
     int32_t epsg;
+    int32_t method;     /* ORC_DEM_* */
+    int32_t pad_;
     double ref_height;
     double xstart, ystart, dx, dy;
     int64_t width, length;
@@ -134,6 +139,16 @@ void orc_xyz_to_lonlat(double a, double e2, const double* xyz, double* llh);
 double orc_dem_interpolate_lonlat(const orc_dem* dem, double lon, double lat);
 double orc_dem_interpolate_xy(const orc_dem* dem, double x, double y);
 double orc_lut2d_eval(const orc_lut2d* lut, double y, double x);
+/* Projections of core/Projections.cpp by EPSG code: 0 ok, 1 projection
+ * failure, -1 unknown code. */
+int orc_proj_forward(int epsg, const double* llh, double* xyz);
+int orc_proj_inverse(int epsg, const double* xyz, double* llh);
+/* 2-D interpolators of core/{Bilinear,Bicubic,NearestNeighbor,Spline2d}Interpolator.cpp at (x, y)
+ * on a float (f32) or double (f64) raster of `length` x `width`. */
+float orc_interp2d_f32(int method, double x, double y, const float* z, int64_t width,
+                       int64_t length);
+double orc_interp2d_f64(int method, double x, double y, const double* z, int64_t width,
+                        int64_t length);
 double orc_bistatic_delay(const double* p, const double* v, const double* x);
 double orc_dry_tropo_tsx(const double* p, const double* llh);
 /* Brent on the reference's own test functions (ids 1..5). */

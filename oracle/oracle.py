@@ -47,8 +47,12 @@ class Grid_t(C.Structure):
                 ("pad_", C.c_int32)]
 
 
+DEM_METHODS = {"bilinear": 1, "bicubic": 2, "nearest": 3, "biquintic": 4}
+
+
 class Dem_t(C.Structure):
-    _fields_ = [("have_raster", C.c_int32), ("epsg", C.c_int32), ("ref_height", C.c_double),
+    _fields_ = [("have_raster", C.c_int32), ("epsg", C.c_int32), ("method", C.c_int32),
+                ("pad_", C.c_int32), ("ref_height", C.c_double),
                 ("xstart", C.c_double), ("ystart", C.c_double), ("dx", C.c_double),
                 ("dy", C.c_double), ("width", C.c_int64), ("length", C.c_int64),
                 ("data", C.c_void_p)]
@@ -106,6 +110,16 @@ def lib():
         h.orc_xyz_to_lonlat.argtypes = [C.c_double, C.c_double, _D, _D]
         h.orc_dem_interpolate_lonlat.restype = C.c_double
         h.orc_dem_interpolate_lonlat.argtypes = [C.POINTER(Dem_t), C.c_double, C.c_double]
+        h.orc_proj_forward.restype = C.c_int
+        h.orc_proj_forward.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        h.orc_proj_inverse.restype = C.c_int
+        h.orc_proj_inverse.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        h.orc_interp2d_f32.restype = C.c_float
+        h.orc_interp2d_f32.argtypes = [C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_int64,
+                                       C.c_int64]
+        h.orc_interp2d_f64.restype = C.c_double
+        h.orc_interp2d_f64.argtypes = [C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_int64,
+                                       C.c_int64]
         h.orc_dem_interpolate_xy.restype = C.c_double
         h.orc_dem_interpolate_xy.argtypes = [C.POINTER(Dem_t), C.c_double, C.c_double]
         h.orc_lut2d_eval.restype = C.c_double
@@ -186,11 +200,12 @@ def grid_struct(geometry) -> Grid_t:
 
 
 def dem_struct(dem) -> Dem_t:
+    method = DEM_METHODS[str(getattr(dem, "interp_method", "bilinear")).lower()]
     if not dem.have_raster:
-        return Dem_t(0, dem.epsg_code, dem.ref_height, 0., 0., 1., 1., 0, 0, None)
+        return Dem_t(0, dem.epsg_code, method, 0, dem.ref_height, 0., 0., 1., 1., 0, 0, None)
     data = np.ascontiguousarray(dem.data, dtype=np.float32)
-    s = Dem_t(1, dem.epsg_code, dem.ref_height, dem.x_start, dem.y_start, dem.delta_x,
-              dem.delta_y, data.shape[1], data.shape[0], data.ctypes.data)
+    s = Dem_t(1, dem.epsg_code, method, 0, dem.ref_height, dem.x_start, dem.y_start,
+              dem.delta_x, dem.delta_y, data.shape[1], data.shape[0], data.ctypes.data)
     s._keep = data
     return s
 

@@ -258,3 +258,45 @@ def test_lines_narrower_than_the_kernel(torch_gpu, precision):
     assert np.abs(ref).max() > 0
     tol = 1e-5 if precision == "plain" else 1e-3
     assert complex_rel_error(z, ref) <= tol
+
+
+def _dem_variant(sc, method, epsg):
+    """The scene's DEM samples re-posted in projection `epsg` (30 m postings
+    for projected codes) around the same centre, interpolated by `method`."""
+    import math
+    from isce3_amd.core import make_projection
+    from isce3_amd.geometry import DEMInterpolator
+    d = sc.dem
+    lon_c = math.radians(d.x_start + 0.5 * d.width * d.delta_x)
+    lat_c = math.radians(d.y_start + 0.5 * d.length * d.delta_y)
+    xc, yc, _ = make_projection(epsg).forward([lon_c, lat_c, 0.0])
+    dx, dy = (d.delta_x, d.delta_y) if epsg == 4326 else (30.0, -30.0)
+    out = DEMInterpolator(d.ref_height, method=method)
+    out.load_dem_array(d.data, xc - 0.5 * d.width * dx, yc - 0.5 * d.length * dy, dx, dy, epsg)
+    return out
+
+
+@pytest.mark.parametrize("method,epsg", [("biquintic", 4326), ("bicubic", 4326),
+                                         ("nearest", 4326), ("biquintic", 32631),
+                                         ("bilinear", 32631), ("bicubic", 6933),
+                                         ("biquintic", 4978)])
+@pytest.mark.parametrize("precision", ["plain", "split"])
+def test_dem_methods_and_projections(torch_gpu, method, epsg, precision):
+    """DEM rasters in UTM / EASE-Grid / geocentric coordinates and the
+    nearest / bicubic / biquintic interpolators (geometry/DEMInterpolator.cpp,
+    core/Projections.cpp, core/*Interpolator.cpp) against the oracle."""
+    import dataclasses
+    sc, raw = scene_and_raw("tiny")
+    sc2 = dataclasses.replace(sc, dem=_dem_variant(sc, method, epsg))
+    print(f"[dem] method={method} epsg={epsg} precision={precision}", flush=True)
+    z, h, err = gpu(sc2, raw, precision=precision)
+    ref, href, rc = orc(sc2, raw, mode=0 if precision == "plain" else 1)
+    assert err == (rc == 6)
+    if precision == "plain":
+        assert_plain_parity(z, ref)
+    else:
+        assert complex_rel_error(z, ref) <= 1e-3
+        assert rms(phase_errors(z, ref)) <= 1e-3
+    np.testing.assert_allclose(h, href, atol=1e-3)
+    # the DEM is in play: heights differ from the flat reference height
+    assert np.nanstd(h) > 1.0

@@ -293,7 +293,7 @@ def test_dry_tropo_tsx_closed_form():
 # ------------------------------------------------------------ DEM / LUT
 def test_dem_bilinear_and_borders():
     data = np.arange(100, dtype=np.float32).reshape(10, 10) * 3.0
-    dem = oracle.Dem_t(1, 4326, 7.0, 10.0, 20.0, 0.1, 0.1, 10, 10, data.ctypes.data)
+    dem = oracle.Dem_t(1, 4326, 1, 0, 7.0, 10.0, 20.0, 0.1, 0.1, 10, 10, data.ctypes.data)
     f = lambda x, y: oracle.lib().orc_dem_interpolate_xy(C.byref(dem), x, y)  # noqa: E731
     # inside: bilinear of a plane is exact: z = 3*(10*row + col)
     for col, row in ((2.25, 3.5), (5.0, 5.0), (7.75, 2.5)):
