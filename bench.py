@@ -457,6 +457,78 @@ def run_rslc(args, torch, np, dev):
     return res
 
 
+# ------------------------------------------------------- presumming
+def run_presum(args, torch, np, dev):
+    """BLU resampling of 16384 dithered-PRF pulses x 16384 range bins
+    (complex64, resident in HBM; two sub-swaths with a 20-bin blanking gap
+    walking across the swath) onto the uniform grid: host weight solves per
+    (output pulse, mask pattern), then one HIP kernel for all output samples.
+    HBM roofline of the kernel: 8 B read + 8 B written per output sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import presum as po
+    from isce3_amd.core import LUT2d
+    from isce3_amd.presum import resample
+    from isce3_amd.product import RadarGridParameters
+    from isce3_amd.sim import EPOCH, SPEED, straight_orbit
+    nt, nr, prf, gap = 16384, 16384, 1520.0, 20
+    rng = np.random.default_rng(17)
+    t = np.sort((np.arange(nt) - nt / 2) / prf + rng.uniform(-0.15, 0.15, nt) / prf)
+    r0, dr = 850e3, 6.25
+    grid = RadarGridParameters(t[0], 0.2384, prf, r0, dr, "right", nt, nr, EPOCH)
+    g = 200 + (np.arange(nt) * 997) % (nr - 400)
+    sw = np.zeros((2, nt, 2), np.int64)
+    sw[0, :, 1] = g
+    sw[1, :, 0], sw[1, :, 1] = g + gap, nr
+    xs = r0 + dr * np.array([-10.0, nr + 10.0])
+    ys = t[0] + np.array([-1.0, 12.0])
+    lut = LUT2d(xs, ys, 50 + 20.0 * (ys[:, None] - t[0]) + 1e-4 * (xs[None, :] - r0))
+    orbit = straight_orbit()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(19)
+    raw = torch.randn((nt, nr), dtype=torch.complex64, device=dev, generator=gen)
+    stream = torch.cuda.current_stream()
+    for _ in range(max(args.warmup, 1)):
+        out = resample(raw, t, grid, sw, orbit, lut)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        out = resample(raw, t, grid, sw, orbit, lut)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    ms = sum(times) / len(times)
+    fd = lambda tt, r: 50 + 20.0 * (tt - t[0]) + 1e-4 * (r - r0)  # noqa: E731
+    # parity on 16 output pulses x the first 1024 bins against the oracle (complex128)
+    rows = np.arange(nt // 2, nt // 2 + 16)
+    ref = po.resample(raw[:, :1024].cpu().numpy(), t, prf, r0, dr, np.minimum(sw, 1024),
+                      lambda tt: SPEED, fd, rows=rows)
+    got = out[nt // 2:nt // 2 + 16, :1024].cpu().numpy()
+    rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    # CPU baseline: the oracle restatement (numpy, one thread) on 64 output pulses
+    nb = 64
+    t0 = time.perf_counter()
+    po.resample(raw[:nb + 16].cpu().numpy(), t, prf, r0, dr, sw, lambda tt: SPEED, fd,
+                rows=np.arange(nb))
+    cpu_s = time.perf_counter() - t0
+    nbytes = nt * nr * 16
+    res = {"workload": f"{nt} dithered pulses x {nr} bins c64, 2 sub-swaths, {gap}-bin walking "
+                       "gap, L = 12 m, Doppler plane",
+           "ms_per_call": ms, "samples_per_s": nt * nr / (ms * 1e-3),
+           "kernel": "presum::resample_kernel (host weights included in ms_per_call)",
+           "roofline_call": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9,
+                             "peak": 8000.0, "unit": "GB/s",
+                             "frac": nbytes / (ms * 1e-3) / 8e12, "traffic": None},
+           "rel_error_vs_oracle": rel,
+           "cpu_baseline": {"value": nb * nr / cpu_s, "unit": "output samples/s",
+                            "cores": 1, "kind": "port",
+                            "sample": f"{nb} output pulses x {nr} bins, numpy oracle"}}
+    log(f"presum: {ms:.2f} ms per {nt}x{nr} resample, rel err {rel:.2e}, "
+        f"CPU {nb * nr / cpu_s:.3g} samples/s")
+    del raw, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def split_kernel_name(opts) -> str:
     """The coherent-sum kernel the split step ran with these launch options
     (kernel width 8)."""
@@ -490,6 +562,9 @@ def main():
     ap.add_argument("--only-rangecomp", action="store_true",
                     help="time the range compressor only (profiling runs)")
     ap.add_argument("--no-rslc", action="store_true")
+    ap.add_argument("--no-presum", action="store_true")
+    ap.add_argument("--only-presum", action="store_true",
+                    help="time the presum resampler only (profiling runs)")
     ap.add_argument("--only-rslc", action="store_true",
                     help="time the RSLC block finisher only (profiling runs)")
     ap.add_argument("--rates", type=lambda s: [float(x) for x in s.split(",")],
@@ -529,6 +604,9 @@ def main():
     if args.only_rslc:
         print(json.dumps({"rslc": run_rslc(args, torch, np, dev)}))
         return
+    if args.only_presum:
+        print(json.dumps({"presum": run_presum(args, torch, np, dev)}))
+        return
 
     out_dir = os.path.join(ROOT, "gpurun_out")
     tuner = autotune.Autotuner(os.path.join(out_dir, "autotune.json")
@@ -552,6 +630,7 @@ def main():
     rc_res = None if args.no_rangecomp else run_rangecomp(args, torch, np, dev)
     torch.cuda.empty_cache()
     rslc_res = None if args.no_rslc else run_rslc(args, torch, np, dev)
+    presum_res = None if args.no_presum else run_presum(args, torch, np, dev)
 
     sp, pl = results["split"], results["plain"]
     achieved = sp["updates"] * FLOPS_PER_UPDATE / (sp["sum_ms"] * 1e-3) / 1e12
@@ -591,6 +670,7 @@ def main():
         "serve": serve_res,
         "rangecomp": rc_res,
         "rslc": rslc_res,
+        "presum": presum_res,
         "cpu_baseline": cpu,
     }
     if cpu:

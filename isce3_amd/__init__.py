@@ -13,8 +13,9 @@ The look-alike host types mirror isce3's module layout:
     isce3_amd.rslc       BackgroundWriter, StatsRealImagFloat32, to_complex32,
                          truncate_mantissa (block finishing on the GPU)
     isce3_amd.planning   plan_processing_blocks and the workflow's parameter helpers
+    isce3_amd.presum     BLU presumming of dithered-PRF pulses (resample, weights)
 """
-from . import (container, core, errors, focus, geometry, planning, product,  # noqa: F401
-               rangecomp, rslc)
+from . import (container, core, errors, focus, geometry, planning, presum,  # noqa: F401
+               product, rangecomp, rslc)
 
 __version__ = "0.1.0"

@@ -53,6 +53,10 @@ class RadarGeometry_t(C.Structure):
     _fields_ = [("grid", Grid_t), ("orbit", Orbit_t), ("doppler", Lut2d_t)]
 
 
+class Acorr_t(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad_", C.c_int32), ("width", C.c_double)]
+
+
 class StatsF32_t(C.Structure):
     _fields_ = [("min", C.c_float), ("max", C.c_float), ("mean", C.c_float),
                 ("pad_", C.c_int32), ("n_valid", C.c_int64), ("real_valued_mean", C.c_double),
@@ -134,6 +138,15 @@ _SIGNATURES = {
     "bp_rslc_finish_block": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
                                        C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                        _P(StatsRealImagF32_t), C.c_void_p]),
+    "bp_acorr_eval": (C.c_double, [_P(Acorr_t), C.c_double]),
+    "bp_presum_weights": (C.c_int, [_P(Acorr_t), C.c_void_p, C.c_int64, C.c_double,
+                                    _P(C.c_int64), C.c_void_p, C.c_int64, _P(C.c_int64)]),
+    "bp_presum_apply_weights": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),
+    "bp_presum_compute_ids": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]),
+    "bp_presum_resample": (C.c_int, [C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                     _P(Grid_t), _P(Orbit_t), _P(Lut2d_t), C.c_double,
+                                     C.c_void_p, C.c_void_p, C.c_void_p]),
     "bp_orbit_interpolate": (C.c_int, [_P(Orbit_t), C.c_double, C.c_int, _P(C.c_double),
                                        _P(C.c_double)]),
     "bp_lonlat_to_xyz": (None, [C.c_double, C.c_double, _P(C.c_double), _P(C.c_double)]),

@@ -475,6 +475,28 @@ class KnabKernel(Kernel):
         return _native.lib().bp_knab_eval(self.width, self._bandwidth, float(x))
 
 
+class BartlettKernel(Kernel):
+    """Triangle of the given width (core/Kernels.h:43-57, Kernels.icc:16-23)."""
+
+    _kind = 1
+
+    def _struct(self) -> _native.Acorr_t:
+        return _native.Acorr_t(self._kind, 0, self.width)
+
+    def __call__(self, x: float) -> float:
+        return _native.lib().bp_acorr_eval(C.byref(self._struct()), float(x))
+
+
+class AzimuthKernel(BartlettKernel):
+    """Autocorrelation of the sinc^2 azimuth antenna pattern, a piecewise
+    cubic of width 2 scale (core/Kernels.h:114-129, Kernels.icc:95-107)."""
+
+    _kind = 0
+
+    def __init__(self, scale: float):
+        super().__init__(2.0 * float(scale))
+
+
 class KernelF32(Kernel):
     """Base of single-precision kernels accepted by backproject."""
 

@@ -1,0 +1,432 @@
+// Presumming / BLU resampling of non-uniformly sampled pulses on gfx950
+// (include/isce3_amd_presum.h).
+//
+// Host side: the BLU weights of isce3::focus::getPresumWeights
+// (cxx/isce3/focus/Presum.icc:9-60) -- the samples within half the
+// autocorrelation width of the output time, their autocorrelation matrix and
+// right-hand side, solved by LDL^T with diagonal pivoting -- and, for the
+// resampler, the distinct valid-sample patterns of each output pulse's window
+// (taken from the sub-swath boundaries, which is where the pattern of a range
+// bin can change) with one weight vector per pattern.
+//
+// Device side: one thread per range bin and chunk of output pulses rebuilds
+// the bin's pattern from the sub-swath table, picks its weights, evaluates
+// the Doppler LUT at (tout, r) and accumulates
+//   sum_k float(w_k) * exp(-2 pi i (t_k - tout) fd) * raw[k][r]
+// in complex64 in pulse order, as nisar/workflows/focus.py:1011-1061 and
+// pybind focus/Presum.cpp:26-66 do.  Consecutive output pulses share most of
+// their input pulses, so a thread's chunk re-reads them from cache.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/isce3_amd_backproject.h"
+#include "../../include/isce3_amd_presum.h"
+#include "bp_math.h"
+
+void bp_set_last_error(const std::string& msg);
+
+namespace presum {
+namespace {
+
+int fail(int code, const std::string& msg)
+{
+    bp_set_last_error(msg);
+    return code;
+}
+
+// core/Kernels.icc:16-23 (Bartlett), :95-107 (azimuth autocorrelation)
+double acorr_eval(const bp_acorr& k, double t)
+{
+    const double hw = std::fabs(k.width / 2.0);
+    if (k.kind == BP_ACORR_BARTLETT) {
+        const double u = std::fabs(t / hw);
+        return u > 1.0 ? 0.0 : 1.0 - u;
+    }
+    const double x = std::fabs(t * 2 / hw);
+    if (x > 2.0) return 0.0;
+    if (x < 1.0) return x * x * (0.75 * x - 1.5) + 1.0;
+    return x * (x * (-0.25 * x + 1.5) - 3.0) + 2.0;
+}
+
+// Solve G w = r for symmetric positive semi-definite G (lower triangle
+// used) by LDL^T with diagonal pivoting: at step k the largest remaining
+// diagonal entry is swapped to k, column k of L is formed from the earlier
+// columns (left-looking), and pivots not above the smallest normal number
+// are treated as zero (their solution component is zero).
+void ldlt_solve(std::vector<double>& g, int n, std::vector<double>& r, std::vector<double>& w)
+{
+    auto G = [&](int i, int j) -> double& { return g[(size_t)i * n + j]; };
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    // make the matrix full-symmetric for row/column swaps
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) G(i, j) = G(j, i);
+    std::vector<double> d(n, 0.0), tmp(n);
+    const double cutoff = std::numeric_limits<double>::min();
+    for (int k = 0; k < n; ++k) {
+        // pivot: largest |diagonal| among k..n-1 (first on ties)
+        int piv = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(G(i, i)) > std::fabs(G(piv, piv))) piv = i;
+        if (piv != k) {
+            for (int j = 0; j < n; ++j) std::swap(G(k, j), G(piv, j));
+            for (int i = 0; i < n; ++i) std::swap(G(i, k), G(i, piv));
+            std::swap(perm[k], perm[piv]);
+        }
+        // D_k = G_kk - sum_j L_kj^2 D_j ; L_ik = (G_ik - sum_j L_ij L_kj D_j) / D_k
+        for (int j = 0; j < k; ++j) tmp[j] = G(k, j) * d[j];
+        double dk = G(k, k);
+        for (int j = 0; j < k; ++j) dk -= G(k, j) * tmp[j];
+        d[k] = dk;
+        for (int i = k + 1; i < n; ++i) {
+            double s = G(i, k);
+            for (int j = 0; j < k; ++j) s -= G(i, j) * tmp[j];
+            G(i, k) = std::fabs(dk) > cutoff ? s / dk : s;
+        }
+    }
+    // P b, L y = P b, D z = y, L^T x = z, w = P^T x
+    std::vector<double> y(n);
+    for (int i = 0; i < n; ++i) y[i] = r[perm[i]];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) y[i] -= G(i, j) * y[j];
+    for (int i = 0; i < n; ++i) y[i] = std::fabs(d[i]) > cutoff ? y[i] / d[i] : 0.0;
+    for (int i = n - 1; i >= 0; --i)
+        for (int j = i + 1; j < n; ++j) y[i] -= G(j, i) * y[j];
+    w.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) w[perm[i]] = y[i];
+}
+
+// getPresumWeights: window [offset, offset + nw) and its weights
+void presum_weights(const bp_acorr& acorr, const double* t, int64_t n, double tout,
+                    int64_t* offset, std::vector<double>& w)
+{
+    const double hw = 0.5 * acorr.width;
+    const double* first = std::lower_bound(t, t + n, tout - hw);
+    const double* last = std::upper_bound(first, t + n, tout + hw);
+    *offset = first - t;
+    const int m = (int)(last - first);
+    std::vector<double> g((size_t)m * m, 0.0), r(m);
+    for (int i = 0; i < m; ++i) {
+        const double xi = first[i];
+        for (int j = i; j < m; ++j) g[(size_t)j * m + i] = acorr_eval(acorr, first[j] - xi);
+        r[i] = acorr_eval(acorr, tout - xi);
+    }
+    ldlt_solve(g, m, r, w);
+}
+
+// Per output pulse: window, distinct patterns (bit k: range bin valid in
+// window pulse k) and one weight vector per pattern.
+struct PulseDesc {
+    double tout;
+    int64_t offset;
+    int32_t nw, pbase, npat, pad_;
+};
+
+struct ResampleArgs {
+    const float2* raw;
+    float2* out;
+    const double* t;           // [nt]
+    const int32_t* swaths;     // [ns][nt][2]
+    const PulseDesc* desc;     // [length]
+    const int64_t* pat_id;     // [total patterns]
+    const float* pat_w;        // [total patterns][kMaxWindow]
+    bp::Lut2d doppler;
+    double r0, dr;
+    int64_t nt, ns, length, width;
+    int32_t pulses_per_thread, pad_;
+};
+
+constexpr int kMaxWindow = 63;
+constexpr int kThreads = 256;
+
+// complex64 product and sum rounded term by term (no fused multiply-add)
+__device__ inline float2 cmul(float2 a, float2 b)
+{
+#pragma clang fp contract(off)
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ inline float2 cadd(float2 a, float2 b)
+{
+#pragma clang fp contract(off)
+    return make_float2(a.x + b.x, a.y + b.y);
+}
+
+__global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
+{
+    const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (j >= a.width) return;
+    const double r = a.r0 + a.dr * (double)j;
+    const int64_t i0 = (int64_t)blockIdx.y * a.pulses_per_thread;
+    const int64_t i1 = min(i0 + (int64_t)a.pulses_per_thread, a.length);
+    for (int64_t i = i0; i < i1; ++i) {
+        const PulseDesc d = a.desc[i];
+        // pattern of this range bin over the window
+        int64_t id = 0;
+        for (int k = 0; k < d.nw; ++k) {
+            const int64_t p = d.offset + k;
+            bool valid = false;
+            for (int64_t s = 0; s < a.ns; ++s) {
+                const int32_t* sw = a.swaths + (s * a.nt + p) * 2;
+                valid |= (j >= sw[0]) & (j < sw[1]);
+            }
+            if (valid) id |= (int64_t)1 << k;
+        }
+        const float* w = nullptr;
+        for (int q = 0; q < d.npat; ++q)
+            if (a.pat_id[d.pbase + q] == id) w = a.pat_w + (int64_t)(d.pbase + q) * kMaxWindow;
+        const double fd = bp::lut2d_eval(a.doppler, d.tout, r);
+        float2 sum = make_float2(0.f, 0.f);
+        if (w != nullptr) {
+            for (int k = 0; k < d.nw; ++k) {
+                const int64_t p = d.offset + k;
+                const double trel = a.t[p] - d.tout;
+                const float ph = (float)(-2 * M_PI * trel * fd);
+                float s, c;
+                sincosf(ph, &s, &c);
+                const float wk = w[k];
+                sum = cadd(sum, cmul(make_float2(wk * c, wk * s), a.raw[p * a.width + j]));
+            }
+        }
+        a.out[i * a.width + j] = sum;
+    }
+}
+
+struct ApplyArgs {
+    float2* out;
+    const double* t;
+    const double* fd;
+    const double* w;
+    const float2* x;
+    int64_t m, n;
+};
+
+__global__ __launch_bounds__(kThreads) void apply_kernel(ApplyArgs a)
+{
+    const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (j >= a.n) return;
+    const double fd = a.fd[j];
+    float2 sum = make_float2(0.f, 0.f);
+    for (int64_t i = 0; i < a.m; ++i) {
+        const float ph = (float)(-2 * M_PI * a.t[i] * fd);
+        float s, c;
+        sincosf(ph, &s, &c);
+        const float wk = (float)a.w[i * a.n + j];
+        sum = cadd(sum, cmul(make_float2(wk * c, wk * s), a.x[i * a.n + j]));
+    }
+    a.out[j] = sum;
+}
+
+struct DevBuf {
+    void* ptr = nullptr;
+    ~DevBuf()
+    {
+        if (ptr) (void)hipFree(ptr);
+    }
+    hipError_t put(const void* src, size_t bytes, hipStream_t s)
+    {
+        if (bytes == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&ptr, bytes);
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(ptr, src, bytes, hipMemcpyHostToDevice, s);
+    }
+};
+
+}  // namespace
+}  // namespace presum
+
+extern "C" {
+
+double bp_acorr_eval(const bp_acorr* acorr, double x)
+{
+    return acorr ? presum::acorr_eval(*acorr, x) : 0.0;
+}
+
+int bp_presum_weights(const bp_acorr* acorr, const double* t, int64_t n, double tout,
+                      int64_t* offset, double* w, int64_t max_w, int64_t* nw)
+{
+    using presum::fail;
+    if (acorr == nullptr || offset == nullptr || nw == nullptr || (n > 0 && t == nullptr))
+        return fail(BP_ERR_INVALID_ARGUMENT, "no storage provided for index offset");
+    if (acorr->kind != BP_ACORR_AZIMUTH && acorr->kind != BP_ACORR_BARTLETT)
+        return fail(BP_ERR_INVALID_ARGUMENT, "unknown autocorrelation kernel");
+    std::vector<double> ww;
+    presum::presum_weights(*acorr, t, n, tout, offset, ww);
+    *nw = (int64_t)ww.size();
+    if (*nw > max_w) return fail(BP_ERR_LENGTH, "weight buffer too short");
+    if (*nw > 0) std::memcpy(w, ww.data(), sizeof(double) * ww.size());
+    return BP_SUCCESS;
+}
+
+int bp_presum_apply_weights(void* out, const double* pulse_times, const double* doppler,
+                            const double* weights, const void* pulses, int64_t m, int64_t n,
+                            void* stream)
+{
+    using presum::fail;
+    if (m < 0 || n < 0) return fail(BP_ERR_LENGTH, "negative dimensions");
+    if (n == 0) return BP_SUCCESS;
+    if (!out || !doppler || (m > 0 && (!pulse_times || !weights || !pulses)))
+        return fail(BP_ERR_INVALID_ARGUMENT, "null buffer");
+    presum::ApplyArgs a{static_cast<float2*>(out), pulse_times, doppler, weights,
+                        static_cast<const float2*>(pulses), m, n};
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(presum::apply_kernel, dim3((unsigned)((n + presum::kThreads - 1) / presum::kThreads)),
+                       dim3(presum::kThreads), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && s == nullptr) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("apply_presum_weights: ") + hipGetErrorString(e));
+    return BP_SUCCESS;
+}
+
+int bp_presum_compute_ids(const uint8_t* mask, int64_t nr, int64_t np, int64_t* ids)
+{
+    using presum::fail;
+    if (np > 63)
+        return fail(BP_ERR_LENGTH, "Hash algorithm requires number of pulses less than number "
+                                   "of bits in identifier (64)");
+    if (nr > 0 && (mask == nullptr || ids == nullptr)) return fail(BP_ERR_INVALID_ARGUMENT, "null buffer");
+    for (int64_t i = 0; i < nr; ++i) {
+        int64_t id = 0;
+        for (int64_t j = 0; j < np; ++j)
+            if (mask[i * np + j]) id |= (int64_t)1 << j;
+        ids[i] = id;
+    }
+    return BP_SUCCESS;
+}
+
+int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* swaths,
+                       int64_t ns, const bp_grid* grid, const bp_orbit* orbit,
+                       const bp_lut2d* doppler, double antenna_length, const void* raw,
+                       void* out, void* stream)
+{
+    using namespace presum;
+    if (!grid || !orbit || !doppler || !t || (ns > 0 && !swaths) || nt < 1)
+        return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    if (grid->length < 0 || grid->width < 0) return fail(BP_ERR_INVALID_ARGUMENT, "negative grid size");
+    if (grid->length == 0 || grid->width == 0) return BP_SUCCESS;
+    if (!raw || !out) return fail(BP_ERR_INVALID_ARGUMENT, "null buffer");
+    if (!(grid->prf > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "PRF must be positive");
+    if (!(antenna_length > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "antenna length must be positive");
+    if (orbit->n < 2 || !orbit->pos || !orbit->vel || !(orbit->dt > 0))
+        return fail(BP_ERR_INVALID_ARGUMENT, "orbit needs at least two state vectors");
+    if (doppler->have_data && (doppler->width < 2 || doppler->length < 2 || !doppler->data))
+        return fail(BP_ERR_INVALID_ARGUMENT, "Doppler LUT2d needs at least 2x2 samples");
+    for (int64_t i = 1; i < nt; ++i)
+        if (!(t[i] >= t[i - 1])) return fail(BP_ERR_INVALID_ARGUMENT, "pulse times must be sorted");
+
+    bp::Orbit ho{orbit->t0, orbit->dt, orbit->n, orbit->method, orbit->pos, orbit->vel};
+    const int64_t L = grid->length, W = grid->width;
+    std::vector<PulseDesc> desc(L);
+    std::vector<int64_t> pat_id;
+    std::vector<float> pat_w;
+    std::vector<double> w, sub_t, wsub;
+    std::vector<int64_t> cuts;
+    for (int64_t i = 0; i < L; ++i) {
+        const double tout = t[0] + (double)i / grid->prf;
+        bp::V3 pos, vel;
+        const int st = bp::orbit_interpolate(ho, tout, bp::kBorderError, &pos, &vel);
+        if (st != bp::kSuccess)
+            return fail(BP_ERR_OUT_OF_RANGE, "output pulse time outside the orbit");
+        const double v = bp::norm(vel);
+        const bp_acorr acor{BP_ACORR_AZIMUTH, 0, 2.0 * (antenna_length / v)};
+        int64_t offset = 0;
+        presum_weights(acor, t, nt, tout, &offset, w);
+        const int nw = (int)w.size();
+        if (nw > kMaxWindow)
+            return fail(BP_ERR_LENGTH, "Hash algorithm requires number of pulses less than "
+                                       "number of bits in identifier (64)");
+        PulseDesc& d = desc[i];
+        d.tout = tout;
+        d.offset = offset;
+        d.nw = nw;
+        d.pbase = (int32_t)pat_id.size();
+        d.npat = 0;
+        d.pad_ = 0;
+        // range bins where the pattern can change: every sub-swath boundary
+        cuts.assign(1, 0);
+        for (int k = 0; k < nw; ++k)
+            for (int64_t s = 0; s < ns; ++s) {
+                const int32_t* sw = swaths + (s * nt + offset + k) * 2;
+                for (int e = 0; e < 2; ++e)
+                    if (sw[e] > 0 && sw[e] < W) cuts.push_back(sw[e]);
+            }
+        std::sort(cuts.begin(), cuts.end());
+        cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+        for (int64_t c : cuts) {
+            int64_t id = 0;
+            for (int k = 0; k < nw; ++k)
+                for (int64_t s = 0; s < ns; ++s) {
+                    const int32_t* sw = swaths + (s * nt + offset + k) * 2;
+                    if (c >= sw[0] && c < sw[1]) {
+                        id |= (int64_t)1 << k;
+                        break;
+                    }
+                }
+            bool seen = false;
+            for (int q = 0; q < d.npat; ++q) seen |= pat_id[d.pbase + q] == id;
+            if (seen) continue;
+            // weights from the valid window pulses only, zeros elsewhere
+            sub_t.clear();
+            for (int k = 0; k < nw; ++k)
+                if (id >> k & 1) sub_t.push_back(t[offset + k]);
+            int64_t joff = 0;
+            presum_weights(acor, sub_t.data(), (int64_t)sub_t.size(), tout, &joff, wsub);
+            if (joff != 0 || wsub.size() != sub_t.size())
+                return fail(BP_ERR_RUNTIME, "presum weights of a mask pattern left its window");
+            pat_id.push_back(id);
+            const size_t base = pat_w.size();
+            pat_w.resize(base + kMaxWindow, 0.f);
+            for (int k = 0, u = 0; k < nw; ++k)
+                if (id >> k & 1) pat_w[base + k] = (float)wsub[u++];
+            ++d.npat;
+        }
+    }
+
+    if (hipSetDevice(device) != hipSuccess) return fail(BP_ERR_HIP, "hipSetDevice failed");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevBuf d_t, d_sw, d_desc, d_pid, d_pw, d_lut;
+    hipError_t e = d_t.put(t, sizeof(double) * nt, s);
+    if (e == hipSuccess) e = d_sw.put(swaths, sizeof(int32_t) * 2 * ns * nt, s);
+    if (e == hipSuccess) e = d_desc.put(desc.data(), sizeof(PulseDesc) * desc.size(), s);
+    if (e == hipSuccess) e = d_pid.put(pat_id.data(), sizeof(int64_t) * pat_id.size(), s);
+    if (e == hipSuccess) e = d_pw.put(pat_w.data(), sizeof(float) * pat_w.size(), s);
+    if (e == hipSuccess && doppler->have_data)
+        e = d_lut.put(doppler->data, sizeof(double) * doppler->width * doppler->length, s);
+    if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
+
+    ResampleArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.raw = static_cast<const float2*>(raw);
+    a.out = static_cast<float2*>(out);
+    a.t = static_cast<const double*>(d_t.ptr);
+    a.swaths = static_cast<const int32_t*>(d_sw.ptr);
+    a.desc = static_cast<const PulseDesc*>(d_desc.ptr);
+    a.pat_id = static_cast<const int64_t*>(d_pid.ptr);
+    a.pat_w = static_cast<const float*>(d_pw.ptr);
+    a.doppler = bp::Lut2d{doppler->have_data, doppler->bounds_error, doppler->ref_value,
+                          doppler->xstart, doppler->ystart, doppler->dx, doppler->dy,
+                          doppler->width, doppler->length,
+                          static_cast<const double*>(d_lut.ptr)};
+    a.r0 = grid->starting_range;
+    a.dr = grid->range_pixel_spacing;
+    a.nt = nt;
+    a.ns = ns;
+    a.length = L;
+    a.width = W;
+    a.pulses_per_thread = 16;
+    const dim3 blocks((unsigned)((W + kThreads - 1) / kThreads),
+                      (unsigned)((L + a.pulses_per_thread - 1) / a.pulses_per_thread));
+    hipLaunchKernelGGL(resample_kernel, blocks, dim3(kThreads), 0, s, a);
+    e = hipGetLastError();
+    // the temporaries are freed on return: wait for the kernel first
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("presum resample: ") + hipGetErrorString(e));
+    return BP_SUCCESS;
+}
+
+}  // extern "C"

@@ -522,3 +522,5 @@ class Scene:
 # isce3.focus.form_linear_chirp)
 from .rangecomp import RangeComp, form_linear_chirp  # noqa: E402,F401
 from .planning import get_radar_velocities, get_sar_duration  # noqa: E402,F401
+from .presum import (apply_presum_weights, compute_ids_from_mask, fill_weights,  # noqa: E402,F401
+                     get_presum_weights, get_unique_ids)

@@ -1,0 +1,118 @@
+"""Presumming on the GPU (include/isce3_amd_presum.h through
+isce3_amd.presum) against the oracle (oracle/presum.py) and the reference's
+own test of apply_presum_weights (pybind focus/presum.py:33-49).
+
+Bar: relative error <= 1e-5 against the complex128 oracle (float32 weights,
+deramp phasors and accumulation as in the reference)."""
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+import presum as po
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - b) / np.linalg.norm(b))
+
+
+def test_apply_weights_reference_case(torch_gpu):
+    """presum.py:33-49 with its prime sizes and tolerances."""
+    from isce3_amd.focus import apply_presum_weights
+    t = np.array([-2.2, -1.1, -0.1, 0.9, 1.9]) / 1910.
+    fd = np.linspace(800, 900, 49999)
+    np.random.seed(12345)
+    N = lambda: np.random.normal(size=(len(t), len(fd)))  # noqa: E731
+    weights = N()
+    raw = (N() + 1j * N()).astype("c8")
+    desired = po.apply_presum_weights(t, fd, weights, raw)
+    out = np.zeros(len(fd), dtype="c8")
+    apply_presum_weights(out, t, fd, weights, raw)
+    print(f"[presum apply] rel={rel(out, desired):.2e}", flush=True)
+    npt.assert_allclose(out, desired, rtol=1e-5, atol=1e-5)
+
+
+def test_apply_weights_device_tensors_and_shape_rules(torch_gpu):
+    torch = torch_gpu
+    from isce3_amd import errors
+    from isce3_amd.focus import apply_presum_weights
+    rng = np.random.default_rng(3)
+    t = rng.uniform(-2e-3, 2e-3, 7)
+    fd = rng.uniform(-500, 500, 1001)
+    w = rng.normal(size=(7, 1001))
+    x = (rng.normal(size=(7, 1001)) + 1j * rng.normal(size=(7, 1001))).astype("c8")
+    out = torch.zeros(1001, dtype=torch.complex64, device="cuda:0")
+    apply_presum_weights(out, torch.from_numpy(t).cuda(), torch.from_numpy(fd).cuda(),
+                         torch.from_numpy(w).cuda(), torch.from_numpy(x).cuda())
+    assert rel(out.cpu().numpy(), po.apply_presum_weights(t, fd, w, x)) < 1e-5
+    with pytest.raises(errors.LengthError):
+        apply_presum_weights(np.zeros(1000, "c8"), t, fd, w, x)
+    with pytest.raises(errors.LengthError):
+        apply_presum_weights(np.zeros(1001, "c8"), t, fd, w[:6], x)
+    with pytest.raises(errors.LengthError):
+        apply_presum_weights(np.zeros(1001, "c8"), t[:, None], fd, w, x)
+
+
+def _dithered_scene(nt=96, nr=300, dither=0.15, gap=20, seed=0, ns=2, walk=True):
+    from isce3_amd.core import LUT2d
+    from isce3_amd.product import RadarGridParameters
+    from isce3_amd.sim import EPOCH, SPEED, straight_orbit
+    rng = np.random.default_rng(seed)
+    prf = 1520.0
+    t = (np.arange(nt) - nt / 2) / prf + rng.uniform(-dither, dither, nt) / prf
+    t.sort()
+    r0, dr = 850e3, 6.25
+    grid = RadarGridParameters(t[0], 0.2384, prf, r0, dr, "right", nt, nr, EPOCH)
+    # sub-swaths: a blanking gap of `gap` bins that walks across the swath
+    # (or stays put, leaving bins with no data at all)
+    g = 40 + (np.arange(nt) * 37) % (nr - 80) if walk else np.full(nt, 120)
+    sw = np.zeros((ns, nt, 2), np.int64)
+    if ns == 1:
+        sw[0, :, 1] = nr
+    else:
+        sw[0, :, 0], sw[0, :, 1] = 0, g
+        sw[1, :, 0], sw[1, :, 1] = g + gap, nr
+        sw[2:, :, :] = sw[1:2, :, :]
+    raw = (rng.normal(size=(nt, nr)) + 1j * rng.normal(size=(nt, nr))).astype("c8")
+    # Doppler plane fd = 50 + 2e4 (t - t0) + 1e-3 (r - r0): bilinear LUTs hold it exactly
+    xs = r0 + dr * np.array([-10.0, nr + 10.0])
+    ys = t[0] + np.array([-0.1, 0.2])
+    data = 50 + 2e4 * (ys[:, None] - t[0]) + 1e-3 * (xs[None, :] - r0)
+    lut = LUT2d(xs, ys, data)
+    fd = lambda tout, r: 50 + 2e4 * (tout - t[0]) + 1e-3 * (r - r0)  # noqa: E731
+    return grid, straight_orbit(), lut, t, sw, raw, fd, SPEED
+
+
+@pytest.mark.parametrize("dither,gap,ns,walk", [(0.15, 20, 2, True), (0.3, 45, 2, True),
+                                                (0.0, 0, 1, True), (0.2, 10, 3, True),
+                                                (0.25, 30, 2, False)])
+def test_resample_matches_oracle(torch_gpu, dither, gap, ns, walk):
+    from isce3_amd.presum import resample
+    grid, orbit, lut, t, sw, raw, fd, speed = _dithered_scene(dither=dither, gap=gap, ns=ns,
+                                                              seed=int(dither * 100) + gap,
+                                                              walk=walk)
+    out = resample(raw, t, grid, sw, orbit, lut, L=12.0)
+    ref = po.resample(raw, t, grid.prf, grid.starting_range, grid.range_pixel_spacing, sw,
+                      lambda tout: speed, fd, L=12.0)
+    e = rel(out, ref)
+    print(f"[presum resample] dither={dither} gap={gap} ns={ns} rel={e:.2e}", flush=True)
+    assert out.shape == ref.shape and out.dtype == np.complex64
+    assert e <= 1e-5
+    # bins outside every sub-swath of the whole window stay zero in both
+    assert np.array_equal(out == 0, ref == 0)
+    if not walk:
+        assert np.all(ref[:, 120:120 + gap] == 0)
+
+
+def test_resample_uniform_grid_is_identity(torch_gpu):
+    """With pulses already on the uniform grid (no dither, no gaps) every
+    output time is an input time, whose BLU weights are the unit vector, and
+    the deramp phase there is zero: the output reproduces the input."""
+    torch = torch_gpu
+    from isce3_amd.presum import resample
+    grid, orbit, lut, t, sw, raw, fd, speed = _dithered_scene(dither=0.0, gap=0, ns=1)
+    t = t[0] + np.arange(len(t)) / grid.prf
+    out = resample(torch.from_numpy(raw).cuda(), t, grid, sw, orbit, lut)
+    assert out.is_cuda
+    npt.assert_allclose(out.cpu().numpy(), raw, rtol=0, atol=2e-5)
