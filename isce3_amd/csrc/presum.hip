@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/isce3_amd_backproject.h"
@@ -124,7 +125,7 @@ void presum_weights(const bp_acorr& acorr, const double* t, int64_t n, double to
 struct PulseDesc {
     double tout;
     int64_t offset;
-    int32_t nw, pbase, npat, pad_;
+    int32_t nw, pbase, npat, wbase;  // weights of pattern q: pat_w[wbase + q nw + k]
 };
 
 struct ResampleArgs {
@@ -134,7 +135,7 @@ struct ResampleArgs {
     const int32_t* swaths;     // [ns][nt][2]
     const PulseDesc* desc;     // [length]
     const int64_t* pat_id;     // [total patterns]
-    const float* pat_w;        // [total patterns][kMaxWindow]
+    const float* pat_w;        // per pulse npat x nw
     bp::Lut2d doppler;
     double r0, dr;
     int64_t nt, ns, length, width;
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
         }
         const float* w = nullptr;
         for (int q = 0; q < d.npat; ++q)
-            if (a.pat_id[d.pbase + q] == id) w = a.pat_w + (int64_t)(d.pbase + q) * kMaxWindow;
+            if (a.pat_id[d.pbase + q] == id) w = a.pat_w + d.wbase + (int64_t)q * d.nw;
         const double fd = bp::lut2d_eval(a.doppler, d.tout, r);
         float2 sum = make_float2(0.f, 0.f);
         if (w != nullptr) {
@@ -220,6 +221,80 @@ __global__ __launch_bounds__(kThreads) void apply_kernel(ApplyArgs a)
         sum = cadd(sum, cmul(make_float2(wk * c, wk * s), a.x[i * a.n + j]));
     }
     a.out[j] = sum;
+}
+
+// Pattern tables of a chunk of output pulses.
+struct Patterns {
+    std::vector<int64_t> id;
+    std::vector<float> w;  // nw per pattern
+    int status = BP_SUCCESS;
+};
+
+// Window, distinct patterns and their weights for output pulses [i0, i1);
+// desc[i].pbase and .wbase are relative to the chunk's tables.
+int plan_pulses(const bp::Orbit& ho, const double* t, int64_t nt, const int32_t* swaths, int64_t ns,
+                int64_t W, double prf, double antenna_length, int64_t i0, int64_t i1,
+                PulseDesc* desc, Patterns& out)
+{
+    std::vector<double> w, sub_t, wsub;
+    std::vector<int64_t> cuts;
+    for (int64_t i = i0; i < i1; ++i) {
+        const double tout = t[0] + (double)i / prf;
+        bp::V3 pos, vel;
+        if (bp::orbit_interpolate(ho, tout, bp::kBorderError, &pos, &vel) != bp::kSuccess)
+            return BP_ERR_OUT_OF_RANGE;
+        const double v = bp::norm(vel);
+        const bp_acorr acor{BP_ACORR_AZIMUTH, 0, 2.0 * (antenna_length / v)};
+        int64_t offset = 0;
+        presum_weights(acor, t, nt, tout, &offset, w);
+        const int nw = (int)w.size();
+        if (nw > kMaxWindow) return BP_ERR_LENGTH;
+        PulseDesc& d = desc[i];
+        d.tout = tout;
+        d.offset = offset;
+        d.nw = nw;
+        d.pbase = (int32_t)out.id.size();
+        d.npat = 0;
+        d.wbase = (int32_t)out.w.size();
+        // range bins where the pattern can change: every sub-swath boundary
+        cuts.assign(1, 0);
+        for (int k = 0; k < nw; ++k)
+            for (int64_t s = 0; s < ns; ++s) {
+                const int32_t* sw = swaths + (s * nt + offset + k) * 2;
+                for (int e = 0; e < 2; ++e)
+                    if (sw[e] > 0 && sw[e] < W) cuts.push_back(sw[e]);
+            }
+        std::sort(cuts.begin(), cuts.end());
+        cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+        for (int64_t c : cuts) {
+            int64_t id = 0;
+            for (int k = 0; k < nw; ++k)
+                for (int64_t s = 0; s < ns; ++s) {
+                    const int32_t* sw = swaths + (s * nt + offset + k) * 2;
+                    if (c >= sw[0] && c < sw[1]) {
+                        id |= (int64_t)1 << k;
+                        break;
+                    }
+                }
+            bool seen = false;
+            for (int q = 0; q < d.npat; ++q) seen |= out.id[d.pbase + q] == id;
+            if (seen) continue;
+            // weights from the valid window pulses only, zeros elsewhere
+            sub_t.clear();
+            for (int k = 0; k < nw; ++k)
+                if (id >> k & 1) sub_t.push_back(t[offset + k]);
+            int64_t joff = 0;
+            presum_weights(acor, sub_t.data(), (int64_t)sub_t.size(), tout, &joff, wsub);
+            if (joff != 0 || wsub.size() != sub_t.size()) return BP_ERR_RUNTIME;
+            out.id.push_back(id);
+            const size_t base = out.w.size();
+            out.w.resize(base + nw, 0.f);
+            for (int k = 0, u = 0; k < nw; ++k)
+                if (id >> k & 1) out.w[base + k] = (float)wsub[u++];
+            ++d.npat;
+        }
+    }
+    return BP_SUCCESS;
 }
 
 struct DevBuf {
@@ -322,96 +397,91 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
     bp::Orbit ho{orbit->t0, orbit->dt, orbit->n, orbit->method, orbit->pos, orbit->vel};
     const int64_t L = grid->length, W = grid->width;
     std::vector<PulseDesc> desc(L);
+    // output pulses are independent: contiguous chunks on host threads, each
+    // with its own pattern tables, concatenated afterwards
+    const int nthreads = (int)std::max<int64_t>(
+        1, std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
+                              (L + 255) / 256}));
+    std::vector<Patterns> parts(nthreads);
+    std::vector<std::thread> pool;
+    for (int p = 0; p < nthreads; ++p) {
+        const int64_t i0 = L * p / nthreads, i1 = L * (p + 1) / nthreads;
+        auto work = [&, p, i0, i1] {
+            parts[p].status = plan_pulses(ho, t, nt, swaths, ns, W, grid->prf, antenna_length, i0,
+                                          i1, desc.data(), parts[p]);
+        };
+        if (p + 1 < nthreads)
+            pool.emplace_back(work);
+        else
+            work();
+    }
+    for (auto& th : pool) th.join();
     std::vector<int64_t> pat_id;
     std::vector<float> pat_w;
-    std::vector<double> w, sub_t, wsub;
-    std::vector<int64_t> cuts;
-    for (int64_t i = 0; i < L; ++i) {
-        const double tout = t[0] + (double)i / grid->prf;
-        bp::V3 pos, vel;
-        const int st = bp::orbit_interpolate(ho, tout, bp::kBorderError, &pos, &vel);
-        if (st != bp::kSuccess)
+    for (int p = 0; p < nthreads; ++p) {
+        if (parts[p].status == BP_ERR_OUT_OF_RANGE)
             return fail(BP_ERR_OUT_OF_RANGE, "output pulse time outside the orbit");
-        const double v = bp::norm(vel);
-        const bp_acorr acor{BP_ACORR_AZIMUTH, 0, 2.0 * (antenna_length / v)};
-        int64_t offset = 0;
-        presum_weights(acor, t, nt, tout, &offset, w);
-        const int nw = (int)w.size();
-        if (nw > kMaxWindow)
+        if (parts[p].status == BP_ERR_LENGTH)
             return fail(BP_ERR_LENGTH, "Hash algorithm requires number of pulses less than "
                                        "number of bits in identifier (64)");
-        PulseDesc& d = desc[i];
-        d.tout = tout;
-        d.offset = offset;
-        d.nw = nw;
-        d.pbase = (int32_t)pat_id.size();
-        d.npat = 0;
-        d.pad_ = 0;
-        // range bins where the pattern can change: every sub-swath boundary
-        cuts.assign(1, 0);
-        for (int k = 0; k < nw; ++k)
-            for (int64_t s = 0; s < ns; ++s) {
-                const int32_t* sw = swaths + (s * nt + offset + k) * 2;
-                for (int e = 0; e < 2; ++e)
-                    if (sw[e] > 0 && sw[e] < W) cuts.push_back(sw[e]);
-            }
-        std::sort(cuts.begin(), cuts.end());
-        cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
-        for (int64_t c : cuts) {
-            int64_t id = 0;
-            for (int k = 0; k < nw; ++k)
-                for (int64_t s = 0; s < ns; ++s) {
-                    const int32_t* sw = swaths + (s * nt + offset + k) * 2;
-                    if (c >= sw[0] && c < sw[1]) {
-                        id |= (int64_t)1 << k;
-                        break;
-                    }
-                }
-            bool seen = false;
-            for (int q = 0; q < d.npat; ++q) seen |= pat_id[d.pbase + q] == id;
-            if (seen) continue;
-            // weights from the valid window pulses only, zeros elsewhere
-            sub_t.clear();
-            for (int k = 0; k < nw; ++k)
-                if (id >> k & 1) sub_t.push_back(t[offset + k]);
-            int64_t joff = 0;
-            presum_weights(acor, sub_t.data(), (int64_t)sub_t.size(), tout, &joff, wsub);
-            if (joff != 0 || wsub.size() != sub_t.size())
-                return fail(BP_ERR_RUNTIME, "presum weights of a mask pattern left its window");
-            pat_id.push_back(id);
-            const size_t base = pat_w.size();
-            pat_w.resize(base + kMaxWindow, 0.f);
-            for (int k = 0, u = 0; k < nw; ++k)
-                if (id >> k & 1) pat_w[base + k] = (float)wsub[u++];
-            ++d.npat;
+        if (parts[p].status != BP_SUCCESS)
+            return fail(BP_ERR_RUNTIME, "presum weights of a mask pattern left its window");
+        const int32_t base = (int32_t)pat_id.size(), wb = (int32_t)pat_w.size();
+        for (int64_t i = L * p / nthreads; i < L * (p + 1) / nthreads; ++i) {
+            desc[i].pbase += base;
+            desc[i].wbase += wb;
         }
+        pat_id.insert(pat_id.end(), parts[p].id.begin(), parts[p].id.end());
+        pat_w.insert(pat_w.end(), parts[p].w.begin(), parts[p].w.end());
     }
 
     if (hipSetDevice(device) != hipSuccess) return fail(BP_ERR_HIP, "hipSetDevice failed");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    DevBuf d_t, d_sw, d_desc, d_pid, d_pw, d_lut;
-    hipError_t e = d_t.put(t, sizeof(double) * nt, s);
-    if (e == hipSuccess) e = d_sw.put(swaths, sizeof(int32_t) * 2 * ns * nt, s);
-    if (e == hipSuccess) e = d_desc.put(desc.data(), sizeof(PulseDesc) * desc.size(), s);
-    if (e == hipSuccess) e = d_pid.put(pat_id.data(), sizeof(int64_t) * pat_id.size(), s);
-    if (e == hipSuccess) e = d_pw.put(pat_w.data(), sizeof(float) * pat_w.size(), s);
-    if (e == hipSuccess && doppler->have_data)
-        e = d_lut.put(doppler->data, sizeof(double) * doppler->width * doppler->length, s);
-    if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
+    // every table in one pinned staging buffer and one device allocation
+    struct Part {
+        const void* src;
+        size_t bytes, off;
+    };
+    Part parts_up[6] = {
+        {t, sizeof(double) * nt, 0},
+        {swaths, sizeof(int32_t) * 2 * ns * nt, 0},
+        {desc.data(), sizeof(PulseDesc) * desc.size(), 0},
+        {pat_id.data(), sizeof(int64_t) * pat_id.size(), 0},
+        {pat_w.data(), sizeof(float) * pat_w.size(), 0},
+        {doppler->have_data ? doppler->data : nullptr,
+         doppler->have_data ? sizeof(double) * doppler->width * doppler->length : 0, 0}};
+    size_t total = 0;
+    for (Part& q : parts_up) {
+        q.off = total;
+        total += (q.bytes + 255) / 256 * 256;
+    }
+    DevBuf d_all;
+    void* h_all = nullptr;
+    hipError_t e = hipHostMalloc(&h_all, total, hipHostMallocDefault);
+    if (e == hipSuccess) {
+        for (const Part& q : parts_up)
+            if (q.bytes) std::memcpy(static_cast<char*>(h_all) + q.off, q.src, q.bytes);
+        e = d_all.put(h_all, total, s);
+    }
+    if (e != hipSuccess) {
+        if (h_all) (void)hipHostFree(h_all);
+        return fail(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
+    }
+    auto dev = [&](int k) { return static_cast<const char*>(d_all.ptr) + parts_up[k].off; };
 
     ResampleArgs a;
     std::memset(&a, 0, sizeof(a));
     a.raw = static_cast<const float2*>(raw);
     a.out = static_cast<float2*>(out);
-    a.t = static_cast<const double*>(d_t.ptr);
-    a.swaths = static_cast<const int32_t*>(d_sw.ptr);
-    a.desc = static_cast<const PulseDesc*>(d_desc.ptr);
-    a.pat_id = static_cast<const int64_t*>(d_pid.ptr);
-    a.pat_w = static_cast<const float*>(d_pw.ptr);
+    a.t = reinterpret_cast<const double*>(dev(0));
+    a.swaths = reinterpret_cast<const int32_t*>(dev(1));
+    a.desc = reinterpret_cast<const PulseDesc*>(dev(2));
+    a.pat_id = reinterpret_cast<const int64_t*>(dev(3));
+    a.pat_w = reinterpret_cast<const float*>(dev(4));
     a.doppler = bp::Lut2d{doppler->have_data, doppler->bounds_error, doppler->ref_value,
                           doppler->xstart, doppler->ystart, doppler->dx, doppler->dy,
                           doppler->width, doppler->length,
-                          static_cast<const double*>(d_lut.ptr)};
+                          reinterpret_cast<const double*>(dev(5))};
     a.r0 = grid->starting_range;
     a.dr = grid->range_pixel_spacing;
     a.nt = nt;
@@ -423,8 +493,9 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
                       (unsigned)((L + a.pulses_per_thread - 1) / a.pulses_per_thread));
     hipLaunchKernelGGL(resample_kernel, blocks, dim3(kThreads), 0, s, a);
     e = hipGetLastError();
-    // the temporaries are freed on return: wait for the kernel first
+    // the tables are freed on return: wait for the kernel first
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipHostFree(h_all);
     if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("presum resample: ") + hipGetErrorString(e));
     return BP_SUCCESS;
 }

@@ -5,13 +5,14 @@
 // (cxx/isce3/math/Stats.cpp:33-165) and the storage encoding -- plus a
 // one-workgroup merge of the per-workgroup statistics.
 //
-// Each thread takes kPerThread samples strided by the workgroup size (every
-// load and store a coalesced 8-byte-per-lane wave access), keeps them in
-// registers, and summarises its valid samples exactly in FP64 by two passes
+// Each thread takes chunks of kPerThread samples strided by the workgroup size
+// (every load and store a coalesced 8-byte-per-lane wave access), keeps them
+// in registers, and summarises its valid samples exactly in FP64 by two passes
 // (mean, then the sum of squared deviations).  Summaries are combined with
-// Chan's pairwise update -- across the wave by lane shuffles, across the
-// workgroup through LDS, across workgroups in a second kernel that merges the
-// partials in a fixed order, so results do not depend on scheduling.
+// Chan's pairwise update -- over a thread's chunks, across the wave by lane
+// shuffles, across the workgroup through LDS, across the (at most 2048)
+// workgroups in a second kernel that merges the partials in a fixed order, so
+// results do not depend on scheduling.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,6 +32,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kPerThread = 16;
 constexpr int64_t kPerBlock = (int64_t)kThreads * kPerThread;
+constexpr int kMaxGrid = 2048;   // workgroups (and partial summaries) at most
 
 // Summary of the valid samples of one component: count, mean, sum of squared
 // deviations, extremes.
@@ -50,7 +52,7 @@ struct FinishArgs {
     const float2* prior;       // [n] or null
     uint32_t* encoded;         // [n] pairs of half (complex32) or null
     Partial* partials;         // [gridDim.x]
-    int64_t n, cols;
+    int64_t n, cols, chunks;   // chunks of kPerBlock samples
     int32_t col_step;          // kThreads % cols
     int32_t data_type;
     uint32_t mantissa_mask;
@@ -136,9 +138,11 @@ __device__ inline Partial block_reduce(Partial p)
     return p;
 }
 
-__global__ __launch_bounds__(kThreads) void finish_kernel(FinishArgs a)
+// One chunk of kPerBlock samples: scale, accumulate, encode and store; the
+// thread's summary of its valid samples in *p (two passes over registers).
+__device__ inline void finish_chunk(const FinishArgs& a, int64_t chunk, Partial* p)
 {
-    const int64_t e0 = (int64_t)blockIdx.x * kPerBlock + threadIdx.x;
+    const int64_t e0 = chunk * kPerBlock + threadIdx.x;
     int64_t col = e0 % a.cols;
     float2 v[kPerThread];
     bool ok[kPerThread];
@@ -167,21 +171,21 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinishArgs a)
         col += a.col_step;
         if (col >= a.cols) col -= a.cols;
     }
-    Partial p = empty_partial();
-    p.n = cnt;
+    *p = empty_partial();
+    p->n = cnt;
     if (cnt > 0) {
-        p.re.mean = sre / (double)cnt;
-        p.im.mean = sim / (double)cnt;
+        p->re.mean = sre / (double)cnt;
+        p->im.mean = sim / (double)cnt;
 #pragma unroll
         for (int j = 0; j < kPerThread; ++j) {
             if (!ok[j]) continue;
-            const double dr = (double)v[j].x - p.re.mean, di = (double)v[j].y - p.im.mean;
-            p.re.m2 += dr * dr;
-            p.im.m2 += di * di;
-            p.re.lo = fminf(p.re.lo, v[j].x);
-            p.re.hi = fmaxf(p.re.hi, v[j].x);
-            p.im.lo = fminf(p.im.lo, v[j].y);
-            p.im.hi = fmaxf(p.im.hi, v[j].y);
+            const double dr = (double)v[j].x - p->re.mean, di = (double)v[j].y - p->im.mean;
+            p->re.m2 += dr * dr;
+            p->im.m2 += di * di;
+            p->re.lo = fminf(p->re.lo, v[j].x);
+            p->re.hi = fmaxf(p->re.hi, v[j].x);
+            p->im.lo = fminf(p->im.lo, v[j].y);
+            p->im.hi = fmaxf(p->im.hi, v[j].y);
         }
     }
     // stores: statistics are of the values before truncation
@@ -198,8 +202,21 @@ __global__ __launch_bounds__(kThreads) void finish_kernel(FinishArgs a)
         }
         if (a.store_z) a.z[e] = x;
     }
-    p = block_reduce(p);
-    if (threadIdx.x == 0) a.partials[blockIdx.x] = p;
+}
+
+// Workgroup g takes chunks g, g + gridDim.x, ... (a fixed assignment, so the
+// merge order and the result do not depend on scheduling) and leaves one
+// partial summary.
+__global__ __launch_bounds__(kThreads) void finish_kernel(FinishArgs a)
+{
+    Partial acc = empty_partial();
+    for (int64_t chunk = blockIdx.x; chunk < a.chunks; chunk += gridDim.x) {
+        Partial p;
+        finish_chunk(a, chunk, &p);
+        merge(acc, p);
+    }
+    acc = block_reduce(acc);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = acc;
 }
 
 __device__ inline void to_stats(const Moments& m, int64_t n, bp_stats_f32& s)
@@ -253,8 +270,8 @@ int bp_rslc_finisher_create(int device, int64_t max_pixels, bp_rslc_finisher** f
     if (f == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     *f = nullptr;
     if (max_pixels < 1) return fail(BP_ERR_INVALID_ARGUMENT, "max_pixels must be > 0");
-    const int64_t blocks = (max_pixels + rslc::kPerBlock - 1) / rslc::kPerBlock;
-    if (blocks > (int64_t)1 << 30) return fail(BP_ERR_INVALID_ARGUMENT, "max_pixels too large");
+    const int64_t blocks = std::min<int64_t>((max_pixels + rslc::kPerBlock - 1) / rslc::kPerBlock,
+                                             rslc::kMaxGrid);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(BP_ERR_HIP, "no HIP device available (isce3_amd needs an MI355X GPU)");
@@ -335,7 +352,8 @@ int bp_rslc_finish_block(bp_rslc_finisher* f, void* z, int64_t rows, int64_t col
     a.mantissa_mask = mask;
     a.store_z = range_cor != nullptr || prior != nullptr ||
                 (data_type == BP_RSLC_COMPLEX64_ZERO_MANTISSA && mask != 0xffffffffu);
-    const int blocks = (int)((n + rslc::kPerBlock - 1) / rslc::kPerBlock);
+    a.chunks = (n + rslc::kPerBlock - 1) / rslc::kPerBlock;
+    const int blocks = (int)std::min<int64_t>(a.chunks, rslc::kMaxGrid);
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(rslc::finish_kernel, dim3(blocks), dim3(rslc::kThreads), 0, s, a);
     hipLaunchKernelGGL(rslc::merge_kernel, dim3(1), dim3(rslc::kThreads), 0, s,
