@@ -158,6 +158,49 @@ __device__ inline float2 cadd(float2 a, float2 b)
     return make_float2(a.x + b.x, a.y + b.y);
 }
 
+// Valid-sample bit of range bin j in input pulse p (any sub-swath).
+__device__ inline bool bin_valid(const ResampleArgs& a, int64_t j, int64_t p)
+{
+    bool valid = false;
+    for (int64_t s = 0; s < a.ns; ++s) {
+        const int32_t* sw = a.swaths + (s * a.nt + p) * 2;
+        valid |= (j >= sw[0]) & (j < sw[1]);
+    }
+    return valid;
+}
+
+// LUT2d<double>::eval (core/LUT2d.cpp:127-159, clamped bilinear) split into
+// the range part, fixed per thread, and the azimuth part, fixed per output
+// pulse.  floor and ceil of one coordinate differ by 0 or exactly 1, so the
+// divisions by (x2 - x1), (y2 - y1) and their product that the bilinear
+// formula performs are divisions by 1.0 and are left out: the values are
+// the same bits.
+struct LutAxis {
+    double v;       // clamped fractional index
+    int64_t i1, i2; // floor, ceil
+};
+
+__device__ inline LutAxis lut_axis(double coord, double start, double step, int64_t n)
+{
+    double v = (coord - start) / step;
+    v = fmin(fmax(v, 0.0), n - 1.0);
+    return {v, (int64_t)floor(v), (int64_t)ceil(v)};
+}
+
+__device__ inline double lut_bilinear(const bp::Lut2d& l, const LutAxis& x, const LutAxis& y)
+{
+#pragma clang fp contract(off)
+    const double* z = l.data;
+    const double q11 = z[y.i1 * l.width + x.i1], q12 = z[y.i2 * l.width + x.i1];
+    const double q21 = z[y.i1 * l.width + x.i2], q22 = z[y.i2 * l.width + x.i2];
+    const double x1 = (double)x.i1, x2 = (double)x.i2, y1 = (double)y.i1, y2 = (double)y.i2;
+    if (y.i1 == y.i2 && x.i1 == x.i2) return q11;
+    if (y.i1 == y.i2) return ((x2 - x.v) * q11) + ((x.v - x1) * q21);
+    if (x.i1 == x.i2) return ((y2 - y.v) * q11) + ((y.v - y1) * q12);
+    return (q11 * ((x2 - x.v) * (y2 - y.v))) + (q21 * ((x.v - x1) * (y2 - y.v))) +
+           (q12 * ((x2 - x.v) * (y.v - y1))) + (q22 * ((x.v - x1) * (y.v - y1)));
+}
+
 __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
 {
     const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -165,23 +208,38 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
     const double r = a.r0 + a.dr * (double)j;
     const int64_t i0 = (int64_t)blockIdx.y * a.pulses_per_thread;
     const int64_t i1 = min(i0 + (int64_t)a.pulses_per_thread, a.length);
+    const bp::Lut2d& lut = a.doppler;
+    LutAxis lx{};
+    if (lut.have_data) lx = lut_axis(r, lut.xstart, lut.dx, lut.width);
+    // validity of this bin in every input pulse the chunk's windows reach,
+    // one bit per pulse from the earliest window start
+    int64_t pfirst = a.desc[i0].offset, pend = pfirst;
+    for (int64_t i = i0; i < i1; ++i) {
+        pfirst = min(pfirst, a.desc[i].offset);
+        pend = max(pend, a.desc[i].offset + (int64_t)a.desc[i].nw);
+    }
+    const int64_t span = pend - pfirst;
+    uint64_t bits = 0;
+    if (span <= 64)
+        for (int64_t p = 0; p < span; ++p)
+            if (bin_valid(a, j, pfirst + p)) bits |= (uint64_t)1 << p;
     for (int64_t i = i0; i < i1; ++i) {
         const PulseDesc d = a.desc[i];
         // pattern of this range bin over the window
         int64_t id = 0;
-        for (int k = 0; k < d.nw; ++k) {
-            const int64_t p = d.offset + k;
-            bool valid = false;
-            for (int64_t s = 0; s < a.ns; ++s) {
-                const int32_t* sw = a.swaths + (s * a.nt + p) * 2;
-                valid |= (j >= sw[0]) & (j < sw[1]);
-            }
-            if (valid) id |= (int64_t)1 << k;
+        if (span <= 64) {
+            const uint64_t m = d.nw >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << d.nw) - 1);
+            id = (int64_t)((bits >> (d.offset - pfirst)) & m);
+        } else {
+            for (int k = 0; k < d.nw; ++k)
+                if (bin_valid(a, j, d.offset + k)) id |= (int64_t)1 << k;
         }
         const float* w = nullptr;
         for (int q = 0; q < d.npat; ++q)
             if (a.pat_id[d.pbase + q] == id) w = a.pat_w + d.wbase + (int64_t)q * d.nw;
-        const double fd = bp::lut2d_eval(a.doppler, d.tout, r);
+        double fd = lut.ref_value;
+        if (lut.have_data)
+            fd = lut_bilinear(lut, lx, lut_axis(d.tout, lut.ystart, lut.dy, lut.length));
         float2 sum = make_float2(0.f, 0.f);
         if (w != nullptr) {
             for (int k = 0; k < d.nw; ++k) {
