@@ -66,7 +66,8 @@ int bp_rslc_finisher_destroy(bp_rslc_finisher* f);
  * z is complex64 rows x cols, row-major, updated in place.  rows * cols above
  * the finisher's max_pixels gives BP_ERR_LENGTH.  The call returns after the
  * statistics are on the host (it synchronises `stream`; NULL = the null
- * stream). */
+ * stream).  Calls on one finisher share its partial-statistics buffer: the
+ * caller serialises them (different finishers are independent). */
 int bp_rslc_finish_block(bp_rslc_finisher* f, void* z, int64_t rows, int64_t cols,
                          const void* range_cor, const void* prior, int data_type,
                          int mantissa_nbits, void* encoded, bp_stats_real_imag_f32* stats,

@@ -462,18 +462,24 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
                               (L + 255) / 256}));
     std::vector<Patterns> parts(nthreads);
     std::vector<std::thread> pool;
-    for (int p = 0; p < nthreads; ++p) {
+    std::vector<char> ran(nthreads, 0);
+    auto work = [&](int p) {
         const int64_t i0 = L * p / nthreads, i1 = L * (p + 1) / nthreads;
-        auto work = [&, p, i0, i1] {
-            parts[p].status = plan_pulses(ho, t, nt, swaths, ns, W, grid->prf, antenna_length, i0,
-                                          i1, desc.data(), parts[p]);
-        };
-        if (p + 1 < nthreads)
-            pool.emplace_back(work);
-        else
-            work();
+        parts[p].status = plan_pulses(ho, t, nt, swaths, ns, W, grid->prf, antenna_length, i0, i1,
+                                      desc.data(), parts[p]);
+        ran[p] = 1;
+    };
+    for (int p = 0; p + 1 < nthreads; ++p) {
+        try {
+            pool.emplace_back(work, p);
+        } catch (...) {
+            break;  // no thread available: the chunk runs below on this thread
+        }
     }
+    work(nthreads - 1);
     for (auto& th : pool) th.join();
+    for (int p = 0; p < nthreads; ++p)
+        if (!ran[p]) work(p);
     std::vector<int64_t> pat_id;
     std::vector<float> pat_w;
     for (int p = 0; p < nthreads; ++p) {
