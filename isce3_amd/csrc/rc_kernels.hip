@@ -206,24 +206,155 @@ __device__ inline void fft(float2* s, const float2* __restrict__ tw)
     if (REM == 3) stockham_pass<8, L>(s, Ns, tw);
 }
 
+// A Stockham pass split into its parts so that the first pass can read from
+// global memory and the last pass of the forward transform can hand its
+// outputs to the first pass of the inverse transform in registers.  Thread t
+// holds butterflies j = t + q T (q < 16 / R), element r of butterfly j being
+// sample j + r L / R on input; outputs go to (j - k) R + k + r Ns, k = j mod Ns.
+template<int R, int L>
+struct Regs {
+    static constexpr int T = L / 16;
+    static constexpr int NB = L / R;
+    static constexpr int Q = NB / T;
+    float2 v[Q][R];
+};
+
+template<int R, int L>
+__device__ inline void pass_load_lds(Regs<R, L>& d, const float2* s)
+{
+#pragma unroll
+    for (int q = 0; q < d.Q; ++q) {
+        const int j = threadIdx.x + q * d.T;
+#pragma unroll
+        for (int r = 0; r < R; ++r) d.v[q][r] = s[pad(j + r * d.NB)];
+    }
+}
+
+template<int R, int L>
+__device__ inline void pass_load_global(Regs<R, L>& d, const float2* x, int n)
+{
+#pragma unroll
+    for (int q = 0; q < d.Q; ++q) {
+        const int j = threadIdx.x + q * d.T;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = j + r * d.NB;
+            d.v[q][r] = i < n ? x[i] : make_float2(0.f, 0.f);
+        }
+    }
+}
+
+template<int R, int L>
+__device__ inline void pass_compute(Regs<R, L>& d, int Ns, const float2* __restrict__ tw)
+{
+#pragma unroll
+    for (int q = 0; q < d.Q; ++q) {
+        const int j = threadIdx.x + q * d.T;
+        const int k = j & (Ns - 1);
+        if (Ns > 1) twiddle<R>(d.v[q], tw, k * (L / (Ns * R)));
+        dft<R>(d.v[q]);
+    }
+}
+
+template<int R, int L>
+__device__ inline void pass_store_lds(const Regs<R, L>& d, float2* s, int Ns)
+{
+#pragma unroll
+    for (int q = 0; q < d.Q; ++q) {
+        const int j = threadIdx.x + q * d.T;
+        const int k = j & (Ns - 1);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[pad(base + r * Ns)] = d.v[q][r];
+    }
+}
+
+// One full pass through LDS (read, barrier, compute, write, barrier).
+template<int R, int L>
+__device__ inline void pass_lds(float2* s, int Ns, const float2* __restrict__ tw)
+{
+    Regs<R, L> d;
+    pass_load_lds(d, s);
+    __syncthreads();
+    pass_compute(d, Ns, tw);
+    pass_store_lds(d, s, Ns);
+    __syncthreads();
+}
+
+template<int REM>
+struct LastRadix {
+    static constexpr int R = REM == 1 ? 2 : (REM == 2 ? 4 : (REM == 3 ? 8 : 16));
+};
+
+// Convolution of one pulse, 2^LOG2L points.  Forward transform: radix-16
+// passes then the remainder radix Rl (16 when LOG2L is a multiple of 4); the
+// inverse runs the radices in the opposite order, so the forward transform's
+// last pass leaves in each thread exactly the spectrum bins the inverse's
+// first pass needs: the spectral product and conjugation happen in
+// registers.  The pulse is read from global memory by the first pass and the
+// crop is written by the last, so LDS holds the data only between passes.
 template<int LOG2L>
 __global__ __launch_bounds__((1 << LOG2L) / 16) void conv_kernel(ConvArgs a)
 {
     constexpr int L = 1 << LOG2L;
     constexpr int T = L / 16;
+    constexpr int Rl = LastRadix<LOG2L % 4>::R;
+    constexpr int N16 = (LOG2L - (Rl == 16 ? 4 : 0)) / 4;  // radix-16 passes besides the last
     __shared__ float2 s[L + L / 16];
     const int64_t b = blockIdx.x;
     const float2* x = a.in + b * a.n_in;
-    for (int i = threadIdx.x; i < L; i += T) s[pad(i)] = i < a.n_in ? x[i] : make_float2(0.f, 0.f);
-    __syncthreads();
-    fft<LOG2L>(s, a.tw);
-    for (int i = threadIdx.x; i < L; i += T) s[pad(i)] = conjf2(cmul(s[pad(i)], a.H[i]));
-    __syncthreads();
-    fft<LOG2L>(s, a.tw);
-    float2* y = a.out + b * a.n_out;
-    for (int i = threadIdx.x; i < a.n_out; i += T) {
-        const int p = a.offset + i;
-        y[i] = p < L ? conjf2(s[pad(p)]) : make_float2(0.f, 0.f);
+
+    // forward: first radix-16 pass from global memory
+    {
+        Regs<16, L> d;
+        pass_load_global(d, x, a.n_in);
+        pass_compute(d, 1, a.tw);
+        pass_store_lds(d, s, 1);
+        __syncthreads();
+    }
+    int Ns = 16;
+#pragma unroll
+    for (int p = 1; p < N16; ++p) {
+        pass_lds<16, L>(s, Ns, a.tw);
+        Ns *= 16;
+    }
+    // forward last pass (radix Rl, Ns = L / Rl) -> product -> inverse first pass (radix Rl, Ns = 1)
+    {
+        Regs<Rl, L> d;
+        pass_load_lds(d, s);
+        __syncthreads();
+        pass_compute(d, Ns, a.tw);
+#pragma unroll
+        for (int q = 0; q < d.Q; ++q) {
+            const int j = threadIdx.x + q * T;   // j < L / Rl = Ns: bins j + r Ns
+#pragma unroll
+            for (int r = 0; r < Rl; ++r) d.v[q][r] = conjf2(cmul(d.v[q][r], a.H[j + r * Ns]));
+        }
+        pass_compute(d, 1, a.tw);
+        pass_store_lds(d, s, 1);
+        __syncthreads();
+    }
+    Ns = Rl;
+#pragma unroll
+    for (int p = 0; p + 1 < N16; ++p) {
+        pass_lds<16, L>(s, Ns, a.tw);
+        Ns *= 16;
+    }
+    // inverse last pass (radix 16, Ns = L / 16): conjugated crop straight to global memory
+    {
+        Regs<16, L> d;
+        pass_load_lds(d, s);
+        pass_compute(d, Ns, a.tw);
+        float2* y = a.out + b * a.n_out;
+        const int j = threadIdx.x;              // outputs j + r L / 16
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int i = j + r * Ns - a.offset;
+            if (i >= 0 && i < a.n_out) y[i] = conjf2(d.v[0][r]);
+        }
+        // outputs past the transform are zero
+        for (int i = max(0, L - a.offset) + (int)threadIdx.x; i < a.n_out; i += T)
+            y[i] = make_float2(0.f, 0.f);
     }
 }
 
