@@ -10,10 +10,12 @@
  * formed in the frequency domain with an FFT of length fft_size, the
  * smallest power of two >= max(1024, chirp_size + input_size - 1) (the
  * reference uses the smallest 2^a 3^b 5^c; both give the same linear
- * convolution).  This build supports fft_size <= 2^19 (524288): up to 2^14
- * one fused kernel per pulse, above that three four-step passes through a
- * device work buffer of at most 512 MiB that the compressor allocates (a run
- * of more pulses than fit goes through it in sub-batches on the stream).
+ * convolution).  Up to fft_size 2^14 one fused transform per pulse; above
+ * that, chirps of at most 2^13 + 1 samples run as overlap-save blocks of 2^14
+ * (any pulse length), longer chirps as three four-step passes through a
+ * device work buffer of at most 512 MiB that the compressor allocates
+ * (fft_size <= 2^19; a run of more pulses than fit goes through it in
+ * sub-batches on the stream).
  *
  * Return codes as in isce3_amd_backproject.h; bp_last_error_message()
  * returns the message of the calling thread's last failure.

@@ -1,8 +1,10 @@
 // C-ABI of range compression (include/isce3_amd_rangecomp.h): argument rules
 // of cxx/isce3/focus/RangeComp.cpp:39-79 and :93-99, the matched filter's
 // spectrum formed on the host in FP64 (RangeComp.cpp:22-37), and the launch
-// of the FFT-convolution kernels (rc_kernels.hip): one fused kernel for FFT
-// lengths up to 2^14, three four-step passes through a work buffer above.
+// of the FFT-convolution kernels (rc_kernels.hip): one fused transform per
+// pulse for convolutions up to 2^14 samples, overlap-save blocks of 2^14 for
+// longer pulses with chirps of at most 2^13 + 1 samples, and three four-step
+// passes through a work buffer for long chirps (transforms up to 2^19).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -74,10 +76,11 @@ struct bp_rangecomp {
     int device = 0;
     int chirp_size = 0, input_size = 0, max_batch = 0, mode = 0;
     int fft_size = 0, log2l = 0, out_size = 0, offset = 0;
-    float2* H = nullptr;    // device [fft_size] (four steps: permuted)
-    float2* tw = nullptr;   // device [fft_size]
-    float2* tw2 = nullptr;  // device [fft_size / 32] (four steps)
-    float2* work = nullptr; // device [work_batch x fft_size] (four steps)
+    int method = 0;         // 0 one transform per pulse, 1 overlap-save blocks, 2 four steps
+    float2* H = nullptr;    // device [L] filter spectrum, L the transform length (four steps: permuted)
+    float2* tw = nullptr;   // device [L]
+    float2* tw2 = nullptr;  // device [L / 32] (four steps)
+    float2* work = nullptr; // device [work_batch x L] (four steps)
     int work_batch = 0;     // pulses per four-step launch
 };
 
@@ -103,18 +106,24 @@ int bp_rangecomp_create(int device, const float* chirp, int chirp_size, int inpu
         return fail(BP_ERR_RUNTIME, "unexpected range compression mode");
     const int64_t full = (int64_t)chirp_size + input_size - 1;
     int log2l = rc::kMinLog2;
-    while (log2l <= rc::kMaxLog2 && ((int64_t)1 << log2l) < full) ++log2l;
-    if (log2l > rc::kMaxLog2)
+    while (log2l < 31 && ((int64_t)1 << log2l) < full) ++log2l;
+    // one fused transform per pulse; overlap-save blocks of 2^14 for longer
+    // pulses with chirps up to 2^13 + 1 samples; four steps otherwise
+    const int method = log2l <= rc::kMaxFusedLog2 ? 0
+                       : (chirp_size - 1 <= rc::kOlsMaxShift ? 1 : 2);
+    if (method == 2 && log2l > rc::kMaxLog2)
         return fail(BP_ERR_INVALID_ARGUMENT,
-                    "range compression of " + std::to_string(full) + " convolution samples exceeds "
-                    "the supported FFT length " + std::to_string(1 << rc::kMaxLog2));
+                    "range compression of " + std::to_string(full) + " convolution samples with a " +
+                    std::to_string(chirp_size) + "-sample chirp exceeds the supported FFT length " +
+                    std::to_string(1 << rc::kMaxLog2));
+    if (log2l >= 31) return fail(BP_ERR_INVALID_ARGUMENT, "pulse too long");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(BP_ERR_HIP, "no HIP device available (isce3_amd needs an MI355X GPU)");
     if (device < 0 || device >= ndev) return fail(BP_ERR_INVALID_ARGUMENT, "invalid device index");
     if (hipSetDevice(device) != hipSuccess) return fail(BP_ERR_HIP, "hipSetDevice failed");
 
-    const int L = 1 << log2l;
+    const int L = method == 1 ? 1 << rc::kOlsLog2 : 1 << log2l;   // transform length
     // matched filter: time-reversed conjugate of the chirp, zero padded to L,
     // transformed, scaled by 1/L (RangeComp.cpp:22-37, :108-115)
     std::vector<std::complex<double>> h(L, 0.0);
@@ -123,7 +132,7 @@ int bp_rangecomp_create(int device, const float* chirp, int chirp_size, int inpu
         h[i] = std::complex<double>(chirp[2 * j], -chirp[2 * j + 1]);
     }
     fft_host(h);
-    const bool four_step = log2l > rc::kMaxFusedLog2;
+    const bool four_step = method == 2;
     const int L2 = L / 32;
     std::vector<float2> Hh(L), twh(L), tw2h(four_step ? L2 : 0);
     for (int i = 0; i < L; ++i) {
@@ -143,8 +152,9 @@ int bp_rangecomp_create(int device, const float* chirp, int chirp_size, int inpu
     r->input_size = input_size;
     r->max_batch = max_batch;
     r->mode = mode;
-    r->fft_size = L;
-    r->log2l = log2l;
+    r->fft_size = 1 << log2l;
+    r->log2l = method == 1 ? rc::kOlsLog2 : log2l;
+    r->method = method;
     r->out_size = output_size(chirp_size, input_size, mode);
     r->offset = mode == BP_RC_FULL ? 0 : (mode == BP_RC_VALID ? chirp_size - 1 : chirp_size / 2);
     if (four_step)
@@ -216,8 +226,19 @@ int bp_rangecomp_run(bp_rangecomp* rc, void* out_dev, const void* in_dev, int ba
     a.work = rc->work;
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipError_t e = hipSuccess;
-    if (rc->log2l <= rc::kMaxFusedLog2) {
+    if (rc->method == 0) {
+        a.shift = 0;
+        a.valid = 1 << rc->log2l;
+        a.nblk = 1;
+        a.blk0 = 0;
         e = rc::launch_conv(a, rc->log2l, batch, s);
+    } else if (rc->method == 1) {
+        // blocks covering full-convolution samples [offset, offset + n_out)
+        a.shift = rc->chirp_size - 1;
+        a.valid = (1 << rc::kOlsLog2) - a.shift;
+        a.blk0 = a.offset / a.valid;
+        a.nblk = (a.offset + a.n_out - 1) / a.valid - a.blk0 + 1;
+        e = rc::launch_conv(a, rc::kOlsLog2, batch, s);
     } else {
         // sub-batches of the work buffer, launched in order on one stream
         for (int b0 = 0; b0 < batch && e == hipSuccess; b0 += rc->work_batch) {

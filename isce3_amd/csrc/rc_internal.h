@@ -8,7 +8,7 @@ namespace rc {
 
 constexpr int kMinLog2 = 10;      // FFT lengths 2^10 .. 2^14: one workgroup per pulse
 constexpr int kMaxFusedLog2 = 14;
-constexpr int kMaxLog2 = 19;      // 2^15 .. 2^19: four steps through a work buffer
+constexpr int kMaxLog2 = 19;      // 2^15 .. 2^19 with long chirps: four steps through a work buffer
 
 struct ConvArgs {
     const float2* in;    // [batch x n_in] pulses
@@ -20,8 +20,15 @@ struct ConvArgs {
     float2* work;        // [batch x L] (four steps)
     int32_t n_in, n_out;
     int32_t offset;      // first output sample in the full convolution
-    int32_t pad_;
+    // Fused kernel blocks (overlap-save): block s of a pulse transforms input
+    // samples [s valid - shift, s valid - shift + L) and yields full-convolution
+    // samples s valid + i - shift for i in [shift, L).  One block per pulse:
+    // shift 0, valid L, nblk 1, blk0 0.
+    int32_t shift, valid, nblk, blk0;
 };
+
+constexpr int kOlsLog2 = 14;             // overlap-save block length 2^14
+constexpr int kOlsMaxShift = 1 << 13;    // chirps up to 2^13 + 1 samples
 
 hipError_t launch_conv(const ConvArgs& a, int log2l, int batch, hipStream_t stream);
 hipError_t launch_four_step(const ConvArgs& a, int log2l, int batch, hipStream_t stream);

@@ -230,16 +230,17 @@ __device__ inline void pass_load_lds(Regs<R, L>& d, const float2* s)
     }
 }
 
+// Samples g0 + i of x (zero outside [0, n)).
 template<int R, int L>
-__device__ inline void pass_load_global(Regs<R, L>& d, const float2* x, int n)
+__device__ inline void pass_load_global(Regs<R, L>& d, const float2* x, int64_t g0, int64_t n)
 {
 #pragma unroll
     for (int q = 0; q < d.Q; ++q) {
         const int j = threadIdx.x + q * d.T;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int i = j + r * d.NB;
-            d.v[q][r] = i < n ? x[i] : make_float2(0.f, 0.f);
+            const int64_t g = g0 + j + r * d.NB;
+            d.v[q][r] = (g >= 0 && g < n) ? x[g] : make_float2(0.f, 0.f);
         }
     }
 }
@@ -301,13 +302,15 @@ __global__ __launch_bounds__((1 << LOG2L) / 16) void conv_kernel(ConvArgs a)
     constexpr int Rl = LastRadix<LOG2L % 4>::R;
     constexpr int N16 = (LOG2L - (Rl == 16 ? 4 : 0)) / 4;  // radix-16 passes besides the last
     __shared__ float2 s[L + L / 16];
-    const int64_t b = blockIdx.x;
+    const int64_t b = blockIdx.x / a.nblk;
+    const int64_t blk = a.blk0 + blockIdx.x % a.nblk;
+    const int64_t o0 = blk * a.valid - a.shift;   // full-convolution index of transform sample 0
     const float2* x = a.in + b * a.n_in;
 
     // forward: first radix-16 pass from global memory
     {
         Regs<16, L> d;
-        pass_load_global(d, x, a.n_in);
+        pass_load_global(d, x, o0, a.n_in);
         pass_compute(d, 1, a.tw);
         pass_store_lds(d, s, 1);
         __syncthreads();
@@ -349,12 +352,14 @@ __global__ __launch_bounds__((1 << LOG2L) / 16) void conv_kernel(ConvArgs a)
         const int j = threadIdx.x;              // outputs j + r L / 16
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int i = j + r * Ns - a.offset;
-            if (i >= 0 && i < a.n_out) y[i] = conjf2(d.v[0][r]);
+            const int t = j + r * Ns;           // transform sample: valid from `shift` on
+            const int64_t i = o0 + t - a.offset;
+            if (t >= a.shift && i >= 0 && i < a.n_out) y[i] = conjf2(d.v[0][r]);
         }
-        // outputs past the transform are zero
-        for (int i = max(0, L - a.offset) + (int)threadIdx.x; i < a.n_out; i += T)
-            y[i] = make_float2(0.f, 0.f);
+        // one block per pulse: outputs past the transform are zero
+        if (a.nblk == 1 && a.shift == 0)
+            for (int i = max(0, L - a.offset) + (int)threadIdx.x; i < a.n_out; i += T)
+                y[i] = make_float2(0.f, 0.f);
     }
 }
 
@@ -512,7 +517,7 @@ hipError_t launch_four_step(const ConvArgs& a, int log2l, int batch, hipStream_t
 hipError_t launch_conv(const ConvArgs& a, int log2l, int batch, hipStream_t stream)
 {
     if (batch <= 0) return hipSuccess;
-    const dim3 grid((unsigned)batch);
+    const dim3 grid((unsigned)((int64_t)batch * a.nblk));
     switch (log2l) {
     case 10: hipLaunchKernelGGL(conv_kernel<10>, grid, dim3(1 << 6), 0, stream, a); break;
     case 11: hipLaunchKernelGGL(conv_kernel<11>, grid, dim3(1 << 7), 0, stream, a); break;

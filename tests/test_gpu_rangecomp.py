@@ -2,7 +2,7 @@
 cxx/isce3/focus/RangeComp.cpp) and the reference's own known answers
 (tests/cxx/isce3/focus/rangecomp.cpp, tests/python/extensions/pybind/focus/
 rangecomp.py).  Tolerance: complex relative error <= 1e-5 against the FP64
-oracle (FP32 FFTs of up to 2^19 points)."""
+oracle (FP32 FFTs of up to 2^19 points, or overlap-save blocks of 2^14)."""
 import numpy as np
 import pytest
 
@@ -110,13 +110,18 @@ def test_device_tensors_in_place_and_batch_limit(torch_gpu):
 
 
 @pytest.mark.parametrize("mode", ["full", "valid", "same"])
-@pytest.mark.parametrize("nin,batch", [(20000, 3), (40000, 2), (100000, 1), (250000, 2),
-                                       (500000, 1)])
-def test_long_pulses_four_step_match_fp64_oracle(torch_gpu, mode, nin, batch):
-    """FFT lengths 2^15 .. 2^19 (the four-step path)."""
+@pytest.mark.parametrize("nin,batch,duration", [(20000, 3, 40e-6), (40000, 2, 40e-6),
+                                                (100000, 1, 40e-6), (250000, 2, 40e-6),
+                                                (500000, 1, 40e-6), (1200000, 1, 40e-6),
+                                                (60000, 2, 341.3e-6)])
+def test_long_pulses_overlap_save_match_fp64_oracle(torch_gpu, mode, nin, batch, duration):
+    """Convolutions past 2^14 samples with chirps of at most 2^13 + 1 samples
+    (overlap-save blocks of 2^14; 961 samples and 8191 samples, next to the
+    2^13 + 1 limit), including a pulse longer than any single transform."""
     from isce3_amd.focus import RangeComp, form_linear_chirp
     rng = np.random.default_rng(nin + 7 * batch)
-    chirp = form_linear_chirp(0.6e12, 40e-6, 24e6)               # 961 samples
+    chirp = form_linear_chirp(0.6e12, duration, 24e6)
+    assert len(chirp) <= 8193
     x = (rng.standard_normal((batch, nin)) + 1j * rng.standard_normal((batch, nin))).astype(np.complex64)
     rc = RangeComp(chirp, nin, maxbatch=batch, mode=RangeComp.Mode[mode.capitalize()])
     assert rc.fft_size >= nin + len(chirp) - 1 > 16384
@@ -130,13 +135,33 @@ def test_long_pulses_four_step_match_fp64_oracle(torch_gpu, mode, nin, batch):
     assert e <= REL
 
 
+@pytest.mark.parametrize("mode", ["full", "valid", "same"])
+@pytest.mark.parametrize("nin,batch,m", [(20000, 3, 9001), (40000, 2, 12001), (150000, 1, 20001)])
+def test_long_chirps_four_step_match_fp64_oracle(torch_gpu, mode, nin, batch, m):
+    """Chirps longer than 2^13 + 1 samples: FFT lengths 2^15 .. 2^18 through
+    the four-step path."""
+    from isce3_amd.focus import RangeComp
+    rng = np.random.default_rng(nin + m)
+    chirp = np.exp(1j * np.pi * 1e-5 * (np.arange(m) - m / 2) ** 2).astype(np.complex64)
+    x = (rng.standard_normal((batch, nin)) + 1j * rng.standard_normal((batch, nin))).astype(np.complex64)
+    rc = RangeComp(chirp, nin, maxbatch=batch, mode=RangeComp.Mode[mode.capitalize()])
+    y = np.zeros((batch, rc.output_size), np.complex64)
+    rc.rangecompress(y, x)
+    ref = rco.rangecompress(chirp, x, MODES[mode], "fp64")
+    e = rel_err(y, ref)
+    print(f"[rangecomp four-step] mode={mode} nin={nin} m={m} fft={rc.fft_size} rel={e:.2e}",
+          flush=True)
+    assert e <= REL
+
+
 def test_long_pulses_more_than_one_work_batch(torch_gpu):
-    """2^19-point FFTs: the 512 MiB work buffer holds 128 pulses, so a batch of
-    130 runs as two sub-batches; pulses either side of the seam are checked."""
-    from isce3_amd.focus import RangeComp, form_linear_chirp
+    """2^19-point FFTs of a long chirp: the 512 MiB work buffer holds 128
+    pulses, so a batch of 130 runs as two sub-batches; pulses either side of
+    the seam are checked."""
+    from isce3_amd.focus import RangeComp
     torch = torch_gpu
-    nin, batch = 300000, 130
-    chirp = form_linear_chirp(0.6e12, 40e-6, 24e6)
+    nin, batch, m = 300000, 130, 10001
+    chirp = np.exp(1j * np.pi * 1e-5 * (np.arange(m) - m / 2) ** 2).astype(np.complex64)
     rc = RangeComp(chirp, nin, maxbatch=batch, mode=RangeComp.Mode.Same)
     assert rc.fft_size == 1 << 19
     g = torch.Generator(device="cuda:0").manual_seed(3)

@@ -18,7 +18,8 @@ def test_constructor_argument_rules():
     with pytest.raises(RuntimeError):
         RangeComp(chirp, 10, mode=7)
     with pytest.raises(errors.InvalidArgument):
-        RangeComp(chirp, 524285)         # 524289-sample convolution > 2^19-point FFTs
+        # a chirp too long for overlap-save blocks and a convolution > 2^19 samples
+        RangeComp(np.ones(10000, np.complex64), 520000)
     with pytest.raises(errors.InvalidArgument):
         RangeComp(np.zeros(0, np.complex64), 10)
 
