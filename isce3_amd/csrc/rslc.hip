@@ -5,8 +5,9 @@
 // (cxx/isce3/math/Stats.cpp:33-165) and the storage encoding -- plus a
 // one-workgroup merge of the per-workgroup statistics.
 //
-// Each thread takes chunks of kPerThread samples strided by the workgroup size
-// (every load and store a coalesced 8-byte-per-lane wave access), keeps them
+// Each thread takes chunks of kPerThread samples, in pairs strided by the
+// workgroup size (every load and store a coalesced 16-byte-per-lane wave
+// access when the buffers allow, 8 bytes otherwise), keeps them
 // in registers, and summarises its valid samples exactly in FP64 by two passes
 // (mean, then the sum of squared deviations).  Summaries are combined with
 // Chan's pairwise update -- over a thread's chunks, across the wave by lane
@@ -53,7 +54,6 @@ struct FinishArgs {
     uint32_t* encoded;         // [n] pairs of half (complex32) or null
     Partial* partials;         // [gridDim.x]
     int64_t n, cols, chunks;   // chunks of kPerBlock samples
-    int32_t col_step;          // kThreads % cols
     int32_t data_type;
     uint32_t mantissa_mask;
     int32_t store_z;
@@ -140,35 +140,61 @@ __device__ inline Partial block_reduce(Partial p)
 
 // One chunk of kPerBlock samples: scale, accumulate, encode and store; the
 // thread's summary of its valid samples in *p (two passes over registers).
+// kPair: each access moves two adjacent samples (16-byte loads and stores of
+// z and prior, 8-byte stores of the encoding); otherwise one sample.
+template<bool kPair>
 __device__ inline void finish_chunk(const FinishArgs& a, int64_t chunk, Partial* p)
 {
-    const int64_t e0 = chunk * kPerBlock + threadIdx.x;
+    constexpr int P = kPair ? 2 : 1;
+    constexpr int J = kPerThread / P;
+    const int64_t e0 = chunk * kPerBlock + P * (int64_t)threadIdx.x;
     int64_t col = e0 % a.cols;
+    const int64_t step = (int64_t)(P * kThreads) % a.cols;
     float2 v[kPerThread];
     bool ok[kPerThread];
     int64_t cnt = 0;
     double sre = 0.0, sim = 0.0;
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-        const int64_t e = e0 + (int64_t)j * kThreads;
-        ok[j] = false;
-        v[j] = make_float2(0.f, 0.f);
-        if (e < a.n) {
-            float2 x = a.z[e];
-            if (a.range_cor) x = cmul_rounded(x, a.range_cor[col]);
+    for (int j = 0; j < J; ++j) {
+        const int64_t e = e0 + (int64_t)j * P * kThreads;
+        float2 x[P], q[P];
+        if (kPair && e + 1 < a.n) {
+            const float4 z4 = *reinterpret_cast<const float4*>(a.z + e);
+            x[0] = make_float2(z4.x, z4.y);
+            x[P - 1] = make_float2(z4.z, z4.w);
             if (a.prior) {
-                const float2 q = a.prior[e];
-                x = make_float2(x.x + q.x, x.y + q.y);
+                const float4 p4 = *reinterpret_cast<const float4*>(a.prior + e);
+                q[0] = make_float2(p4.x, p4.y);
+                q[P - 1] = make_float2(p4.z, p4.w);
             }
-            v[j] = x;
-            ok[j] = !(isnan(x.x) || isnan(x.y));
-            if (ok[j]) {
-                ++cnt;
-                sre += (double)x.x;
-                sim += (double)x.y;
+        } else {
+#pragma unroll
+            for (int u = 0; u < P; ++u) {
+                x[u] = e + u < a.n ? a.z[e + u] : make_float2(0.f, 0.f);
+                if (a.prior) q[u] = e + u < a.n ? a.prior[e + u] : make_float2(0.f, 0.f);
             }
         }
-        col += a.col_step;
+        int64_t cu = col;
+#pragma unroll
+        for (int u = 0; u < P; ++u) {
+            const int k = P * j + u;
+            ok[k] = false;
+            v[k] = make_float2(0.f, 0.f);
+            if (e + u < a.n) {
+                float2 y = x[u];
+                if (a.range_cor) y = cmul_rounded(y, a.range_cor[cu]);
+                if (a.prior) y = make_float2(y.x + q[u].x, y.y + q[u].y);
+                v[k] = y;
+                ok[k] = !(isnan(y.x) || isnan(y.y));
+                if (ok[k]) {
+                    ++cnt;
+                    sre += (double)y.x;
+                    sim += (double)y.y;
+                }
+            }
+            if (++cu == a.cols) cu = 0;
+        }
+        col += step;
         if (col >= a.cols) col -= a.cols;
     }
     *p = empty_partial();
@@ -177,42 +203,60 @@ __device__ inline void finish_chunk(const FinishArgs& a, int64_t chunk, Partial*
         p->re.mean = sre / (double)cnt;
         p->im.mean = sim / (double)cnt;
 #pragma unroll
-        for (int j = 0; j < kPerThread; ++j) {
-            if (!ok[j]) continue;
-            const double dr = (double)v[j].x - p->re.mean, di = (double)v[j].y - p->im.mean;
+        for (int k = 0; k < kPerThread; ++k) {
+            if (!ok[k]) continue;
+            const double dr = (double)v[k].x - p->re.mean, di = (double)v[k].y - p->im.mean;
             p->re.m2 += dr * dr;
             p->im.m2 += di * di;
-            p->re.lo = fminf(p->re.lo, v[j].x);
-            p->re.hi = fmaxf(p->re.hi, v[j].x);
-            p->im.lo = fminf(p->im.lo, v[j].y);
-            p->im.hi = fmaxf(p->im.hi, v[j].y);
+            p->re.lo = fminf(p->re.lo, v[k].x);
+            p->re.hi = fmaxf(p->re.hi, v[k].x);
+            p->im.lo = fminf(p->im.lo, v[k].y);
+            p->im.hi = fmaxf(p->im.hi, v[k].y);
         }
     }
     // stores: statistics are of the values before truncation
 #pragma unroll
-    for (int j = 0; j < kPerThread; ++j) {
-        const int64_t e = e0 + (int64_t)j * kThreads;
-        if (e >= a.n) continue;
-        float2 x = v[j];
-        if (a.data_type == BP_RSLC_COMPLEX32) {
-            a.encoded[e] = to_half_pair(x);
-        } else if (a.data_type == BP_RSLC_COMPLEX64_ZERO_MANTISSA) {
-            x.x = __uint_as_float(__float_as_uint(x.x) & a.mantissa_mask);
-            x.y = __uint_as_float(__float_as_uint(x.y) & a.mantissa_mask);
+    for (int j = 0; j < J; ++j) {
+        const int64_t e = e0 + (int64_t)j * P * kThreads;
+        float2 x[P];
+        uint32_t h[P];
+#pragma unroll
+        for (int u = 0; u < P; ++u) {
+            x[u] = v[P * j + u];
+            if (a.data_type == BP_RSLC_COMPLEX32) {
+                h[u] = to_half_pair(x[u]);
+            } else if (a.data_type == BP_RSLC_COMPLEX64_ZERO_MANTISSA) {
+                x[u].x = __uint_as_float(__float_as_uint(x[u].x) & a.mantissa_mask);
+                x[u].y = __uint_as_float(__float_as_uint(x[u].y) & a.mantissa_mask);
+            }
         }
-        if (a.store_z) a.z[e] = x;
+        if (kPair && e + 1 < a.n) {
+            if (a.data_type == BP_RSLC_COMPLEX32)
+                *reinterpret_cast<uint2*>(a.encoded + e) = make_uint2(h[0], h[P - 1]);
+            if (a.store_z)
+                *reinterpret_cast<float4*>(a.z + e) =
+                    make_float4(x[0].x, x[0].y, x[P - 1].x, x[P - 1].y);
+        } else {
+#pragma unroll
+            for (int u = 0; u < P; ++u) {
+                if (e + u >= a.n) continue;
+                if (a.data_type == BP_RSLC_COMPLEX32) a.encoded[e + u] = h[u];
+                if (a.store_z) a.z[e + u] = x[u];
+            }
+        }
     }
 }
 
 // Workgroup g takes chunks g, g + gridDim.x, ... (a fixed assignment, so the
 // merge order and the result do not depend on scheduling) and leaves one
 // partial summary.
+template<bool kPair>
 __global__ __launch_bounds__(kThreads) void finish_kernel(FinishArgs a)
 {
     Partial acc = empty_partial();
     for (int64_t chunk = blockIdx.x; chunk < a.chunks; chunk += gridDim.x) {
         Partial p;
-        finish_chunk(a, chunk, &p);
+        finish_chunk<kPair>(a, chunk, &p);
         merge(acc, p);
     }
     acc = block_reduce(acc);
@@ -347,7 +391,6 @@ int bp_rslc_finish_block(bp_rslc_finisher* f, void* z, int64_t rows, int64_t col
     a.partials = f->partials;
     a.n = n;
     a.cols = cols;
-    a.col_step = (int32_t)(rslc::kThreads % cols);
     a.data_type = data_type;
     a.mantissa_mask = mask;
     a.store_z = range_cor != nullptr || prior != nullptr ||
@@ -355,7 +398,14 @@ int bp_rslc_finish_block(bp_rslc_finisher* f, void* z, int64_t rows, int64_t col
     a.chunks = (n + rslc::kPerBlock - 1) / rslc::kPerBlock;
     const int blocks = (int)std::min<int64_t>(a.chunks, rslc::kMaxGrid);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(rslc::finish_kernel, dim3(blocks), dim3(rslc::kThreads), 0, s, a);
+    // two samples per access when every buffer is aligned for it
+    const bool pair = reinterpret_cast<uintptr_t>(z) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(prior) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(encoded) % 8 == 0;
+    if (pair)
+        hipLaunchKernelGGL(rslc::finish_kernel<true>, dim3(blocks), dim3(rslc::kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(rslc::finish_kernel<false>, dim3(blocks), dim3(rslc::kThreads), 0, s, a);
     hipLaunchKernelGGL(rslc::merge_kernel, dim3(1), dim3(rslc::kThreads), 0, s,
                        (const rslc::Partial*)f->partials, blocks, f->d_stats);
     hipError_t e = hipGetLastError();
