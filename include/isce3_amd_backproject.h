@@ -268,6 +268,33 @@ int bp_scene_destroy(bp_scene* scene);
 int bp_scene_focus(bp_scene* scene, bp_out_block* blocks, int n_blocks,
                    const bp_options* options, void* stream, int* any_failed, bp_stats* stats);
 
+/* ------------------------------------------------------------ serving
+ * A continuous-batching server over one scene: requests (one output block
+ * each) are queued by bp_serve_submit from any host thread; a worker thread
+ * takes every pending request that fits the scene (max_pixels, max_blocks),
+ * focuses them with one bp_scene_focus call on its own stream, completes
+ * their tickets and starts on the requests that arrived meanwhile.  The
+ * server owns the scene while it exists (no other calls on the scene).
+ * bp_server_destroy finishes every accepted request first.
+ *
+ * No reference counterpart: the tile-serving entry points of the build's
+ * serving workload (many tile requests against resident raw sets). */
+typedef struct bp_server bp_server;
+
+int bp_server_create(bp_scene* scene, const bp_options* options, int max_pending,
+                     bp_server** server);
+int bp_server_destroy(bp_server* server);
+/* Queue one block (its out_dev / height_dev device buffers must stay valid
+ * until the ticket completes).  BP_ERR_LENGTH when max_pending requests are
+ * waiting or the block alone exceeds the scene's max_pixels. */
+int bp_serve_submit(bp_server* server, const bp_out_block* block, uint64_t* ticket);
+/* Non-blocking: *done = 1 once the request has been focused (its result code
+ * returned, *failed = some pixel failed to converge), else 0.  A completed
+ * ticket is forgotten after it has been reported. */
+int bp_serve_poll(bp_server* server, uint64_t ticket, int* done, int* failed);
+/* Block until the request has been focused; returns its result code. */
+int bp_serve_wait(bp_server* server, uint64_t ticket, int* failed);
+
 /* ---------------------------------------------------- host-side helpers
  * Host implementations of the same FP64 primitives the kernels use; they back
  * the Python mirror's Orbit.interpolate, Ellipsoid and DEM/LUT lookups. */

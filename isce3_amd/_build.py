@@ -36,8 +36,8 @@ def _run(cmd):
 def build_hip():
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = [os.path.join(CSRC, f)
-            for f in ("bp_kernels.hip", "bp_api.hip", "rc_kernels.hip", "rc_api.hip",
-                      "rslc.hip", "presum.hip")]
+            for f in ("bp_kernels.hip", "bp_api.hip", "bp_serve.hip", "rc_kernels.hip",
+                      "rc_api.hip", "rslc.hip", "presum.hip")]
     out = os.path.join(LIBDIR, "libisce3_amd.so")
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-o", tmp, *srcs])

@@ -128,6 +128,11 @@ _SIGNATURES = {
     "bp_scene_destroy": (C.c_int, [C.c_void_p]),
     "bp_scene_focus": (C.c_int, [C.c_void_p, _P(OutBlock_t), C.c_int, _P(Options_t), C.c_void_p,
                                  _P(C.c_int), _P(Stats_t)]),
+    "bp_server_create": (C.c_int, [C.c_void_p, _P(Options_t), C.c_int, _P(C.c_void_p)]),
+    "bp_server_destroy": (C.c_int, [C.c_void_p]),
+    "bp_serve_submit": (C.c_int, [C.c_void_p, _P(OutBlock_t), _P(C.c_uint64)]),
+    "bp_serve_poll": (C.c_int, [C.c_void_p, C.c_uint64, _P(C.c_int), _P(C.c_int)]),
+    "bp_serve_wait": (C.c_int, [C.c_void_p, C.c_uint64, _P(C.c_int)]),
     "bp_rangecomp_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                       _P(C.c_void_p)]),
     "bp_rangecomp_destroy": (C.c_int, [C.c_void_p]),

@@ -1023,6 +1023,16 @@ int bp_scene_destroy(bp_scene* sc)
     return BP_SUCCESS;
 }
 
+}  // extern "C"
+
+namespace bp {
+int scene_device(const bp_scene* s) { return s->ctx->device; }
+int64_t scene_max_pixels(const bp_scene* s) { return s->max_pixels; }
+int scene_max_blocks(const bp_scene* s) { return s->max_blocks; }
+}  // namespace bp
+
+extern "C" {
+
 int bp_scene_focus(bp_scene* sc, bp_out_block* blocks, int n_blocks, const bp_options* options,
                    void* stream, int* any_failed, bp_stats* stats)
 {

@@ -174,4 +174,9 @@ int sum_blocks_per_cu(int precision, int width, bool staged, bool pair);
 // as graph kernel nodes report it.
 const void* kernel_fn(int which, int width, bool staged, bool pair);
 
+// Scene facts the serving queue needs (bp_api.hip).
+int scene_device(const struct bp_scene* scene);
+int64_t scene_max_pixels(const struct bp_scene* scene);
+int scene_max_blocks(const struct bp_scene* scene);
+
 }  // namespace bp

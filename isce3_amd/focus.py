@@ -518,6 +518,91 @@ class Scene:
         self.close()
 
 
+class TileServer:
+    """Continuous batching of tile requests against one :class:`Scene`
+    (bp_server_* / bp_serve_* in include/isce3_amd_backproject.h): a native
+    worker thread focuses every pending request that fits the scene in one
+    step and starts on the requests that arrived meanwhile.  ``submit``
+    returns a ticket at once; ``poll`` / ``wait`` report its completion (the
+    per-request failure flag).  The server owns the scene until
+    :meth:`close`, which finishes every accepted request first."""
+
+    def __init__(self, scene: Scene, options: Optional[dict] = None, max_pending: int = 1024):
+        self._h = None
+        if scene._h is None:
+            raise errors.RuntimeError_("scene is closed")
+        self.scene = scene
+        opt = _options_struct(options)
+        h = C.c_void_p()
+        _native.check(_native.lib().bp_server_create(scene._h, C.byref(opt), int(max_pending),
+                                                     C.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, out, grid, height=None) -> int:
+        """Queue one output block (``out`` a complex64 device tensor of the
+        grid's shape, kept alive until the ticket completes)."""
+        if self._h is None:
+            raise errors.RuntimeError_("server is closed")
+        torch = _torch()
+        if not _is_torch(out) or not out.is_cuda or out.dtype != torch.complex64 \
+                or not out.is_contiguous() or _shape_of(out) != (grid.length, grid.width):
+            raise errors.InvalidArgument(
+                "outputs must be C-contiguous complex64 device tensors of the grid shape")
+        if height is not None and (not _is_torch(height) or height.dtype != torch.float32 or
+                                   not height.is_contiguous() or
+                                   _shape_of(height) != (grid.length, grid.width)):
+            raise errors.InvalidArgument(
+                "heights must be C-contiguous float32 device tensors of the grid shape")
+        blk = _native.OutBlock_t()
+        blk.grid = self.scene._grid_struct(grid)
+        blk.out_dev = out.data_ptr()
+        blk.height_dev = height.data_ptr() if height is not None else None
+        # the worker runs on its own stream: work queued on the caller's
+        # stream (e.g. the buffers' allocation fills) completes first
+        with torch.cuda.device(self.scene.device):
+            torch.cuda.current_stream().synchronize()
+        t = C.c_uint64()
+        _native.check(_native.lib().bp_serve_submit(self._h, C.byref(blk), C.byref(t)))
+        self._keep[t.value] = (out, height)
+        return int(t.value)
+
+    def poll(self, ticket: int):
+        """None while pending, else the request's failure flag."""
+        done, failed = C.c_int(0), C.c_int(0)
+        _native.check(_native.lib().bp_serve_poll(self._h, C.c_uint64(ticket), C.byref(done),
+                                                  C.byref(failed)))
+        if not done.value:
+            return None
+        self._keep.pop(ticket, None)
+        return bool(failed.value)
+
+    def wait(self, ticket: int) -> bool:
+        """Block until the request is focused; returns its failure flag."""
+        failed = C.c_int(0)
+        _native.check(_native.lib().bp_serve_wait(self._h, C.c_uint64(ticket), C.byref(failed)))
+        self._keep.pop(ticket, None)
+        return bool(failed.value)
+
+    def close(self):
+        if self._h is not None:
+            _native.lib().bp_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
 # range compression, the step before backprojection (isce3.focus.RangeComp,
 # isce3.focus.form_linear_chirp)
 from .rangecomp import RangeComp, form_linear_chirp  # noqa: E402,F401

@@ -1,0 +1,80 @@
+"""The native tile server (bp_server_* / bp_serve_*): requests submitted from
+several host threads, batched by the worker as they arrive, give every block
+the result of bp_backproject_blocks bit for bit; closing the server finishes
+every accepted request; oversized requests and unknown tickets are
+rejected."""
+import threading
+
+import numpy as np
+import pytest
+
+from helpers import scene_and_raw
+from test_gpu_blocks import uneven_blocks
+from test_gpu_scene import eager
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["plain", "split"])
+def test_concurrent_requests_match_eager_blocks(torch_gpu, precision):
+    torch = torch_gpu
+    from isce3_amd.focus import Scene, TileServer
+    sc, raw = scene_and_raw("small")
+    raw_dev = torch.from_numpy(raw.copy()).cuda()
+    grids = uneven_blocks(sc)
+    opts = {"tile_rows": 2}
+    ref = eager(sc, raw_dev, grids, precision, opts)
+    scene = Scene(raw_dev, sc.in_geometry(), sc.orbit, sc.doppler, sc.dem, sc.fc, sc.ds,
+                  sc.kernel, sc.dry_tropo_model, precision=precision, max_pixels=1 << 15,
+                  max_blocks=4)
+    outs = [torch.full(tuple(g.shape), 7 + 7j, dtype=torch.complex64, device="cuda:0")
+            for g in grids]
+    hs = [torch.zeros(tuple(g.shape), dtype=torch.float32, device="cuda:0") for g in grids]
+    flags = [None] * len(grids)
+    with TileServer(scene, options=opts) as server:
+        def client(idx):
+            tickets = [(i, server.submit(outs[i], grids[i], hs[i])) for i in idx]
+            for i, t in tickets:
+                flags[i] = server.wait(t)
+        threads = [threading.Thread(target=client, args=(list(range(k, len(grids), 3)),))
+                   for k in range(3)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+    scene.close()
+    print(f"[tileserver] precision={precision} blocks={len(grids)}", flush=True)
+    torch.cuda.synchronize()
+    for o, h, z0, h0 in zip(outs, hs, ref[0], ref[1]):
+        np.testing.assert_array_equal(o.cpu().numpy(), z0)
+        np.testing.assert_array_equal(h.cpu().numpy(), h0)
+    assert flags == ref[2]
+
+
+def test_close_drains_and_rejections(torch_gpu):
+    torch = torch_gpu
+    from isce3_amd import errors
+    from isce3_amd.focus import Scene, TileServer
+    sc, raw = scene_and_raw("small")
+    raw_dev = torch.from_numpy(raw.copy()).cuda()
+    grids = uneven_blocks(sc)
+    ref = eager(sc, raw_dev, grids, "split", None)
+    scene = Scene(raw_dev, sc.in_geometry(), sc.orbit, sc.doppler, sc.dem, sc.fc, sc.ds,
+                  sc.kernel, sc.dry_tropo_model, precision="split", max_pixels=1 << 15,
+                  max_blocks=8)
+    server = TileServer(scene, max_pending=64)
+    outs = [torch.zeros(tuple(g.shape), dtype=torch.complex64, device="cuda:0") for g in grids]
+    tickets = [server.submit(o, g) for o, g in zip(outs, grids)]
+    big = sc.out_grid
+    with pytest.raises(errors.LengthError):
+        server.submit(torch.zeros(tuple(big.shape), dtype=torch.complex64, device="cuda:0"), big)
+    # poll until the first request completes, then close with the rest in flight
+    while server.poll(tickets[0]) is None:
+        pass
+    with pytest.raises(errors.InvalidArgument):
+        server.poll(tickets[0])          # reported once, then forgotten
+    server.close()
+    scene.close()
+    torch.cuda.synchronize()
+    for o, z0 in zip(outs, ref[0]):
+        np.testing.assert_array_equal(o.cpu().numpy(), z0)
