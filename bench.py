@@ -346,9 +346,10 @@ def run_serve(args, torch, np, dev, tuner):
 # ------------------------------------------------------- range compression
 def run_rangecomp(args, torch, np, dev):
     """Range compression of 16384 pulses x 15000 samples (complex64) with a
-    961-sample linear chirp, "same" mode, inputs resident in HBM: one fused
-    FFT-convolution launch (2^14-point transforms).  HBM-bound: the roofline
-    counts the pulses read and the compressed pulses written."""
+    961-sample linear chirp, "same" mode, inputs resident in HBM: one
+    FFT-convolution launch (the compressor's plan: overlap-save blocks of
+    2^12 points for this shape).  HBM-bound: the roofline counts the pulses
+    read and the compressed pulses written."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import rangecomp as rco
     from isce3_amd.focus import RangeComp, form_linear_chirp
@@ -383,8 +384,9 @@ def run_rangecomp(args, torch, np, dev):
     rco.rangecompress(chirp, xb, rco.SAME, "fp32")
     cpu_s = time.perf_counter() - t0
     res = {"workload": f"{na} pulses x {nr} samples c64, {len(chirp)}-sample chirp, same mode",
-           "fft_size": rc.fft_size, "ms": ms, "pulses_per_s": na / (ms * 1e-3),
-           "kernel": f"rc::conv_kernel<{int(np.log2(rc.fft_size))}>",
+           "fft_size": rc.fft_size, "plan": list(rc.plan), "ms": ms,
+           "pulses_per_s": na / (ms * 1e-3),
+           "kernel": f"rc::conv_kernel<{int(np.log2(rc.plan[1]))}>",
            "roofline": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": 8000.0,
                         "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 8e12,
                         "traffic": None},

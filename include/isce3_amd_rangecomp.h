@@ -10,12 +10,15 @@
  * formed in the frequency domain with an FFT of length fft_size, the
  * smallest power of two >= max(1024, chirp_size + input_size - 1) (the
  * reference uses the smallest 2^a 3^b 5^c; both give the same linear
- * convolution).  Up to fft_size 2^14 one fused transform per pulse; above
- * that, chirps of at most 2^13 + 1 samples run as overlap-save blocks of 2^14
- * (any pulse length), longer chirps as three four-step passes through a
- * device work buffer of at most 512 MiB that the compressor allocates
- * (fft_size <= 2^19; a run of more pulses than fit goes through it in
- * sub-batches on the stream).
+ * convolution).  The compressor runs each pulse as one fused transform of
+ * fft_size points (fft_size <= 2^14) or as overlap-save blocks of 2^b points,
+ * 10 <= b <= 14, for chirps of at most 2^(b-1) + 1 samples (any pulse
+ * length), whichever its cost model of the kernel rates fastest; chirps longer
+ * than 2^13 + 1 samples with fft_size above 2^14 run as three four-step
+ * passes through a device work buffer of at most 512 MiB that the compressor
+ * allocates (fft_size <= 2^19; a run of more pulses than fit goes through it
+ * in sub-batches on the stream).  All plans give the same convolution to
+ * FP32 rounding.
  *
  * Return codes as in isce3_amd_backproject.h; bp_last_error_message()
  * returns the message of the calling thread's last failure.
@@ -44,6 +47,12 @@ int bp_rangecomp_destroy(bp_rangecomp* rc);
 /* chirpSize, inputSize, fftSize, maxBatch, mode, outputSize, firstValidSample
  * (RangeComp.h:57-91) into sizes[0..6]. */
 int bp_rangecomp_sizes(const bp_rangecomp* rc, int32_t* sizes);
+
+/* The compressor's plan (no counterpart in the reference; for tests and
+ * tuning): plan[0] = 0 one fused transform per pulse, 1 overlap-save blocks,
+ * 2 four steps; plan[1] = transform length; plan[2] = transforms per pulse
+ * (1 for four steps). */
+int bp_rangecomp_plan(const bp_rangecomp* rc, int32_t* plan);
 
 /* RangeComp::rangecompress (RangeComp.cpp:93-131): in_dev device complex64
  * [batch x input_size], out_dev device complex64 [batch x output_size].

@@ -137,6 +137,7 @@ _SIGNATURES = {
                                       _P(C.c_void_p)]),
     "bp_rangecomp_destroy": (C.c_int, [C.c_void_p]),
     "bp_rangecomp_sizes": (C.c_int, [C.c_void_p, _P(C.c_int32)]),
+    "bp_rangecomp_plan": (C.c_int, [C.c_void_p, _P(C.c_int32)]),
     "bp_rangecomp_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "bp_rslc_finisher_create": (C.c_int, [C.c_int, C.c_int64, _P(C.c_void_p)]),
     "bp_rslc_finisher_destroy": (C.c_int, [C.c_void_p]),

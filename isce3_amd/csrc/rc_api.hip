@@ -2,9 +2,10 @@
 // of cxx/isce3/focus/RangeComp.cpp:39-79 and :93-99, the matched filter's
 // spectrum formed on the host in FP64 (RangeComp.cpp:22-37), and the launch
 // of the FFT-convolution kernels (rc_kernels.hip): one fused transform per
-// pulse for convolutions up to 2^14 samples, overlap-save blocks of 2^14 for
-// longer pulses with chirps of at most 2^13 + 1 samples, and three four-step
-// passes through a work buffer for long chirps (transforms up to 2^19).
+// pulse, or overlap-save blocks of 2^10 .. 2^14 points, whichever a cost
+// model of the kernel on MI355X rates fastest, and three four-step passes
+// through a work buffer for convolutions past 2^14 samples with chirps longer
+// than 2^13 + 1 samples (transforms up to 2^19).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -70,6 +71,57 @@ void fft_host(std::vector<std::complex<double>>& x)
     }
 }
 
+// How a compressor runs its pulses: one fused transform of 2^tlog2 points per
+// pulse (method 0), overlap-save blocks of 2^tlog2 points (method 1, nblk per
+// pulse), or the four-step passes (method 2).
+struct Plan {
+    int method, tlog2, nblk;
+};
+
+// Modelled cost of the fused kernel on MI355X, measured over 16384-pulse
+// batches: picoseconds per transform point (forward and inverse transform and
+// spectral product) for 2^10 .. 2^14-point transforms -- the smaller
+// transforms run several workgroups per CU and hide their barriers and LDS
+// latency -- and the rate at which the kernel streams pulses in and out when
+// the transforms are cheap enough (bytes per picosecond = TB/s).
+constexpr double kPointPs[5] = {2.9, 3.0, 3.4, 4.1, 5.1};
+constexpr double kStreamBytesPerPs = 4.1;
+
+// Overlap-save blocks of 2^b points covering full-convolution samples
+// [offset, offset + n_out) of a chirp of m samples.
+int ols_blocks(int b, int m, int offset, int n_out)
+{
+    const int valid = (1 << b) - (m - 1);
+    return (offset + n_out - 1) / valid - offset / valid + 1;
+}
+
+// The plan of least modelled time per pulse: the larger of the time to stream
+// the pulse in and its output out, and the transform work.  Overlap-save
+// blocks of 2^b points need a chirp of at most 2^(b-1) + 1 samples; chirps
+// longer than 2^13 + 1 samples with convolutions past 2^14 take four steps.
+Plan plan_transform(int chirp_size, int input_size, int mode, int log2l)
+{
+    const int n_out = output_size(chirp_size, input_size, mode);
+    const int offset = mode == BP_RC_FULL ? 0 : (mode == BP_RC_VALID ? chirp_size - 1 : chirp_size / 2);
+    const double stream_ps = 8.0 * ((double)input_size + n_out) / kStreamBytesPerPs;
+    Plan best{2, log2l, 1};
+    double best_t = 0., best_pts = 0.;
+    auto consider = [&](int method, int b, int nblk) {
+        const double pts = (double)nblk * (double)(1 << b);
+        const double t = std::max(stream_ps, pts * kPointPs[b - rc::kMinLog2]);
+        if (best.method == 2 || t < best_t || (t == best_t && pts < best_pts)) {
+            best = Plan{method, b, nblk};
+            best_t = t;
+            best_pts = pts;
+        }
+    };
+    if (log2l <= rc::kMaxFusedLog2) consider(0, log2l, 1);
+    for (int b = rc::kMinLog2; b <= std::min(rc::kMaxFusedLog2, log2l - 1); ++b)
+        if (chirp_size - 1 <= (1 << (b - 1)))
+            consider(1, b, ols_blocks(b, chirp_size, offset, n_out));
+    return best;
+}
+
 }  // namespace
 
 struct bp_rangecomp {
@@ -77,6 +129,7 @@ struct bp_rangecomp {
     int chirp_size = 0, input_size = 0, max_batch = 0, mode = 0;
     int fft_size = 0, log2l = 0, out_size = 0, offset = 0;
     int method = 0;         // 0 one transform per pulse, 1 overlap-save blocks, 2 four steps
+    int nblk = 1;           // transforms per pulse (methods 0 and 1)
     float2* H = nullptr;    // device [L] filter spectrum, L the transform length (four steps: permuted)
     float2* tw = nullptr;   // device [L]
     float2* tw2 = nullptr;  // device [L / 32] (four steps)
@@ -107,23 +160,22 @@ int bp_rangecomp_create(int device, const float* chirp, int chirp_size, int inpu
     const int64_t full = (int64_t)chirp_size + input_size - 1;
     int log2l = rc::kMinLog2;
     while (log2l < 31 && ((int64_t)1 << log2l) < full) ++log2l;
-    // one fused transform per pulse; overlap-save blocks of 2^14 for longer
-    // pulses with chirps up to 2^13 + 1 samples; four steps otherwise
-    const int method = log2l <= rc::kMaxFusedLog2 ? 0
-                       : (chirp_size - 1 <= rc::kOlsMaxShift ? 1 : 2);
+    if (log2l >= 31) return fail(BP_ERR_INVALID_ARGUMENT, "pulse too long");
+    const Plan plan = plan_transform(chirp_size, input_size, mode, log2l);
+    const int method = plan.method;
+    const int tlog2 = plan.tlog2;   // log2 of the transform length
     if (method == 2 && log2l > rc::kMaxLog2)
         return fail(BP_ERR_INVALID_ARGUMENT,
                     "range compression of " + std::to_string(full) + " convolution samples with a " +
                     std::to_string(chirp_size) + "-sample chirp exceeds the supported FFT length " +
                     std::to_string(1 << rc::kMaxLog2));
-    if (log2l >= 31) return fail(BP_ERR_INVALID_ARGUMENT, "pulse too long");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(BP_ERR_HIP, "no HIP device available (isce3_amd needs an MI355X GPU)");
     if (device < 0 || device >= ndev) return fail(BP_ERR_INVALID_ARGUMENT, "invalid device index");
     if (hipSetDevice(device) != hipSuccess) return fail(BP_ERR_HIP, "hipSetDevice failed");
 
-    const int L = method == 1 ? 1 << rc::kOlsLog2 : 1 << log2l;   // transform length
+    const int L = 1 << tlog2;
     // matched filter: time-reversed conjugate of the chirp, zero padded to L,
     // transformed, scaled by 1/L (RangeComp.cpp:22-37, :108-115)
     std::vector<std::complex<double>> h(L, 0.0);
@@ -153,8 +205,9 @@ int bp_rangecomp_create(int device, const float* chirp, int chirp_size, int inpu
     r->max_batch = max_batch;
     r->mode = mode;
     r->fft_size = 1 << log2l;
-    r->log2l = method == 1 ? rc::kOlsLog2 : log2l;
+    r->log2l = tlog2;
     r->method = method;
+    r->nblk = plan.nblk;
     r->out_size = output_size(chirp_size, input_size, mode);
     r->offset = mode == BP_RC_FULL ? 0 : (mode == BP_RC_VALID ? chirp_size - 1 : chirp_size / 2);
     if (four_step)
@@ -204,6 +257,15 @@ int bp_rangecomp_sizes(const bp_rangecomp* rc, int32_t* sizes)
     return BP_SUCCESS;
 }
 
+int bp_rangecomp_plan(const bp_rangecomp* rc, int32_t* plan)
+{
+    if (rc == nullptr || plan == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    plan[0] = rc->method;
+    plan[1] = 1 << rc->log2l;
+    plan[2] = rc->method == 2 ? 1 : rc->nblk;
+    return BP_SUCCESS;
+}
+
 int bp_rangecomp_run(bp_rangecomp* rc, void* out_dev, const void* in_dev, int batch,
                      void* stream)
 {
@@ -235,10 +297,10 @@ int bp_rangecomp_run(bp_rangecomp* rc, void* out_dev, const void* in_dev, int ba
     } else if (rc->method == 1) {
         // blocks covering full-convolution samples [offset, offset + n_out)
         a.shift = rc->chirp_size - 1;
-        a.valid = (1 << rc::kOlsLog2) - a.shift;
+        a.valid = (1 << rc->log2l) - a.shift;
         a.blk0 = a.offset / a.valid;
         a.nblk = (a.offset + a.n_out - 1) / a.valid - a.blk0 + 1;
-        e = rc::launch_conv(a, rc::kOlsLog2, batch, s);
+        e = rc::launch_conv(a, rc->log2l, batch, s);
     } else {
         // sub-batches of the work buffer, launched in order on one stream
         for (int b0 = 0; b0 < batch && e == hipSuccess; b0 += rc->work_batch) {

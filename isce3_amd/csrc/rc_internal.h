@@ -27,8 +27,6 @@ struct ConvArgs {
     int32_t shift, valid, nblk, blk0;
 };
 
-constexpr int kOlsLog2 = 14;             // overlap-save block length 2^14
-constexpr int kOlsMaxShift = 1 << 13;    // chirps up to 2^13 + 1 samples
 
 hipError_t launch_conv(const ConvArgs& a, int log2l, int batch, hipStream_t stream);
 hipError_t launch_four_step(const ConvArgs& a, int log2l, int batch, hipStream_t stream);

@@ -77,6 +77,9 @@ class RangeComp:
         (self._chirp_size, self._input_size, self._fft_size, self._maxbatch, m,
          self._output_size, self._first_valid) = list(sizes)
         self._mode = Mode(m)
+        plan = (C.c_int32 * 3)()
+        _native.check(_native.lib().bp_rangecomp_plan(h, plan))
+        self._plan = (("fused", "overlap-save", "four-step")[plan[0]], int(plan[1]), int(plan[2]))
 
     chirp_size = property(lambda self: self._chirp_size)
     input_size = property(lambda self: self._input_size)
@@ -85,6 +88,9 @@ class RangeComp:
     mode = property(lambda self: self._mode)
     output_size = property(lambda self: self._output_size)
     first_valid_sample = property(lambda self: self._first_valid)
+    #: how the pulses run: (kind, transform length, transforms per pulse),
+    #: kind "fused" (one transform per pulse), "overlap-save" or "four-step"
+    plan = property(lambda self: self._plan)
 
     def rangecompress(self, out, in_, stream=None) -> None:
         """Compress a batch of pulses: ``in_`` of shape (batch, input_size) or

@@ -2,7 +2,8 @@
 cxx/isce3/focus/RangeComp.cpp) and the reference's own known answers
 (tests/cxx/isce3/focus/rangecomp.cpp, tests/python/extensions/pybind/focus/
 rangecomp.py).  Tolerance: complex relative error <= 1e-5 against the FP64
-oracle (FP32 FFTs of up to 2^19 points, or overlap-save blocks of 2^14)."""
+oracle (FP32 FFTs of up to 2^19 points, or overlap-save blocks of 2^10 to
+2^14 points)."""
 import numpy as np
 import pytest
 
@@ -116,8 +117,9 @@ def test_device_tensors_in_place_and_batch_limit(torch_gpu):
                                                 (60000, 2, 341.3e-6)])
 def test_long_pulses_overlap_save_match_fp64_oracle(torch_gpu, mode, nin, batch, duration):
     """Convolutions past 2^14 samples with chirps of at most 2^13 + 1 samples
-    (overlap-save blocks of 2^14; 961 samples and 8191 samples, next to the
-    2^13 + 1 limit), including a pulse longer than any single transform."""
+    (overlap-save blocks; 961 samples and 8191 samples, next to the 2^13 + 1
+    limit of 2^14-point blocks), including a pulse longer than any single
+    transform."""
     from isce3_amd.focus import RangeComp, form_linear_chirp
     rng = np.random.default_rng(nin + 7 * batch)
     chirp = form_linear_chirp(0.6e12, duration, 24e6)
@@ -132,6 +134,33 @@ def test_long_pulses_overlap_save_match_fp64_oracle(torch_gpu, mode, nin, batch,
     print(f"[rangecomp long] mode={mode} nin={nin} batch={batch} fft={rc.fft_size} "
           f"rel={e:.2e}", flush=True)
     assert y.shape == ref.shape
+    assert e <= REL
+
+
+@pytest.mark.parametrize("m,nin,kind,length,nblk", [
+    (65, 2000, "overlap-save", 1 << 10, 3), (481, 12000, "overlap-save", 1 << 11, 8),
+    (961, 15000, "overlap-save", 1 << 12, 5), (2001, 15000, "overlap-save", 1 << 13, 3),
+    (8193, 60000, "overlap-save", 1 << 14, 8), (481, 3000, "fused", 1 << 12, 1),
+    (4801, 4801, "fused", 1 << 14, 1), (9601, 9601, "four-step", 1 << 15, 1)])
+def test_every_transform_plan_matches_fp64_oracle(torch_gpu, m, nin, kind, length, nblk):
+    """The compressor's cost model picks one fused transform per pulse,
+    overlap-save blocks of 2^10 .. 2^14 points, or four steps; each plan (the
+    block seams included) gives the FP64 oracle's "same" convolution, and the
+    reported fft_size stays the smallest power of two that holds the full
+    convolution."""
+    from isce3_amd.focus import RangeComp
+    rng = np.random.default_rng(m + nin)
+    chirp = np.exp(1j * np.pi * 3e-5 * (np.arange(m) - m / 2) ** 2).astype(np.complex64)
+    x = (rng.standard_normal((3, nin)) + 1j * rng.standard_normal((3, nin))).astype(np.complex64)
+    rc = RangeComp(chirp, nin, maxbatch=3, mode=RangeComp.Mode.Same)
+    print(f"[rangecomp plan] m={m} nin={nin} plan={rc.plan}", flush=True)
+    assert rc.plan == (kind, length, nblk)
+    assert rc.fft_size == 1 << int(np.ceil(np.log2(max(1024, m + nin - 1))))
+    y = np.zeros((3, rc.output_size), np.complex64)
+    rc.rangecompress(y, x)
+    ref = rco.rangecompress(chirp, x, rco.SAME, "fp64")
+    e = rel_err(y, ref)
+    print(f"[rangecomp plan] rel={e:.2e}", flush=True)
     assert e <= REL
 
 
