@@ -83,9 +83,12 @@ struct Plan {
 // spectral product) for 2^10 .. 2^14-point transforms -- the smaller
 // transforms run several workgroups per CU and hide their barriers and LDS
 // latency -- and the rate at which the kernel streams pulses in and out when
-// the transforms are cheap enough (bytes per picosecond = TB/s).
-constexpr double kPointPs[5] = {2.9, 3.0, 3.4, 4.1, 5.1};
+// the transforms are cheap enough (bytes per picosecond = TB/s).  The two
+// overlap imperfectly: a plan costs the larger of its streaming and transform
+// times plus kOverlapLoss of the smaller.
+constexpr double kPointPs[5] = {3.1, 3.0, 3.4, 4.1, 5.1};
 constexpr double kStreamBytesPerPs = 4.1;
+constexpr double kOverlapLoss = 0.15;
 
 // Overlap-save blocks of 2^b points covering full-convolution samples
 // [offset, offset + n_out) of a chirp of m samples.
@@ -95,8 +98,8 @@ int ols_blocks(int b, int m, int offset, int n_out)
     return (offset + n_out - 1) / valid - offset / valid + 1;
 }
 
-// The plan of least modelled time per pulse: the larger of the time to stream
-// the pulse in and its output out, and the transform work.  Overlap-save
+// The plan of least modelled time per pulse (streaming the pulse in and its
+// output out against the transform work).  Overlap-save
 // blocks of 2^b points need a chirp of at most 2^(b-1) + 1 samples; chirps
 // longer than 2^13 + 1 samples with convolutions past 2^14 take four steps.
 Plan plan_transform(int chirp_size, int input_size, int mode, int log2l)
@@ -108,7 +111,8 @@ Plan plan_transform(int chirp_size, int input_size, int mode, int log2l)
     double best_t = 0., best_pts = 0.;
     auto consider = [&](int method, int b, int nblk) {
         const double pts = (double)nblk * (double)(1 << b);
-        const double t = std::max(stream_ps, pts * kPointPs[b - rc::kMinLog2]);
+        const double work_ps = pts * kPointPs[b - rc::kMinLog2];
+        const double t = std::max(stream_ps, work_ps) + kOverlapLoss * std::min(stream_ps, work_ps);
         if (best.method == 2 || t < best_t || (t == best_t && pts < best_pts)) {
             best = Plan{method, b, nblk};
             best_t = t;
