@@ -9,18 +9,27 @@
 // (taken from the sub-swath boundaries, which is where the pattern of a range
 // bin can change) with one weight vector per pattern.
 //
-// Device side: one thread per range bin and chunk of output pulses rebuilds
-// the bin's pattern from the sub-swath table, picks its weights, evaluates
-// the Doppler LUT at (tout, r) and accumulates
+// Device side: one thread per range bin and 16 output pulses rebuilds the
+// bin's pattern from the sub-swath table, picks its weights, evaluates the
+// Doppler LUT at (tout, r) and accumulates
 //   sum_k float(w_k) * exp(-2 pi i (t_k - tout) fd) * raw[k][r]
 // in complex64 in pulse order, as nisar/workflows/focus.py:1011-1061 and
-// pybind focus/Presum.cpp:26-66 do.  Consecutive output pulses share most of
-// their input pulses, so a thread's chunk re-reads them from cache.
+// pybind focus/Presum.cpp:26-66 do (the deramp from the phase in turns,
+// reduced in FP64, by the hardware sine and cosine, where the reference
+// rounds the phase in radians to float and calls sinf/cosf).  Consecutive
+// output pulses share most of their input pulses, so a thread's pulses
+// re-read them from cache.
+//
+// The output pulses go in launches of 2048: host threads plan the next
+// launches' patterns and weights while the GPU resamples the earlier ones.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -54,20 +63,30 @@ double acorr_eval(const bp_acorr& k, double t)
     return x * (x * (-0.25 * x + 1.5) - 3.0) + 2.0;
 }
 
-// Solve G w = r for symmetric positive semi-definite G (lower triangle
-// used) by LDL^T with diagonal pivoting: at step k the largest remaining
-// diagonal entry is swapped to k, column k of L is formed from the earlier
-// columns (left-looking), and pivots not above the smallest normal number
-// are treated as zero (their solution component is zero).
-void ldlt_solve(std::vector<double>& g, int n, std::vector<double>& r, std::vector<double>& w)
+// Host buffers of the weight solves, reused from solve to solve.
+struct SolveScratch {
+    std::vector<double> g, r, d, tmp, y;
+    std::vector<int> perm;
+};
+
+// Solve G w = r (G = s.g, n x n, lower triangle used; r = s.r) for symmetric
+// positive semi-definite G by LDL^T with diagonal pivoting: at step k the
+// largest remaining diagonal entry is swapped to k, column k of L is formed
+// from the earlier columns (left-looking), and pivots not above the smallest
+// normal number are treated as zero (their solution component is zero).
+void ldlt_solve(SolveScratch& s, int n, std::vector<double>& w)
 {
+    std::vector<double>& g = s.g;
     auto G = [&](int i, int j) -> double& { return g[(size_t)i * n + j]; };
-    std::vector<int> perm(n);
-    for (int i = 0; i < n; ++i) perm[i] = i;
+    s.perm.resize(n);
+    for (int i = 0; i < n; ++i) s.perm[i] = i;
     // make the matrix full-symmetric for row/column swaps
     for (int i = 0; i < n; ++i)
         for (int j = i + 1; j < n; ++j) G(i, j) = G(j, i);
-    std::vector<double> d(n, 0.0), tmp(n);
+    s.d.assign(n, 0.0);
+    s.tmp.resize(n);
+    std::vector<double>& d = s.d;
+    std::vector<double>& tmp = s.tmp;
     const double cutoff = std::numeric_limits<double>::min();
     for (int k = 0; k < n; ++k) {
         // pivot: largest |diagonal| among k..n-1 (first on ties)
@@ -77,7 +96,7 @@ void ldlt_solve(std::vector<double>& g, int n, std::vector<double>& r, std::vect
         if (piv != k) {
             for (int j = 0; j < n; ++j) std::swap(G(k, j), G(piv, j));
             for (int i = 0; i < n; ++i) std::swap(G(i, k), G(i, piv));
-            std::swap(perm[k], perm[piv]);
+            std::swap(s.perm[k], s.perm[piv]);
         }
         // D_k = G_kk - sum_j L_kj^2 D_j ; L_ik = (G_ik - sum_j L_ij L_kj D_j) / D_k
         for (int j = 0; j < k; ++j) tmp[j] = G(k, j) * d[j];
@@ -85,47 +104,77 @@ void ldlt_solve(std::vector<double>& g, int n, std::vector<double>& r, std::vect
         for (int j = 0; j < k; ++j) dk -= G(k, j) * tmp[j];
         d[k] = dk;
         for (int i = k + 1; i < n; ++i) {
-            double s = G(i, k);
-            for (int j = 0; j < k; ++j) s -= G(i, j) * tmp[j];
-            G(i, k) = std::fabs(dk) > cutoff ? s / dk : s;
+            double v = G(i, k);
+            for (int j = 0; j < k; ++j) v -= G(i, j) * tmp[j];
+            G(i, k) = std::fabs(dk) > cutoff ? v / dk : v;
         }
     }
     // P b, L y = P b, D z = y, L^T x = z, w = P^T x
-    std::vector<double> y(n);
-    for (int i = 0; i < n; ++i) y[i] = r[perm[i]];
+    std::vector<double>& y = s.y;
+    y.resize(n);
+    for (int i = 0; i < n; ++i) y[i] = s.r[s.perm[i]];
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < i; ++j) y[i] -= G(i, j) * y[j];
     for (int i = 0; i < n; ++i) y[i] = std::fabs(d[i]) > cutoff ? y[i] / d[i] : 0.0;
     for (int i = n - 1; i >= 0; --i)
         for (int j = i + 1; j < n; ++j) y[i] -= G(j, i) * y[j];
     w.assign(n, 0.0);
-    for (int i = 0; i < n; ++i) w[perm[i]] = y[i];
+    for (int i = 0; i < n; ++i) w[s.perm[i]] = y[i];
 }
 
-// getPresumWeights: window [offset, offset + nw) and its weights
-void presum_weights(const bp_acorr& acorr, const double* t, int64_t n, double tout,
-                    int64_t* offset, std::vector<double>& w)
+// getPresumWeights' window: the samples within half the kernel width of tout,
+// [*offset, *offset + return value).
+int64_t presum_window(const bp_acorr& acorr, const double* t, int64_t n, double tout,
+                      int64_t* offset)
 {
     const double hw = 0.5 * acorr.width;
     const double* first = std::lower_bound(t, t + n, tout - hw);
     const double* last = std::upper_bound(first, t + n, tout + hw);
     *offset = first - t;
-    const int m = (int)(last - first);
-    std::vector<double> g((size_t)m * m, 0.0), r(m);
+    return last - first;
+}
+
+// getPresumWeights: window [offset, offset + nw) and its weights
+void presum_weights(const bp_acorr& acorr, const double* t, int64_t n, double tout,
+                    int64_t* offset, std::vector<double>& w, SolveScratch& s)
+{
+    const int m = (int)presum_window(acorr, t, n, tout, offset);
+    const double* first = t + *offset;
+    s.g.assign((size_t)m * m, 0.0);
+    s.r.resize(m);
     for (int i = 0; i < m; ++i) {
         const double xi = first[i];
-        for (int j = i; j < m; ++j) g[(size_t)j * m + i] = acorr_eval(acorr, first[j] - xi);
-        r[i] = acorr_eval(acorr, tout - xi);
+        for (int j = i; j < m; ++j) s.g[(size_t)j * m + i] = acorr_eval(acorr, first[j] - xi);
+        s.r[i] = acorr_eval(acorr, tout - xi);
     }
-    ldlt_solve(g, m, r, w);
+    ldlt_solve(s, m, w);
 }
 
 // Per output pulse: window, distinct patterns (bit k: range bin valid in
 // window pulse k) and one weight vector per pattern.
+// LUT2d<double>::eval (core/LUT2d.cpp:127-159, clamped bilinear) split into
+// the range part, fixed per thread, and the azimuth part, fixed per output
+// pulse (formed on the host).  floor and ceil of one coordinate differ by 0
+// or exactly 1, so the divisions by (x2 - x1), (y2 - y1) and their product
+// that the bilinear formula performs are divisions by 1.0 and are left out:
+// the values are the same bits.
+struct LutAxis {
+    double v;       // clamped fractional index
+    int64_t i1, i2; // floor, ceil
+};
+
+__host__ __device__ inline LutAxis lut_axis(double coord, double start, double step, int64_t n)
+{
+    double v = (coord - start) / step;
+    v = fmin(fmax(v, 0.0), n - 1.0);
+    return {v, (int64_t)floor(v), (int64_t)ceil(v)};
+}
+
 struct PulseDesc {
     double tout;
     int64_t offset;
     int32_t nw, pbase, npat, wbase;  // weights of pattern q: pat_w[wbase + q nw + k]
+    LutAxis ly;                      // Doppler LUT azimuth axis at tout
 };
 
 struct ResampleArgs {
@@ -133,17 +182,21 @@ struct ResampleArgs {
     float2* out;
     const double* t;           // [nt]
     const int32_t* swaths;     // [ns][nt][2]
-    const PulseDesc* desc;     // [length]
-    const int64_t* pat_id;     // [total patterns]
+    const PulseDesc* desc;     // [rows]: output pulses row0 .. row0 + rows - 1
+    const int64_t* pat_id;     // the rows' patterns
     const float* pat_w;        // per pulse npat x nw
     bp::Lut2d doppler;
     double r0, dr;
-    int64_t nt, ns, length, width;
+    int64_t nt, ns, row0, rows, width;
     int32_t pulses_per_thread, pad_;
 };
 
 constexpr int kMaxWindow = 63;
 constexpr int kThreads = 256;
+constexpr int kPulsesPerThread = 16;   // output pulses per thread
+constexpr int64_t kFirstChunk = 512;   // output pulses of the first launch,
+constexpr int64_t kChunk = 2048;       // of the others (multiples of kPulsesPerThread)
+constexpr int64_t kItem = 32;          // output pulses a host thread plans at a time
 
 // complex64 product and sum rounded term by term (no fused multiply-add)
 __device__ inline float2 cmul(float2 a, float2 b)
@@ -169,24 +222,6 @@ __device__ inline bool bin_valid(const ResampleArgs& a, int64_t j, int64_t p)
     return valid;
 }
 
-// LUT2d<double>::eval (core/LUT2d.cpp:127-159, clamped bilinear) split into
-// the range part, fixed per thread, and the azimuth part, fixed per output
-// pulse.  floor and ceil of one coordinate differ by 0 or exactly 1, so the
-// divisions by (x2 - x1), (y2 - y1) and their product that the bilinear
-// formula performs are divisions by 1.0 and are left out: the values are
-// the same bits.
-struct LutAxis {
-    double v;       // clamped fractional index
-    int64_t i1, i2; // floor, ceil
-};
-
-__device__ inline LutAxis lut_axis(double coord, double start, double step, int64_t n)
-{
-    double v = (coord - start) / step;
-    v = fmin(fmax(v, 0.0), n - 1.0);
-    return {v, (int64_t)floor(v), (int64_t)ceil(v)};
-}
-
 __device__ inline double lut_bilinear(const bp::Lut2d& l, const LutAxis& x, const LutAxis& y)
 {
 #pragma clang fp contract(off)
@@ -206,8 +241,9 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
     const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (j >= a.width) return;
     const double r = a.r0 + a.dr * (double)j;
+    // output pulses [i0, i1), i relative to row0
     const int64_t i0 = (int64_t)blockIdx.y * a.pulses_per_thread;
-    const int64_t i1 = min(i0 + (int64_t)a.pulses_per_thread, a.length);
+    const int64_t i1 = min(i0 + (int64_t)a.pulses_per_thread, a.rows);
     const bp::Lut2d& lut = a.doppler;
     LutAxis lx{};
     if (lut.have_data) lx = lut_axis(r, lut.xstart, lut.dx, lut.width);
@@ -234,25 +270,30 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
             for (int k = 0; k < d.nw; ++k)
                 if (bin_valid(a, j, d.offset + k)) id |= (int64_t)1 << k;
         }
+        // the pulse's patterns are distinct: stop at the match
         const float* w = nullptr;
         for (int q = 0; q < d.npat; ++q)
-            if (a.pat_id[d.pbase + q] == id) w = a.pat_w + d.wbase + (int64_t)q * d.nw;
+            if (a.pat_id[d.pbase + q] == id) {
+                w = a.pat_w + d.wbase + (int64_t)q * d.nw;
+                break;
+            }
         double fd = lut.ref_value;
-        if (lut.have_data)
-            fd = lut_bilinear(lut, lx, lut_axis(d.tout, lut.ystart, lut.dy, lut.length));
+        if (lut.have_data) fd = lut_bilinear(lut, lx, d.ly);
         float2 sum = make_float2(0.f, 0.f);
         if (w != nullptr) {
             for (int k = 0; k < d.nw; ++k) {
                 const int64_t p = d.offset + k;
+                // deramp exp(-2 pi i (t_k - tout) fd): the phase in turns,
+                // reduced to [0, 1) in FP64, then the hardware sine and
+                // cosine of 2 pi times it
                 const double trel = a.t[p] - d.tout;
-                const float ph = (float)(-2 * M_PI * trel * fd);
-                float s, c;
-                sincosf(ph, &s, &c);
+                const float turns = (float)__builtin_amdgcn_fract(-trel * fd);
+                const float s = __builtin_amdgcn_sinf(turns), c = __builtin_amdgcn_cosf(turns);
                 const float wk = w[k];
                 sum = cadd(sum, cmul(make_float2(wk * c, wk * s), a.raw[p * a.width + j]));
             }
         }
-        a.out[i * a.width + j] = sum;
+        a.out[(a.row0 + i) * a.width + j] = sum;
     }
 }
 
@@ -289,12 +330,12 @@ struct Patterns {
 };
 
 // Window, distinct patterns and their weights for output pulses [i0, i1);
-// desc[i].pbase and .wbase are relative to the chunk's tables.
+// desc[i].pbase and .wbase are relative to out's tables.
 int plan_pulses(const bp::Orbit& ho, const double* t, int64_t nt, const int32_t* swaths, int64_t ns,
-                int64_t W, double prf, double antenna_length, int64_t i0, int64_t i1,
-                PulseDesc* desc, Patterns& out)
+                int64_t W, double prf, double antenna_length, const bp_lut2d& lut, int64_t i0,
+                int64_t i1, PulseDesc* desc, Patterns& out, SolveScratch& scratch)
 {
-    std::vector<double> w, sub_t, wsub;
+    std::vector<double> sub_t, wsub;
     std::vector<int64_t> cuts;
     for (int64_t i = i0; i < i1; ++i) {
         const double tout = t[0] + (double)i / prf;
@@ -303,10 +344,11 @@ int plan_pulses(const bp::Orbit& ho, const double* t, int64_t nt, const int32_t*
             return BP_ERR_OUT_OF_RANGE;
         const double v = bp::norm(vel);
         const bp_acorr acor{BP_ACORR_AZIMUTH, 0, 2.0 * (antenna_length / v)};
+        // the window of the full weight vector (focus.py takes its length)
         int64_t offset = 0;
-        presum_weights(acor, t, nt, tout, &offset, w);
-        const int nw = (int)w.size();
-        if (nw > kMaxWindow) return BP_ERR_LENGTH;
+        const int64_t nw64 = presum_window(acor, t, nt, tout, &offset);
+        if (nw64 > kMaxWindow) return BP_ERR_LENGTH;
+        const int nw = (int)nw64;
         PulseDesc& d = desc[i];
         d.tout = tout;
         d.offset = offset;
@@ -314,6 +356,7 @@ int plan_pulses(const bp::Orbit& ho, const double* t, int64_t nt, const int32_t*
         d.pbase = (int32_t)out.id.size();
         d.npat = 0;
         d.wbase = (int32_t)out.w.size();
+        d.ly = lut.have_data ? lut_axis(tout, lut.ystart, lut.dy, lut.length) : LutAxis{0.0, 0, 0};
         // range bins where the pattern can change: every sub-swath boundary
         cuts.assign(1, 0);
         for (int k = 0; k < nw; ++k)
@@ -342,7 +385,7 @@ int plan_pulses(const bp::Orbit& ho, const double* t, int64_t nt, const int32_t*
             for (int k = 0; k < nw; ++k)
                 if (id >> k & 1) sub_t.push_back(t[offset + k]);
             int64_t joff = 0;
-            presum_weights(acor, sub_t.data(), (int64_t)sub_t.size(), tout, &joff, wsub);
+            presum_weights(acor, sub_t.data(), (int64_t)sub_t.size(), tout, &joff, wsub, scratch);
             if (joff != 0 || wsub.size() != sub_t.size()) return BP_ERR_RUNTIME;
             out.id.push_back(id);
             const size_t base = out.w.size();
@@ -355,19 +398,40 @@ int plan_pulses(const bp::Orbit& ho, const double* t, int64_t nt, const int32_t*
     return BP_SUCCESS;
 }
 
-struct DevBuf {
-    void* ptr = nullptr;
-    ~DevBuf()
+struct Part {
+    const void* src;
+    size_t bytes;
+};
+
+// Pinned staging buffer and its device copy for a set of tables, freed on
+// destruction (after the kernels that read them have finished).
+struct Staging {
+    void* host = nullptr;
+    void* dev = nullptr;
+    ~Staging()
     {
-        if (ptr) (void)hipFree(ptr);
+        if (dev) (void)hipFree(dev);
+        if (host) (void)hipHostFree(host);
     }
-    hipError_t put(const void* src, size_t bytes, hipStream_t s)
+    // Packs the parts at 256-byte aligned offsets (into off) and enqueues the
+    // upload on s.
+    hipError_t upload(const Part* parts, int n, size_t* off, hipStream_t s)
     {
-        if (bytes == 0) return hipSuccess;
-        hipError_t e = hipMalloc(&ptr, bytes);
+        size_t total = 0;
+        for (int k = 0; k < n; ++k) {
+            off[k] = total;
+            total += (parts[k].bytes + 255) / 256 * 256;
+        }
+        if (total == 0) return hipSuccess;
+        hipError_t e = hipHostMalloc(&host, total, hipHostMallocDefault);
         if (e != hipSuccess) return e;
-        return hipMemcpyAsync(ptr, src, bytes, hipMemcpyHostToDevice, s);
+        for (int k = 0; k < n; ++k)
+            if (parts[k].bytes) std::memcpy(static_cast<char*>(host) + off[k], parts[k].src, parts[k].bytes);
+        e = hipMalloc(&dev, total);
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, s);
     }
+    const char* at(size_t off) const { return static_cast<const char*>(dev) + off; }
 };
 
 }  // namespace
@@ -389,7 +453,8 @@ int bp_presum_weights(const bp_acorr* acorr, const double* t, int64_t n, double 
     if (acorr->kind != BP_ACORR_AZIMUTH && acorr->kind != BP_ACORR_BARTLETT)
         return fail(BP_ERR_INVALID_ARGUMENT, "unknown autocorrelation kernel");
     std::vector<double> ww;
-    presum::presum_weights(*acorr, t, n, tout, offset, ww);
+    presum::SolveScratch scratch;
+    presum::presum_weights(*acorr, t, n, tout, offset, ww, scratch);
     *nw = (int64_t)ww.size();
     if (*nw > max_w) return fail(BP_ERR_LENGTH, "weight buffer too short");
     if (*nw > 0) std::memcpy(w, ww.data(), sizeof(double) * ww.size());
@@ -455,113 +520,174 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
     bp::Orbit ho{orbit->t0, orbit->dt, orbit->n, orbit->method, orbit->pos, orbit->vel};
     const int64_t L = grid->length, W = grid->width;
     std::vector<PulseDesc> desc(L);
-    // output pulses are independent: contiguous chunks on host threads, each
-    // with its own pattern tables, concatenated afterwards
-    const int nthreads = (int)std::max<int64_t>(
-        1, std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
-                              (L + 255) / 256}));
-    std::vector<Patterns> parts(nthreads);
-    std::vector<std::thread> pool;
-    std::vector<char> ran(nthreads, 0);
-    auto work = [&](int p) {
-        const int64_t i0 = L * p / nthreads, i1 = L * (p + 1) / nthreads;
-        parts[p].status = plan_pulses(ho, t, nt, swaths, ns, W, grid->prf, antenna_length, i0, i1,
-                                      desc.data(), parts[p]);
-        ran[p] = 1;
+    // Output pulses in launches (chunks) of kFirstChunk, then kChunk pulses,
+    // each split into items of kItem pulses.  Host threads -- and this thread
+    // while it waits for a chunk -- take the items in order, each item with
+    // its own pattern tables, while the GPU resamples the chunks already
+    // planned.
+    std::vector<int64_t> chunk_start{0};
+    for (int64_t c0 = std::min(L, kFirstChunk); c0 < L; c0 = std::min(L, c0 + kChunk))
+        chunk_start.push_back(c0);
+    chunk_start.push_back(L);
+    const int64_t nchunk = (int64_t)chunk_start.size() - 1;
+    struct Item {
+        int64_t i0, i1, chunk;
     };
-    for (int p = 0; p + 1 < nthreads; ++p) {
-        try {
-            pool.emplace_back(work, p);
-        } catch (...) {
-            break;  // no thread available: the chunk runs below on this thread
-        }
+    std::vector<Item> items;
+    std::vector<int64_t> first_item(nchunk + 1);
+    for (int64_t c = 0; c < nchunk; ++c) {
+        first_item[c] = (int64_t)items.size();
+        for (int64_t i0 = chunk_start[c]; i0 < chunk_start[c + 1]; i0 += kItem)
+            items.push_back({i0, std::min(chunk_start[c + 1], i0 + kItem), c});
     }
-    work(nthreads - 1);
-    for (auto& th : pool) th.join();
-    for (int p = 0; p < nthreads; ++p)
-        if (!ran[p]) work(p);
-    std::vector<int64_t> pat_id;
-    std::vector<float> pat_w;
-    for (int p = 0; p < nthreads; ++p) {
-        if (parts[p].status == BP_ERR_OUT_OF_RANGE)
-            return fail(BP_ERR_OUT_OF_RANGE, "output pulse time outside the orbit");
-        if (parts[p].status == BP_ERR_LENGTH)
-            return fail(BP_ERR_LENGTH, "Hash algorithm requires number of pulses less than "
-                                       "number of bits in identifier (64)");
-        if (parts[p].status != BP_SUCCESS)
-            return fail(BP_ERR_RUNTIME, "presum weights of a mask pattern left its window");
-        const int32_t base = (int32_t)pat_id.size(), wb = (int32_t)pat_w.size();
-        for (int64_t i = L * p / nthreads; i < L * (p + 1) / nthreads; ++i) {
-            desc[i].pbase += base;
-            desc[i].wbase += wb;
+    first_item[nchunk] = (int64_t)items.size();
+    const int64_t nitems = (int64_t)items.size();
+    const int nworkers = (int)std::max<int64_t>(
+        0, std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
+                              nitems}) - 1);
+    std::vector<Patterns> item_pat(nitems);
+    std::vector<int64_t> done(nchunk, 0);
+    std::vector<SolveScratch> scratch(nworkers + 1);   // the last one is this thread's
+    std::mutex mu;
+    std::condition_variable cv;
+    std::atomic<bool> abort{false};
+    std::atomic<int64_t> next{0};
+    auto run_item = [&](int64_t k, SolveScratch& sc) {
+        const Item& it = items[k];
+        Patterns& pt = item_pat[k];
+        pt.status = abort ? BP_ERR_RUNTIME
+                          : plan_pulses(ho, t, nt, swaths, ns, W, grid->prf, antenna_length,
+                                        *doppler, it.i0, it.i1, desc.data(), pt, sc);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            ++done[it.chunk];
         }
-        pat_id.insert(pat_id.end(), parts[p].id.begin(), parts[p].id.end());
-        pat_w.insert(pat_w.end(), parts[p].w.begin(), parts[p].w.end());
+        cv.notify_all();
+    };
+    std::vector<std::thread> pool;
+    for (int p = 0; p < nworkers; ++p) {
+        try {
+            pool.emplace_back([&, p] {
+                for (int64_t k = next++; k < nitems; k = next++) run_item(k, scratch[p]);
+            });
+        } catch (...) {
+            break;  // fewer threads: this thread takes more items
+        }
     }
 
-    if (hipSetDevice(device) != hipSuccess) return fail(BP_ERR_HIP, "hipSetDevice failed");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    // every table in one pinned staging buffer and one device allocation
-    struct Part {
-        const void* src;
-        size_t bytes, off;
+    // every failure below stops the planners, waits for the launches already
+    // made and frees the staging buffers (destructors, after the wait); an
+    // exception still stops and joins the planners
+    struct Joiner {
+        std::vector<std::thread>& pool;
+        std::atomic<bool>& abort;
+        ~Joiner()
+        {
+            abort = true;
+            for (auto& th : pool)
+                if (th.joinable()) th.join();
+        }
     };
-    Part parts_up[6] = {
-        {t, sizeof(double) * nt, 0},
-        {swaths, sizeof(int32_t) * 2 * ns * nt, 0},
-        {desc.data(), sizeof(PulseDesc) * desc.size(), 0},
-        {pat_id.data(), sizeof(int64_t) * pat_id.size(), 0},
-        {pat_w.data(), sizeof(float) * pat_w.size(), 0},
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Staging shared;
+    std::vector<Staging> staged(nchunk);
+    Joiner joiner{pool, abort};
+    auto finish = [&](int code, const std::string& msg) {
+        abort = true;
+        for (auto& th : pool) th.join();
+        pool.clear();
+        const hipError_t e = hipStreamSynchronize(s);
+        if (code == BP_SUCCESS && e != hipSuccess)
+            return fail(BP_ERR_HIP, std::string("presum resample: ") + hipGetErrorString(e));
+        return code == BP_SUCCESS ? BP_SUCCESS : fail(code, msg);
+    };
+    if (hipSetDevice(device) != hipSuccess) return finish(BP_ERR_HIP, "hipSetDevice failed");
+    // tables every chunk reads: pulse times, sub-swaths, Doppler samples
+    const Part shared_parts[3] = {
+        {t, sizeof(double) * nt},
+        {swaths, sizeof(int32_t) * 2 * ns * nt},
         {doppler->have_data ? doppler->data : nullptr,
-         doppler->have_data ? sizeof(double) * doppler->width * doppler->length : 0, 0}};
-    size_t total = 0;
-    for (Part& q : parts_up) {
-        q.off = total;
-        total += (q.bytes + 255) / 256 * 256;
-    }
-    DevBuf d_all;
-    void* h_all = nullptr;
-    hipError_t e = hipHostMalloc(&h_all, total, hipHostMallocDefault);
-    if (e == hipSuccess) {
-        for (const Part& q : parts_up)
-            if (q.bytes) std::memcpy(static_cast<char*>(h_all) + q.off, q.src, q.bytes);
-        e = d_all.put(h_all, total, s);
-    }
-    if (e != hipSuccess) {
-        if (h_all) (void)hipHostFree(h_all);
-        return fail(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
-    }
-    auto dev = [&](int k) { return static_cast<const char*>(d_all.ptr) + parts_up[k].off; };
+         doppler->have_data ? sizeof(double) * doppler->width * doppler->length : 0}};
+    size_t shared_off[3];
+    hipError_t e = shared.upload(shared_parts, 3, shared_off, s);
+    if (e != hipSuccess) return finish(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
 
     ResampleArgs a;
     std::memset(&a, 0, sizeof(a));
     a.raw = static_cast<const float2*>(raw);
     a.out = static_cast<float2*>(out);
-    a.t = reinterpret_cast<const double*>(dev(0));
-    a.swaths = reinterpret_cast<const int32_t*>(dev(1));
-    a.desc = reinterpret_cast<const PulseDesc*>(dev(2));
-    a.pat_id = reinterpret_cast<const int64_t*>(dev(3));
-    a.pat_w = reinterpret_cast<const float*>(dev(4));
+    a.t = reinterpret_cast<const double*>(shared.at(shared_off[0]));
+    a.swaths = reinterpret_cast<const int32_t*>(shared.at(shared_off[1]));
     a.doppler = bp::Lut2d{doppler->have_data, doppler->bounds_error, doppler->ref_value,
                           doppler->xstart, doppler->ystart, doppler->dx, doppler->dy,
                           doppler->width, doppler->length,
-                          reinterpret_cast<const double*>(dev(5))};
+                          reinterpret_cast<const double*>(shared.at(shared_off[2]))};
     a.r0 = grid->starting_range;
     a.dr = grid->range_pixel_spacing;
     a.nt = nt;
     a.ns = ns;
-    a.length = L;
     a.width = W;
-    a.pulses_per_thread = 16;
-    const dim3 blocks((unsigned)((W + kThreads - 1) / kThreads),
-                      (unsigned)((L + a.pulses_per_thread - 1) / a.pulses_per_thread));
-    hipLaunchKernelGGL(resample_kernel, blocks, dim3(kThreads), 0, s, a);
-    e = hipGetLastError();
-    // the tables are freed on return: wait for the kernel first
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipHostFree(h_all);
-    if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("presum resample: ") + hipGetErrorString(e));
-    return BP_SUCCESS;
+    a.pulses_per_thread = kPulsesPerThread;
+    std::vector<int64_t> pat_id;
+    std::vector<float> pat_w;
+    for (int64_t c = 0; c < nchunk; ++c) {
+        const int64_t need = first_item[c + 1] - first_item[c];
+        for (;;) {
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (done[c] == need) break;
+            }
+            const int64_t k = next++;
+            if (k < nitems) {
+                run_item(k, scratch[nworkers]);
+                continue;
+            }
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return done[c] == need; });
+            break;
+        }
+        pat_id.clear();
+        pat_w.clear();
+        for (int64_t k = first_item[c]; k < first_item[c + 1]; ++k) {
+            const Patterns& pt = item_pat[k];
+            if (pt.status == BP_ERR_OUT_OF_RANGE)
+                return finish(BP_ERR_OUT_OF_RANGE, "output pulse time outside the orbit");
+            if (pt.status == BP_ERR_LENGTH)
+                return finish(BP_ERR_LENGTH, "Hash algorithm requires number of pulses less than "
+                                             "number of bits in identifier (64)");
+            if (pt.status != BP_SUCCESS)
+                return finish(BP_ERR_RUNTIME, "presum weights of a mask pattern left its window");
+            // the chunk's tables: its items' tables one after the other
+            const int32_t base = (int32_t)pat_id.size(), wb = (int32_t)pat_w.size();
+            for (int64_t i = items[k].i0; i < items[k].i1; ++i) {
+                desc[i].pbase += base;
+                desc[i].wbase += wb;
+            }
+            pat_id.insert(pat_id.end(), pt.id.begin(), pt.id.end());
+            pat_w.insert(pat_w.end(), pt.w.begin(), pt.w.end());
+        }
+        const int64_t c0 = chunk_start[c], rows = chunk_start[c + 1] - c0;
+        const Part parts[3] = {{desc.data() + c0, sizeof(PulseDesc) * rows},
+                               {pat_id.data(), sizeof(int64_t) * pat_id.size()},
+                               {pat_w.data(), sizeof(float) * pat_w.size()}};
+        size_t off[3];
+        e = staged[c].upload(parts, 3, off, s);
+        if (e != hipSuccess)
+            return finish(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
+        a.desc = reinterpret_cast<const PulseDesc*>(staged[c].at(off[0]));
+        a.pat_id = reinterpret_cast<const int64_t*>(staged[c].at(off[1]));
+        a.pat_w = reinterpret_cast<const float*>(staged[c].at(off[2]));
+        a.row0 = c0;
+        a.rows = rows;
+        const dim3 blocks((unsigned)((W + kThreads - 1) / kThreads),
+                          (unsigned)((rows + kPulsesPerThread - 1) / kPulsesPerThread));
+        hipLaunchKernelGGL(resample_kernel, blocks, dim3(kThreads), 0, s, a);
+        e = hipGetLastError();
+        if (e != hipSuccess)
+            return finish(BP_ERR_HIP, std::string("presum resample: ") + hipGetErrorString(e));
+    }
+    // the staging buffers are freed on return: wait for the kernels first
+    return finish(BP_SUCCESS, "");
 }
 
 }  // extern "C"

@@ -77,7 +77,7 @@ def _dithered_scene(nt=96, nr=300, dither=0.15, gap=20, seed=0, ns=2, walk=True)
     raw = (rng.normal(size=(nt, nr)) + 1j * rng.normal(size=(nt, nr))).astype("c8")
     # Doppler plane fd = 50 + 2e4 (t - t0) + 1e-3 (r - r0): bilinear LUTs hold it exactly
     xs = r0 + dr * np.array([-10.0, nr + 10.0])
-    ys = t[0] + np.array([-0.1, 0.2])
+    ys = t[0] + np.array([-0.1, t[-1] - t[0] + 0.2])   # the LUT spans every output time
     data = 50 + 2e4 * (ys[:, None] - t[0]) + 1e-3 * (xs[None, :] - r0)
     lut = LUT2d(xs, ys, data)
     fd = lambda tout, r: 50 + 2e4 * (tout - t[0]) + 1e-3 * (r - r0)  # noqa: E731
@@ -103,6 +103,22 @@ def test_resample_matches_oracle(torch_gpu, dither, gap, ns, walk):
     assert np.array_equal(out == 0, ref == 0)
     if not walk:
         assert np.all(ref[:, 120:120 + gap] == 0)
+
+
+def test_resample_across_launch_chunks(torch_gpu):
+    """4500 output pulses run as launches of 512, 2048 and 1940 pulses
+    (planned on host threads while the GPU works): the pulses either side of
+    each launch boundary and the ragged end match the oracle."""
+    from isce3_amd.presum import resample
+    grid, orbit, lut, t, sw, raw, fd, speed = _dithered_scene(nt=4500, nr=96, gap=12, seed=3)
+    out = resample(raw, t, grid, sw, orbit, lut, L=12.0)
+    rows = np.r_[0:3, 509:515, 2557:2563, 4497:4500]
+    ref = po.resample(raw, t, grid.prf, grid.starting_range, grid.range_pixel_spacing, sw,
+                      lambda tout: speed, fd, L=12.0, rows=rows)
+    e = rel(out[rows], ref)
+    print(f"[presum chunks] rel={e:.2e}", flush=True)
+    assert e <= 1e-5
+    assert np.array_equal(out[rows] == 0, ref == 0)
 
 
 def test_resample_uniform_grid_is_identity(torch_gpu):
