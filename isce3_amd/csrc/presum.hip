@@ -197,6 +197,7 @@ constexpr int kPulsesPerThread = 16;   // output pulses per thread
 constexpr int64_t kFirstChunk = 512;   // output pulses of the first launch,
 constexpr int64_t kChunk = 2048;       // of the others (multiples of kPulsesPerThread)
 constexpr int64_t kItem = 32;          // output pulses a host thread plans at a time
+constexpr int kTapGroup = 8;           // window taps loaded together
 
 // complex64 product and sum rounded term by term (no fused multiply-add)
 __device__ inline float2 cmul(float2 a, float2 b)
@@ -212,34 +213,62 @@ __device__ inline float2 cadd(float2 a, float2 b)
 }
 
 // Valid-sample bit of range bin j in input pulse p (any sub-swath).
+// A wave-uniform read-only pointer as a constant-address-space pointer: its
+// loads go through the scalar data cache (one load per wavefront instead of
+// one vector memory instruction of 64 addresses).  Only for data no kernel
+// writes while it runs, read at wave-uniform indices.
+// (The host compilation pass of the kernels keeps the generic pointer.)
+template<class T>
+__device__ inline auto uniform_ptr(const T* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (const __attribute__((address_space(4))) T*)(p);
+#else
+    return p;
+#endif
+}
+
+// Valid-sample bit of range bin j in input pulse p (any sub-swath); p is
+// wave-uniform.
 __device__ inline bool bin_valid(const ResampleArgs& a, int64_t j, int64_t p)
 {
     bool valid = false;
     for (int64_t s = 0; s < a.ns; ++s) {
-        const int32_t* sw = a.swaths + (s * a.nt + p) * 2;
+        const auto* sw = uniform_ptr(a.swaths) + (s * a.nt + p) * 2;
         valid |= (j >= sw[0]) & (j < sw[1]);
     }
     return valid;
 }
 
-__device__ inline double lut_bilinear(const bp::Lut2d& l, const LutAxis& x, const LutAxis& y)
+// The four samples around (x, y) ...
+struct LutCorners {
+    double q11, q12, q21, q22;
+};
+
+__device__ inline LutCorners lut_corners(const bp::Lut2d& l, const LutAxis& x, const LutAxis& y)
+{
+    const double* z = l.data;
+    return {z[y.i1 * l.width + x.i1], z[y.i2 * l.width + x.i1], z[y.i1 * l.width + x.i2],
+            z[y.i2 * l.width + x.i2]};
+}
+
+// ... and the reference's bilinear combination of them
+__device__ inline double lut_combine(const LutCorners& q, const LutAxis& x, const LutAxis& y)
 {
 #pragma clang fp contract(off)
-    const double* z = l.data;
-    const double q11 = z[y.i1 * l.width + x.i1], q12 = z[y.i2 * l.width + x.i1];
-    const double q21 = z[y.i1 * l.width + x.i2], q22 = z[y.i2 * l.width + x.i2];
     const double x1 = (double)x.i1, x2 = (double)x.i2, y1 = (double)y.i1, y2 = (double)y.i2;
-    if (y.i1 == y.i2 && x.i1 == x.i2) return q11;
-    if (y.i1 == y.i2) return ((x2 - x.v) * q11) + ((x.v - x1) * q21);
-    if (x.i1 == x.i2) return ((y2 - y.v) * q11) + ((y.v - y1) * q12);
-    return (q11 * ((x2 - x.v) * (y2 - y.v))) + (q21 * ((x.v - x1) * (y2 - y.v))) +
-           (q12 * ((x2 - x.v) * (y.v - y1))) + (q22 * ((x.v - x1) * (y.v - y1)));
+    if (y.i1 == y.i2 && x.i1 == x.i2) return q.q11;
+    if (y.i1 == y.i2) return ((x2 - x.v) * q.q11) + ((x.v - x1) * q.q21);
+    if (x.i1 == x.i2) return ((y2 - y.v) * q.q11) + ((y.v - y1) * q.q12);
+    return (q.q11 * ((x2 - x.v) * (y2 - y.v))) + (q.q21 * ((x.v - x1) * (y2 - y.v))) +
+           (q.q12 * ((x2 - x.v) * (y.v - y1))) + (q.q22 * ((x.v - x1) * (y.v - y1)));
 }
 
 __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
 {
     const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (j >= a.width) return;
+    const uint32_t jj = (uint32_t)j;   // widths stay below 2^31
     const double r = a.r0 + a.dr * (double)j;
     // output pulses [i0, i1), i relative to row0
     const int64_t i0 = (int64_t)blockIdx.y * a.pulses_per_thread;
@@ -247,12 +276,15 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
     const bp::Lut2d& lut = a.doppler;
     LutAxis lx{};
     if (lut.have_data) lx = lut_axis(r, lut.xstart, lut.dx, lut.width);
+    LutCorners q{};
+    int64_t qy1 = -1, qy2 = -1;
     // validity of this bin in every input pulse the chunk's windows reach,
     // one bit per pulse from the earliest window start
-    int64_t pfirst = a.desc[i0].offset, pend = pfirst;
+    const auto* desc = uniform_ptr(a.desc);
+    int64_t pfirst = desc[i0].offset, pend = pfirst;
     for (int64_t i = i0; i < i1; ++i) {
-        pfirst = min(pfirst, a.desc[i].offset);
-        pend = max(pend, a.desc[i].offset + (int64_t)a.desc[i].nw);
+        pfirst = min(pfirst, desc[i].offset);
+        pend = max(pend, desc[i].offset + (int64_t)desc[i].nw);
     }
     const int64_t span = pend - pfirst;
     uint64_t bits = 0;
@@ -260,7 +292,7 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
         for (int64_t p = 0; p < span; ++p)
             if (bin_valid(a, j, pfirst + p)) bits |= (uint64_t)1 << p;
     for (int64_t i = i0; i < i1; ++i) {
-        const PulseDesc d = a.desc[i];
+        const PulseDesc d = desc[i];
         // pattern of this range bin over the window
         int64_t id = 0;
         if (span <= 64) {
@@ -270,30 +302,61 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
             for (int k = 0; k < d.nw; ++k)
                 if (bin_valid(a, j, d.offset + k)) id |= (int64_t)1 << k;
         }
-        // the pulse's patterns are distinct: stop at the match
-        const float* w = nullptr;
+        // the pulse's patterns are distinct: stop at the match (-1: none)
+        int pat = -1;
         for (int q = 0; q < d.npat; ++q)
-            if (a.pat_id[d.pbase + q] == id) {
-                w = a.pat_w + d.wbase + (int64_t)q * d.nw;
+            if (uniform_ptr(a.pat_id)[d.pbase + q] == id) {
+                pat = q;
                 break;
             }
+        // a wavefront whose bins share one pattern reads its weights through
+        // the scalar cache
+        const int pat0 = __builtin_amdgcn_readfirstlane(pat);
+        const bool uniform_pat = __all(pat == pat0);
         double fd = lut.ref_value;
-        if (lut.have_data) fd = lut_bilinear(lut, lx, d.ly);
+        if (lut.have_data) {
+            // the corners are re-read only when the pulse moves to other LUT rows
+            if (d.ly.i1 != qy1 || d.ly.i2 != qy2) {
+                q = lut_corners(lut, lx, d.ly);
+                qy1 = d.ly.i1;
+                qy2 = d.ly.i2;
+            }
+            fd = lut_combine(q, lx, d.ly);
+        }
         float2 sum = make_float2(0.f, 0.f);
-        if (w != nullptr) {
-            for (int k = 0; k < d.nw; ++k) {
-                const int64_t p = d.offset + k;
-                // deramp exp(-2 pi i (t_k - tout) fd): the phase in turns,
-                // reduced to [0, 1) in FP64, then the hardware sine and
-                // cosine of 2 pi times it
-                const double trel = a.t[p] - d.tout;
-                const float turns = (float)__builtin_amdgcn_fract(-trel * fd);
-                const float s = __builtin_amdgcn_sinf(turns), c = __builtin_amdgcn_cosf(turns);
-                const float wk = w[k];
-                sum = cadd(sum, cmul(make_float2(wk * c, wk * s), a.raw[p * a.width + j]));
+        if (pat >= 0) {
+            // taps in groups of kTapGroup: the group's loads first, then the
+            // products, accumulated in tap order
+            const float2* row = a.raw + d.offset * a.width;   // input pulse d.offset
+            const auto* tp = uniform_ptr(a.t) + d.offset;
+            const float* w = a.pat_w + d.wbase + (int64_t)pat * d.nw;
+            const auto* w0 = uniform_ptr(a.pat_w) + d.wbase + (int64_t)pat0 * d.nw;
+            for (int k0 = 0; k0 < d.nw; k0 += kTapGroup) {
+                float2 x[kTapGroup];
+                float wv[kTapGroup];
+                double tk[kTapGroup];
+#pragma unroll
+                for (int u = 0; u < kTapGroup; ++u)
+                    if (k0 + u < d.nw) {
+                        x[u] = row[jj];
+                        row += a.width;
+                        wv[u] = uniform_pat ? w0[k0 + u] : w[k0 + u];
+                        tk[u] = tp[k0 + u];
+                    }
+#pragma unroll
+                for (int u = 0; u < kTapGroup; ++u)
+                    if (k0 + u < d.nw) {
+                        // deramp exp(-2 pi i (t_k - tout) fd): the phase in
+                        // turns, reduced to [0, 1) in FP64, then the hardware
+                        // sine and cosine of 2 pi times it
+                        const double trel = tk[u] - d.tout;
+                        const float turns = (float)__builtin_amdgcn_fract(-trel * fd);
+                        const float sn = __builtin_amdgcn_sinf(turns), cs = __builtin_amdgcn_cosf(turns);
+                        sum = cadd(sum, cmul(make_float2(wv[u] * cs, wv[u] * sn), x[u]));
+                    }
             }
         }
-        a.out[(a.row0 + i) * a.width + j] = sum;
+        (a.out + (a.row0 + i) * a.width)[jj] = sum;
     }
 }
 
