@@ -9,7 +9,7 @@
 // (taken from the sub-swath boundaries, which is where the pattern of a range
 // bin can change) with one weight vector per pattern.
 //
-// Device side: one thread per range bin and 16 output pulses rebuilds the
+// Device side: one thread per range bin and 8 output pulses rebuilds the
 // bin's pattern from the sub-swath table, picks its weights, evaluates the
 // Doppler LUT at (tout, r) and accumulates
 //   sum_k float(w_k) * exp(-2 pi i (t_k - tout) fd) * raw[k][r]
@@ -193,11 +193,11 @@ struct ResampleArgs {
 
 constexpr int kMaxWindow = 63;
 constexpr int kThreads = 256;
-constexpr int kPulsesPerThread = 16;   // output pulses per thread
+constexpr int kPulsesPerThread = 8;    // output pulses per thread
 constexpr int64_t kFirstChunk = 512;   // output pulses of the first launch,
 constexpr int64_t kChunk = 2048;       // of the others (multiples of kPulsesPerThread)
 constexpr int64_t kItem = 32;          // output pulses a host thread plans at a time
-constexpr int kTapGroup = 8;           // window taps loaded together
+constexpr int kTapGroup = 4;           // window taps loaded together
 
 // complex64 product and sum rounded term by term (no fused multiply-add)
 __device__ inline float2 cmul(float2 a, float2 b)
