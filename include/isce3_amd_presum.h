@@ -70,9 +70,12 @@ int bp_presum_compute_ids(const uint8_t* mask, int64_t nr, int64_t np, int64_t* 
  *   swaths  host int32 [ns][nt][2], [start, stop) valid range bins
  *   raw     device complex64 [nt][grid->width]
  *   out     device complex64 [grid->length][grid->width]
- * Weights are solved on the host per (output pulse, distinct mask pattern);
- * the sums over all output pulses run in one kernel.  BP_ERR_LENGTH when a
- * window holds more than 63 pulses (the mask hash's limit). */
+ * Weights are solved on host threads per (output pulse, distinct mask
+ * pattern) while the sums run on the GPU, in launches of up to 2048 output
+ * pulses on `stream`; the call returns when they have finished.  The tables
+ * go up through pinned staging buffers that a process-wide pool keeps for
+ * later calls (a few MB per concurrent call at 16384 pulses).  BP_ERR_LENGTH
+ * when a window holds more than 63 pulses (the mask hash's limit). */
 int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* swaths,
                        int64_t ns, const bp_grid* grid, const bp_orbit* orbit,
                        const bp_lut2d* doppler, double antenna_length, const void* raw,

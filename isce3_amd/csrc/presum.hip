@@ -466,18 +466,17 @@ struct Part {
     size_t bytes;
 };
 
-// Pinned staging buffer and its device copy for a set of tables, freed on
-// destruction (after the kernels that read them have finished).
+// Pinned staging buffer and its device copy for a set of tables.  Buffers
+// live in a process-wide pool for later calls on the same device -- pinning
+// and freeing host memory costs more than the copies -- and the pool is never
+// freed.
 struct Staging {
+    int device = 0;
     void* host = nullptr;
     void* dev = nullptr;
-    ~Staging()
-    {
-        if (dev) (void)hipFree(dev);
-        if (host) (void)hipHostFree(host);
-    }
+    size_t capacity = 0;
     // Packs the parts at 256-byte aligned offsets (into off) and enqueues the
-    // upload on s.
+    // upload on s (the current device is `device`).
     hipError_t upload(const Part* parts, int n, size_t* off, hipStream_t s)
     {
         size_t total = 0;
@@ -486,15 +485,69 @@ struct Staging {
             total += (parts[k].bytes + 255) / 256 * 256;
         }
         if (total == 0) return hipSuccess;
-        hipError_t e = hipHostMalloc(&host, total, hipHostMallocDefault);
-        if (e != hipSuccess) return e;
+        if (total > capacity) {
+            if (dev) (void)hipFree(dev);
+            if (host) (void)hipHostFree(host);
+            dev = host = nullptr;
+            capacity = 0;
+            const size_t cap = std::max<size_t>(total + total / 2, size_t(1) << 20);
+            hipError_t e = hipHostMalloc(&host, cap, hipHostMallocDefault);
+            if (e != hipSuccess) return e;
+            e = hipMalloc(&dev, cap);
+            if (e != hipSuccess) return e;
+            capacity = cap;
+        }
         for (int k = 0; k < n; ++k)
             if (parts[k].bytes) std::memcpy(static_cast<char*>(host) + off[k], parts[k].src, parts[k].bytes);
-        e = hipMalloc(&dev, total);
-        if (e != hipSuccess) return e;
         return hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, s);
     }
     const char* at(size_t off) const { return static_cast<const char*>(dev) + off; }
+};
+
+struct StagingPool {
+    std::mutex mu;
+    std::vector<Staging*> idle;
+    Staging* take(int device)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t k = 0; k < idle.size(); ++k)
+            if (idle[k]->device == device) {
+                Staging* b = idle[k];
+                idle.erase(idle.begin() + k);
+                return b;
+            }
+        auto* b = new Staging();
+        b->device = device;
+        return b;
+    }
+    void give(Staging* b)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        idle.push_back(b);
+    }
+};
+
+StagingPool& staging_pool()
+{
+    static auto* p = new StagingPool();   // lives until the process ends
+    return *p;
+}
+
+// The staging buffers one call holds; they go back to the pool once the
+// call's stream has finished with them.
+struct Leases {
+    hipStream_t stream;
+    std::vector<Staging*> held;
+    Staging* take(int device)
+    {
+        held.push_back(staging_pool().take(device));
+        return held.back();
+    }
+    ~Leases()
+    {
+        if (!held.empty()) (void)hipStreamSynchronize(stream);
+        for (Staging* b : held) staging_pool().give(b);
+    }
 };
 
 }  // namespace
@@ -638,9 +691,9 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
         }
     }
 
-    // every failure below stops the planners, waits for the launches already
-    // made and frees the staging buffers (destructors, after the wait); an
-    // exception still stops and joins the planners
+    // every failure below stops the planners and waits for the launches
+    // already made; the staging buffers go back to the pool after the wait
+    // (destructors), and an exception still stops and joins the planners
     struct Joiner {
         std::vector<std::thread>& pool;
         std::atomic<bool>& abort;
@@ -652,8 +705,7 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
         }
     };
     hipStream_t s = static_cast<hipStream_t>(stream);
-    Staging shared;
-    std::vector<Staging> staged(nchunk);
+    Leases leases{s, {}};
     Joiner joiner{pool, abort};
     auto finish = [&](int code, const std::string& msg) {
         abort = true;
@@ -672,19 +724,20 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
         {doppler->have_data ? doppler->data : nullptr,
          doppler->have_data ? sizeof(double) * doppler->width * doppler->length : 0}};
     size_t shared_off[3];
-    hipError_t e = shared.upload(shared_parts, 3, shared_off, s);
+    Staging* shared = leases.take(device);
+    hipError_t e = shared->upload(shared_parts, 3, shared_off, s);
     if (e != hipSuccess) return finish(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
 
     ResampleArgs a;
     std::memset(&a, 0, sizeof(a));
     a.raw = static_cast<const float2*>(raw);
     a.out = static_cast<float2*>(out);
-    a.t = reinterpret_cast<const double*>(shared.at(shared_off[0]));
-    a.swaths = reinterpret_cast<const int32_t*>(shared.at(shared_off[1]));
+    a.t = reinterpret_cast<const double*>(shared->at(shared_off[0]));
+    a.swaths = reinterpret_cast<const int32_t*>(shared->at(shared_off[1]));
     a.doppler = bp::Lut2d{doppler->have_data, doppler->bounds_error, doppler->ref_value,
                           doppler->xstart, doppler->ystart, doppler->dx, doppler->dy,
                           doppler->width, doppler->length,
-                          reinterpret_cast<const double*>(shared.at(shared_off[2]))};
+                          reinterpret_cast<const double*>(shared->at(shared_off[2]))};
     a.r0 = grid->starting_range;
     a.dr = grid->range_pixel_spacing;
     a.nt = nt;
@@ -734,12 +787,13 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
                                {pat_id.data(), sizeof(int64_t) * pat_id.size()},
                                {pat_w.data(), sizeof(float) * pat_w.size()}};
         size_t off[3];
-        e = staged[c].upload(parts, 3, off, s);
+        Staging* staged = leases.take(device);
+        e = staged->upload(parts, 3, off, s);
         if (e != hipSuccess)
             return finish(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
-        a.desc = reinterpret_cast<const PulseDesc*>(staged[c].at(off[0]));
-        a.pat_id = reinterpret_cast<const int64_t*>(staged[c].at(off[1]));
-        a.pat_w = reinterpret_cast<const float*>(staged[c].at(off[2]));
+        a.desc = reinterpret_cast<const PulseDesc*>(staged->at(off[0]));
+        a.pat_id = reinterpret_cast<const int64_t*>(staged->at(off[1]));
+        a.pat_w = reinterpret_cast<const float*>(staged->at(off[2]));
         a.row0 = c0;
         a.rows = rows;
         const dim3 blocks((unsigned)((W + kThreads - 1) / kThreads),
@@ -749,7 +803,7 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
         if (e != hipSuccess)
             return finish(BP_ERR_HIP, std::string("presum resample: ") + hipGetErrorString(e));
     }
-    // the staging buffers are freed on return: wait for the kernels first
+    // the staging buffers go back to the pool on return, after the kernels
     return finish(BP_SUCCESS, "");
 }
 
