@@ -41,6 +41,7 @@ FLOPS_PER_UPDATE = 101  # algorithmic flops per pixel-pulse update (FMA = 2)
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sum_split_kernel_two_lines_pmc.md")
+STAGE_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_rangecomp_rslc_presum.md")
 
 
 def log(msg):
@@ -56,6 +57,18 @@ def hbm_bytes_per_launch():
     except OSError:
         return None
     m = re.search(r"HBM bytes per launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def stage_traffic(key):
+    """HBM bytes per launch (per call for presum) of the range-compression,
+    RSLC or presum kernel on its bench workload, from the committed counter
+    summary; None when it is absent."""
+    try:
+        text = open(STAGE_TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"HBM bytes per (?:launch|call) \(" + key + r"\):\s*([0-9.eE+]+)", text)
     return float(m.group(1)) if m else None
 
 
@@ -389,7 +402,7 @@ def run_rangecomp(args, torch, np, dev):
            "kernel": f"rc::conv_kernel<{int(np.log2(rc.plan[1]))}>",
            "roofline": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": 8000.0,
                         "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 8e12,
-                        "traffic": None},
+                        "traffic": stage_traffic("rangecomp")},
            "rel_error_vs_fp64_oracle": rel,
            "cpu_baseline": {"value": nb / cpu_s, "unit": "pulses/s", "cores": 1, "kind": "port",
                             "sample": f"{nb} pulses, numpy complex64 FFTs"}}
@@ -447,7 +460,8 @@ def run_rslc(args, torch, np, dev):
            "ms": ms, "samples_per_s": n * n / (ms * 1e-3),
            "kernel": "rslc::finish_kernel (+ one-workgroup merge_kernel)",
            "roofline": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": 8000.0,
-                        "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 8e12, "traffic": None},
+                        "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 8e12,
+                        "traffic": stage_traffic("rslc")},
            "bit_exact_vs_oracle": bit_exact, "n_valid": st.n_valid,
            "cpu_baseline": {"value": nb * n / cpu_s, "unit": "samples/s", "cores": 1,
                             "kind": "port", "sample": f"{nb} x {n} samples, numpy oracle"}}
@@ -519,7 +533,8 @@ def run_presum(args, torch, np, dev):
            "kernel": "presum::resample_kernel (host weights included in ms_per_call)",
            "roofline_call": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9,
                              "peak": 8000.0, "unit": "GB/s",
-                             "frac": nbytes / (ms * 1e-3) / 8e12, "traffic": None},
+                             "frac": nbytes / (ms * 1e-3) / 8e12,
+                             "traffic": stage_traffic("presum")},
            "rel_error_vs_oracle": rel,
            "cpu_baseline": {"value": nb * nr / cpu_s, "unit": "output samples/s",
                             "cores": 1, "kind": "port",
