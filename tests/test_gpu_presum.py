@@ -121,6 +121,42 @@ def test_resample_across_launch_chunks(torch_gpu):
     assert np.array_equal(out[rows] == 0, ref == 0)
 
 
+def test_resample_concurrent_calls_share_the_staging_pool(torch_gpu):
+    """Two host threads resampling different scenes at once, each on its own
+    stream, take distinct staging buffers from the pool: each result equals
+    the same call made alone."""
+    import threading
+    torch = torch_gpu
+    from isce3_amd.presum import resample
+    scenes = [_dithered_scene(nt=2600, nr=128, gap=14, seed=s) for s in (5, 6)]
+    alone = [resample(torch.from_numpy(sc[5]).cuda(), sc[3], sc[0], sc[4], sc[1], sc[2]).cpu()
+             for sc in scenes]
+    got = [None, None]
+    errors_seen = []
+
+    def work(k):
+        try:
+            sc = scenes[k]
+            stream = torch.cuda.Stream()
+            with torch.cuda.stream(stream):
+                for _ in range(3):
+                    out = resample(torch.from_numpy(sc[5]).cuda(), sc[3], sc[0], sc[4], sc[1],
+                                   sc[2])
+                stream.synchronize()
+                got[k] = out.cpu()
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors_seen.append(e)
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors_seen
+    for k in range(2):
+        assert torch.equal(got[k], alone[k])
+
+
 def test_resample_uniform_grid_is_identity(torch_gpu):
     """With pulses already on the uniform grid (no dither, no gaps) every
     output time is an input time, whose BLU weights are the unit vector, and
