@@ -157,6 +157,28 @@ def test_resample_concurrent_calls_share_the_staging_pool(torch_gpu):
         assert torch.equal(got[k], alone[k])
 
 
+def test_resample_error_after_launches_leaves_the_library_usable(torch_gpu):
+    """An orbit that ends inside the scene: the first launch (output pulses
+    0-511) is planned and running when the planning of a later launch finds
+    an output time past the orbit.  The call raises OutOfRange after the
+    running launch has finished; the next call (full orbit) is unaffected."""
+    from isce3_amd import errors
+    from isce3_amd.core import Orbit
+    from isce3_amd.presum import resample
+    from isce3_amd.sim import EPOCH, SPEED, WGS84_A, ALTITUDE
+    grid, orbit, lut, t, sw, raw, fd, speed = _dithered_scene(nt=2600, nr=96, gap=12, seed=8)
+    ts = np.arange(-0.9, 0.31, 0.1)                    # output times run to +0.85 s
+    pos = np.array([WGS84_A + ALTITUDE, 0.0, 0.0])[None, :] + ts[:, None] * np.array([0, 0, SPEED])
+    short = Orbit.from_arrays(EPOCH, float(ts[0]), 0.1, pos, np.repeat([[0.0, 0.0, SPEED]], len(ts), 0))
+    with pytest.raises(errors.OutOfRange):
+        resample(raw, t, grid, sw, short, lut, L=12.0)
+    out = resample(raw, t, grid, sw, orbit, lut, L=12.0)
+    rows = np.r_[0:3, 2597:2600]
+    ref = po.resample(raw, t, grid.prf, grid.starting_range, grid.range_pixel_spacing, sw,
+                      lambda tout: speed, fd, L=12.0, rows=rows)
+    assert rel(out[rows], ref) <= 1e-5
+
+
 def test_resample_uniform_grid_is_identity(torch_gpu):
     """With pulses already on the uniform grid (no dither, no gaps) every
     output time is an input time, whose BLU weights are the unit vector, and
