@@ -569,6 +569,9 @@ def main():
     ap.add_argument("--options", type=json.loads, default=None,
                     help="fixed launch options (JSON) for the full block instead of the "
                          "autotuner's, e.g. '{\"tile_rows\": 4}'")
+    ap.add_argument("--tune-cache", default=None, metavar="JSON",
+                    help="file the autotuner keeps its choices in (default: "
+                         "$ISCE3_AMD_TUNE_CACHE or ~/.cache/isce3_amd/autotune.json)")
     ap.add_argument("--no-tune", action="store_true",
                     help="default launch options for the full block (profiling runs)")
     ap.add_argument("--only", choices=["plain", "split"], default=None,
@@ -625,9 +628,7 @@ def main():
         print(json.dumps({"presum": run_presum(args, torch, np, dev)}))
         return
 
-    out_dir = os.path.join(ROOT, "gpurun_out")
-    tuner = autotune.Autotuner(os.path.join(out_dir, "autotune.json")
-                               if os.path.isdir(out_dir) else None)
+    tuner = autotune.Autotuner(args.tune_cache)
     sc, raw_host, results = run_full(args, torch, np, dev, tuner)
     if args.only:
         r = results[args.only]
