@@ -175,6 +175,12 @@ struct PulseDesc {
     int64_t offset;
     int32_t nw, pbase, npat, wbase;  // weights of pattern q: pat_w[wbase + q nw + k]
     LutAxis ly;                      // Doppler LUT azimuth axis at tout
+    // device addresses, formed on the host when the launch's tables are placed
+    const float2* raw_row;           // input pulse `offset`, bin 0
+    float2* out_row;                 // this output pulse, bin 0
+    const double* t_row;             // the window's pulse times
+    const int64_t* ids;              // this pulse's npat pattern ids
+    const float* w;                  // this pulse's weights, npat x nw
 };
 
 struct ResampleArgs {
@@ -197,7 +203,8 @@ constexpr int kPulsesPerThread = 8;    // output pulses per thread
 constexpr int64_t kFirstChunk = 512;   // output pulses of the first launch,
 constexpr int64_t kChunk = 2048;       // of the others (multiples of kPulsesPerThread)
 constexpr int64_t kItem = 32;          // output pulses a host thread plans at a time
-constexpr int kTapGroup = 4;           // window taps loaded together
+constexpr int kTapGroup = 4;           // window taps loaded together (longer windows)
+constexpr int kFixedTaps = 8;          // windows up to this many taps: unrolled per count
 
 // complex64 product and sum rounded term by term (no fused multiply-add)
 __device__ inline float2 cmul(float2 a, float2 b)
@@ -264,6 +271,34 @@ __device__ inline double lut_combine(const LutCorners& q, const LutAxis& x, cons
            (q.q12 * ((x2 - x.v) * (y.v - y1))) + (q.q22 * ((x.v - x1) * (y.v - y1)));
 }
 
+// One output sample from a window of N taps (N fixed at compile time): the
+// taps' loads first, then the products, accumulated in tap order.
+template<int N, class WU, class TU>
+__device__ inline float2 window_sum(const float2* row, int64_t stride, uint32_t jj, const float* w,
+                                    WU w0, bool uniform_pat, TU tp, double tout, double fd)
+{
+    float2 x[N];
+    float wv[N];
+    double tk[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        x[k] = (row + k * stride)[jj];
+        wv[k] = uniform_pat ? w0[k] : w[k];
+        tk[k] = tp[k];
+    }
+    float2 sum = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        // deramp exp(-2 pi i (t_k - tout) fd): the phase in turns, reduced to
+        // [0, 1) in FP64, then the hardware sine and cosine of 2 pi times it
+        const double trel = tk[k] - tout;
+        const float turns = (float)__builtin_amdgcn_fract(-trel * fd);
+        const float sn = __builtin_amdgcn_sinf(turns), cs = __builtin_amdgcn_cosf(turns);
+        sum = cadd(sum, cmul(make_float2(wv[k] * cs, wv[k] * sn), x[k]));
+    }
+    return sum;
+}
+
 __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
 {
     const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -305,7 +340,7 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
         // the pulse's patterns are distinct: stop at the match (-1: none)
         int pat = -1;
         for (int q = 0; q < d.npat; ++q)
-            if (uniform_ptr(a.pat_id)[d.pbase + q] == id) {
+            if (uniform_ptr(d.ids)[q] == id) {
                 pat = q;
                 break;
             }
@@ -324,13 +359,26 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
             fd = lut_combine(q, lx, d.ly);
         }
         float2 sum = make_float2(0.f, 0.f);
-        if (pat >= 0) {
+        const float2* row = d.raw_row;
+        const auto* tp = uniform_ptr(d.t_row);
+        const float* w = d.w + (int64_t)max(pat, 0) * d.nw;
+        const auto* w0 = uniform_ptr(d.w) + (int64_t)max(pat0, 0) * d.nw;
+        if (pat >= 0 && d.nw <= kFixedTaps) {
+            // windows of up to kFixedTaps pulses with the tap count fixed at
+            // compile time
+            switch (d.nw) {
+#define PRESUM_TAPS(N)                                                                        \
+    case N:                                                                                   \
+        sum = window_sum<N>(row, a.width, jj, w, w0, uniform_pat, tp, d.tout, fd);            \
+        break;
+                PRESUM_TAPS(1) PRESUM_TAPS(2) PRESUM_TAPS(3) PRESUM_TAPS(4)
+                PRESUM_TAPS(5) PRESUM_TAPS(6) PRESUM_TAPS(7) PRESUM_TAPS(8)
+#undef PRESUM_TAPS
+            default: break;
+            }
+        } else if (pat >= 0) {
             // taps in groups of kTapGroup: the group's loads first, then the
             // products, accumulated in tap order
-            const float2* row = a.raw + d.offset * a.width;   // input pulse d.offset
-            const auto* tp = uniform_ptr(a.t) + d.offset;
-            const float* w = a.pat_w + d.wbase + (int64_t)pat * d.nw;
-            const auto* w0 = uniform_ptr(a.pat_w) + d.wbase + (int64_t)pat0 * d.nw;
             for (int k0 = 0; k0 < d.nw; k0 += kTapGroup) {
                 float2 x[kTapGroup];
                 float wv[kTapGroup];
@@ -356,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
                     }
             }
         }
-        (a.out + (a.row0 + i) * a.width)[jj] = sum;
+        d.out_row[jj] = sum;
     }
 }
 
@@ -475,31 +523,42 @@ struct Staging {
     void* host = nullptr;
     void* dev = nullptr;
     size_t capacity = 0;
-    // Packs the parts at 256-byte aligned offsets (into off) and enqueues the
-    // upload on s (the current device is `device`).
-    hipError_t upload(const Part* parts, int n, size_t* off, hipStream_t s)
+    size_t pending = 0;   // bytes placed by the last reserve()
+    // Places the parts at 256-byte aligned offsets (into off), growing the
+    // buffers if needed (the current device is `device`); then send() packs
+    // and enqueues the upload on s.
+    hipError_t reserve(const Part* parts, int n, size_t* off)
     {
         size_t total = 0;
         for (int k = 0; k < n; ++k) {
             off[k] = total;
             total += (parts[k].bytes + 255) / 256 * 256;
         }
-        if (total == 0) return hipSuccess;
-        if (total > capacity) {
-            if (dev) (void)hipFree(dev);
-            if (host) (void)hipHostFree(host);
-            dev = host = nullptr;
-            capacity = 0;
-            const size_t cap = std::max<size_t>(total + total / 2, size_t(1) << 20);
-            hipError_t e = hipHostMalloc(&host, cap, hipHostMallocDefault);
-            if (e != hipSuccess) return e;
-            e = hipMalloc(&dev, cap);
-            if (e != hipSuccess) return e;
-            capacity = cap;
-        }
+        pending = total;
+        if (total == 0 || total <= capacity) return hipSuccess;
+        if (dev) (void)hipFree(dev);
+        if (host) (void)hipHostFree(host);
+        dev = host = nullptr;
+        capacity = 0;
+        const size_t cap = std::max<size_t>(total + total / 2, size_t(1) << 20);
+        hipError_t e = hipHostMalloc(&host, cap, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+        e = hipMalloc(&dev, cap);
+        if (e != hipSuccess) return e;
+        capacity = cap;
+        return hipSuccess;
+    }
+    hipError_t send(const Part* parts, int n, const size_t* off, hipStream_t s)
+    {
+        if (pending == 0) return hipSuccess;
         for (int k = 0; k < n; ++k)
             if (parts[k].bytes) std::memcpy(static_cast<char*>(host) + off[k], parts[k].src, parts[k].bytes);
-        return hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, s);
+        return hipMemcpyAsync(dev, host, pending, hipMemcpyHostToDevice, s);
+    }
+    hipError_t upload(const Part* parts, int n, size_t* off, hipStream_t s)
+    {
+        const hipError_t e = reserve(parts, n, off);
+        return e != hipSuccess ? e : send(parts, n, off, s);
     }
     const char* at(size_t off) const { return static_cast<const char*>(dev) + off; }
 };
@@ -788,12 +847,24 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
                                {pat_w.data(), sizeof(float) * pat_w.size()}};
         size_t off[3];
         Staging* staged = leases.take(device);
-        e = staged->upload(parts, 3, off, s);
+        e = staged->reserve(parts, 3, off);
         if (e != hipSuccess)
             return finish(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
         a.desc = reinterpret_cast<const PulseDesc*>(staged->at(off[0]));
         a.pat_id = reinterpret_cast<const int64_t*>(staged->at(off[1]));
         a.pat_w = reinterpret_cast<const float*>(staged->at(off[2]));
+        // the device addresses each pulse's sum reads and writes
+        for (int64_t i = c0; i < c0 + rows; ++i) {
+            PulseDesc& d = desc[i];
+            d.raw_row = a.raw + d.offset * W;
+            d.out_row = a.out + i * W;
+            d.t_row = a.t + d.offset;
+            d.ids = a.pat_id + d.pbase;
+            d.w = a.pat_w + d.wbase;
+        }
+        e = staged->send(parts, 3, off, s);
+        if (e != hipSuccess)
+            return finish(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
         a.row0 = c0;
         a.rows = rows;
         const dim3 blocks((unsigned)((W + kThreads - 1) / kThreads),
