@@ -524,6 +524,7 @@ struct Staging {
     void* dev = nullptr;
     size_t capacity = 0;
     size_t pending = 0;   // bytes placed by the last reserve()
+    hipEvent_t ready = nullptr;   // the last upload's completion
     // Places the parts at 256-byte aligned offsets (into off), growing the
     // buffers if needed (the current device is `device`); then send() packs
     // and enqueues the upload on s.
@@ -548,17 +549,26 @@ struct Staging {
         capacity = cap;
         return hipSuccess;
     }
-    hipError_t send(const Part* parts, int n, const size_t* off, hipStream_t s)
+    // The copy goes on stream `copy`; `s` waits for it (an event kept with
+    // the buffer), so copies for later launches overlap the kernels on s.
+    hipError_t send(const Part* parts, int n, const size_t* off, hipStream_t copy, hipStream_t s)
     {
         if (pending == 0) return hipSuccess;
         for (int k = 0; k < n; ++k)
             if (parts[k].bytes) std::memcpy(static_cast<char*>(host) + off[k], parts[k].src, parts[k].bytes);
-        return hipMemcpyAsync(dev, host, pending, hipMemcpyHostToDevice, s);
+        hipError_t e = hipMemcpyAsync(dev, host, pending, hipMemcpyHostToDevice, copy);
+        if (e != hipSuccess || copy == s) return e;
+        if (ready == nullptr) {
+            e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+            if (e != hipSuccess) return e;
+        }
+        e = hipEventRecord(ready, copy);
+        return e != hipSuccess ? e : hipStreamWaitEvent(s, ready, 0);
     }
-    hipError_t upload(const Part* parts, int n, size_t* off, hipStream_t s)
+    hipError_t upload(const Part* parts, int n, size_t* off, hipStream_t copy, hipStream_t s)
     {
         const hipError_t e = reserve(parts, n, off);
-        return e != hipSuccess ? e : send(parts, n, off, s);
+        return e != hipSuccess ? e : send(parts, n, off, copy, s);
     }
     const char* at(size_t off) const { return static_cast<const char*>(dev) + off; }
 };
@@ -584,6 +594,20 @@ struct StagingPool {
         std::lock_guard<std::mutex> lk(mu);
         idle.push_back(b);
     }
+    // One non-blocking stream per device for the uploads (nullptr if it
+    // cannot be created: the uploads then go on the caller's stream); the
+    // current device is `device`.
+    std::vector<std::pair<int, hipStream_t>> copy_streams;
+    hipStream_t copy_stream(int device)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (const auto& cs : copy_streams)
+            if (cs.first == device) return cs.second;
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        copy_streams.emplace_back(device, st);
+        return st;
+    }
 };
 
 StagingPool& staging_pool()
@@ -593,9 +617,10 @@ StagingPool& staging_pool()
 }
 
 // The staging buffers one call holds; they go back to the pool once the
-// call's stream has finished with them.
+// call's stream and the copy stream have finished with them.
 struct Leases {
     hipStream_t stream;
+    hipStream_t copy;
     std::vector<Staging*> held;
     Staging* take(int device)
     {
@@ -604,7 +629,10 @@ struct Leases {
     }
     ~Leases()
     {
-        if (!held.empty()) (void)hipStreamSynchronize(stream);
+        if (!held.empty()) {
+            (void)hipStreamSynchronize(stream);
+            if (copy != stream) (void)hipStreamSynchronize(copy);
+        }
         for (Staging* b : held) staging_pool().give(b);
     }
 };
@@ -764,7 +792,10 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
         }
     };
     hipStream_t s = static_cast<hipStream_t>(stream);
-    Leases leases{s, {}};
+    const bool device_set = hipSetDevice(device) == hipSuccess;
+    hipStream_t copy = device_set ? staging_pool().copy_stream(device) : nullptr;
+    if (copy == nullptr) copy = s;
+    Leases leases{s, copy, {}};
     Joiner joiner{pool, abort};
     auto finish = [&](int code, const std::string& msg) {
         abort = true;
@@ -775,7 +806,7 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
             return fail(BP_ERR_HIP, std::string("presum resample: ") + hipGetErrorString(e));
         return code == BP_SUCCESS ? BP_SUCCESS : fail(code, msg);
     };
-    if (hipSetDevice(device) != hipSuccess) return finish(BP_ERR_HIP, "hipSetDevice failed");
+    if (!device_set) return finish(BP_ERR_HIP, "hipSetDevice failed");
     // tables every chunk reads: pulse times, sub-swaths, Doppler samples
     const Part shared_parts[3] = {
         {t, sizeof(double) * nt},
@@ -784,7 +815,7 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
          doppler->have_data ? sizeof(double) * doppler->width * doppler->length : 0}};
     size_t shared_off[3];
     Staging* shared = leases.take(device);
-    hipError_t e = shared->upload(shared_parts, 3, shared_off, s);
+    hipError_t e = shared->upload(shared_parts, 3, shared_off, copy, s);
     if (e != hipSuccess) return finish(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
 
     ResampleArgs a;
@@ -862,7 +893,7 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
             d.ids = a.pat_id + d.pbase;
             d.w = a.pat_w + d.wbase;
         }
-        e = staged->send(parts, 3, off, s);
+        e = staged->send(parts, 3, off, copy, s);
         if (e != hipSuccess)
             return finish(BP_ERR_HIP, std::string("presum upload: ") + hipGetErrorString(e));
         a.row0 = c0;
