@@ -138,10 +138,15 @@ __device__ inline Partial block_reduce(Partial p)
     return p;
 }
 
+// Native 4 x float vector for the streaming (non-temporal) accesses: the
+// block is read once and written once, so its lines need not stay in L2.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 // One chunk of kPerBlock samples: scale, accumulate, encode and store; the
 // thread's summary of its valid samples in *p (two passes over registers).
 // kPair: each access moves two adjacent samples (16-byte loads and stores of
-// z and prior, 8-byte stores of the encoding); otherwise one sample.
+// z and prior, 8-byte stores of the encoding, all non-temporal); otherwise
+// one sample.
 template<bool kPair>
 __device__ inline void finish_chunk(const FinishArgs& a, int64_t chunk, Partial* p)
 {
@@ -159,11 +164,12 @@ __device__ inline void finish_chunk(const FinishArgs& a, int64_t chunk, Partial*
         const int64_t e = e0 + (int64_t)j * P * kThreads;
         float2 x[P], q[P];
         if (kPair && e + 1 < a.n) {
-            const float4 z4 = *reinterpret_cast<const float4*>(a.z + e);
+            const v4f z4 = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(a.z + e));
             x[0] = make_float2(z4.x, z4.y);
             x[P - 1] = make_float2(z4.z, z4.w);
             if (a.prior) {
-                const float4 p4 = *reinterpret_cast<const float4*>(a.prior + e);
+                const v4f p4 =
+                    __builtin_nontemporal_load(reinterpret_cast<const v4f*>(a.prior + e));
                 q[0] = make_float2(p4.x, p4.y);
                 q[P - 1] = make_float2(p4.z, p4.w);
             }
@@ -232,10 +238,12 @@ __device__ inline void finish_chunk(const FinishArgs& a, int64_t chunk, Partial*
         }
         if (kPair && e + 1 < a.n) {
             if (a.data_type == BP_RSLC_COMPLEX32)
-                *reinterpret_cast<uint2*>(a.encoded + e) = make_uint2(h[0], h[P - 1]);
+                __builtin_nontemporal_store(
+                    (unsigned long long)h[0] | ((unsigned long long)h[P - 1] << 32),
+                    reinterpret_cast<unsigned long long*>(a.encoded + e));
             if (a.store_z)
-                *reinterpret_cast<float4*>(a.z + e) =
-                    make_float4(x[0].x, x[0].y, x[P - 1].x, x[P - 1].y);
+                __builtin_nontemporal_store(v4f{x[0].x, x[0].y, x[P - 1].x, x[P - 1].y},
+                                            reinterpret_cast<v4f*>(a.z + e));
         } else {
 #pragma unroll
             for (int u = 0; u < P; ++u) {
