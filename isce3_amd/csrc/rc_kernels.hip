@@ -30,6 +30,10 @@ __device__ inline float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
 __device__ inline int pad(int i) { return i + (i >> 4); }
 
+// Native 2 x float vector for the non-temporal stores of the compressed
+// pulses (written once, not read again by the kernel).
+typedef float v2f __attribute__((ext_vector_type(2)));
+
 // Forward DFTs (sign -1) of R values in registers.
 __device__ inline void dft2(float2& a, float2& b)
 {
@@ -354,7 +358,10 @@ __global__ __launch_bounds__((1 << LOG2L) / 16) void conv_kernel(ConvArgs a)
         for (int r = 0; r < 16; ++r) {
             const int t = j + r * Ns;           // transform sample: valid from `shift` on
             const int64_t i = o0 + t - a.offset;
-            if (t >= a.shift && i >= 0 && i < a.n_out) y[i] = conjf2(d.v[0][r]);
+            if (t >= a.shift && i >= 0 && i < a.n_out) {
+                const float2 c = conjf2(d.v[0][r]);
+                __builtin_nontemporal_store(v2f{c.x, c.y}, reinterpret_cast<v2f*>(y + i));
+            }
         }
         // one block per pulse: outputs past the transform are zero
         if (a.nblk == 1 && a.shift == 0)
