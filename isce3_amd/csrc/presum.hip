@@ -220,6 +220,10 @@ __device__ inline float2 cadd(float2 a, float2 b)
 }
 
 // Valid-sample bit of range bin j in input pulse p (any sub-swath).
+// Native 2 x float vector for the non-temporal stores of the output samples
+// (written once, not read again by the kernel).
+typedef float v2f __attribute__((ext_vector_type(2)));
+
 // A wave-uniform read-only pointer as a constant-address-space pointer: its
 // loads go through the scalar data cache (one load per wavefront instead of
 // one vector memory instruction of 64 addresses).  Only for data no kernel
@@ -404,7 +408,7 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
                     }
             }
         }
-        d.out_row[jj] = sum;
+        __builtin_nontemporal_store(v2f{sum.x, sum.y}, reinterpret_cast<v2f*>(d.out_row + jj));
     }
 }
 
