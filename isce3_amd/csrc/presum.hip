@@ -20,8 +20,9 @@
 // output pulses share most of their input pulses, so a thread's pulses
 // re-read them from cache.
 //
-// The output pulses go in launches of 2048: host threads plan the next
-// launches' patterns and weights while the GPU resamples the earlier ones.
+// The output pulses go in launches of 512, then 2048: host threads plan the
+// next launches' patterns and weights while the GPU resamples the earlier
+// ones, and the launches' tables go up on a copy stream beside them.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
