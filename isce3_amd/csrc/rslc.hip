@@ -11,7 +11,7 @@
 // in registers, and summarises its valid samples exactly in FP64 by two passes
 // (mean, then the sum of squared deviations).  Summaries are combined with
 // Chan's pairwise update -- over a thread's chunks, across the wave by lane
-// shuffles, across the workgroup through LDS, across the (at most 2048)
+// shuffles, across the workgroup through LDS, across the (at most 8192)
 // workgroups in a second kernel that merges the partials in a fixed order, so
 // results do not depend on scheduling.
 #include <hip/hip_runtime.h>
@@ -33,7 +33,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kPerThread = 16;
 constexpr int64_t kPerBlock = (int64_t)kThreads * kPerThread;
-constexpr int kMaxGrid = 2048;   // workgroups (and partial summaries) at most
+constexpr int kMaxGrid = 8192;   // workgroups (and partial summaries) at most
 
 // Summary of the valid samples of one component: count, mean, sum of squared
 // deviations, extremes.
