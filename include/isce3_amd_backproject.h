@@ -285,14 +285,21 @@ int bp_server_create(bp_scene* scene, const bp_options* options, int max_pending
                      bp_server** server);
 int bp_server_destroy(bp_server* server);
 /* Queue one block (its out_dev / height_dev device buffers must stay valid
- * until the ticket completes).  BP_ERR_LENGTH when max_pending requests are
- * waiting or the block alone exceeds the scene's max_pixels. */
+ * until the ticket completes).  The block is checked as bp_backproject_blocks
+ * checks its blocks (shape, PRF > 0, output buffer): a bad request is refused
+ * here with BP_ERR_INVALID_ARGUMENT and never joins a batch.  BP_ERR_LENGTH
+ * when max_pending requests are waiting or the block alone exceeds the
+ * scene's max_pixels. */
 int bp_serve_submit(bp_server* server, const bp_out_block* block, uint64_t* ticket);
 /* Non-blocking: *done = 1 once the request has been focused (its result code
  * returned, *failed = some pixel failed to converge), else 0.  A completed
  * ticket is forgotten after it has been reported. */
 int bp_serve_poll(bp_server* server, uint64_t ticket, int* done, int* failed);
-/* Block until the request has been focused; returns its result code. */
+/* Block until the request has been focused; returns its result code.  A
+ * ticket has one consumer: when another thread's wait or poll reports it
+ * first, this wait returns BP_ERR_INVALID_ARGUMENT ("unknown ticket").
+ * bp_server_destroy lets every thread blocked here return before it frees the
+ * server; calls that start after destroy has begun are not allowed. */
 int bp_serve_wait(bp_server* server, uint64_t ticket, int* failed);
 
 /* ---------------------------------------------------- host-side helpers

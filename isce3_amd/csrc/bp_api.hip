@@ -30,11 +30,31 @@ int fail(int code, const std::string& msg)
 // Shared with the range-compression entry points (rc_api.hip).
 void bp_set_last_error(const std::string& msg) { g_msg = msg; }
 
+namespace bp {
+
+int check_out_block(const bp_out_block& b)
+{
+    const bp_grid& go = b.grid;
+    if (go.length < 0 || go.width < 0) return fail(BP_ERR_INVALID_ARGUMENT, "negative grid size");
+    if (go.length > INT32_MAX || go.width > INT32_MAX)
+        return fail(BP_ERR_INVALID_ARGUMENT, "grid size exceeds int range");
+    if (!(go.prf > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "PRF must be positive");
+    if (go.length * go.width > 0 && b.out_dev == nullptr)
+        return fail(BP_ERR_INVALID_ARGUMENT, "out is null");
+    return BP_SUCCESS;
+}
+
+}  // namespace bp
+
 namespace {
 
+// A device buffer.  `fixed` buffers are views of a region the library does
+// not own (the static scratch of bp_backproject_noalloc): they are never
+// grown or freed, and a call that needs more than they hold fails.
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool fixed = false;
 };
 
 }  // namespace
@@ -71,6 +91,10 @@ int ensure(DevBuf& b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
     if (b.bytes >= bytes) return BP_SUCCESS;
+    if (b.fixed)
+        return fail(BP_ERR_LENGTH, "call needs " + std::to_string(bytes) +
+                                       " bytes of a fixed scratch buffer of " +
+                                       std::to_string(b.bytes));
     if (b.ptr) {
         (void)hipFree(b.ptr);
         b.ptr = nullptr;
@@ -494,15 +518,10 @@ int check_blocks(bp_out_block* blocks, int n_blocks, int64_t* npix)
         return fail(BP_ERR_INVALID_ARGUMENT, "invalid output block list");
     *npix = 0;
     for (int b = 0; b < n_blocks; ++b) {
-        const bp_grid& go = blocks[b].grid;
         blocks[b].failed = 0;
-        if (go.length < 0 || go.width < 0) return fail(BP_ERR_INVALID_ARGUMENT, "negative grid size");
-        if (go.length > INT32_MAX || go.width > INT32_MAX)
-            return fail(BP_ERR_INVALID_ARGUMENT, "grid size exceeds int range");
-        if (!(go.prf > 0)) return fail(BP_ERR_INVALID_ARGUMENT, "PRF must be positive");
-        if (go.length * go.width > 0 && blocks[b].out_dev == nullptr)
-            return fail(BP_ERR_INVALID_ARGUMENT, "out is null");
-        *npix += go.length * go.width;
+        const int rc = bp::check_out_block(blocks[b]);
+        if (rc != BP_SUCCESS) return rc;
+        *npix += blocks[b].grid.length * blocks[b].grid.width;
     }
     return BP_SUCCESS;
 }
@@ -724,8 +743,10 @@ int finish_step(const int32_t* hflags, int n_grids, const std::vector<int>& bloc
             BP_HIP_CHECK(hipEventElapsedTime(&ms, ev[1], ev[2]));
             stats->sum_ms = ms;
         }
-        BP_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[2]));
-        stats->step_ms = ms;
+        if (ev[0] && ev[2]) {  // calls without events report no times
+            BP_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[2]));
+            stats->step_ms = ms;
+        }
         stats->updates = (int64_t)hupd;
         stats->any_failed = hflags[0] != 0;
     }

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/isce3_amd_backproject.h"
 #include "bp_math.h"
 
 namespace bp {
@@ -176,6 +177,10 @@ const void* kernel_fn(int which, int width, bool staged, bool pair);
 
 // Scene facts the serving queue needs (bp_api.hip).
 int scene_device(const struct bp_scene* scene);
+// Argument rules of one output block (shape, PRF, output buffer), shared by
+// the batched calls and the serving queue's admission; returns BP_SUCCESS or
+// a negative code with the thread's message set.
+int check_out_block(const bp_out_block& block);
 int64_t scene_max_pixels(const struct bp_scene* scene);
 int scene_max_blocks(const struct bp_scene* scene);
 

@@ -38,6 +38,7 @@ struct Result {
     bool done = false;
     int code = BP_SUCCESS;
     int failed = 0;
+    std::string msg;   // the batch's error message when code < 0
 };
 
 }  // namespace
@@ -57,6 +58,8 @@ struct bp_server {
     std::unordered_map<uint64_t, Result> results;
     uint64_t next_ticket = 1;
     bool stopping = false;
+    int waiters = 0;                  // threads inside bp_serve_wait
+    std::condition_variable left_cv;  // a waiter left (destroy waits for none)
     std::thread worker;
 
     void run()
@@ -84,6 +87,7 @@ struct bp_server {
             int any = 0;
             const int code = bp_scene_focus(scene, batch.data(), (int)batch.size(), &options,
                                             stream, &any, nullptr);
+            const std::string msg = code < 0 ? std::string(bp_last_error_message()) : std::string();
             {
                 std::lock_guard<std::mutex> lk(mu);
                 for (size_t i = 0; i < batch.size(); ++i) {
@@ -91,6 +95,7 @@ struct bp_server {
                     r.done = true;
                     r.code = code < 0 ? code : (batch[i].failed ? BP_FAILED_TO_CONVERGE : BP_SUCCESS);
                     r.failed = batch[i].failed;
+                    r.msg = msg;
                 }
             }
             done_cv.notify_all();
@@ -138,6 +143,13 @@ int bp_server_destroy(bp_server* s)
     }
     s->work_cv.notify_all();
     if (s->worker.joinable()) s->worker.join();
+    // every accepted request is complete now; let the threads still inside
+    // bp_serve_wait return before the server goes away
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->done_cv.notify_all();
+        s->left_cv.wait(lk, [&] { return s->waiters == 0; });
+    }
     (void)hipSetDevice(s->device);
     (void)hipStreamDestroy(s->stream);
     delete s;
@@ -148,13 +160,13 @@ int bp_serve_submit(bp_server* s, const bp_out_block* block, uint64_t* ticket)
 {
     if (s == nullptr || block == nullptr || ticket == nullptr)
         return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
-    if (block->grid.length < 0 || block->grid.width < 0)
-        return fail(BP_ERR_INVALID_ARGUMENT, "negative block shape");
+    // the per-block rules of the batched call, checked here so that a bad
+    // request is refused alone instead of failing the batch it would join
+    const int bad = bp::check_out_block(*block);
+    if (bad != BP_SUCCESS) return bad;
     if (block_pixels(*block) > s->max_pixels)
         return fail(BP_ERR_LENGTH, "request of " + std::to_string(block_pixels(*block)) +
                                        " pixels exceeds the scene's max_pixels");
-    if (block_pixels(*block) > 0 && block->out_dev == nullptr)
-        return fail(BP_ERR_INVALID_ARGUMENT, "null output buffer");
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stopping) return fail(BP_ERR_RUNTIME, "server is shutting down");
@@ -180,8 +192,9 @@ int bp_serve_poll(bp_server* s, uint64_t ticket, int* done, int* failed)
     if (!it->second.done) return BP_SUCCESS;
     if (failed) *failed = it->second.failed;
     const int code = it->second.code;
+    const std::string msg = it->second.msg;
     s->results.erase(it);
-    if (code < 0) return fail(code, "focusing the request's batch failed");
+    if (code < 0) return fail(code, "focusing the request's batch failed: " + msg);
     return code;
 }
 
@@ -191,12 +204,26 @@ int bp_serve_wait(bp_server* s, uint64_t ticket, int* failed)
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->results.find(ticket);
     if (it == s->results.end()) return fail(BP_ERR_INVALID_ARGUMENT, "unknown ticket");
-    s->done_cv.wait(lk, [&] { return s->results.at(ticket).done; });
+    // A ticket has one consumer: if another thread's wait or poll reports it
+    // first, the entry disappears and this wait returns "unknown ticket".
+    ++s->waiters;
+    s->done_cv.wait(lk, [&] {
+        auto f = s->results.find(ticket);
+        return f == s->results.end() || f->second.done;
+    });
+    --s->waiters;
     it = s->results.find(ticket);
-    if (failed) *failed = it->second.failed;
-    const int code = it->second.code;
-    s->results.erase(it);
-    if (code < 0) return fail(code, "focusing the request's batch failed");
+    int code = BP_ERR_INVALID_ARGUMENT;
+    std::string msg = "unknown ticket (reported to another caller)";
+    if (it != s->results.end()) {
+        if (failed) *failed = it->second.failed;
+        code = it->second.code;
+        msg = "focusing the request's batch failed: " + it->second.msg;
+        s->results.erase(it);
+    }
+    if (s->waiters == 0) s->left_cv.notify_all();
+    lk.unlock();
+    if (code < 0) return fail(code, msg);
     return code;
 }
 

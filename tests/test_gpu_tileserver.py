@@ -78,3 +78,67 @@ def test_close_drains_and_rejections(torch_gpu):
     torch.cuda.synchronize()
     for o, z0 in zip(outs, ref[0]):
         np.testing.assert_array_equal(o.cpu().numpy(), z0)
+
+
+def test_bad_request_refused_alone(torch_gpu):
+    """A request with PRF 0 is refused at submission; the requests around it
+    are batched and focused as if it had never been sent."""
+    torch = torch_gpu
+    from isce3_amd import errors
+    from isce3_amd.focus import Scene, TileServer
+    sc, raw = scene_and_raw("small")
+    raw_dev = torch.from_numpy(raw.copy()).cuda()
+    grids = uneven_blocks(sc)
+    ref = eager(sc, raw_dev, grids, "split", None)
+    scene = Scene(raw_dev, sc.in_geometry(), sc.orbit, sc.doppler, sc.dem, sc.fc, sc.ds,
+                  sc.kernel, sc.dry_tropo_model, precision="split", max_pixels=1 << 15,
+                  max_blocks=8)
+    bad = grids[4].copy()
+    bad.prf = 0.0
+    outs = [torch.zeros(tuple(g.shape), dtype=torch.complex64, device="cuda:0") for g in grids]
+    with TileServer(scene) as server:
+        tickets = []
+        for i, (o, g) in enumerate(zip(outs, grids)):
+            tickets.append(server.submit(o, g))
+            if i == 2:
+                with pytest.raises(errors.InvalidArgument, match="PRF"):
+                    server.submit(torch.zeros(tuple(bad.shape), dtype=torch.complex64,
+                                              device="cuda:0"), bad)
+        flags = [server.wait(t) for t in tickets]
+    scene.close()
+    torch.cuda.synchronize()
+    assert flags == ref[2]
+    for o, z0 in zip(outs, ref[0]):
+        np.testing.assert_array_equal(o.cpu().numpy(), z0)
+
+
+def test_two_waiters_on_one_ticket(torch_gpu):
+    """A ticket has one consumer: of two threads waiting on it, one gets the
+    result and the other an 'unknown ticket' error; neither hangs or crashes."""
+    torch = torch_gpu
+    from isce3_amd import errors
+    from isce3_amd.focus import Scene, TileServer
+    sc, raw = scene_and_raw("small")
+    raw_dev = torch.from_numpy(raw.copy()).cuda()
+    g = uneven_blocks(sc)[4]
+    scene = Scene(raw_dev, sc.in_geometry(), sc.orbit, sc.doppler, sc.dem, sc.fc, sc.ds,
+                  sc.kernel, sc.dry_tropo_model, precision="split", max_pixels=1 << 15,
+                  max_blocks=4)
+    out = torch.zeros(tuple(g.shape), dtype=torch.complex64, device="cuda:0")
+    results = []
+    with TileServer(scene) as server:
+        t = server.submit(out, g)
+
+        def waiter():
+            try:
+                results.append(("ok", server.wait(t)))
+            except errors.InvalidArgument as e:
+                results.append(("unknown", str(e)))
+        threads = [threading.Thread(target=waiter) for _ in range(2)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(60)
+        assert not any(th.is_alive() for th in threads)
+    scene.close()
+    assert sorted(r[0] for r in results) == ["ok", "unknown"]

@@ -174,9 +174,12 @@ def test_brent_reference_functions():
     assert oracle.lib().orc_brent_testfn(1, 5.0, 6.0, 1e-12, C.byref(r)) == 11
 
 
-@pytest.mark.skipif(oracle.ref_lib() is None, reason="oracle/_ref not built (no reference tree)")
 def test_brent_matches_reference_build_on_geometry_like_function():
+    # loaded inside the test (not at collection) so that collecting the suite
+    # maps no reference-built library into the process
     ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (no reference tree)")
     f = lambda x: math.cos(3 * x) - x / 7 + 0.1  # noqa: E731
     cb = ref.FN(lambda x, _c: f(x))
     r = C.c_double()
