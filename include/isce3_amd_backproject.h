@@ -22,6 +22,7 @@
 #ifndef ISCE3_AMD_BACKPROJECT_H
 #define ISCE3_AMD_BACKPROJECT_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -202,6 +203,50 @@ int bp_backproject(bp_context* ctx, const bp_radar_geometry* out_geometry,
                    const bp_options* options, void* stream, int* any_failed,
                    bp_stats* stats);
 
+/* What bp_backproject_host moved (filled when non-null). */
+typedef struct {
+    int64_t h2d_bytes;      /* pulse bytes copied host to device */
+    int64_t window_pulses;  /* pulses copied, summed over bands */
+    int32_t bands;          /* bands of output lines focused (1 unless the window
+                               exceeded max_window_bytes) */
+    int32_t pad_;
+    double copy_ms;         /* device time of the pulse copies (copy stream) */
+} bp_host_stats;
+
+/*
+ * Focus from range-compressed pulses in host memory -- an array or a memory
+ * map of a raster file -- moving only the pulses the output needs.
+ *
+ * Replaces isce3::cuda::focus::backproject with host input
+ * (cxx/isce3/cuda/focus/Backproject.cu:641-688: the geometry first, then the
+ * pulses [kstart_min, kstop_max) in batches of `batch`), as bound by
+ * python/extensions/pybind_isce3/cuda/focus/Backproject.cpp:87-116 and called
+ * per block with the whole raster by python/packages/nisar/workflows/
+ * focus.py:2001.  The union [kmin, kmax) of the output pixels' pulse windows
+ * is copied to the device in chunks of at most `batch` pulses (and at most
+ * 64 MiB) through pinned staging on a copy stream; no pulse outside it is
+ * read.  When that window exceeds max_window_bytes of device memory, the
+ * block is focused in bands of output lines, each with its own window.  The
+ * results equal bp_backproject's with the whole raster on the device, bit for
+ * bit, in every case.
+ *
+ *   in_host            host, complex64 [in.grid.length x in.grid.width]
+ *   batch              pulses per host-to-device copy (>= 1, else BP_ERR_DOMAIN)
+ *   max_window_bytes   device bytes for pulse windows (0: half the free memory);
+ *                      BP_ERR_LENGTH when the window of one tile row of output
+ *                      lines exceeds it
+ *   out_dev, height_dev  device buffers as in bp_backproject
+ *   host_stats         bytes and pulses moved, or NULL
+ */
+int bp_backproject_host(bp_context* ctx, const bp_radar_geometry* out_geometry,
+                        const bp_radar_geometry* in_geometry, const bp_dem* dem, double fc,
+                        double ds, const bp_kernel* kernel, int dry_tropo_model,
+                        const bp_rdr2geo_params* rdr2geo, const bp_geo2rdr_params* geo2rdr,
+                        const void* in_host, int64_t batch, int64_t max_window_bytes,
+                        void* out_dev, float* height_dev, int precision,
+                        const bp_options* options, void* stream, int* any_failed,
+                        bp_stats* stats, bp_host_stats* host_stats);
+
 /* One output block of bp_backproject_blocks. */
 typedef struct {
     bp_grid grid;        /* output radar grid of this block */
@@ -323,6 +368,10 @@ int bp_proj_forward(int epsg, const double* llh, double* xyz);
 int bp_proj_inverse(int epsg, const double* xyz, double* llh);
 /* LUT2d<double>::eval (core/LUT2d.cpp:127-159). */
 double bp_lut2d_eval(const bp_lut2d* lut, double y, double x);
+/* sin and cos of 2 pi t in FP64 for a phase t in turns: the phase rotation
+ * exp(j 2 pi fc tau) of the plain sum (Backproject.cpp:53-55 forms cos/sin of
+ * 2 pi fc tau in double). */
+void bp_sincos_turns(double t, double* s, double* c);
 /* KnabKernel<double>::operator() (core/Kernels.icc:29-52). */
 double bp_knab_eval(double width, double bandwidth, double x);
 /* TabulatedKernel<float> table fill from a Knab kernel (core/Kernels.icc:113-135). */

@@ -105,6 +105,11 @@ class Stats_t(C.Structure):
                 ("any_failed", C.c_int32), ("pad_", C.c_int32), ("step_ms", C.c_double)]
 
 
+class HostStats_t(C.Structure):
+    _fields_ = [("h2d_bytes", C.c_int64), ("window_pulses", C.c_int64), ("bands", C.c_int32),
+                ("pad_", C.c_int32), ("copy_ms", C.c_double)]
+
+
 _P = C.POINTER
 _SIGNATURES = {
     "bp_context_create": (C.c_int, [C.c_int, _P(C.c_void_p)]),
@@ -162,6 +167,7 @@ _SIGNATURES = {
     "bp_proj_forward": (C.c_int, [C.c_int, _P(C.c_double), _P(C.c_double)]),
     "bp_proj_inverse": (C.c_int, [C.c_int, _P(C.c_double), _P(C.c_double)]),
     "bp_lut2d_eval": (C.c_double, [_P(Lut2d_t), C.c_double, C.c_double]),
+    "bp_sincos_turns": (None, [C.c_double, _P(C.c_double), _P(C.c_double)]),
     "bp_knab_eval": (C.c_double, [C.c_double, C.c_double, C.c_double]),
     "bp_tabulate_knab": (C.c_int, [C.c_double, C.c_double, C.c_int, _P(C.c_float)]),
     "bp_rdr2geo_bracket": (C.c_int, [C.c_double, C.c_double, C.c_double, _P(Orbit_t), _P(Dem_t),

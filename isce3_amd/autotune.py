@@ -10,8 +10,8 @@ The tunable options are those of ``bp_options`` (include/isce3_amd_backproject.h
     staged         split taps gathered from global memory (0) or read from
                    pulse windows staged in LDS (1)
     lane_rows      output lines per lane of the split sum (2: adjacent lines
-                   share one gather; kernel width 8 with staged 0 only --
-                   other combinations raise InvalidArgument and score as
+                   share one gather; staged 0 only -- with staged 1 the call
+                   raises InvalidArgument and the candidate scores as
                    infinitely slow)
 
 The plain kernel only has ``tile_rows``.  A tuning run times candidates with

@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/isce3_amd_backproject.h"
@@ -63,8 +65,10 @@ struct DevBuf {
 // blocks: orbits, Doppler tables, DEM, kernel tables and (plain precision)
 // the platform state at every pulse.
 struct StaticData {
-    DevBuf table, poly, poly2, in_pos, in_vel, out_pos, out_vel, in_dop, out_dop, dem, pulse_pv;
+    DevBuf table, table2, poly, poly2, in_pos, in_vel, out_pos, out_vel, in_dop, out_dop, dem,
+        pulse_pv;
     std::vector<float> poly_host, poly2_host;
+    std::vector<double> table2_host;
 };
 
 // Per-pixel records, batch header + output descriptors, and the flags of one
@@ -74,11 +78,31 @@ struct Scratch {
     DevBuf geom, split, flags, desc;
 };
 
+// Buffers of the host-input path (bp_backproject_host): device pulse windows,
+// band descriptors and windows, pinned staging, a copy stream and its events.
+// Created on first use and kept by the context.
+struct HostPath {
+    DevBuf win[2];            // device copies of band pulse windows
+    DevBuf band_desc;         // batch header + descriptor of every band
+    DevBuf band_win;          // [kmin, kmax) of every line band (int32 pairs)
+    unsigned char* stage[2] = {nullptr, nullptr};  // pinned staging of pulse chunks
+    size_t stage_bytes = 0;
+    int32_t* h_win = nullptr; // pinned copy of band_win
+    size_t h_win_words = 0;
+    hipStream_t copy = nullptr;
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};  // copy out of staging j finished
+    hipEvent_t win_ev = nullptr;                  // band windows on the host
+    hipEvent_t copied_ev = nullptr;               // a band's pulses on the device
+    hipEvent_t sum_ev[2] = {nullptr, nullptr};    // sum reading window buffer w finished
+    std::vector<hipEvent_t> timing;               // per band: copy start/end, sum start/end
+};
+
 struct bp_context {
     int device = 0;
     int n_cu = 0;
     StaticData st;
     Scratch sc;
+    HostPath hp;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     std::vector<unsigned char> desc_host;
     std::vector<int> desc_block;
@@ -290,15 +314,17 @@ void build_polyphase(const bp_kernel& k, int width, std::vector<float>& out)
     }
 }
 
-// Two-line weight table (W = 8): kPairPolyRows + 1 rows of 9 entries at
-// fractional offsets q / kPairPolyRows, in two copies: copy 0 holds the 8
-// weights in entries 0-7 and a zero in entry 8, copy 1 a zero in entry 0 and
-// the weights in entries 1-8.  A row is its 9 entries, the 9 differences to
-// the next row of the same copy, and 2 floats of padding.
-void build_pair_table(const bp_kernel& k, std::vector<float>& out)
+// Two-line weight table of kernel width W: Q = 2^pair_q_log2(W) rows at
+// fractional offsets t0 = tmin + q/Q (tmin = 0 for even W, -1/2 for odd W, as
+// the one-line table), each row pair_row_floats(W) floats, in two copies:
+// copy 0 holds the W weights in entries 0..W-1 and a zero in entry W, copy 1
+// a zero in entry 0 and the weights in entries 1..W.  Rows are read at the
+// nearest offset, so there are no differences to the next row.
+void build_pair_table(const bp_kernel& k, int W, std::vector<float>& out)
 {
-    constexpr int W = 8;
-    const int Q = bp::kPairPolyRows;
+    const int Q = 1 << bp::pair_q_log2(W);
+    const int RF = bp::pair_row_floats(W);
+    const double tmin = (W % 2 == 0) ? 0.0 : -0.5;
     const double hw = k.halfwidth;
     const double dx = hw / (k.n - 1.0);
     const float rdx = (float)(1.0 / dx);
@@ -309,23 +335,33 @@ void build_pair_table(const bp_kernel& k, std::vector<float>& out)
         int i = std::min((int)std::floor(axn), k.n - 2);
         return (double)k.table[i] + (axn - i) * (double)(k.table[i + 1] - k.table[i]);
     };
-    std::vector<float> rows((size_t)(Q + 1) * W);
-    for (int q = 0; q <= Q; ++q)
-        for (int m = 0; m < W; ++m) rows[(size_t)q * W + m] = (float)weight((m - W / 2) + (double)q / Q);
-    out.assign((size_t)bp::kPairTableFloats, 0.f);
-    for (int o = 0; o < 2; ++o) {
-        for (int q = 0; q <= Q; ++q) {
-            float* r = &out[((size_t)o * (Q + 1) + q) * bp::kPairRowFloats];
-            for (int e = 0; e < 9; ++e) {
-                const int m = e - o;
-                if (m < 0 || m >= W) continue;
-                const float w0 = rows[(size_t)q * W + m];
-                const float w1 = q < Q ? rows[(size_t)(q + 1) * W + m] : w0;
-                r[e] = w0;
-                r[9 + e] = w1 - w0;
-            }
+    out.assign((size_t)bp::pair_table_floats(W), 0.f);
+    for (int q = 0; q < Q; ++q) {
+        const double t0 = tmin + (double)q / Q;
+        for (int m = 0; m < W; ++m) {
+            const float w = (float)weight((m - W / 2) + t0);
+            out[((size_t)0 * Q + q) * RF + m] = w;
+            out[((size_t)1 * Q + q) * RF + m + 1] = w;
         }
     }
+}
+
+// Plain-sum table of TabulatedKernel<float> (core/Kernels.icc:137-154): entry
+// i holds (t_i, t_{i+1} - t_i) as doubles with the difference formed in float,
+// as the reference's evaluation forms it; entry n - 1 holds (t_{n-2} +
+// (t_{n-1} - t_{n-2}), t_{n-1} - t_{n-2}), what the reference's clamp of the
+// index to n - 2 evaluates for |x| at the half width.
+void build_plain_table(const bp_kernel& k, std::vector<double>& out)
+{
+    const int n = k.n;
+    out.assign((size_t)2 * n, 0.0);
+    for (int i = 0; i + 1 < n; ++i) {
+        out[2 * i] = (double)k.table[i];
+        out[2 * i + 1] = (double)(float)(k.table[i + 1] - k.table[i]);
+    }
+    const double d = (double)(float)(k.table[n - 1] - k.table[n - 2]);
+    out[2 * (n - 1)] = (double)k.table[n - 2] + d;
+    out[2 * (n - 1) + 1] = d;
 }
 
 }  // namespace
@@ -370,17 +406,33 @@ namespace {
 
 void free_static(StaticData& s)
 {
-    DevBuf* bufs[] = {&s.table, &s.poly, &s.poly2, &s.in_pos, &s.in_vel, &s.out_pos,
+    DevBuf* bufs[] = {&s.table, &s.table2, &s.poly, &s.poly2, &s.in_pos, &s.in_vel, &s.out_pos,
                       &s.out_vel, &s.in_dop, &s.out_dop, &s.dem, &s.pulse_pv};
     for (DevBuf* b : bufs)
-        if (b->ptr) (void)hipFree(b->ptr);
+        if (b->ptr && !b->fixed) (void)hipFree(b->ptr);
 }
 
 void free_scratch(Scratch& s)
 {
     DevBuf* bufs[] = {&s.geom, &s.split, &s.flags, &s.desc};
     for (DevBuf* b : bufs)
-        if (b->ptr) (void)hipFree(b->ptr);
+        if (b->ptr && !b->fixed) (void)hipFree(b->ptr);
+}
+
+void free_host_path(HostPath& h)
+{
+    DevBuf* bufs[] = {&h.win[0], &h.win[1], &h.band_desc, &h.band_win};
+    for (DevBuf* b : bufs)
+        if (b->ptr && !b->fixed) (void)hipFree(b->ptr);
+    for (auto* p : h.stage)
+        if (p) (void)hipHostFree(p);
+    if (h.h_win) (void)hipHostFree(h.h_win);
+    hipEvent_t evs[] = {h.stage_ev[0], h.stage_ev[1], h.win_ev, h.copied_ev, h.sum_ev[0], h.sum_ev[1]};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : h.timing)
+        if (e) (void)hipEventDestroy(e);
+    if (h.copy) (void)hipStreamDestroy(h.copy);
 }
 
 }  // namespace
@@ -392,6 +444,7 @@ extern "C" int bp_context_destroy(bp_context* c)
     (void)hipDeviceSynchronize();
     free_static(c->st);
     free_scratch(c->sc);
+    free_host_path(c->hp);
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
     delete c;
@@ -446,8 +499,12 @@ int check_call(const CallArgs& a, int* width)
     if (k->n < 2 || k->table == nullptr || !(k->halfwidth > 0))
         return fail(BP_ERR_INVALID_ARGUMENT, "kernel table needs at least two samples");
     *width = (int)std::ceil(2.0 * k->halfwidth);
-    if (*width != 8 && *width != 9)
-        return fail(BP_ERR_INVALID_ARGUMENT, "kernel width must be 8 or 9 (got " + std::to_string(*width) + ")");
+    if (*width < bp::kMinKernelWidth || *width > bp::kMaxKernelWidth)
+        return fail(BP_ERR_INVALID_ARGUMENT,
+                    "kernel width must be in [" + std::to_string(bp::kMinKernelWidth) + ", " +
+                        std::to_string(bp::kMaxKernelWidth) + "] taps (got " +
+                        std::to_string(*width) + ")");
+    if (k->n > (1 << 20)) return fail(BP_ERR_INVALID_ARGUMENT, "kernel table exceeds 2^20 samples");
     const bp_dem* dem = a.dem;
     if (dem->have_raster) {
         bp::Proj pj;
@@ -504,11 +561,13 @@ int check_options(const bp_options* options, Opts* o)
 // lane_rows 0 takes them where offered.
 int check_lane_rows(Opts& o, int precision, int width)
 {
-    const bool offered = precision == BP_PRECISION_SPLIT && width == 8 && !o.staged;
+    const bool offered = precision == BP_PRECISION_SPLIT && !o.staged;
+    if (o.staged && (precision != BP_PRECISION_SPLIT || (width != 8 && width != 9)))
+        return fail(BP_ERR_INVALID_ARGUMENT, "staged 1 needs split precision and kernel width 8 or 9");
     if (o.lane_rows == 0) o.lane_rows = offered ? 2 : 1;
     if (o.lane_rows == 2 && !offered)
         return fail(BP_ERR_INVALID_ARGUMENT,
-                    "lane_rows 2 needs split precision, kernel width 8 and staged 0");
+                    "lane_rows 2 needs split precision and staged 0");
     return BP_SUCCESS;
 }
 
@@ -596,11 +655,14 @@ int upload_static(StaticData& st, const CallArgs& a, int width, hipStream_t s)
         build_polyphase(*a.kernel, width, st.poly_host);
         if ((rc = upload(st.poly, st.poly_host.data(), sizeof(float) * st.poly_host.size(), s)))
             return rc;
-        if (width == 8) {
-            build_pair_table(*a.kernel, st.poly2_host);
-            if ((rc = upload(st.poly2, st.poly2_host.data(), sizeof(float) * st.poly2_host.size(), s)))
-                return rc;
-        }
+        build_pair_table(*a.kernel, width, st.poly2_host);
+        if ((rc = upload(st.poly2, st.poly2_host.data(), sizeof(float) * st.poly2_host.size(), s)))
+            return rc;
+    } else {
+        build_plain_table(*a.kernel, st.table2_host);
+        if ((rc = upload(st.table2, st.table2_host.data(),
+                         sizeof(double) * st.table2_host.size(), s)))
+            return rc;
     }
     return BP_SUCCESS;
 }
@@ -611,7 +673,8 @@ int pulse_states(StaticData& st, const CallArgs& a, int32_t* d_flags, hipStream_
     if (a.precision != BP_PRECISION_PLAIN) return BP_SUCCESS;
     const bp_grid& gi = a.in_geometry->grid;
     int rc;
-    if ((rc = ensure(st.pulse_pv, sizeof(double) * 6 * std::max<int64_t>(gi.length, 1)))) return rc;
+    if ((rc = ensure(st.pulse_pv, sizeof(double) * bp::kPulseRecDoubles * std::max<int64_t>(gi.length, 1))))
+        return rc;
     if (gi.length > 0)
         BP_HIP_CHECK(bp::launch_pulse_orbit(dev_orbit(a.in_geometry->orbit, st.in_pos, st.in_vel),
                                             gi.sensing_start, 1.0 / gi.prf, (int)gi.length,
@@ -695,7 +758,10 @@ void build_step(Step& sp, const StaticData& st, const Scratch& sc, const CallArg
     sa.table = static_cast<const float*>(st.table.ptr);
     sa.poly = static_cast<const float*>(st.poly.ptr);
     sa.poly2 = static_cast<const float*>(st.poly2.ptr);
-    sa.zscale2 = (float)(-(double)bp::kPairPolyRows * sa.inv_k);
+    sa.zscale2 = (float)(-(double)(1 << bp::pair_q_log2(width)) * sa.inv_k);
+    sa.inv_dtau = 1.0 / dtau;
+    sa.exact_width = 2.0 * a.kernel->halfwidth == (double)width;
+    sa.tab2 = static_cast<const double2*>(st.table2.ptr);
 
     sp.width = width;
     sp.precision = a.precision;
@@ -711,6 +777,21 @@ void build_step(Step& sp, const StaticData& st, const Scratch& sc, const CallArg
     sp.ev[0] = sp.ev[1] = sp.ev[2] = nullptr;
 }
 
+// The coherent sum of a step over the descriptors at `batch`, reading pulse k
+// at rc + k * nr.
+int enqueue_sum(const Step& sp, const bp::BatchHeader* batch, const float2* rc, int plain_grid,
+                hipStream_t s)
+{
+    bp::SumArgs sa = sp.sa;
+    sa.batch = batch;
+    sa.rc = rc;
+    if (sp.precision == BP_PRECISION_PLAIN)
+        BP_HIP_CHECK(bp::launch_sum_plain(sa, sp.width, plain_grid, s));
+    else
+        BP_HIP_CHECK(bp::launch_sum_split(sa, sp.width, sp.sum_grid, sp.staged, sp.pair, s));
+    return BP_SUCCESS;
+}
+
 // Enqueue the launches of one step: clear the flags, per-pixel geometry,
 // coherent sum (with optional event records around the two passes).
 int enqueue_step(const Step& sp, int plain_grid, hipStream_t s)
@@ -720,10 +801,8 @@ int enqueue_step(const Step& sp, int plain_grid, hipStream_t s)
     BP_HIP_CHECK(bp::launch_geometry(sp.ga, sp.geom_grid, s));
     if (sp.precision == BP_PRECISION_SPLIT) BP_HIP_CHECK(bp::launch_fit(sp.ga, sp.fit_grid, s));
     if (sp.ev[1]) BP_HIP_CHECK(hipEventRecord(sp.ev[1], s));
-    if (sp.precision == BP_PRECISION_PLAIN)
-        BP_HIP_CHECK(bp::launch_sum_plain(sp.sa, sp.width, plain_grid, s));
-    else
-        BP_HIP_CHECK(bp::launch_sum_split(sp.sa, sp.width, sp.sum_grid, sp.staged, sp.pair, s));
+    const int rc = enqueue_sum(sp, sp.sa.batch, sp.sa.rc, plain_grid, s);
+    if (rc != BP_SUCCESS) return rc;
     if (sp.ev[2]) BP_HIP_CHECK(hipEventRecord(sp.ev[2], s));
     return BP_SUCCESS;
 }
@@ -814,7 +893,335 @@ int run_blocks(bp_context* c, const CallArgs& a, bp_out_block* blocks, int n_blo
                        any_failed, stats);
 }
 
+// ------------------------------------------------------------ host input
+//
+// Range-compressed pulses in host memory (an array or a memory map of a
+// raster file): the geometry pass runs first, the union [kmin, kmax) of the
+// pixels' pulse windows comes back per band of tile rows, and only those
+// pulses are copied to the device, in chunks of at most `batch` pulses through
+// two pinned staging buffers on a copy stream (the host reads chunk j + 1
+// while chunk j is copied).  When the window of the whole block exceeds the
+// device budget, the block is focused in bands of output lines with smaller
+// windows; a band starts on a tile row, so its wavefront tiles, records and
+// per-pixel arithmetic are those of the whole block and every result equals
+// the resident call's.  Bands alternate between two window buffers when two
+// fit the budget, so the copy of band b + 1 overlaps the sum of band b.
+
+constexpr size_t kStageBytes = size_t(64) << 20;   // per pinned staging buffer
+
+int host_path_init(HostPath& h)
+{
+    if (h.copy) return BP_SUCCESS;
+    BP_HIP_CHECK(hipStreamCreateWithFlags(&h.copy, hipStreamNonBlocking));  // default priority
+    hipEvent_t* evs[] = {&h.stage_ev[0], &h.stage_ev[1], &h.win_ev, &h.copied_ev, &h.sum_ev[0],
+                         &h.sum_ev[1]};
+    for (hipEvent_t* e : evs) BP_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    return BP_SUCCESS;
+}
+
+int ensure_pinned(unsigned char*& p, size_t& have, size_t bytes)
+{
+    if (have >= bytes && p) return BP_SUCCESS;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    have = 0;
+    if (hipHostMalloc(reinterpret_cast<void**>(&p), bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+        return fail(BP_ERR_OUT_OF_MEMORY, "hipHostMalloc of pulse staging failed");
+    }
+    have = bytes;
+    return BP_SUCCESS;
+}
+
+// memcpy of `bytes` from src to dst, split over up to 8 host threads for
+// large copies (reading a memory-mapped raster is the slow side).
+void host_copy(unsigned char* dst, const unsigned char* src, size_t bytes)
+{
+    const size_t kPerThread = size_t(16) << 20;
+    unsigned nt = (unsigned)std::min<size_t>(8, (bytes + kPerThread - 1) / kPerThread);
+    nt = std::max(1u, std::min(nt, std::max(1u, std::thread::hardware_concurrency())));
+    if (nt == 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t part = (bytes + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) {
+        const size_t o = std::min(bytes, t * part), n = std::min(bytes - o, part);
+        try {
+            th.emplace_back([=] { std::memcpy(dst + o, src + o, n); });
+        } catch (...) {
+            std::memcpy(dst + o, src + o, n);  // no thread: copy here
+        }
+    }
+    std::memcpy(dst, src, std::min(bytes, part));
+    for (auto& x : th) x.join();
+}
+
+struct Band {
+    int64_t r0, r1;   // output lines
+    int32_t k0, k1;   // pulses [k0, k1) (k0 >= k1: none)
+};
+
+int run_host(bp_context* c, const CallArgs& a, bp_out_block* blk, int64_t batch,
+             int64_t max_window_bytes, const bp_options* options, void* stream, int* any_failed,
+             bp_stats* stats, bp_host_stats* hstats)
+{
+    if (any_failed) *any_failed = 0;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    if (hstats) std::memset(hstats, 0, sizeof(*hstats));
+    if (!c) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    int width = 0, rc;
+    if ((rc = check_call(a, &width)) != BP_SUCCESS) return rc;
+    if (batch < 1) return fail(BP_ERR_DOMAIN, "batch size must be > 0");
+    if (max_window_bytes < 0) return fail(BP_ERR_INVALID_ARGUMENT, "max_window_bytes must be >= 0");
+    int64_t npix = 0;
+    if ((rc = check_blocks(blk, 1, &npix)) != BP_SUCCESS) return rc;
+    Opts o;
+    if ((rc = check_options(options, &o)) != BP_SUCCESS) return rc;
+    if ((rc = check_lane_rows(o, a.precision, width)) != BP_SUCCESS) return rc;
+    const bp_grid& gi = a.in_geometry->grid;
+    if (npix > 0 && gi.length * gi.width > 0 && a.in_dev == nullptr)
+        return fail(BP_ERR_INVALID_ARGUMENT, "in is null");
+    if (npix == 0) return BP_SUCCESS;
+
+    BP_HIP_CHECK(hipSetDevice(c->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HostPath& hp = c->hp;
+    if ((rc = host_path_init(hp))) return rc;
+
+    const int n_flag = kFlagWords + 1;
+    if ((rc = ensure(c->sc.flags, sizeof(int32_t) * n_flag))) return rc;
+    c->desc_host.assign(sizeof(bp::BatchHeader) + sizeof(bp::OutDesc), 0);
+    bp::BatchHeader hdr;
+    if ((rc = plan_batch(blk, 1, o, static_cast<int32_t*>(c->sc.flags.ptr) + kFlagWords,
+                         c->desc_host.data(), 1, &hdr, c->desc_block)) != BP_SUCCESS)
+        return rc;
+    bp::OutDesc whole;
+    std::memcpy(&whole, c->desc_host.data() + sizeof(bp::BatchHeader), sizeof(whole));
+
+    if ((rc = upload_static(c->st, a, width, s))) return rc;
+    if ((rc = upload(c->sc.desc, c->desc_host.data(), c->desc_host.size(), s))) return rc;
+    if ((rc = ensure(c->sc.geom, sizeof(bp::PixGeom) * npix))) return rc;
+    if (a.precision == BP_PRECISION_SPLIT && (rc = ensure(c->sc.split, sizeof(bp::PixSplit) * npix)))
+        return rc;
+    BP_HIP_CHECK(bp::launch_clear(static_cast<int32_t*>(c->sc.flags.ptr), n_flag, s));
+    if ((rc = pulse_states(c->st, a, static_cast<int32_t*>(c->sc.flags.ptr), s))) return rc;
+
+    Step sp;
+    build_step(sp, c->st, c->sc, a, width, o, c->n_cu);
+    sp.geom_grid = (int)hdr.n_geom_groups;
+    sp.fit_grid = (int)hdr.n_groups;
+
+    // geometry, the band windows (read back while the fit runs), the fit
+    const int band_rows = whole.map.tile_rows * whole.map.lane_rows;
+    const int64_t length = whole.map.length, owidth = whole.map.width;
+    const int64_t n_tr = (length + band_rows - 1) / band_rows;
+    if (stats) BP_HIP_CHECK(hipEventRecord(c->ev[0], s));
+    BP_HIP_CHECK(bp::launch_geometry(sp.ga, sp.geom_grid, s));
+    if ((rc = ensure(hp.band_win, sizeof(int32_t) * 2 * n_tr))) return rc;
+    BP_HIP_CHECK(bp::launch_band_window(static_cast<const bp::PixGeom*>(c->sc.geom.ptr), length,
+                                        owidth, band_rows, n_tr,
+                                        static_cast<int32_t*>(hp.band_win.ptr), s));
+    if (hp.h_win_words < (size_t)(2 * n_tr)) {
+        unsigned char* p = reinterpret_cast<unsigned char*>(hp.h_win);
+        size_t have = hp.h_win_words * sizeof(int32_t);
+        if ((rc = ensure_pinned(p, have, sizeof(int32_t) * 2 * n_tr))) {
+            hp.h_win = nullptr;
+            hp.h_win_words = 0;
+            return rc;
+        }
+        hp.h_win = reinterpret_cast<int32_t*>(p);
+        hp.h_win_words = have / sizeof(int32_t);
+    }
+    BP_HIP_CHECK(hipMemcpyAsync(hp.h_win, hp.band_win.ptr, sizeof(int32_t) * 2 * n_tr,
+                                hipMemcpyDeviceToHost, s));
+    BP_HIP_CHECK(hipEventRecord(hp.win_ev, s));
+    if (a.precision == BP_PRECISION_SPLIT) BP_HIP_CHECK(bp::launch_fit(sp.ga, sp.fit_grid, s));
+    if (stats) BP_HIP_CHECK(hipEventRecord(c->ev[1], s));
+    BP_HIP_CHECK(hipEventSynchronize(hp.win_ev));
+
+    // bands of tile rows whose union window fits the budget
+    const size_t line_bytes = sizeof(float) * 2 * (size_t)gi.width;
+    size_t budget = (size_t)max_window_bytes;
+    if (budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        BP_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+        budget = free_b / 2;
+    }
+    std::vector<Band> bands;
+    {
+        Band cur{0, 0, INT32_MAX, INT32_MIN};
+        for (int64_t t = 0; t < n_tr; ++t) {
+            const int32_t lo = hp.h_win[2 * t], hi = hp.h_win[2 * t + 1];
+            if (lo < hi && (size_t)(hi - lo) * line_bytes > budget)
+                return fail(BP_ERR_LENGTH, "the pulse window of " + std::to_string(band_rows) +
+                                               " output lines (" + std::to_string(hi - lo) +
+                                               " pulses) exceeds max_window_bytes");
+            const int32_t nlo = lo < hi ? std::min(cur.k0, lo) : cur.k0;
+            const int32_t nhi = lo < hi ? std::max(cur.k1, hi) : cur.k1;
+            if (cur.r1 > cur.r0 && nlo < nhi && (size_t)(nhi - nlo) * line_bytes > budget) {
+                bands.push_back(cur);
+                cur = Band{cur.r1, cur.r1, INT32_MAX, INT32_MIN};
+                if (lo < hi) {
+                    cur.k0 = lo;
+                    cur.k1 = hi;
+                }
+            } else {
+                cur.k0 = nlo;
+                cur.k1 = nhi;
+            }
+            cur.r1 = std::min(length, (t + 1) * (int64_t)band_rows);
+        }
+        bands.push_back(cur);
+    }
+    size_t max_band = 16;
+    for (const Band& b : bands)
+        if (b.k0 < b.k1) max_band = std::max(max_band, (size_t)(b.k1 - b.k0) * line_bytes);
+    const int nbuf = bands.size() > 1 && 2 * max_band <= budget ? 2 : 1;
+    for (int w = 0; w < nbuf; ++w)
+        if ((rc = ensure(hp.win[w], max_band))) return rc;
+    const int64_t chunk_lines = std::max<int64_t>(
+        1, std::min<int64_t>(batch, (int64_t)(kStageBytes / std::max<size_t>(line_bytes, 1))));
+    {
+        size_t have = hp.stage_bytes, have1 = hp.stage_bytes;
+        const size_t want = (size_t)chunk_lines * line_bytes;
+        if ((rc = ensure_pinned(hp.stage[0], have, want))) return rc;
+        if ((rc = ensure_pinned(hp.stage[1], have1, want))) return rc;
+        hp.stage_bytes = std::min(have, have1);
+    }
+
+    // descriptors of every band: the whole block's, cut to the band's lines
+    const size_t stride = (sizeof(bp::BatchHeader) + sizeof(bp::OutDesc) + 255) & ~size_t(255);
+    std::vector<unsigned char> bdesc(stride * bands.size(), 0);
+    std::vector<int> band_groups(bands.size());
+    for (size_t b = 0; b < bands.size(); ++b) {
+        bp::OutDesc d = whole;
+        const int64_t r0 = bands[b].r0;
+        d.map = make_map(bands[b].r1 - r0, owidth, o.tile_rows, o.order, o.lane_rows);
+        d.t0 = whole.t0 + r0 * whole.dt;
+        d.seq_base = 0;
+        d.rec_base = whole.rec_base + r0 * owidth;
+        d.out = whole.out + r0 * owidth;
+        d.height = whole.height ? whole.height + r0 * owidth : nullptr;
+        bp::BatchHeader h;
+        std::memset(&h, 0, sizeof(h));
+        h.n_seq = d.map.n_tiles;
+        h.n_groups = (h.n_seq + 3) / 4;
+        h.n_geom_groups = (h.n_seq * o.lane_rows + 3) / 4;
+        h.n_grids = 1;
+        h.lane_rows = o.lane_rows;
+        std::memcpy(bdesc.data() + b * stride, &h, sizeof(h));
+        std::memcpy(bdesc.data() + b * stride + sizeof(h), &d, sizeof(d));
+        band_groups[b] = (int)h.n_groups;
+    }
+    if ((rc = upload(hp.band_desc, bdesc.data(), bdesc.size(), s))) return rc;
+
+    const bool timed = stats != nullptr || hstats != nullptr;
+    while (timed && hp.timing.size() < 4 * bands.size()) {
+        hipEvent_t e;
+        BP_HIP_CHECK(hipEventCreate(&e));
+        hp.timing.push_back(e);
+    }
+    // copies on the copy stream (they write only the context's window
+    // buffers, which no earlier call still reads: every call ends with both
+    // streams drained), sums on the caller's stream
+    const unsigned char* src = static_cast<const unsigned char*>(a.in_dev);
+    int64_t h2d = 0, pulses = 0, chunk_no = 0;
+    bool used[2] = {false, false};
+    for (size_t b = 0; b < bands.size(); ++b) {
+        const Band& bd = bands[b];
+        const int w = (int)(b % nbuf);
+        unsigned char* dwin = static_cast<unsigned char*>(hp.win[w].ptr);
+        if (used[w]) BP_HIP_CHECK(hipStreamWaitEvent(hp.copy, hp.sum_ev[w], 0));
+        if (timed) BP_HIP_CHECK(hipEventRecord(hp.timing[4 * b], hp.copy));
+        for (int64_t k = bd.k0; k < bd.k1; k += chunk_lines) {
+            const int64_t n = std::min<int64_t>(chunk_lines, bd.k1 - k);
+            const int j = (int)(chunk_no++ & 1);
+            // staging j is free once its previous copy has finished
+            BP_HIP_CHECK(hipEventSynchronize(hp.stage_ev[j]));
+            const size_t bytes = (size_t)n * line_bytes;
+            host_copy(hp.stage[j], src + (size_t)k * line_bytes, bytes);
+            BP_HIP_CHECK(hipMemcpyAsync(dwin + (size_t)(k - bd.k0) * line_bytes, hp.stage[j],
+                                        bytes, hipMemcpyHostToDevice, hp.copy));
+            BP_HIP_CHECK(hipEventRecord(hp.stage_ev[j], hp.copy));
+            h2d += (int64_t)bytes;
+        }
+        if (bd.k0 < bd.k1) pulses += bd.k1 - bd.k0;
+        if (timed) BP_HIP_CHECK(hipEventRecord(hp.timing[4 * b + 1], hp.copy));
+        BP_HIP_CHECK(hipEventRecord(hp.copied_ev, hp.copy));
+        BP_HIP_CHECK(hipStreamWaitEvent(s, hp.copied_ev, 0));
+        // pulse k of the band at dwin + (k - k0) * line_bytes; the sum reads
+        // only pulses of its pixels' windows, all inside [k0, k1)
+        const int64_t k0 = bd.k0 < bd.k1 ? bd.k0 : 0;
+        const float2* rc_base = reinterpret_cast<const float2*>(
+            reinterpret_cast<uintptr_t>(dwin) - (uintptr_t)((size_t)k0 * line_bytes));
+        const auto* batch_b = reinterpret_cast<const bp::BatchHeader*>(
+            static_cast<unsigned char*>(hp.band_desc.ptr) + b * stride);
+        if (timed) BP_HIP_CHECK(hipEventRecord(hp.timing[4 * b + 2], s));
+        if ((rc = enqueue_sum(sp, batch_b, rc_base, band_groups[b], s))) return rc;
+        if (timed) BP_HIP_CHECK(hipEventRecord(hp.timing[4 * b + 3], s));
+        BP_HIP_CHECK(hipEventRecord(hp.sum_ev[w], s));
+        used[w] = true;
+    }
+    if (stats) BP_HIP_CHECK(hipEventRecord(c->ev[2], s));
+
+    c->flags_host.assign(n_flag, 0);
+    BP_HIP_CHECK(hipMemcpyAsync(c->flags_host.data(), c->sc.flags.ptr, sizeof(int32_t) * n_flag,
+                                hipMemcpyDeviceToHost, s));
+    BP_HIP_CHECK(hipStreamSynchronize(s));
+    BP_HIP_CHECK(hipStreamSynchronize(hp.copy));
+    double sum_ms = 0., copy_ms = 0.;
+    if (timed) {
+        for (size_t b = 0; b < bands.size(); ++b) {
+            float ms = 0.f;
+            BP_HIP_CHECK(hipEventElapsedTime(&ms, hp.timing[4 * b], hp.timing[4 * b + 1]));
+            copy_ms += ms;
+            BP_HIP_CHECK(hipEventElapsedTime(&ms, hp.timing[4 * b + 2], hp.timing[4 * b + 3]));
+            sum_ms += ms;
+        }
+    }
+    if (hstats) {
+        hstats->h2d_bytes = h2d;
+        hstats->window_pulses = pulses;
+        hstats->bands = (int32_t)bands.size();
+        hstats->copy_ms = copy_ms;
+    }
+    const hipEvent_t ev[3] = {stats ? c->ev[0] : nullptr, nullptr, stats ? c->ev[2] : nullptr};
+    rc = finish_step(c->flags_host.data(), hdr.n_grids, c->desc_block, blk, ev, any_failed, stats);
+    if (stats && c->ev[1]) {
+        float ms = 0.f;
+        BP_HIP_CHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        stats->geometry_ms = ms;
+        stats->sum_ms = sum_ms;
+    }
+    return rc;
+}
+
 }  // namespace
+
+extern "C" int bp_backproject_host(bp_context* c, const bp_radar_geometry* og,
+                                   const bp_radar_geometry* ig, const bp_dem* dem, double fc,
+                                   double ds, const bp_kernel* kernel, int dry_tropo_model,
+                                   const bp_rdr2geo_params* r2g, const bp_geo2rdr_params* g2r,
+                                   const void* in_host, int64_t batch, int64_t max_window_bytes,
+                                   void* out_dev, float* height_dev, int precision,
+                                   const bp_options* options, void* stream, int* any_failed,
+                                   bp_stats* stats, bp_host_stats* host_stats)
+{
+    if (any_failed) *any_failed = 0;
+    if (!og) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    bp_out_block blk;
+    std::memset(&blk, 0, sizeof(blk));
+    blk.grid = og->grid;
+    blk.out_dev = out_dev;
+    blk.height_dev = height_dev;
+    CallArgs a{&og->orbit, &og->doppler, ig, dem, fc, ds, kernel, dry_tropo_model, r2g, g2r,
+               in_host, precision};
+    return run_host(c, a, &blk, batch, max_window_bytes, options, stream, any_failed, stats,
+                    host_stats);
+}
 
 // ------------------------------------------------------------------ scenes
 
@@ -1196,6 +1603,14 @@ int bp_proj_inverse(int epsg, const double* xyz, double* llh)
 double bp_lut2d_eval(const bp_lut2d* lut, double y, double x)
 {
     return bp::lut2d_eval(host_lut(*lut), y, x);
+}
+
+void bp_sincos_turns(double t, double* s, double* c)
+{
+    double ss = 0., cc = 0.;
+    bp::sincos_turns(t, &ss, &cc);
+    if (s) *s = ss;
+    if (c) *c = cc;
 }
 
 double bp_knab_eval(double width, double bandwidth, double x)

@@ -76,12 +76,29 @@ struct TileMap {
     int32_t lane_rows;          // output lines per lane: 1, or 2 (split sum, W = 8)
 };
 
-// Rows of the two-line weight table (split sum with lane_rows = 2): 9 taps
-// per row at Q = 2^8 fractional offsets, in two copies shifted by one tap.
-constexpr int kPairPolyLog2 = 8;
-constexpr int kPairPolyRows = 1 << kPairPolyLog2;
-constexpr int kPairRowFloats = 20;   // 9 weights, 9 deltas, 2 padding (80 bytes)
-constexpr int kPairTableFloats = 2 * (kPairPolyRows + 1) * kPairRowFloats;
+// Interpolation kernel widths (taps, ceil of the kernel width) with built
+// coherent-sum kernels.
+constexpr int kMinKernelWidth = 2;
+constexpr int kMaxKernelWidth = 16;
+
+// Two-line weight table of kernel width W (split sum with lane_rows = 2):
+// Q = 2^pair_q_log2(W) rows at fractional offsets tmin + q/Q, each holding
+// the W + 1 entries of a (W + 1)-bin window (padded to whole float4s), in two
+// copies -- the W weights followed by a zero (copy 0) or preceded by one
+// (copy 1).  A pixel reads the row nearest to its offset (no interpolation
+// between rows).  Tables stay within 64 KB of LDS, so that two 512-thread
+// workgroups fit one CU.
+__host__ __device__ constexpr int pair_row_floats(int w) { return (w + 1 + 3) / 4 * 4; }
+__host__ __device__ constexpr int pair_q_log2(int w) { return pair_row_floats(w) <= 16 ? 9 : 8; }
+__host__ __device__ constexpr int pair_table_floats(int w)
+{
+    return 2 * (1 << pair_q_log2(w)) * pair_row_floats(w);
+}
+
+// Doubles per pulse of the plain sum's per-pulse record: platform position
+// (3), velocity (3), 2 / (|v|^2 - c^2) (the bistatic delay's factor) and one
+// of padding.
+constexpr int kPulseRecDoubles = 8;
 
 // One output grid of a (batched) call.  Wavefront tiles of all grids are
 // numbered consecutively: grid g owns [seq_base, seq_base + map.n_tiles).
@@ -147,14 +164,16 @@ struct SumArgs {
     float rdx;                  // float(1/dx) of the tabulated kernel
     float zscale;               // -Q * inv_k (split polyphase coordinate per cycle)
     const float2* rc;           // range-compressed pulses [na x nr]
-    const double* pulse_pv;     // per-pulse position/velocity [na x 6]
+    const double* pulse_pv;     // per-pulse records [na x kPulseRecDoubles]
     const PixGeom* geom;
     const PixSplit* split;
     const float* table;         // tabulated kernel (plain)
     const float* poly;          // polyphase weights (split): (rows+1) x poly_row_floats(W)
-    const float* poly2;         // two-line weights (split, lane_rows 2): kPairTableFloats
-    float zscale2;              // -2^kPairPolyLog2 * inv_k
-    int32_t pad2_;
+    const float* poly2;         // two-line weights (split, lane_rows 2): pair_table_floats(W)
+    float zscale2;              // -2^pair_q_log2(W) * inv_k
+    int32_t exact_width;        // plain: 2 * halfwidth == W (no tap lies past the half width)
+    double inv_dtau;            // 1 / dtau
+    const double2* tab2;        // plain: (t_i, t_{i+1} - t_i) per table entry, as doubles
 };
 
 // Host launchers (defined in bp_kernels.hip).  `grid` is the number of
@@ -170,6 +189,9 @@ hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_
 hipError_t launch_sum_split(const SumArgs& args, int width, int grid, bool staged, bool pair,
                             hipStream_t stream);
 int sum_blocks_per_cu(int precision, int width, bool staged, bool pair);
+// [kmin, kmax) of each band of band_rows output lines (n_bands workgroups).
+hipError_t launch_band_window(const PixGeom* geom, int64_t length, int64_t width, int band_rows,
+                              int64_t n_bands, int32_t* win, hipStream_t stream);
 // Host-side handle of a kernel (0 geometry, 4 geometry for generic DEMs,
 // 1 plain sum, 2 split sum, 3 fit),
 // as graph kernel nodes report it.

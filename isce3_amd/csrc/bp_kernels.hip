@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <limits.h>
 
+#include <utility>
+
 #include "bp_internal.h"
 
 namespace bp {
@@ -95,17 +97,24 @@ __device__ inline bool tile_pixel(const TileMap& m, int64_t tile, int lane, int 
 }
 
 // TabulatedKernel<float>::operator() with the reference's FP64 index math
-// (core/Kernels.icc:137-154).
-__device__ inline float tab_eval(const float* tab, int ntab, double hw, float rdx, double x)
+// (core/Kernels.icc:137-154): axn = |x| * float(1/dx) in FP64, i = floor(axn),
+// K = t_i + (axn - i) (t_{i+1} - t_i) rounded to float.  The table holds each
+// entry as the pair (t_i, t_{i+1} - t_i) in FP64 (the difference formed in
+// float, as the reference does), so one 16-byte LDS read gives both; entry
+// n - 1 holds the value the reference's clamp to i = n - 2 gives at |x| = hw.
+// kCheck: x may lie past the half width (then K = 0, and the entry index is
+// clamped into the table).  Without it |x| <= hw, so axn <= (n - 1)(1 + 2^-23)
+// and floor(axn) <= n - 1 for tables of up to 2^20 entries.
+template<bool kCheck>
+__device__ inline float tab_weight(const double2* tab2, int ntab, double hw, double rdx, double x)
 {
     const double ax = fabs(x);
-    if (ax > hw) return 0.f;
-    const double axn = ax * (double)rdx;
-    int i = (int)floor(axn);
-    i = min(i, ntab - 2);
-    const float t0 = tab[i];
-    const float dt = tab[i + 1] - t0;
-    return (float)((double)t0 + (axn - i) * (double)dt);
+    const double axn = ax * rdx;
+    int i = (int)axn;  // axn >= 0: truncation is floor
+    if (kCheck) i = min(i, ntab - 1);
+    const double2 e = tab2[i];
+    const float w = (float)fma(__builtin_amdgcn_fract(axn), e.y, e.x);
+    return kCheck && ax > hw ? 0.f : w;
 }
 
 }  // namespace
@@ -118,13 +127,17 @@ __global__ void pulse_orbit_kernel(Orbit orbit, double t0, double dt, int na, do
     V3 p, v;
     const int st = orbit_interpolate(orbit, t0 + k * dt, kBorderError, &p, &v);
     if (st != kSuccess) atomicOr(&flags[1], 1);
-    double* o = pv + 6 * (int64_t)k;
+    double* o = pv + kPulseRecDoubles * (int64_t)k;
     o[0] = p.x;
     o[1] = p.y;
     o[2] = p.z;
     o[3] = v.x;
     o[4] = v.y;
     o[5] = v.z;
+    // bistatic delay tau = (r.v - c |r|) * 2 / (|v|^2 - c^2) (BistaticDelay.icc:9-17):
+    // the factor once per pulse instead of a division per pixel and pulse
+    o[6] = 2. / (dot(v, v) - kC * kC);
+    o[7] = 0.;
 }
 
 __global__ void clear_kernel(int32_t* words, int n)
@@ -242,7 +255,7 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
 // over the sum wavefront's union window [kw0, kw1) with a degree-4 Chebyshev
 // interpolant evaluated in FP64, re-expanded as forward differences at kw0.
 __device__ inline PixSplit fit_pixel(const GeomArgs& a, const PixGeom& g, bool has_win, int kw0,
-                                     int kw1, double q_rows)
+                                     int kw1, double q_rows, double z_round)
 {
     const V3 x{g.x[0], g.x[1], g.x[2]};
     PixSplit sp;
@@ -364,7 +377,9 @@ __device__ inline PixSplit fit_pixel(const GeomArgs& a, const PixGeom& g, bool h
         // koff whole bins of offset keep z >= 0 over the window, so that the
         // sum kernel takes floor(z) and fract(z) by truncation and v_fract
         const double koff = ceil(bnd) + 2.0;
-        sp.zoff = (float)(-q_rows * (f_ref - c_ref * inv_k + shift - koff));
+        // z_round: 1/2 row for tables read at the nearest row (truncation of
+        // z + 1/2 rounds), 0 for tables interpolated between rows
+        sp.zoff = (float)(-q_rows * (f_ref - c_ref * inv_k + shift - koff) + z_round);
         sp.cbase = (int32_t)n_ref - a.kwidth / 2 - (odd ? 1 : 0) + (int32_t)koff;
         const double c_lo = n_ref + ceil(f_ref - bnd + shift) - (a.kwidth / 2) - (odd ? 1 : 0) - 1;
         const double c_hi = n_ref + ceil(f_ref + bnd + shift) - (a.kwidth / 2) - (odd ? 1 : 0) + 1;
@@ -395,7 +410,8 @@ __device__ inline void fit_tile(const GeomArgs& a, const OutDesc* grids, int n_g
     const int kw0 = wave_min_i(ks);
     const int kw1 = wave_max_i(ke);
     // polyphase rows of the weight table the sum reads for this tiling
-    const double q_rows = lr == 2 ? (double)kPairPolyRows : (double)kPolyphaseRows;
+    const double q_rows = lr == 2 ? (double)(1 << pair_q_log2(a.kwidth)) : (double)kPolyphaseRows;
+    const double z_round = lr == 2 ? 0.5 : 0.0;
 #pragma unroll 1
     for (int j = 0; j < lr; ++j) {
         int64_t row = 0, col = 0;
@@ -403,7 +419,7 @@ __device__ inline void fit_tile(const GeomArgs& a, const OutDesc* grids, int n_g
             const int64_t rec = d.rec_base + row * d.map.width + col;
             const PixGeom g = a.geom[rec];
             const bool has_win = g.status == kPixOk && g.kstop > g.kstart;
-            a.split[rec] = fit_pixel(a, g, has_win, kw0, kw1, q_rows);
+            a.split[rec] = fit_pixel(a, g, has_win, kw0, kw1, q_rows, z_round);
         }
     }
 }
@@ -468,9 +484,19 @@ __device__ inline void for_each_tile(const BatchHeader* batch, Body&& body)
 
 // Reference-semantics coherent sum of one wavefront tile (Backproject.cpp:30-63):
 // FP64 bistatic delay, range-bin position and phase per pulse; float tabulated
-// taps with FP64 index math and a complex<float> inner product; complex<double> sum.
-template<int W>
-__device__ inline void plain_tile(const SumArgs& a, const float* tab, const OutDesc& od,
+// taps with FP64 index math and a complex<float> inner product (interp1d,
+// core/detail/Interp1d.h:23-106: first tap ceil(u) - W/2 for even W, round(u)
+// - W/2 for odd W, taps outside the line zero); complex<double> sum.
+//
+// Per pulse, everything the W taps share is formed once: the delay from the
+// pulse record (its 2/(|v|^2 - c^2) factor precomputed), u by a product with
+// 1/dtau, and the first tap's offset x0 = low - u; tap m is then at x0 + m
+// (exact: both terms are within a few bins of each other for u >= W, so the
+// differences are exact, as the reference's (m + low) - u is).  The phase
+// 2 pi fc tau comes from sincos_turns of fc tau in FP64.
+// kExact: 2 * halfwidth == W, so no tap lies past the half width.
+template<int W, bool kExact>
+__device__ inline void plain_tile(const SumArgs& a, const double2* tab2, const OutDesc& od,
                                   int64_t tile, int lane)
 {
     int64_t row = 0, col = 0;
@@ -486,41 +512,56 @@ __device__ inline void plain_tile(const SumArgs& a, const float* tab, const OutD
     const int kw1 = __builtin_amdgcn_readfirstlane(wave_max_i(ke));
     const V3 x{g.x[0], g.x[1], g.x[2]};
     const double tau_atm = ok ? g.tau_atm : 0.;
+    const double nr_d = (double)a.nr;
+    const double hw = a.halfwidth, rdx = (double)a.rdx;
+    const int ntab = a.ntab;
 
     double acc_re = 0., acc_im = 0.;
     for (int k = kw0; k < kw1; ++k) {
         if (k < ks || k >= ke) continue;
-        const double* pv = a.pulse_pv + 6 * (int64_t)k;
-        const V3 p{pv[0], pv[1], pv[2]};
-        const V3 v{pv[3], pv[4], pv[5]};
-        const double tau = tau_atm + bistatic_delay(p, v, x);
-        const double u = (tau - a.swst) / a.dtau;
-        const long i0 = (W % 2 == 0) ? (long)ceil(u) : (long)round(u);
-        const long low = i0 - W / 2;
-        const float2* line = a.rc + (int64_t)k * a.nr;
+        const double* pv = a.pulse_pv + kPulseRecDoubles * (int64_t)k;
+        const V3 r{x.x - pv[0], x.y - pv[1], x.z - pv[2]};
+        const double rv = r.x * pv[3] + r.y * pv[4] + r.z * pv[5];
+        const double rn = sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+        const double tau = tau_atm + (rv - kC * rn) * pv[6];
+        const double u = (tau - a.swst) * a.inv_dtau;
+        const double low = ((W % 2 == 0) ? ceil(u) : round(u)) - (double)(W / 2);
+        const double x0 = low - u;
+        float w[W];
+#pragma unroll
+        for (int m = 0; m < W; ++m) {
+            const double xm = x0 + (double)m;
+            // only the end taps can reach past a half width below W / 2
+            w[m] = (!kExact && (m == 0 || m == W - 1))
+                       ? tab_weight<true>(tab2, ntab, hw, rdx, xm)
+                       : tab_weight<false>(tab2, ntab, hw, rdx, xm);
+        }
         float sre = 0.f, sim = 0.f;
-        if (low >= 0 && low + W <= a.nr) {
+        if (low >= 0. && low + (double)W <= nr_d) {
+            const float2* p = a.rc + (int64_t)k * a.nr + (int64_t)low;
 #pragma unroll
             for (int m = 0; m < W; ++m) {
-                const float w = tab_eval(tab, a.ntab, a.halfwidth, a.rdx, (double)(m + low) - u);
-                const float2 d = line[low + m];
-                sre += w * d.x;
-                sim += w * d.y;
+                const float2 d = p[m];
+                sre = fmaf(w[m], d.x, sre);
+                sim = fmaf(w[m], d.y, sim);
             }
         } else {
+            const float2* line = a.rc + (int64_t)k * a.nr;
 #pragma unroll
             for (int m = 0; m < W; ++m) {
-                const long j = low + m;
-                const float w = tab_eval(tab, a.ntab, a.halfwidth, a.rdx, (double)j - u);
-                const float2 d = (j >= 0 && j < a.nr) ? line[j] : make_float2(0.f, 0.f);
-                sre += w * d.x;
-                sim += w * d.y;
+                const double j = low + (double)m;
+                const float2 d = (j >= 0. && j < nr_d) ? line[(int64_t)j] : make_float2(0.f, 0.f);
+                sre = fmaf(w[m], d.x, sre);
+                sim = fmaf(w[m], d.y, sim);
             }
         }
         double sn, cs;
-        sincospi(2.0 * a.fc * tau, &sn, &cs);
-        acc_re += (double)sre * cs - (double)sim * sn;
-        acc_im += (double)sre * sn + (double)sim * cs;
+        sincos_turns(a.fc * tau, &sn, &cs);
+        const double dre = (double)sre, dim = (double)sim;
+        acc_re = fma(dre, cs, acc_re);
+        acc_re = fma(-dim, sn, acc_re);
+        acc_im = fma(dre, sn, acc_im);
+        acc_im = fma(dim, cs, acc_im);
     }
     if (valid) {
         const float nanf_ = __builtin_nanf("");
@@ -529,19 +570,22 @@ __device__ inline void plain_tile(const SumArgs& a, const float* tab, const OutD
 }
 
 // One 256-thread workgroup per group of four tiles (measured faster than the
-// persistent walk of the split kernel for this FP64-bound kernel, which needs
-// ~90 VGPRs); workgroups past the batch's groups return at once.
+// persistent walk of the split kernel for this FP64-bound kernel); workgroups
+// past the batch's groups return at once.  The (t_i, t_{i+1} - t_i) table is
+// copied to LDS when it fits kMaxTab2Lds entries.
+constexpr int kMaxTab2Lds = 4096;
+
 template<int W>
 __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
 {
     const int64_t ng = uniform64(a.batch->n_groups);
     if ((int64_t)blockIdx.x >= ng) return;
-    extern __shared__ float s_tab[];
-    const bool tab_in_lds = a.ntab <= kMaxTableLds;
+    extern __shared__ double2 s_tab2[];
+    const bool tab_in_lds = a.ntab <= kMaxTab2Lds;
     if (tab_in_lds)
-        for (int i = threadIdx.x; i < a.ntab; i += blockDim.x) s_tab[i] = a.table[i];
+        for (int i = threadIdx.x; i < a.ntab; i += blockDim.x) s_tab2[i] = a.tab2[i];
     __syncthreads();
-    const float* tab = tab_in_lds ? s_tab : a.table;
+    const double2* tab2 = tab_in_lds ? s_tab2 : a.tab2;
     const OutDesc* grids = batch_grids(a.batch);
     const int n_grids = __builtin_amdgcn_readfirstlane(a.batch->n_grids);
     const int64_t n_seq = uniform64(a.batch->n_seq);
@@ -549,7 +593,10 @@ __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
     const int64_t gseq = group_tile(blockIdx.x, ng, &lane);
     if (gseq < n_seq) {
         const OutDesc d = load_desc(grids, find_grid(grids, n_grids, gseq));
-        plain_tile<W>(a, tab, d, gseq - d.seq_base, lane);
+        if (a.exact_width)
+            plain_tile<W, true>(a, tab2, d, gseq - d.seq_base, lane);
+        else
+            plain_tile<W, false>(a, tab2, d, gseq - d.seq_base, lane);
     }
 }
 
@@ -931,58 +978,69 @@ __device__ inline void split_tile(const SumArgs& a, const float* s_poly, char* s
     }
 }
 
-// Two output lines per lane (lane_rows = 2, W = 8).  The lane's pixels A and
-// B are vertically adjacent: at any pulse their first taps differ by at most
-// one range bin (adjacent lines are one azimuth sample apart, so their ranges
-// to a pulse differ by a few centimetres), and one 9-bin gather from the
-// smaller first tap serves both.  The weights come from a table of 9-tap rows
-// in two copies, the 8 weights preceded (copy 1) or followed (copy 0) by a
-// zero, so that each pixel picks its alignment inside the shared 9 bins by
-// its row offset alone.  This halves the gather traffic per update, the
-// bound of the one-line kernel.  Where the first taps of some lane's pixels
-// are further apart, that pulse is gathered per pixel.
+// Two output lines per lane (lane_rows = 2).  The lane's pixels A and B are
+// vertically adjacent: at any pulse their first taps differ by at most one
+// range bin (adjacent lines are one azimuth sample apart, so their ranges to
+// a pulse differ by a few centimetres), and one (W + 1)-bin gather from the
+// smaller first tap serves both.  The weights come from a table of
+// (W + 1)-entry rows in two copies, the W weights followed (copy 0) or
+// preceded (copy 1) by a zero, so that each pixel picks its alignment inside
+// the shared bins by its row offset alone.  This halves the gather traffic
+// per update.  Where the first taps of some lane's pixels are further apart,
+// that pulse is gathered per pixel.  Each pixel reads the table row nearest
+// to its fractional offset (the fit's coordinate offset holds the half row
+// that turns truncation into rounding): with 2^9 rows per bin (2^8 for the
+// widest kernels) the offset error stays below 1/1024 (1/512) bin and no
+// interpolation between rows is needed.
 struct PairPulse {
     float ph;   // phase, revolutions in [0, 1)
-    float qf;   // fraction between weight rows
     int row;    // byte offset of the weight row (copy 0)
     int c0;     // first tap's range bin
 };
 
-constexpr int kPairRowBytes = 4 * kPairRowFloats;
-constexpr int kPairCopyBytes = (kPairPolyRows + 1) * kPairRowBytes;
+template<int W>
+struct PairTable {
+    static constexpr int kNE = W + 1;                  // entries per row
+    static constexpr int kRF = pair_row_floats(W);     // floats per row
+    static constexpr int kQL = pair_q_log2(W);
+    static constexpr int kRowBytes = 4 * kRF;
+    static constexpr int kCopyBytes = (1 << kQL) * kRowBytes;
+    static constexpr int kFloats = pair_table_floats(W);
+};
 
+template<int W>
 __device__ inline PairPulse pair_pulse(const PhaseState& st, float zscale, float zoff, int cbase)
 {
+    using T = PairTable<W>;
     PairPulse r;
     r.ph = (float)__builtin_amdgcn_fract(st.p);
     const float z = fmaf((float)st.p, zscale, zoff);  // >= 0 over the window
     const uint32_t m = (uint32_t)z;
-    r.qf = __builtin_amdgcn_fractf(z);
-    r.row = (int)(m & (kPairPolyRows - 1)) * kPairRowBytes;
-    r.c0 = cbase - (int)(m >> kPairPolyLog2);
+    r.row = (int)(m & ((1u << T::kQL) - 1)) * T::kRowBytes;
+    r.c0 = cbase - (int)(m >> T::kQL);
     return r;
 }
 
-// Interpolate the 9 bins `d` with the weight row at byte offset `row`
-// (9 weights, 9 deltas, 2 padding), rotate by the phase and accumulate.
-__device__ inline void pair_update(const float* s_pair, int row, float qf, float ph,
-                                   const float2 (&d)[9], float& cre, float& cim)
+// Interpolate the W + 1 bins `d` with the weight row at byte offset `row`,
+// rotate by the phase and accumulate.
+template<int W>
+__device__ inline void pair_update(const float* s_pair, int row, float ph,
+                                   const float2 (&d)[W + 1], float& cre, float& cim)
 {
+    using T = PairTable<W>;
     const float4* r4 = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_pair) + row);
-    const float4 A = r4[0], B = r4[1], C = r4[2], D = r4[3], E = r4[4];
-    float w[9];
-    w[0] = fmaf(qf, C.y, A.x);
-    w[1] = fmaf(qf, C.z, A.y);
-    w[2] = fmaf(qf, C.w, A.z);
-    w[3] = fmaf(qf, D.x, A.w);
-    w[4] = fmaf(qf, D.y, B.x);
-    w[5] = fmaf(qf, D.z, B.y);
-    w[6] = fmaf(qf, D.w, B.z);
-    w[7] = fmaf(qf, E.x, B.w);
-    w[8] = fmaf(qf, E.y, C.x);
+    float w[T::kRF];
+#pragma unroll
+    for (int i = 0; i < T::kRF / 4; ++i) {
+        const float4 v = r4[i];
+        w[4 * i + 0] = v.x;
+        w[4 * i + 1] = v.y;
+        w[4 * i + 2] = v.z;
+        w[4 * i + 3] = v.w;
+    }
     float sre = 0.f, sim = 0.f;
 #pragma unroll
-    for (int m = 0; m < 9; ++m) {
+    for (int m = 0; m < T::kNE; ++m) {
         sre = fmaf(w[m], d[m].x, sre);
         sim = fmaf(w[m], d[m].y, sim);
     }
@@ -994,24 +1052,25 @@ __device__ inline void pair_update(const float* s_pair, int row, float qf, float
     cim = fmaf(sim, cs, cim);
 }
 
-// The 8 taps from bin c0 (guarded: zero outside [0, nr)), and a ninth zero.
-template<bool kGuard>
-__device__ inline void pair_taps_one(const float2* line, int c0, int nr, float2 (&d)[9])
+// The W taps from bin c0 (guarded: zero outside [0, nr)), and a zero.
+template<int W, bool kGuard>
+__device__ inline void pair_taps_one(const float2* line, int c0, int nr, float2 (&d)[W + 1])
 {
-    float2 e[8];
-    split_taps<8, kGuard>(line, c0, nr, e);
+    float2 e[W];
+    split_taps<W, kGuard>(line, c0, nr, e);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) d[m] = e[m];
-    d[8] = make_float2(0.f, 0.f);
+    for (int m = 0; m < W; ++m) d[m] = e[m];
+    d[W] = make_float2(0.f, 0.f);
 }
 
-// The 9 bins from `base` (base + 9 <= nr).
-__device__ inline void pair_taps_shared(const float2* line, int base, float2 (&d)[9])
+// The W + 1 bins from `base` (base + W + 1 <= nr).
+template<int W>
+__device__ inline void pair_taps_shared(const float2* line, int base, float2 (&d)[W + 1])
 {
     const float2* p = reinterpret_cast<const float2*>(reinterpret_cast<const char*>(line) +
                                                       (uint32_t)base * (uint32_t)sizeof(float2));
 #pragma unroll
-    for (int m = 0; m < 9; ++m) d[m] = p[m];
+    for (int m = 0; m <= W; ++m) d[m] = p[m];
 }
 
 struct PairLine {
@@ -1020,26 +1079,26 @@ struct PairLine {
 };
 
 // Pulses [k_lo, k_hi): each pixel gathers its own taps where it integrates.
-template<bool kGuard>
+template<int W, bool kGuard>
 __device__ inline void pair_masked(const SumArgs& a, const float* s_pair, PairLine& A,
                                    PairLine& B, int k_lo, int k_hi)
 {
     const int nr = (int)a.nr;
     for (int k = k_lo; k < k_hi; ++k) {
         const float2* line = a.rc + (int64_t)k * nr;
-        const PairPulse pa = pair_pulse(A.s.st, a.zscale2, A.s.zoff, A.s.cbase);
-        const PairPulse pb = pair_pulse(B.s.st, a.zscale2, B.s.zoff, B.s.cbase);
+        const PairPulse pa = pair_pulse<W>(A.s.st, a.zscale2, A.s.zoff, A.s.cbase);
+        const PairPulse pb = pair_pulse<W>(B.s.st, a.zscale2, B.s.zoff, B.s.cbase);
         A.s.st.advance();
         B.s.st.advance();
         if (k >= A.ks && k < A.ke) {
-            float2 d[9];
-            pair_taps_one<kGuard>(line, pa.c0, nr, d);
-            pair_update(s_pair, pa.row, pa.qf, pa.ph, d, A.s.cre, A.s.cim);
+            float2 d[W + 1];
+            pair_taps_one<W, kGuard>(line, pa.c0, nr, d);
+            pair_update<W>(s_pair, pa.row, pa.ph, d, A.s.cre, A.s.cim);
         }
         if (k >= B.ks && k < B.ke) {
-            float2 d[9];
-            pair_taps_one<kGuard>(line, pb.c0, nr, d);
-            pair_update(s_pair, pb.row, pb.qf, pb.ph, d, B.s.cre, B.s.cim);
+            float2 d[W + 1];
+            pair_taps_one<W, kGuard>(line, pb.c0, nr, d);
+            pair_update<W>(s_pair, pb.row, pb.ph, d, B.s.cre, B.s.cim);
         }
         if (++A.s.count >= a.chunk) {
             A.s.fold();
@@ -1049,52 +1108,46 @@ __device__ inline void pair_masked(const SumArgs& a, const float* s_pair, PairLi
 }
 
 // Core pulses [k0, k1), every pixel of every lane integrating: one shared
-// 9-bin gather per lane and pulse (no software prefetch: the two lines' state
-// leaves no registers for a second pulse in flight).
+// (W + 1)-bin gather per lane and pulse (no software prefetch: the two lines'
+// state leaves no registers for a second pulse in flight).
+template<int W>
 __device__ inline void pair_core(const SumArgs& a, const float* s_pair, PairLine& A, PairLine& B,
                                  int k0, int k1, uint64_t act_a, uint64_t act_b)
 {
+    using T = PairTable<W>;
     const int nr = (int)a.nr;
-    auto next = [&](PairPulse& pa, PairPulse& pb, int& base, float2 (&d)[9], int k) {
-        pa = pair_pulse(A.s.st, a.zscale2, A.s.zoff, A.s.cbase);
-        pb = pair_pulse(B.s.st, a.zscale2, B.s.zoff, B.s.cbase);
+    for (int k = k0; k < k1; ++k) {
+        const PairPulse pa = pair_pulse<W>(A.s.st, a.zscale2, A.s.zoff, A.s.cbase);
+        const PairPulse pb = pair_pulse<W>(B.s.st, a.zscale2, B.s.zoff, B.s.cbase);
         A.s.st.advance();
         B.s.st.advance();
-        base = min(min(pa.c0, pb.c0), nr - 9);
-        pair_taps_shared(a.rc + (int64_t)k * nr, base, d);
-    };
-    auto consume = [&](const PairPulse& pa, const PairPulse& pb, int base, const float2 (&d)[9],
-                       int k) {
+        const int base = min(min(pa.c0, pb.c0), nr - (W + 1));
+        const float2* line = a.rc + (int64_t)k * nr;
+        float2 d[W + 1];
+        pair_taps_shared<W>(line, base, d);
         const uint32_t oa = (uint32_t)(pa.c0 - base), ob = (uint32_t)(pb.c0 - base);
         const uint64_t bad = (__builtin_amdgcn_ballot_w64(oa > 1u) & act_a) |
                              (__builtin_amdgcn_ballot_w64(ob > 1u) & act_b);
         if (bad == 0) {
-            pair_update(s_pair, pa.row + (int)min(oa, 1u) * kPairCopyBytes, pa.qf, pa.ph, d,
-                        A.s.cre, A.s.cim);
-            pair_update(s_pair, pb.row + (int)min(ob, 1u) * kPairCopyBytes, pb.qf, pb.ph, d,
-                        B.s.cre, B.s.cim);
+            pair_update<W>(s_pair, pa.row + (int)min(oa, 1u) * T::kCopyBytes, pa.ph, d, A.s.cre,
+                           A.s.cim);
+            pair_update<W>(s_pair, pb.row + (int)min(ob, 1u) * T::kCopyBytes, pb.ph, d, B.s.cre,
+                           B.s.cim);
         } else {
-            const float2* line = a.rc + (int64_t)k * nr;
-            float2 e[9];
-            pair_taps_one<false>(line, pa.c0, nr, e);
-            pair_update(s_pair, pa.row, pa.qf, pa.ph, e, A.s.cre, A.s.cim);
-            pair_taps_one<false>(line, pb.c0, nr, e);
-            pair_update(s_pair, pb.row, pb.qf, pb.ph, e, B.s.cre, B.s.cim);
+            float2 e[W + 1];
+            pair_taps_one<W, false>(line, pa.c0, nr, e);
+            pair_update<W>(s_pair, pa.row, pa.ph, e, A.s.cre, A.s.cim);
+            pair_taps_one<W, false>(line, pb.c0, nr, e);
+            pair_update<W>(s_pair, pb.row, pb.ph, e, B.s.cre, B.s.cim);
         }
         if (++A.s.count >= a.chunk) {
             A.s.fold();
             B.s.fold();
         }
-    };
-    for (int k = k0; k < k1; ++k) {
-        PairPulse pa, pb;
-        int base;
-        float2 d[9];
-        next(pa, pb, base, d, k);
-        consume(pa, pb, base, d, k);
     }
 }
 
+template<int W>
 __device__ inline void pair_tile(const SumArgs& a, const float* s_pair, const OutDesc& od,
                                  int64_t tile, int lane)
 {
@@ -1115,22 +1168,19 @@ __device__ inline void pair_tile(const SumArgs& a, const float* s_pair, const Ou
     // is discarded), so that the shared gather always serves the lane
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-        const int src = win[j] ? j : (win[1 - j] ? 1 - j : -1);
+        // selects, not an index into sp[] (a run-time index would place the
+        // records in scratch memory)
+        const bool own = win[j], other = !win[j] && win[1 - j];
+        const PixSplit& r0 = sp[j];
+        const PixSplit& r1 = sp[1 - j];
         SplitAcc& s = L[j].s;
-        if (src >= 0) {
-            const PixSplit& r = sp[src];
-            s.st.p = r.p0;
-            s.st.d1 = r.d1;
-            s.st.d2 = r.d2;
-            s.st.d3 = r.d3;
-            s.st.d4 = r.d4;
-            s.zoff = r.zoff;
-            s.cbase = r.cbase;
-        } else {
-            s.st.p = s.st.d1 = s.st.d2 = s.st.d3 = s.st.d4 = 0.;
-            s.zoff = 0.f;
-            s.cbase = 0;
-        }
+        s.st.p = own ? r0.p0 : (other ? r1.p0 : 0.);
+        s.st.d1 = own ? r0.d1 : (other ? r1.d1 : 0.);
+        s.st.d2 = own ? r0.d2 : (other ? r1.d2 : 0.);
+        s.st.d3 = own ? r0.d3 : (other ? r1.d3 : 0.);
+        s.st.d4 = own ? r0.d4 : (other ? r1.d4 : 0.);
+        s.zoff = own ? r0.zoff : (other ? r1.zoff : 0.f);
+        s.cbase = own ? r0.cbase : (other ? r1.cbase : 0);
         s.cre = s.cim = s.acc_re = s.acc_im = s.cmp_re = s.cmp_im = 0.f;
         s.count = 0;
     }
@@ -1147,16 +1197,16 @@ __device__ inline void pair_tile(const SumArgs& a, const float* s_pair, const Ou
         kf1 = kw1;
     }
     const bool edge = (win[0] && sp[0].edge) || (win[1] && sp[1].edge);
-    const bool guard = a.nr < 9 || __builtin_amdgcn_readfirstlane(wave_max_i(edge ? 1 : 0)) != 0;
+    const bool guard = a.nr < W + 1 || __builtin_amdgcn_readfirstlane(wave_max_i(edge ? 1 : 0)) != 0;
     if (kw0 < kw1) {
         if (guard) {
-            pair_masked<true>(a, s_pair, L[0], L[1], kw0, kw1);
+            pair_masked<W, true>(a, s_pair, L[0], L[1], kw0, kw1);
         } else {
-            pair_masked<false>(a, s_pair, L[0], L[1], kw0, kf0);
+            pair_masked<W, false>(a, s_pair, L[0], L[1], kw0, kf0);
             if (kf0 < kf1)
-                pair_core(a, s_pair, L[0], L[1], kf0, kf1, __builtin_amdgcn_ballot_w64(win[0]),
-                          __builtin_amdgcn_ballot_w64(win[1]));
-            pair_masked<false>(a, s_pair, L[0], L[1], max(kf1, kw0), kw1);
+                pair_core<W>(a, s_pair, L[0], L[1], kf0, kf1, __builtin_amdgcn_ballot_w64(win[0]),
+                             __builtin_amdgcn_ballot_w64(win[1]));
+            pair_masked<W, false>(a, s_pair, L[0], L[1], max(kf1, kw0), kw1);
         }
         L[0].s.fold();
         L[1].s.fold();
@@ -1175,9 +1225,9 @@ template<int W, bool kStaged, bool kPair>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
 void sum_split_kernel(SumArgs a)
 {
-    static_assert(!kPair || (W == 8 && !kStaged), "two-line lanes: W = 8, gathered taps");
+    static_assert(!(kPair && kStaged), "two-line lanes gather their taps");
     constexpr int ROW = poly_row_floats(W);
-    constexpr int kTable = kPair ? kPairTableFloats : (kPolyphaseRows + 1) * ROW;
+    constexpr int kTable = kPair ? PairTable<W>::kFloats : (kPolyphaseRows + 1) * ROW;
     // static LDS arrays: row and slot offsets fold into the ds_read offsets
     __shared__ float4 s_poly4[kTable / 4];
     __shared__ float4 s_win4[kStaged ? kSumWaves * kWinWaveBytes / 16 : 1];
@@ -1187,14 +1237,61 @@ void sum_split_kernel(SumArgs a)
     __syncthreads();
     char* s_win = reinterpret_cast<char*>(s_win4);
     for_each_tile(a.batch, [&](const OutDesc& d, int64_t tile, int lane) {
-        if (kPair)
-            pair_tile(a, s_poly, d, tile, lane);
+        if constexpr (kPair)
+            pair_tile<W>(a, s_poly, d, tile, lane);
         else
             split_tile<W, kStaged>(a, s_poly, s_win, d, tile, lane);
     });
 }
 
+// Pulse window [kmin, kmax) of every band of `band_rows` output lines of one
+// grid whose records start at `geom`: the union of the windows of the band's
+// pixels that integrate pulses (kmin > kmax when none does).  One workgroup
+// of 256 threads per band.
+__global__ __launch_bounds__(256) void band_window_kernel(const PixGeom* geom, int64_t length,
+                                                          int64_t width, int band_rows,
+                                                          int32_t* win)
+{
+    const int64_t b = blockIdx.x;
+    const int64_t r0 = b * band_rows;
+    const int64_t r1 = min(length, r0 + band_rows);
+    int lo = INT_MAX, hi = INT_MIN;
+    for (int64_t i = r0 * width + threadIdx.x; i < r1 * width; i += blockDim.x) {
+        const PixGeom& g = geom[i];
+        if (g.status == kPixOk && g.kstop > g.kstart) {
+            lo = min(lo, g.kstart);
+            hi = max(hi, g.kstop);
+        }
+    }
+    lo = wave_min_i(lo);
+    hi = wave_max_i(hi);
+    __shared__ int s_lo[4], s_hi[4];
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_lo[wave] = lo;
+        s_hi[wave] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            lo = min(lo, s_lo[w]);
+            hi = max(hi, s_hi[w]);
+        }
+        win[2 * b] = lo;
+        win[2 * b + 1] = hi;
+    }
+}
+
 // ------------------------------------------------------------------ launchers
+
+hipError_t launch_band_window(const PixGeom* geom, int64_t length, int64_t width, int band_rows,
+                              int64_t n_bands, int32_t* win, hipStream_t stream)
+{
+    if (n_bands <= 0) return hipSuccess;
+    hipLaunchKernelGGL(band_window_kernel, dim3((unsigned)n_bands), dim3(256), 0, stream, geom,
+                       length, width, band_rows, win);
+    return hipGetLastError();
+}
 
 hipError_t launch_pulse_orbit(const Orbit& orbit, double t0, double dt, int na, double* pv,
                               int32_t* flags, hipStream_t stream)
@@ -1231,21 +1328,65 @@ hipError_t launch_fit(const GeomArgs& args, int grid, hipStream_t stream)
     return hipGetLastError();
 }
 
+namespace {
+
+// Kernels of every width kMinKernelWidth..kMaxKernelWidth, indexed by width.
+using Widths = std::make_integer_sequence<int, kMaxKernelWidth - kMinKernelWidth + 1>;
+
+template<int... I>
+const void* plain_fn(int w, std::integer_sequence<int, I...>)
+{
+    static const void* const fns[] = {
+        reinterpret_cast<const void*>(&sum_plain_kernel<I + kMinKernelWidth>)...};
+    return fns[w - kMinKernelWidth];
+}
+
+template<int... I>
+const void* split_fn(int w, bool pair, std::integer_sequence<int, I...>)
+{
+    static const void* const one[] = {
+        reinterpret_cast<const void*>(&sum_split_kernel<I + kMinKernelWidth, false, false>)...};
+    static const void* const two[] = {
+        reinterpret_cast<const void*>(&sum_split_kernel<I + kMinKernelWidth, false, true>)...};
+    return (pair ? two : one)[w - kMinKernelWidth];
+}
+
+bool width_ok(int w) { return w >= kMinKernelWidth && w <= kMaxKernelWidth; }
+
+// Dynamic LDS of the plain sum: the (t_i, t_{i+1} - t_i) table when it fits.
+size_t plain_lds(int ntab) { return ntab <= kMaxTab2Lds ? sizeof(double2) * ntab : 0; }
+
+}  // namespace
+
+// Host-side handle of a kernel (0 geometry, 4 geometry for generic DEMs,
+// 1 plain sum, 2 split sum, 3 fit), as graph kernel nodes report it; null for
+// a width or variant that is not built.
+const void* kernel_fn(int which, int width, bool staged, bool pair)
+{
+    switch (which) {
+    case 0: return reinterpret_cast<const void*>(&geometry_kernel<false>);
+    case 4: return reinterpret_cast<const void*>(&geometry_kernel<true>);
+    case 3: return reinterpret_cast<const void*>(&fit_kernel);
+    case 1: return width_ok(width) ? plain_fn(width, Widths{}) : nullptr;
+    default:
+        if (staged) {
+            // LDS-staged pulse windows: widths 8 and 9, one line per lane
+            if (pair) return nullptr;
+            if (width == 8) return reinterpret_cast<const void*>(&sum_split_kernel<8, true, false>);
+            if (width == 9) return reinterpret_cast<const void*>(&sum_split_kernel<9, true, false>);
+            return nullptr;
+        }
+        return width_ok(width) ? split_fn(width, pair, Widths{}) : nullptr;
+    }
+}
+
 hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_t stream)
 {
     if (grid <= 0) return hipSuccess;
-    const size_t lds = args.ntab <= kMaxTableLds ? sizeof(float) * args.ntab : 0;
-    switch (width) {
-    case 8:
-        hipLaunchKernelGGL(sum_plain_kernel<8>, dim3(grid), dim3(256), lds, stream, args);
-        break;
-    case 9:
-        hipLaunchKernelGGL(sum_plain_kernel<9>, dim3(grid), dim3(256), lds, stream, args);
-        break;
-    default:
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    const void* fn = kernel_fn(1, width, false, false);
+    if (fn == nullptr) return hipErrorInvalidValue;
+    void* kargs[] = {const_cast<SumArgs*>(&args)};
+    return hipLaunchKernel(fn, dim3(grid), dim3(256), kargs, plain_lds(args.ntab), stream);
 }
 
 // Workgroups one CU can hold for the chosen sum kernel (256 threads for the
@@ -1253,13 +1394,13 @@ hipError_t launch_sum_plain(const SumArgs& args, int width, int grid, hipStream_
 int sum_blocks_per_cu(int precision, int width, bool staged, bool pair)
 {
     int n = 0;
-    hipError_t e = hipSuccess;
+    hipError_t e = hipErrorInvalidValue;
     if (precision == 0) {
-        const size_t lds = sizeof(float) * kMaxTableLds;
-        e = width == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_plain_kernel<8>, 256, lds)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sum_plain_kernel<9>, 256, lds);
+        const void* fn = kernel_fn(1, width, false, false);
+        if (fn) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256, plain_lds(2048));
     } else {
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel_fn(2, width, staged, pair), 512, 0);
+        const void* fn = kernel_fn(2, width, staged, pair);
+        if (fn) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 512, 0);
     }
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -1269,42 +1410,16 @@ int sum_blocks_per_cu(int precision, int width, bool staged, bool pair)
 }
 
 // Persistent grid: `grid` workgroups of 512 threads (a multiple of 8, one
-// share per XCD), sized by the caller from the device's CU count and occupancy.
-const void* kernel_fn(int which, int width, bool staged, bool pair)
-{
-    switch (which) {
-    case 0: return reinterpret_cast<const void*>(&geometry_kernel<false>);
-    case 4: return reinterpret_cast<const void*>(&geometry_kernel<true>);
-    case 3: return reinterpret_cast<const void*>(&fit_kernel);
-    case 1: return width == 8 ? reinterpret_cast<const void*>(&sum_plain_kernel<8>)
-                              : reinterpret_cast<const void*>(&sum_plain_kernel<9>);
-    default:
-        if (width == 8 && pair) return reinterpret_cast<const void*>(&sum_split_kernel<8, false, true>);
-        if (width == 8)
-            return staged ? reinterpret_cast<const void*>(&sum_split_kernel<8, true, false>)
-                          : reinterpret_cast<const void*>(&sum_split_kernel<8, false, false>);
-        return staged ? reinterpret_cast<const void*>(&sum_split_kernel<9, true, false>)
-                      : reinterpret_cast<const void*>(&sum_split_kernel<9, false, false>);
-    }
-}
-
+// share per XCD), sized by the caller from the device's CU count and
+// occupancy.  The weight tables and window rings are static LDS arrays.
 hipError_t launch_sum_split(const SumArgs& args, int width, int grid, bool staged, bool pair,
                             hipStream_t stream)
 {
     if (grid <= 0) return hipSuccess;
-    if ((width != 8 && width != 9) || (pair && (width != 8 || staged))) return hipErrorInvalidValue;
-    const size_t lds = 0;  // the weight table and window rings are static LDS arrays
-    if (pair)
-        hipLaunchKernelGGL((sum_split_kernel<8, false, true>), dim3(grid), dim3(512), lds, stream, args);
-    else if (width == 8 && staged)
-        hipLaunchKernelGGL((sum_split_kernel<8, true, false>), dim3(grid), dim3(512), lds, stream, args);
-    else if (width == 8)
-        hipLaunchKernelGGL((sum_split_kernel<8, false, false>), dim3(grid), dim3(512), lds, stream, args);
-    else if (staged)
-        hipLaunchKernelGGL((sum_split_kernel<9, true, false>), dim3(grid), dim3(512), lds, stream, args);
-    else
-        hipLaunchKernelGGL((sum_split_kernel<9, false, false>), dim3(grid), dim3(512), lds, stream, args);
-    return hipGetLastError();
+    const void* fn = kernel_fn(2, width, staged, pair);
+    if (fn == nullptr) return hipErrorInvalidValue;
+    void* kargs[] = {const_cast<SumArgs*>(&args)};
+    return hipLaunchKernel(fn, dim3(grid), dim3(512), kargs, 0, stream);
 }
 
 }  // namespace bp

@@ -848,6 +848,47 @@ BP_HD double bistatic_delay(V3 p, V3 v, V3 x)
     return 2. * (dot(r, v) - kC * norm(r)) / (dot(v, v) - (kC * kC));
 }
 
+// sin(2 pi t) and cos(2 pi t) in FP64 for a phase t in turns.  t is reduced
+// to r = t - q/4 with q the nearest integer to 4t (exact for |t| < 2^50), so
+// that |2 pi r| <= pi/4; sin and cos of x = 2 pi r come from their Taylor
+// series through x^17 and x^18 (truncation below 1e-16 on that interval), and
+// the quadrant q mod 4 rotates the pair.  Error about 2 ulp; no table, no
+// division.
+BP_HD void sincos_turns(double t, double* s, double* c)
+{
+    const double q = rint(4.0 * t);
+    const double r = fma(-0.25, q, t);
+    const double x = r * 6.283185307179586476925286766559;
+    const double x2 = x * x;
+    // sin x = x (1 + x^2 (-1/3! + x^2 (1/5! - ...)))
+    double ps = 2.8114572543455207632e-15;          //  1/17!
+    ps = fma(ps, x2, -7.6471637318198164759e-13);   // -1/15!
+    ps = fma(ps, x2, 1.6059043836821614599e-10);    //  1/13!
+    ps = fma(ps, x2, -2.5052108385441718775e-08);   // -1/11!
+    ps = fma(ps, x2, 2.7557319223985890653e-06);    //  1/9!
+    ps = fma(ps, x2, -1.9841269841269841270e-04);   // -1/7!
+    ps = fma(ps, x2, 8.3333333333333333333e-03);    //  1/5!
+    ps = fma(ps, x2, -1.6666666666666666667e-01);   // -1/3!
+    const double sx = fma(x * x2, ps, x);
+    // cos x = 1 + x^2 (-1/2! + x^2 (1/4! - ...))
+    double pc = 1.5619206968586225686e-16;          //  1/18!
+    pc = fma(pc, x2, -4.7794773323873852974e-14);   // -1/16!
+    pc = fma(pc, x2, 1.1470745597729724714e-11);    //  1/14!
+    pc = fma(pc, x2, -2.0876756987868098979e-09);   // -1/12!
+    pc = fma(pc, x2, 2.7557319223985890653e-07);    //  1/10!
+    pc = fma(pc, x2, -2.4801587301587301587e-05);   // -1/8!
+    pc = fma(pc, x2, 1.3888888888888888889e-03);    //  1/6!
+    pc = fma(pc, x2, -4.1666666666666666667e-02);   //  1/4!
+    pc = fma(pc, x2, 0.5);                          // (sign folded below)
+    const double cx = fma(-x2, pc, 1.0);
+    // quadrant: (sin, cos) of x + q pi/2
+    const int64_t iq = (int64_t)q & 3;
+    const double sa = (iq & 1) ? cx : sx;
+    const double ca = (iq & 1) ? sx : cx;
+    *s = (iq & 2) ? -sa : sa;
+    *c = ((iq + 1) & 2) ? -ca : ca;
+}
+
 // TSX dry troposphere delay (focus/DryTroposphereModel.icc:9-29).
 BP_HD double dry_tropo_tsx(V3 p, V3 llh)
 {

@@ -4,8 +4,11 @@ Signature, argument checks, error types and the return value follow
 python/extensions/pybind_isce3/focus/Backproject.cpp:29-166 (CPU) and
 python/extensions/pybind_isce3/cuda/focus/Backproject.cpp:25-116 (GPU, which
 adds ``batch``).  The work runs in HIP kernels on the current GPU through
-the C-ABI in include/isce3_amd_backproject.h; host arrays are staged through
-device buffers, device tensors (torch, ROCm) are used in place.
+the C-ABI in include/isce3_amd_backproject.h.  Device tensors (torch, ROCm)
+are used in place; pulses in host memory (numpy arrays, memory maps, CPU
+tensors) are read only where the output needs them: the pixels' pulse window
+is copied to the device in chunks of ``batch`` pulses, as the reference's GPU
+path does (cxx/isce3/cuda/focus/Backproject.cu:641-688).
 """
 from __future__ import annotations
 
@@ -123,7 +126,7 @@ def backproject(out, out_geometry: RadarGeometry, signal_data, in_geometry: Rada
                 dry_tropo_model: str = "tsx", rdr2geo_params: Optional[dict] = None,
                 geo2rdr_params: Optional[dict] = None, batch=1024, height=None, *,
                 precision: str = "plain", options: Optional[dict] = None, stream=None,
-                stats: Optional[dict] = None) -> bool:
+                stats: Optional[dict] = None, max_window_bytes: int = 0) -> bool:
     """Focus in azimuth via time-domain backprojection.
 
     Parameters follow isce3.focus.backproject: ``out`` (complex64, written in
@@ -131,15 +134,19 @@ def backproject(out, out_geometry: RadarGeometry, signal_data, in_geometry: Rada
     ``in_geometry``, ``dem``, ``fc`` (Hz), ``ds`` (desired azimuth resolution,
     m), ``kernel`` (TabulatedKernelF32), ``dry_tropo_model`` ("tsx" or
     "nodelay"), ``rdr2geo_params`` {tol_height, look_min, look_max},
-    ``geo2rdr_params`` {tol_aztime, time_start, time_end}, ``batch`` (accepted
-    for the GPU signature; all pulses are resident so it only has to be >= 1)
-    and ``height`` (optional float32 output).  Arrays may be numpy arrays or
-    ROCm torch tensors.
+    ``geo2rdr_params`` {tol_aztime, time_start, time_end}, ``batch`` (pulses
+    per host-to-device copy when ``signal_data`` is in host memory; >= 1) and
+    ``height`` (optional float32 output).  Arrays may be numpy arrays (memory
+    maps included) or torch tensors; host pulses outside the output's pulse
+    window are never read.
 
     Extensions: ``precision`` "plain" (reference semantics, FP64 inner loop)
     or "split" (FP64 per-pixel reference + FP32 deltas), ``options`` tiling
     dict {tile_rows, chunk, order, blocks_per_cu, staged, lane_rows}, ``stream`` (torch stream or raw hipStream_t int)
-    and ``stats`` (dict filled with updates, geometry_ms, sum_ms).
+    ``stats`` (dict filled with updates, geometry_ms, sum_ms, and for host
+    pulses h2d_bytes, window_pulses, bands, copy_ms) and ``max_window_bytes``
+    (device bytes for the pulse window of host pulses, 0 = half the free
+    memory; a larger window is focused in bands of output lines).
 
     Returns True if any pixel failed to converge (those pixels are NaN).
     """
@@ -177,7 +184,7 @@ def backproject(out, out_geometry: RadarGeometry, signal_data, in_geometry: Rada
     with torch.cuda.device(device):
         return _run(torch, device, out, out_geometry, signal_data, in_geometry, dem, fc, ds,
                     kernel, tropo, r2g, g2r, height, _PRECISION[precision], options, stream,
-                    stats)
+                    stats, int(batch), int(max_window_bytes))
 
 
 def _check_out_array(a, what, complex_=True):
@@ -221,6 +228,19 @@ def _device_input(torch, device, sig):
     return torch.from_numpy(h).to(f"cuda:{device}", non_blocking=False)
 
 
+def _host_pulses(torch, sig) -> np.ndarray:
+    """C-contiguous complex64 view of pulses in host memory (numpy array,
+    memory map or CPU tensor); copied only when the layout or type differs."""
+    if _is_torch(sig):
+        sig = sig.contiguous()
+        if sig.dtype != torch.complex64:
+            sig = sig.to(torch.complex64)
+        return sig.numpy()
+    if isinstance(sig, np.ndarray) and sig.dtype == np.complex64 and sig.flags.c_contiguous:
+        return sig
+    return np.ascontiguousarray(sig, dtype=np.complex64)
+
+
 def _device_output(torch, device, a, dtype):
     """Device buffer written by the kernels: the array itself when it is a
     device tensor, else a staging tensor copied back by _copy_back."""
@@ -259,7 +279,11 @@ def _fill_stats(stats, st):
 
 
 def _run(torch, device, out, ogeom, sig, igeom, dem, fc, ds, kernel, tropo, r2g, g2r, height,
-         precision, options, stream, stats):
+         precision, options, stream, stats, batch=1024, max_window_bytes=0):
+    if not (_is_torch(sig) and sig.is_cuda):
+        return _run_host(torch, device, out, ogeom, sig, igeom, dem, fc, ds, kernel, tropo, r2g,
+                         g2r, height, precision, options, stream, stats, batch,
+                         max_window_bytes)
     hstream = _stream_handle(torch, stream)
     d_in = _device_input(torch, device, sig)
     d_out = _device_output(torch, device, out, torch.complex64)
@@ -284,6 +308,41 @@ def _run(torch, device, out, ogeom, sig, igeom, dem, fc, ds, kernel, tropo, r2g,
     _copy_back(torch, out, d_out)
     _copy_back(torch, height, d_h)
     _fill_stats(stats, st)
+    return bool(failed.value)
+
+
+def _run_host(torch, device, out, ogeom, sig, igeom, dem, fc, ds, kernel, tropo, r2g, g2r,
+              height, precision, options, stream, stats, batch, max_window_bytes):
+    """Pulses in host memory: bp_backproject_host copies the pixels' pulse
+    window only (in chunks of ``batch`` pulses)."""
+    hstream = _stream_handle(torch, stream)
+    h_in = _host_pulses(torch, sig)
+    d_out = _device_output(torch, device, out, torch.complex64)
+    d_h = _device_output(torch, device, height, torch.float32)
+    og = geometry_struct(ogeom)
+    ig = geometry_struct(igeom)
+    dm = dem._struct()
+    kr = kernel._struct()
+    opt = _options_struct(options)
+    st = _native.Stats_t()
+    hst = _native.HostStats_t()
+    failed = C.c_int(0)
+    ctx = _contexts.get(device)
+    _keep = (ogeom, igeom, dem, kernel, h_in)
+    rc = _native.lib().bp_backproject_host(
+        ctx, C.byref(og), C.byref(ig), C.byref(dm), float(fc), float(ds), C.byref(kr),
+        int(tropo), C.byref(r2g), C.byref(g2r), C.c_void_p(h_in.ctypes.data), int(batch),
+        int(max_window_bytes), C.c_void_p(d_out.data_ptr()),
+        C.c_void_p(d_h.data_ptr()) if d_h is not None else None, int(precision), C.byref(opt),
+        C.c_void_p(hstream), C.byref(failed), C.byref(st), C.byref(hst))
+    del _keep
+    _native.check(rc)
+    _copy_back(torch, out, d_out)
+    _copy_back(torch, height, d_h)
+    _fill_stats(stats, st)
+    if stats is not None:
+        stats.update(h2d_bytes=int(hst.h2d_bytes), window_pulses=int(hst.window_pulses),
+                     bands=int(hst.bands), copy_ms=float(hst.copy_ms))
     return bool(failed.value)
 
 

@@ -10,8 +10,8 @@ Look-alikes of
     (python/packages/isce3/core/types.py:5, 38-42, 116-171), run on the GPU;
   * ``nisar.workflows.focus.BackgroundWriter`` (focus.py:817-914): range
     correction, mixed-mode accumulation, block statistics and encoding in one
-    HIP pass per block (include/isce3_amd_rslc.h), the dataset write in a
-    background thread.  The dataset is any object with ``dtype``,
+    HIP pass per block (include/isce3_amd_rslc.h), the dataset write on a
+    worker thread.  The dataset is any object with ``dtype``,
     ``write_direct(array, dest_sel=...)`` and ``__getitem__`` (an
     ``h5py.Dataset`` in the workflow).
 """
@@ -21,7 +21,7 @@ import ctypes as C
 import logging
 import math
 import threading
-from queue import Empty, Queue
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -302,9 +302,9 @@ def truncate_mantissa(z, significant_bits: int = 10) -> None:
 # ---------------------------------------------------------------- writer
 
 def unpack_slices(slices):
-    """Hashable key of a (rows, cols) slice pair (focus.py:806-814)."""
-    rows, cols = slices
-    return ((rows.start, rows.stop, rows.step), (cols.start, cols.stop, cols.step))
+    """Hashable form of a (rows, cols) pair of slices: one (start, stop, step)
+    triple per axis, usable as a dictionary key."""
+    return tuple((s.start, s.stop, s.step) for s in slices)
 
 
 def _decode_c32(a) -> np.ndarray:
@@ -317,60 +317,62 @@ def _decode_c32(a) -> np.ndarray:
     return a.astype(np.complex64)
 
 
-class BackgroundWriter:
-    """Compute statistics and write RSLC blocks in a background thread
-    (nisar.workflows.focus.BackgroundWriter, focus.py:817-914, on the worker
-    queue of isce3.io.background.BackgroundWriter).
+# storage type of the dataset each encoding writes to
+_STORAGE = {"complex64": np.dtype(np.complex64), "complex64_zero_mantissa": np.dtype(np.complex64),
+            "complex32": complex32}
 
-    ``write(z, block)`` scales ``z`` by ``range_cor[block[1]]`` in place, adds
-    the data already written at ``block`` when that block was visited before,
-    records the block statistics and writes the encoded block with
-    ``dset.write_direct(..., dest_sel=block)``; all but the dataset access run
-    in one HIP pass.  ``queue_write`` hands the same call to the thread."""
+
+def _stored_dtype(dset):
+    """The dataset's element type, or None when the dataset cannot describe
+    it (h5py raises TypeError for some compound types, e.g. float16 pairs)."""
+    try:
+        return np.dtype(dset.dtype)
+    except TypeError:
+        return None
+
+
+class BackgroundWriter:
+    """Finish and store focused RSLC blocks, optionally in a background thread.
+
+    Same constructor, methods and errors as nisar.workflows.focus
+    .BackgroundWriter (focus.py:817-914).  ``write(z, block)`` runs one HIP
+    pass over the block (include/isce3_amd_rslc.h): ``z`` scaled in place by
+    ``range_cor[cols]``, plus what is already stored at ``block`` when the
+    block was written before (mixed-mode focusing visits blocks twice), the
+    block's statistics, and its storage encoding; the encoded block goes to
+    ``dset.write_direct(..., dest_sel=block)``.  ``queue_write`` runs the same
+    call on one worker thread (at most ``num_work_queue`` calls waiting when
+    positive); ``notify_finished`` waits for them and re-raises the first
+    error one of them raised.  ``stats`` merges the statistics of every block
+    written.  Writes are serialised by a lock, so calls from the caller's
+    thread and the worker may mix."""
 
     def __init__(self, range_cor, dset, data_type, mantissa_nbits=10, device: int = 0,
                  num_work_queue: int = 0, timeout: float = 1.0):
-        log.debug(f"Preparing BackgroundWriter with data_type={data_type}"
-                  f" and mantissa_nbits={mantissa_nbits}")
         if data_type not in DATA_TYPES:
             raise ValueError(f"Requested invalid data_type {data_type}")
+        stored = _stored_dtype(dset)
+        fits = stored == _STORAGE[data_type] if stored is not None else data_type == "complex32"
+        if not fits:
+            raise TypeError(f"Requested {data_type} encoding but the dataset stores "
+                            f"{stored if stored is not None else 'an undescribed type'}")
+        mantissa_nbits = int(mantissa_nbits)
+        if data_type == "complex64_zero_mantissa" and not 0 < mantissa_nbits <= 23:
+            raise ValueError(f"Require 0 < significant_bits={mantissa_nbits} <= 23")
+        log.debug("RSLC writer: %s encoding (%d mantissa bits), dataset %s", data_type,
+                  mantissa_nbits, stored)
         self.range_cor = range_cor
         self.dset = dset
         self.data_type = data_type
-        self.mantissa_nbits = int(mantissa_nbits)
+        self.mantissa_nbits = mantissa_nbits
         self.device = int(device)
         self._visited_blocks = {}
-        try:
-            h5type = dset.dtype
-        except TypeError:
-            h5type = "unknown"
-        c8match = data_type.startswith("complex64") and h5type == np.complex64
-        c4match = data_type == "complex32" and (h5type == "unknown" or h5type == complex32)
-        if not c8match and not c4match:
-            raise TypeError(f"Requested {data_type} encoding but type of HDF5 "
-                            f"dataset is {h5type}")
-        if data_type == "complex64_zero_mantissa" and not 0 < self.mantissa_nbits <= 23:
-            raise ValueError(f"Require 0 < significant_bits={self.mantissa_nbits} <= 23")
         self._finisher = None
-        self._timeout = timeout
-        self._finished = threading.Event()
-        self._queue = Queue(num_work_queue)
-        self._error = None
-        self._thread = threading.Thread(target=self._consume, daemon=True)
-        self._thread.start()
-
-    def _consume(self):
-        while not self._finished.is_set() and threading.main_thread().is_alive():
-            try:
-                args, kw = self._queue.get(timeout=self._timeout)
-            except Empty:
-                continue
-            try:
-                self.write(*args, **kw)
-            except BaseException as e:  # noqa: BLE001 - re-raised by notify_finished
-                if self._error is None:
-                    self._error = e
-            self._queue.task_done()
+        self._lock = threading.Lock()
+        self._pool = ThreadPoolExecutor(max_workers=1, thread_name_prefix="isce3_amd-rslc")
+        self._slots = threading.BoundedSemaphore(num_work_queue) if num_work_queue > 0 else None
+        self._pending = []
+        self._closed = False
 
     @property
     def stats(self) -> StatsRealImagFloat32:
@@ -379,47 +381,63 @@ class BackgroundWriter:
             total.update(s)
         return total
 
+    def block_stats(self, block) -> StatsRealImagFloat32:
+        """Statistics of the last write of ``block``."""
+        return self._visited_blocks[unpack_slices(block)]
+
     def readback(self, block) -> np.ndarray:
+        """The block as stored, decoded to complex64."""
         return _decode_c32(self.dset[block])
 
+    def _finisher_for(self, pixels: int) -> BlockFinisher:
+        if self._finisher is None or self._finisher.max_pixels < pixels:
+            self._finisher = BlockFinisher(max(pixels, 1 << 20), self.device)
+        return self._finisher
+
     def write(self, z, block):
-        _, cols = block
-        npx = int(np.prod(z.shape))
-        if self._finisher is None or self._finisher.max_pixels < npx:
-            self._finisher = BlockFinisher(max(npx, 1 << 20), self.device)
-        rc = self.range_cor[cols]
         key = unpack_slices(block)
-        prior = None
-        if key in self._visited_blocks:
-            log.debug("reading back SLC data at block %s", block)
-            prior = self.readback(block)
-        enc, s = self._finisher.finish(z, range_cor=rc, prior=prior, data_type=self.data_type,
-                                       mantissa_nbits=self.mantissa_nbits)
-        amax = np.max(np.abs([s.real.max, s.real.min, s.imag.max, s.imag.min]))
-        log.debug(f"scaled max component = {amax}")
-        self._visited_blocks[key] = s
-        if _is_torch(enc):
-            import torch
-            enc = (enc.view(torch.int32).cpu().numpy().view(complex32)
-                   if self.data_type == "complex32" else enc.cpu().numpy())
-        self.dset.write_direct(enc, dest_sel=block)
+        cols = block[1]
+        with self._lock:
+            prior = self.readback(block) if key in self._visited_blocks else None
+            enc, st = self._finisher_for(int(np.prod(z.shape))).finish(
+                z, range_cor=self.range_cor[cols], prior=prior, data_type=self.data_type,
+                mantissa_nbits=self.mantissa_nbits)
+            self._visited_blocks[key] = st
+            if _is_torch(enc):
+                import torch
+                enc = (enc.view(torch.int32).cpu().numpy().view(complex32)
+                       if self.data_type == "complex32" else enc.cpu().numpy())
+            self.dset.write_direct(enc, dest_sel=block)
+        log.debug("RSLC block %s written%s; real [%g, %g], imag [%g, %g]", key,
+                  " (accumulated)" if prior is not None else "", st.real.min, st.real.max,
+                  st.imag.min, st.imag.max)
+
+    def _run(self, z, block):
+        try:
+            self.write(z, block)
+        finally:
+            if self._slots is not None:
+                self._slots.release()
 
     def queue_write(self, z, block):
-        if self._finished.is_set():
-            raise RuntimeError("Attempted to queue_work after notify_finished!")
-        self._queue.put(((z, block), {}))
+        if self._closed:
+            raise RuntimeError("queue_write called after notify_finished")
+        if self._slots is not None:
+            self._slots.acquire()
+        self._pending.append(self._pool.submit(self._run, z, block))
 
     def notify_finished(self):
-        self._queue.join()
-        self._finished.set()
-        if self._thread.is_alive():
-            self._thread.join()
-        if self._error is not None:
-            err, self._error = self._error, None
-            raise err
+        """Wait for every queued write; re-raise the first error one raised."""
+        self._closed = True
+        pending, self._pending = self._pending, []
+        errors_ = [f.exception() for f in pending]
+        self._pool.shutdown(wait=True)
+        first = next((e for e in errors_ if e is not None), None)
+        if first is not None:
+            raise first
 
     def __del__(self):
         try:
-            self._finished.set()
-        except Exception:  # noqa: BLE001
+            self._pool.shutdown(wait=False)
+        except Exception:  # noqa: BLE001 - interpreter shutdown
             pass

@@ -18,6 +18,9 @@ Scenes (pulses x range bins -> output grid):
          requests are cut (random_tiles)
   pt     1024 x 1024 -> 129 x 129 chip around one target, Knab width 9
   tiny   256 x 512 -> 48 x 40, 2 targets, fractal DEM (fast parity checks)
+  strip  16384 x 1024 -> 16384 x 128, 6 targets, flat DEM, ds = 24 m: a long
+         raster whose aperture (~950 pulses) is a small part of it (out-of-core
+         host input)
 """
 from __future__ import annotations
 
@@ -202,6 +205,14 @@ def make_scene(config: str, seed: Optional[int] = None, *, ds: Optional[float] =
         dem = _dem_around(orbit, out_grid, 128, 30.0, 0.8, 200.0, seed)
         rel = [(12.4, 9.7), (35.2, 30.1)]
         ntg, step, noise, kw = 2, None, 0.01, kw or 8.0
+    elif config == "strip":
+        seed = 13 if seed is None else seed
+        in_grid, out_grid = _center_grid(16384, 1024, 16384, 128)
+        dem = DEMInterpolator(0.0)
+        rel = [(1000.4, 20.3), (3000.7, 64.0), (6000.2, 100.6), (9000.9, 40.2),
+               (12000.5, 80.8), (15200.1, 60.5)]
+        ntg, step, noise, kw = 6, None, 0.01, kw or 8.0
+        ds = ds or 24.0
     else:
         raise ValueError(f"unknown scene config {config!r}")
     ds = 6.0 if ds is None else float(ds)

@@ -1,0 +1,125 @@
+"""Pulses in host memory (numpy arrays, read-only memory maps of a raster
+file): only the output's pulse window -- the union of its pixels' coherent
+processing windows, Backproject.cpp:175-193 -- is copied to the device, in
+chunks of `batch` pulses, and every result equals the device-resident call's
+bit for bit, with the window copied whole and in bands of output lines under
+a small device budget."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import complex_rel_error, scene_and_raw
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def strip(tmp_path_factory):
+    """The 16384 x 1024 strip raster in a file, opened read-only as a memory map."""
+    sc, raw = scene_and_raw("strip")
+    path = tmp_path_factory.mktemp("raster") / "rc.c64"
+    mm = np.memmap(path, dtype=np.complex64, mode="w+", shape=raw.shape)
+    mm[:] = raw
+    mm.flush()
+    del mm
+    ro = np.memmap(path, dtype=np.complex64, mode="r", shape=raw.shape)
+    return sc, raw, ro
+
+
+def blocks(sc):
+    """64 x 128 blocks near the start, middle and end of the strip, each
+    holding a point target."""
+    g = sc.out_grid
+    return [g.offset_and_resize(r0, 0, 64, 128) for r0 in (970, 5970, 15170)]
+
+
+def resident(torch, sc, d_raw, grid, precision, options=None):
+    from isce3_amd.focus import backproject
+    out = torch.empty(tuple(grid.shape), dtype=torch.complex64, device="cuda:0")
+    h = torch.empty(tuple(grid.shape), dtype=torch.float32, device="cuda:0")
+    failed = backproject(out, sc.out_geometry(grid), d_raw, sc.in_geometry(), sc.dem, sc.fc,
+                         sc.ds, sc.kernel, sc.dry_tropo_model, height=h, precision=precision,
+                         options=options)
+    return out.cpu().numpy(), h.cpu().numpy(), failed
+
+
+def from_host(sc, host, grid, precision, **kw):
+    from isce3_amd.focus import backproject
+    out = np.full(grid.shape, 9 + 9j, np.complex64)
+    h = np.zeros(grid.shape, np.float32)
+    st = {}
+    failed = backproject(out, sc.out_geometry(grid), host, sc.in_geometry(), sc.dem, sc.fc,
+                         sc.ds, sc.kernel, sc.dry_tropo_model, kw.pop("batch", 1024), h,
+                         precision=precision, stats=st, **kw)
+    return out, h, failed, st
+
+
+def pulse_window(sc, raw, grid):
+    """[min kstart, max kstop) over the block's pixels, from the oracle's
+    per-pixel windows."""
+    _, _, diag, rc = oracle.backproject(sc.out_geometry(grid), raw, sc.in_geometry(), sc.dem,
+                                        sc.fc, sc.ds, sc.kernel, sc.dry_tropo_model,
+                                        want_diag=True)
+    assert rc == 0
+    ks = [d.kstart for d in diag if d.status == 0 and d.kstop > d.kstart]
+    ke = [d.kstop for d in diag if d.status == 0 and d.kstop > d.kstart]
+    return min(ks), max(ke)
+
+
+@pytest.mark.parametrize("precision", ["plain", "split"])
+def test_memmap_copies_the_window_only(torch_gpu, strip, precision):
+    torch = torch_gpu
+    sc, raw, mm = strip
+    d_raw = torch.from_numpy(raw.copy()).cuda()
+    line = sc.in_grid.width * 8
+    for g in blocks(sc):
+        print(f"[host] {precision} block at {g.sensing_start:.3f} s", flush=True)
+        z0, h0, f0 = resident(torch, sc, d_raw, g, precision)
+        z, h, f, st = from_host(sc, mm, g, precision, batch=100)
+        np.testing.assert_array_equal(z, z0)
+        np.testing.assert_array_equal(h, h0)
+        assert f == f0 and not f
+        k0, k1 = pulse_window(sc, raw, g)
+        assert st["bands"] == 1
+        assert st["window_pulses"] == k1 - k0
+        assert st["h2d_bytes"] == (k1 - k0) * line
+        # the aperture is a small part of the raster
+        assert 16 * (k1 - k0) < sc.in_grid.length
+        assert st["updates"] > 0
+
+
+@pytest.mark.parametrize("precision,options", [("plain", None), ("split", None),
+                                               ("split", {"lane_rows": 1, "tile_rows": 4})])
+def test_bands_under_a_small_budget(torch_gpu, strip, precision, options):
+    """A device budget of ~1000 pulses (one aperture plus a few lines) splits
+    each block into bands of output lines; results stay bit-identical."""
+    torch = torch_gpu
+    from isce3_amd import errors
+    sc, raw, mm = strip
+    d_raw = torch.from_numpy(raw.copy()).cuda()
+    line = sc.in_grid.width * 8
+    g = blocks(sc)[1]
+    z0, h0, _ = resident(torch, sc, d_raw, g, precision, options)
+    k0, k1 = pulse_window(sc, raw, g)
+    budget = (k1 - k0 - 40) * line
+    z, h, f, st = from_host(sc, mm, g, precision, batch=37, max_window_bytes=budget,
+                            options=options)
+    print(f"[host] {precision} {options}: {st['bands']} bands, {st['window_pulses']} pulses",
+          flush=True)
+    np.testing.assert_array_equal(z, z0)
+    np.testing.assert_array_equal(h, h0)
+    assert st["bands"] >= 2
+    assert st["h2d_bytes"] == st["window_pulses"] * line
+    # a budget below one tile row's aperture is refused
+    with pytest.raises(errors.LengthError, match="max_window_bytes"):
+        from_host(sc, mm, g, precision, max_window_bytes=100 * line, options=options)
+
+
+def test_numpy_input_matches_oracle(torch_gpu, strip):
+    sc, raw, _ = strip
+    g = blocks(sc)[2]
+    z, _, _, st = from_host(sc, raw, g, "plain")
+    ref, _, _, rc = oracle.backproject(sc.out_geometry(g), raw, sc.in_geometry(), sc.dem, sc.fc,
+                                       sc.ds, sc.kernel, sc.dry_tropo_model, kernel_bandwidth=0.8)
+    assert rc == 0
+    assert complex_rel_error(z, ref) <= 1e-5

@@ -110,6 +110,14 @@ def test_two_lines_per_lane_rejected_where_unsupported(torch_gpu):
     with pytest.raises(errors.InvalidArgument):
         run_gpu("tiny", "plain", options={"lane_rows": 2})
     with pytest.raises(errors.InvalidArgument):
-        run_gpu("pt", "split", options={"lane_rows": 2})       # kernel width 9
-    with pytest.raises(errors.InvalidArgument):
         run_gpu("tiny", "split", options={"lane_rows": 2, "staged": 1})
+
+
+def test_two_lines_per_lane_odd_kernel_width(torch_gpu):
+    """The point-target scene uses Knab width 9 (the workflow's default
+    halfwidth 4): the two-line kernel serves it against the FP64 oracle."""
+    out, h, err, _ = run_gpu("pt", "split", options={"lane_rows": 2})
+    ref, href, _, rc = oracle_run("pt", 1)
+    assert err == (rc == 6)
+    assert complex_rel_error(out, ref) <= SPLIT_REL
+    assert rms(phase_errors(out, ref)) <= SPLIT_PHS

@@ -230,8 +230,8 @@ def test_background_writer_accumulates_revisited_blocks(torch_gpu, data_type):
     assert np.array_equal(got2.view(view), e2.view(view))
     total = w.stats
     assert total.n_valid == 2 * 48 * 160
-    check_stats(w._visited_blocks[((0, 48, None), (0, 160, None))], st1b)
-    check_stats(w._visited_blocks[((48, 96, None), (0, 160, None))], st2)
+    check_stats(w.block_stats(b1), st1b)
+    check_stats(w.block_stats(b2), st2)
 
 
 def test_full_size_block(torch_gpu):

@@ -107,3 +107,50 @@ def test_plan_processing_blocks_native_doppler_shifts_times():
     dt_expected = -fd * WAVELENGTH * r / (2 * SPEED ** 2)
     assert (a1 - a0) == pytest.approx(dt_expected, rel=0.05)
     assert (b1 - b0) == pytest.approx(dt_expected, rel=0.05)
+
+
+def test_corner_reskews_are_shared_between_blocks(monkeypatch):
+    """Each corner pixel of the block lattice is reskewed once, and the plan
+    equals the per-block evaluation of its four corners."""
+    sc = make_scene("small")
+    grid, orbit = sc.out_grid, sc.orbit
+    calls = []
+    real = geometry.rdr2rdr
+
+    def counting(t, r, *a, **kw):
+        calls.append((t, r))
+        return real(t, r, *a, **kw)
+
+    monkeypatch.setattr(geometry, "rdr2rdr", counting)
+    c = cfg(na=64, nr=64)
+    plan = planning.plan_processing_blocks(c, grid, core.LUT2d(), sc.dem, orbit)
+    assert len(calls) == len(set(calls))
+    n_az, n_rg = math.ceil(grid.length / 64), math.ceil(grid.width / 64)
+    assert len(calls) == (2 * n_az) * (2 * n_rg)
+    # brute force: the four corners of every block, each reskewed again
+    r2g, g2r = planning.get_rdr2geo_params(c), planning.get_geo2rdr_params(c, orbit)
+    for (rows, cols), (t0, t1) in plan:
+        times = []
+        for u in (rows.start, rows.stop - 1):
+            for v in (cols.start, cols.stop - 1):
+                t, _ = real(grid.sensing_start + u / grid.prf,
+                            grid.starting_range + v * grid.range_pixel_spacing, orbit,
+                            grid.lookside, core.LUT2d(), grid.wavelength, sc.dem,
+                            doppler_out=core.LUT2d(), rdr2geo_params=r2g, geo2rdr_params=g2r)
+                times.append(t)
+        sub = grid[rows, cols]
+        half = 0.55 * planning.get_sar_duration(sub.sensing_mid, sub.end_range, orbit,
+                                                core.Ellipsoid(), 6.0, grid.wavelength)
+        assert t0 == pytest.approx(min(times) - half, abs=1e-9)
+        assert t1 == pytest.approx(max(times) + half, abs=1e-9)
+
+
+def test_failed_reskew_raises_runtime_error(monkeypatch):
+    sc = make_scene("small")
+
+    def boom(*a, **kw):
+        raise RuntimeError("no root")
+
+    monkeypatch.setattr(geometry, "rdr2rdr", boom)
+    with pytest.raises(RuntimeError, match="imaging geometry"):
+        planning.plan_processing_blocks(cfg(), sc.out_grid, core.LUT2d(), sc.dem, sc.orbit)
