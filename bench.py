@@ -36,6 +36,9 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads stay on adjacent cores (read when the
+# OpenMP runtime starts, so before anything loads it)
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 FLOPS_PER_UPDATE = 101  # algorithmic flops per pixel-pulse update (FMA = 2)
 PEAK_FP32_TFLOPS = 157.3
@@ -168,17 +171,47 @@ def accuracy(args, np, sc, raw_np, results):
     return acc
 
 
+def host_cpu():
+    """Host description for the CPU baseline: logical CPUs (nproc), the CPU
+    model and the cores per socket and sockets from lscpu."""
+    import subprocess
+    info = {"nproc": os.cpu_count()}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], capture_output=True, text=True,
+                                           check=True).stdout.strip())
+    except Exception:  # noqa: BLE001 - description only
+        pass
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("Model name", "Core(s) per socket", "Socket(s)", "Thread(s) per core"):
+                info[k.lower().replace("(s)", "s").replace(" ", "_")] = v.strip()
+    except Exception:  # noqa: BLE001 - description only
+        pass
+    return info
+
+
+def cpu_threads():
+    """OpenMP threads of the CPU baseline: the host's logical CPUs, capped at
+    the 16-core CPU share that a one-GPU box gives this process."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 def cpu_baseline(args, np, sc, raw_np):
+    """The oracle in reference mode (the CPU restatement of the reference
+    backprojector, OpenMP over pixels, OMP_PROC_BIND=close) on a band of output
+    rows at the centre of the grid: one warm-up run, then the median of three
+    timed runs, the band sized so that one run takes about a quarter of
+    --cpu-seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    nth = min(16, os.cpu_count() or 1)
+    nth = cpu_threads()
     W = sc.out_grid.width
-    rows = 1
-    t_total = 0.0
-    upd_total = 0
     row0 = sc.out_grid.length // 2
-    first = row0
-    while True:
+
+    def run(rows):
         sub = np.arange(row0 * W, (row0 + rows) * W, dtype=np.int64)
         t0 = time.perf_counter()
         _, _, diag, _ = oracle.backproject(sc.out_geometry(), raw_np, sc.in_geometry(),
@@ -186,17 +219,26 @@ def cpu_baseline(args, np, sc, raw_np):
                                            sc.dry_tropo_model, subset=sub, nthreads=nth,
                                            want_diag=True)
         dt = time.perf_counter() - t0
-        t_total += dt
-        upd_total += int(sum(d.kstop - d.kstart for d in diag if d.status == 0))
-        row0 += rows
-        if t_total >= args.cpu_seconds or row0 >= sc.out_grid.length:
-            break
-        rows = max(1, min(64, int(rows * max(1.0, (args.cpu_seconds - t_total) / max(dt, 1e-3)))))
-    cpu = {"value": upd_total / t_total, "unit": "pixel-pulse updates/s", "cores": nth,
+        return dt, int(sum(d.kstop - d.kstart for d in diag if d.status == 0))
+
+    dt1, _ = run(1)  # sizing run
+    rows = int(max(1, min(sc.out_grid.length - row0, round(0.25 * args.cpu_seconds / dt1))))
+    run(rows)        # warm-up at the timed size
+    times = []
+    upd = 0
+    for _ in range(3):
+        dt, upd = run(rows)
+        times.append(dt)
+    t_med = float(np.median(times))
+    host = host_cpu()
+    cpu = {"value": upd / t_med, "unit": "pixel-pulse updates/s", "cores": nth,
            "kind": "port",
-           "sample": f"output rows {first}-{row0 - 1} of the 4096 x 4096 grid "
-                     f"({upd_total} updates), oracle reference mode, {t_total:.1f} s"}
-    log(f"cpu baseline: {cpu['value']:.3e} updates/s on {nth} threads")
+           "sample": f"output rows {row0}-{row0 + rows - 1} of the {sc.out_grid.length} x {W} "
+                     f"grid ({upd} updates), oracle reference mode, {nth} OpenMP threads "
+                     f"(OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND', '')}), median of 3 "
+                     f"after a warm-up: {', '.join(f'{t:.2f}' for t in times)} s",
+           "host": host}
+    log(f"cpu baseline: {cpu['value']:.3e} updates/s on {nth} threads ({host})")
     return cpu
 
 
@@ -329,7 +371,7 @@ def run_serve(args, torch, np, dev, tuner):
 
     rows = []
     for rate in args.rates:
-        n = max(20, int(args.serve_seconds * rate))
+        n = max(20, min(args.serve_max_requests, int(args.serve_seconds * rate)))
         row = {"rate_per_s": rate, "requests": n}
         for label, precision, policy in (("split_batched", "split", "batch"),
                                          ("plain_fcfs", "plain", "fcfs")):
@@ -588,9 +630,12 @@ def main():
     ap.add_argument("--only-rslc", action="store_true",
                     help="time the RSLC block finisher only (profiling runs)")
     ap.add_argument("--rates", type=lambda s: [float(x) for x in s.split(",")],
-                    default=[20.0, 50.0, 100.0])
+                    default=[20.0, 50.0, 100.0, 400.0],
+                    help="request rates (the last default one saturates the GPU)")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
-                    help="requests per rate = rate x this (at least 20)")
+                    help="requests per rate = rate x this (at least 20, at most "
+                         "--serve-max-requests)")
+    ap.add_argument("--serve-max-requests", type=int, default=400)
     ap.add_argument("--serve-budget", type=float, default=200e9)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of this many MiB first, as the "

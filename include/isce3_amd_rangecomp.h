@@ -6,19 +6,20 @@
  * RangeComp.cpp:39-131) and its binding isce3.focus.RangeComp
  * (python/extensions/pybind_isce3/focus/RangeComp.cpp:37-101): the output of
  * a pulse is the discrete convolution of the pulse with the time-reversed
- * complex conjugate of the chirp, cropped by the mode.  The convolution is
- * formed in the frequency domain with an FFT of length fft_size, the
- * smallest power of two >= max(1024, chirp_size + input_size - 1) (the
- * reference uses the smallest 2^a 3^b 5^c; both give the same linear
- * convolution).  The compressor runs each pulse as one fused transform of
- * fft_size points (fft_size <= 2^14) or as overlap-save blocks of 2^b points,
+ * complex conjugate of the chirp, cropped by the mode.  The reported
+ * fft_size (bp_rangecomp_sizes) is the reference's FFT length, the smallest
+ * 2^a 3^b 5^c >= chirp_size + input_size - 1 (RangeComp.cpp:60).  The
+ * convolution is formed here with power-of-two transforms: L, the smallest
+ * power of two >= max(1024, chirp_size + input_size - 1), gives the same
+ * linear convolution.  The compressor runs each pulse as one fused transform
+ * of L points (L <= 2^14) or as overlap-save blocks of 2^b points,
  * 10 <= b <= 14, for chirps of at most 2^(b-1) + 1 samples (any pulse
  * length), whichever its cost model of the kernel rates fastest; chirps longer
- * than 2^13 + 1 samples with fft_size above 2^14 run as three four-step
- * passes through a device work buffer of at most 512 MiB that the compressor
- * allocates (fft_size <= 2^19; a run of more pulses than fit goes through it
- * in sub-batches on the stream).  All plans give the same convolution to
- * FP32 rounding.
+ * than 2^13 + 1 samples with L above 2^14 run as three four-step passes
+ * through a device work buffer of at most 512 MiB that the compressor
+ * allocates (L <= 2^19; a run of more pulses than fit goes through it in
+ * sub-batches on the stream).  bp_rangecomp_plan reports the choice and its
+ * transform length.  All plans give the same convolution to FP32 rounding.
  *
  * Return codes as in isce3_amd_backproject.h; bp_last_error_message()
  * returns the message of the calling thread's last failure.

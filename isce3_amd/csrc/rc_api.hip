@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <complex>
 #include <cstring>
 #include <string>
@@ -131,7 +132,8 @@ Plan plan_transform(int chirp_size, int input_size, int mode, int log2l)
 struct bp_rangecomp {
     int device = 0;
     int chirp_size = 0, input_size = 0, max_batch = 0, mode = 0;
-    int fft_size = 0, log2l = 0, out_size = 0, offset = 0;
+    int fft_size = 0;       // the reference's FFT length (2^a 3^b 5^c), reported only
+    int log2l = 0, out_size = 0, offset = 0;
     int method = 0;         // 0 one transform per pulse, 1 overlap-save blocks, 2 four steps
     int nblk = 1;           // transforms per pulse (methods 0 and 1)
     float2* H = nullptr;    // device [L] filter spectrum, L the transform length (four steps: permuted)
@@ -145,6 +147,22 @@ namespace {
 
 // Bytes of the four-step work buffer at most (it holds whole padded pulses).
 constexpr size_t kWorkBytes = size_t(512) << 20;
+
+// Smallest m = 2^a 3^b 5^c >= n (1 for n <= 1): the FFT length the reference's
+// compressor uses (RangeComp.cpp:60, fft/FFTUtil.icc:37-73).  Only reported:
+// the transforms here run at power-of-two lengths.
+int64_t next_fast_power(int64_t n)
+{
+    if (n <= 1) return 1;
+    int64_t best = INT64_MAX;
+    for (int64_t p5 = 1; p5 < 2 * n; p5 *= 5)
+        for (int64_t p35 = p5; p35 < 2 * n; p35 *= 3) {
+            int64_t m = p35;
+            while (m < n) m *= 2;
+            best = std::min(best, m);
+        }
+    return best;
+}
 
 }  // namespace
 
@@ -208,7 +226,7 @@ int bp_rangecomp_create(int device, const float* chirp, int chirp_size, int inpu
     r->input_size = input_size;
     r->max_batch = max_batch;
     r->mode = mode;
-    r->fft_size = 1 << log2l;
+    r->fft_size = (int)next_fast_power(full);
     r->log2l = tlog2;
     r->method = method;
     r->nblk = plan.nblk;

@@ -103,15 +103,31 @@ def test_tsx_troposphere(torch_gpu, precision):
     assert rms(phase_errors(z, ref)) <= tol
 
 
+@pytest.mark.parametrize("squint", ["mild", "strong"])
 @pytest.mark.parametrize("precision", ["plain", "split"])
-def test_nonzero_doppler_luts(torch_gpu, precision):
+def test_nonzero_doppler_luts(torch_gpu, precision, squint):
+    """Doppler LUTs varying in azimuth time and slant range: 11-19 Hz, and
+    500-1100 Hz (squint ~0.3-0.5 deg, targets ~10 km ahead of broadside) over a
+    raster DEM that reaches there.  The oracle's squinted geometry is pinned by
+    the invariants of test_oracle_pins.test_squinted_geometry_invariants."""
+    import dataclasses
+    from isce3_amd import sim
     from isce3_amd.container import RadarGeometry
     from isce3_amd.core import LUT2d
+    from isce3_amd.geometry import DEMInterpolator
     sc, raw = scene_and_raw("tiny")
     g = sc.in_grid
     xs = g.starting_range + np.linspace(-2e3, g.width * g.range_pixel_spacing + 2e3, 6)
     ys = g.sensing_start + np.linspace(-1.0, g.length / g.prf + 1.0, 5)
-    data = 15.0 + 4.0 * np.sin(np.add.outer(ys * 0.7, xs * 1e-4))
+    if squint == "mild":
+        data = 15.0 + 4.0 * np.sin(np.add.outer(ys * 0.7, xs * 1e-4))
+    else:
+        data = 800.0 + 300.0 * np.sin(np.add.outer((ys - ys[0]) * 0.9, (xs - xs[0]) * 2e-4))
+        dem = DEMInterpolator(0.0, method="bilinear")
+        step = sc.dem.delta_x
+        dem.load_dem_array(sim.fractal_dem(1024, 0.8, 200.0, seed=21),
+                           sc.dem.x_start - 200 * step, -0.05, step, step, 4326)
+        sc = dataclasses.replace(sc, dem=dem)
     dop = LUT2d(xs, ys, data)
     ig = RadarGeometry(g, sc.orbit, dop)
     og = RadarGeometry(sc.out_grid, sc.orbit, dop)
@@ -279,7 +295,8 @@ def _dem_variant(sc, method, epsg):
 @pytest.mark.parametrize("method,epsg", [("biquintic", 4326), ("bicubic", 4326),
                                          ("nearest", 4326), ("biquintic", 32631),
                                          ("bilinear", 32631), ("bicubic", 6933),
-                                         ("biquintic", 4978)])
+                                         ("biquintic", 4978), ("bilinear", 3031),
+                                         ("biquintic", 3413)])
 @pytest.mark.parametrize("precision", ["plain", "split"])
 def test_dem_methods_and_projections(torch_gpu, method, epsg, precision):
     """DEM rasters in UTM / EASE-Grid / geocentric coordinates and the

@@ -57,7 +57,7 @@ def test_compressed_chirp_closed_form(torch_gpu):
     chirprate, duration, fs = 100.0, 2.0, 2400.0
     chirp = form_linear_chirp(chirprate, duration, fs)
     rc = RangeComp(chirp, len(chirp))
-    assert rc.fft_size == 16384
+    assert rc.fft_size == 10000 and rc.plan == ("fused", 16384, 1)
     y = np.zeros(rc.output_size, np.complex64)
     rc.rangecompress(y, chirp)
     y = y / fs
@@ -146,8 +146,8 @@ def test_every_transform_plan_matches_fp64_oracle(torch_gpu, m, nin, kind, lengt
     """The compressor's cost model picks one fused transform per pulse,
     overlap-save blocks of 2^10 .. 2^14 points, or four steps; each plan (the
     block seams included) gives the FP64 oracle's "same" convolution, and the
-    reported fft_size stays the smallest power of two that holds the full
-    convolution."""
+    reported fft_size is the reference's length, the smallest 2^a 3^b 5^c
+    that holds the full convolution (RangeComp.cpp:60)."""
     from isce3_amd.focus import RangeComp
     rng = np.random.default_rng(m + nin)
     chirp = np.exp(1j * np.pi * 3e-5 * (np.arange(m) - m / 2) ** 2).astype(np.complex64)
@@ -155,7 +155,7 @@ def test_every_transform_plan_matches_fp64_oracle(torch_gpu, m, nin, kind, lengt
     rc = RangeComp(chirp, nin, maxbatch=3, mode=RangeComp.Mode.Same)
     print(f"[rangecomp plan] m={m} nin={nin} plan={rc.plan}", flush=True)
     assert rc.plan == (kind, length, nblk)
-    assert rc.fft_size == 1 << int(np.ceil(np.log2(max(1024, m + nin - 1))))
+    assert rc.fft_size == rco.next_fast_power(m + nin - 1)
     y = np.zeros((3, rc.output_size), np.complex64)
     rc.rangecompress(y, x)
     ref = rco.rangecompress(chirp, x, rco.SAME, "fp64")
@@ -192,7 +192,8 @@ def test_long_pulses_more_than_one_work_batch(torch_gpu):
     nin, batch, m = 300000, 130, 10001
     chirp = np.exp(1j * np.pi * 1e-5 * (np.arange(m) - m / 2) ** 2).astype(np.complex64)
     rc = RangeComp(chirp, nin, maxbatch=batch, mode=RangeComp.Mode.Same)
-    assert rc.fft_size == 1 << 19
+    assert rc.fft_size == rco.next_fast_power(nin + m - 1)
+    assert rc.plan == ("four-step", 1 << 19, 1)
     g = torch.Generator(device="cuda:0").manual_seed(3)
     d_in = torch.randn((batch, nin), dtype=torch.complex64, device="cuda:0", generator=g)
     d_out = torch.empty((batch, rc.output_size), dtype=torch.complex64, device="cuda:0")

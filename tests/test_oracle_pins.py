@@ -321,3 +321,92 @@ def test_lut2d_bilinear_clamped():
     assert ev(-3.0, -50.0) == 1.0
     zero = oracle.Lut_t(0, 1, 0.0, 0, 0, 1, 1, 0, 0, None)
     assert oracle.lib().orc_lut2d_eval(C.byref(zero), 3.0, 4.0) == 0.0
+
+
+# ------------------------------------------------------------ squinted geometry
+def _squint_lut(sc):
+    """A Doppler centroid of 500-1100 Hz (squint ~0.3-0.5 deg) varying along both
+    azimuth time and slant range, as a bilinear LUT over the scene."""
+    from isce3_amd.core import LUT2d
+    g = sc.in_grid
+    xs = g.starting_range + np.linspace(-2e3, g.width * g.range_pixel_spacing + 2e3, 6)
+    ys = g.sensing_start + np.linspace(-1.0, g.length / g.prf + 1.0, 5)
+    data = 800.0 + 300.0 * np.sin(np.add.outer((ys - ys[0]) * 0.9, (xs - xs[0]) * 2e-4))
+    return LUT2d(xs, ys, data)
+
+
+def test_squinted_geometry_invariants():
+    """rdr2geo_bracket at a non-zero Doppler (Rdr2Geo.icc:200-211) and
+    geo2rdr_bracket against the same Doppler LUT (Geo2Rdr.icc:185-238), on a
+    fractal raster DEM.  At the rdr2geo solution x for (t, r, fD(t, r)):
+    |x - p(t)| = r; the Doppler of x seen from p(t), 2/lambda v.(x-p)/|x-p|,
+    equals fD(t, r) (a sign error in the squint's along-track offset gives -fD);
+    the ellipsoidal height of x equals the DEM at (lon, lat); x lies on the
+    right of the track; and geo2rdr(x) with the LUT returns (t, r).  The host
+    library's solvers return the same points."""
+    import dataclasses
+    from isce3_amd import geometry, sim
+    sc = sim.make_scene("tiny")
+    # a 1024^2 fractal raster (30 m posting) reaching ~10 km north of the
+    # broadside footprint, where the squinted targets lie
+    dem = geometry.DEMInterpolator(0.0, method="bilinear")
+    step = sc.dem.delta_x
+    dem.load_dem_array(sim.fractal_dem(1024, 0.8, 200.0, seed=21), sc.dem.x_start - 200 * step,
+                       -0.05, step, step, 4326)
+    sc = dataclasses.replace(sc, dem=dem)
+    lut = _squint_lut(sc)
+    lam = sc.out_grid.wavelength
+    o = oracle.orbit_struct(sc.orbit)
+    d = oracle.dem_struct(sc.dem)
+    ls = oracle.lut_struct(lut)
+    r2g = oracle.R2G_t(1e-5, 0.0, math.pi / 2)
+    g2r = oracle.G2R_t(1e-7, 0, 0, 0.0, 0.0)
+    lib = oracle.lib()
+    g = sc.in_grid
+    heights = []
+    for t in g.sensing_start + np.array([0.02, 0.08, 0.15]):
+        for r in g.starting_range + g.range_pixel_spacing * np.array([20.0, 250.5, 480.0]):
+            fd = lib.orc_lut2d_eval(C.byref(ls), t, r)
+            assert 400.0 < fd < 1200.0
+            x = D3()
+            assert lib.orc_rdr2geo_bracket(t, r, fd, C.byref(o), C.byref(d), lam, -1,
+                                           C.byref(r2g), x) == 0
+            x = np.array(x)
+            p, v = D3(), D3()
+            assert lib.orc_orbit_interpolate(C.byref(o), t, 0, p, v) == 0
+            p, v = np.array(p), np.array(v)
+            los = x - p
+            rng = np.linalg.norm(los)
+            assert abs(rng - r) < 1e-6
+            assert abs(2.0 / lam * np.dot(v, los) / rng - fd) < 1e-6
+            # target ahead of the antenna by r sin(squint) along the track
+            assert abs(np.dot(los, v / np.linalg.norm(v)) - r * fd * lam
+                       / (2 * np.linalg.norm(v))) < 1e-6
+            llh = D3()
+            lib.orc_xyz_to_lonlat(6378137.0, 0.006694379990141317, _d3(x), llh)
+            h_dem = lib.orc_dem_interpolate_lonlat(C.byref(d), llh[0], llh[1])
+            assert abs(llh[2] - h_dem) < 1e-3
+            heights.append(h_dem)
+            # right looking: (x - p) x v . p > 0 (the look-side test of Geo2Rdr.icc:232-235)
+            assert np.dot(np.cross(los, v), p) > 0
+            ta, ra = C.c_double(), C.c_double()
+            assert lib.orc_geo2rdr_bracket(_d3(x), C.byref(o), C.byref(ls), lam, -1,
+                                           C.byref(g2r), C.byref(ta), C.byref(ra)) == 0
+            assert abs(ta.value - t) < 1e-6 and abs(ra.value - r) < 1e-4
+            # the product's host solvers agree with the oracle
+            xh = geometry.rdr2geo_bracket(t, r, sc.orbit, "right", fd, lam, sc.dem)
+            np.testing.assert_allclose(xh, x, rtol=0, atol=1e-6)
+            th, rh = geometry.geo2rdr_bracket(x, sc.orbit, lut, lam, "right")
+            assert abs(th - ta.value) < 1e-9 and abs(rh - ra.value) < 1e-6
+    # the DEM is in play (not a constant reference height)
+    assert np.std(heights) > 1.0
+    # zero Doppler puts the target broadside: a different point, |dx| ~ r sin(squint)
+    x0 = D3()
+    t, r = g.sensing_start + 0.08, g.starting_range + 250.5 * g.range_pixel_spacing
+    assert lib.orc_rdr2geo_bracket(t, r, 0.0, C.byref(o), C.byref(d), lam, -1, C.byref(r2g),
+                                   x0) == 0
+    fd = lib.orc_lut2d_eval(C.byref(ls), t, r)
+    xs_ = D3()
+    lib.orc_rdr2geo_bracket(t, r, fd, C.byref(o), C.byref(d), lam, -1, C.byref(r2g), xs_)
+    shift = np.linalg.norm(np.array(xs_) - np.array(x0))
+    assert 0.5 * r * fd * lam / (2 * 7500.0) < shift < 2.0 * r * fd * lam / (2 * 7500.0)
