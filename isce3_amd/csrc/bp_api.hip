@@ -318,12 +318,15 @@ void build_polyphase(const bp_kernel& k, int width, std::vector<float>& out)
 // fractional offsets t0 = tmin + q/Q (tmin = 0 for even W, -1/2 for odd W, as
 // the one-line table), each row pair_row_floats(W) floats, in two copies:
 // copy 0 holds the W weights in entries 0..W-1 and a zero in entry W, copy 1
-// a zero in entry 0 and the weights in entries 1..W.  Rows are read at the
-// nearest offset, so there are no differences to the next row.
+// a zero in entry 0 and the weights in entries 1..W.  Entries interpolated in
+// the sum (pair_lerp_lo .. + pair_lerp_n) hold the weight at t0 and, after the
+// W + 1 entries, its difference to the weight at t0 + 1/Q; the others hold the
+// weight at t0 + 1/(2Q), the middle of the row's interval.
 void build_pair_table(const bp_kernel& k, int W, std::vector<float>& out)
 {
     const int Q = 1 << bp::pair_q_log2(W);
     const int RF = bp::pair_row_floats(W);
+    const int lo = bp::pair_lerp_lo(W), nl = bp::pair_lerp_n(W);
     const double tmin = (W % 2 == 0) ? 0.0 : -0.5;
     const double hw = k.halfwidth;
     const double dx = hw / (k.n - 1.0);
@@ -336,12 +339,22 @@ void build_pair_table(const bp_kernel& k, int W, std::vector<float>& out)
         return (double)k.table[i] + (axn - i) * (double)(k.table[i + 1] - k.table[i]);
     };
     out.assign((size_t)bp::pair_table_floats(W), 0.f);
-    for (int q = 0; q < Q; ++q) {
-        const double t0 = tmin + (double)q / Q;
-        for (int m = 0; m < W; ++m) {
-            const float w = (float)weight((m - W / 2) + t0);
-            out[((size_t)0 * Q + q) * RF + m] = w;
-            out[((size_t)1 * Q + q) * RF + m + 1] = w;
+    for (int o = 0; o < 2; ++o) {
+        for (int q = 0; q < Q; ++q) {
+            float* r = &out[((size_t)o * Q + q) * RF];
+            const double t0 = tmin + (double)q / Q;
+            for (int e = 0; e <= W; ++e) {
+                const int m = e - o;  // tap of entry e in this copy
+                if (m < 0 || m >= W) continue;
+                const double x = (m - W / 2) + t0;
+                if (e >= lo && e < lo + nl) {
+                    const float w0 = (float)weight(x), w1 = (float)weight(x + 1.0 / Q);
+                    r[e] = w0;
+                    r[W + 1 + (e - lo)] = w1 - w0;
+                } else {
+                    r[e] = (float)weight(x + 0.5 / Q);
+                }
+            }
         }
     }
 }
@@ -557,14 +570,16 @@ int check_options(const bp_options* options, Opts* o)
     return BP_SUCCESS;
 }
 
-// Two lines per lane need the split kernel of width 8 with gathered taps;
-// lane_rows 0 takes them where offered.
+// Two lines per lane need the split kernel with gathered taps; lane_rows 0
+// takes them where offered for kernel widths up to 9 (wider two-line kernels
+// exceed the 128 registers of four waves per SIMD and spill, so one line per
+// lane is their default).
 int check_lane_rows(Opts& o, int precision, int width)
 {
     const bool offered = precision == BP_PRECISION_SPLIT && !o.staged;
     if (o.staged && (precision != BP_PRECISION_SPLIT || (width != 8 && width != 9)))
         return fail(BP_ERR_INVALID_ARGUMENT, "staged 1 needs split precision and kernel width 8 or 9");
-    if (o.lane_rows == 0) o.lane_rows = offered ? 2 : 1;
+    if (o.lane_rows == 0) o.lane_rows = offered && width <= 9 ? 2 : 1;
     if (o.lane_rows == 2 && !offered)
         return fail(BP_ERR_INVALID_ARGUMENT,
                     "lane_rows 2 needs split precision and staged 0");

@@ -83,13 +83,26 @@ constexpr int kMaxKernelWidth = 16;
 
 // Two-line weight table of kernel width W (split sum with lane_rows = 2):
 // Q = 2^pair_q_log2(W) rows at fractional offsets tmin + q/Q, each holding
-// the W + 1 entries of a (W + 1)-bin window (padded to whole float4s), in two
-// copies -- the W weights followed by a zero (copy 0) or preceded by one
-// (copy 1).  A pixel reads the row nearest to its offset (no interpolation
-// between rows).  Tables stay within 64 KB of LDS, so that two 512-thread
-// workgroups fit one CU.
-__host__ __device__ constexpr int pair_row_floats(int w) { return (w + 1 + 3) / 4 * 4; }
-__host__ __device__ constexpr int pair_q_log2(int w) { return pair_row_floats(w) <= 16 ? 9 : 8; }
+// the W + 1 entries of a (W + 1)-bin window followed by the differences to the
+// next row of entries [pair_lerp_lo(W), pair_lerp_lo(W) + pair_lerp_n(W)),
+// padded to whole float4s, in two copies -- the W weights followed by a zero
+// (copy 0) or preceded by one (copy 1).  A pixel reads row floor(Q t) and
+// interpolates the entries that have differences linearly between the rows;
+// entries without differences hold the weight at the middle of the row's
+// interval.  Q is the largest power of two up to 512 that keeps the table
+// within 64 KB of LDS, so that two 512-thread workgroups fit one CU.
+__host__ __device__ constexpr int pair_lerp_n(int w) { return w + 1; }
+__host__ __device__ constexpr int pair_lerp_lo(int w) { return (w + 1 - pair_lerp_n(w)) / 2; }
+__host__ __device__ constexpr int pair_row_floats(int w)
+{
+    return (w + 1 + pair_lerp_n(w) + 3) / 4 * 4;
+}
+__host__ __device__ constexpr int pair_q_log2(int w)
+{
+    int q = 9;
+    while (q > 4 && 2 * (1 << q) * pair_row_floats(w) * 4 > 65536) --q;
+    return q;
+}
 __host__ __device__ constexpr int pair_table_floats(int w)
 {
     return 2 * (1 << pair_q_log2(w)) * pair_row_floats(w);

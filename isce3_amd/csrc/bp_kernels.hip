@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
 // over the sum wavefront's union window [kw0, kw1) with a degree-4 Chebyshev
 // interpolant evaluated in FP64, re-expanded as forward differences at kw0.
 __device__ inline PixSplit fit_pixel(const GeomArgs& a, const PixGeom& g, bool has_win, int kw0,
-                                     int kw1, double q_rows, double z_round)
+                                     int kw1, double q_rows)
 {
     const V3 x{g.x[0], g.x[1], g.x[2]};
     PixSplit sp;
@@ -377,9 +377,7 @@ __device__ inline PixSplit fit_pixel(const GeomArgs& a, const PixGeom& g, bool h
         // koff whole bins of offset keep z >= 0 over the window, so that the
         // sum kernel takes floor(z) and fract(z) by truncation and v_fract
         const double koff = ceil(bnd) + 2.0;
-        // z_round: 1/2 row for tables read at the nearest row (truncation of
-        // z + 1/2 rounds), 0 for tables interpolated between rows
-        sp.zoff = (float)(-q_rows * (f_ref - c_ref * inv_k + shift - koff) + z_round);
+        sp.zoff = (float)(-q_rows * (f_ref - c_ref * inv_k + shift - koff));
         sp.cbase = (int32_t)n_ref - a.kwidth / 2 - (odd ? 1 : 0) + (int32_t)koff;
         const double c_lo = n_ref + ceil(f_ref - bnd + shift) - (a.kwidth / 2) - (odd ? 1 : 0) - 1;
         const double c_hi = n_ref + ceil(f_ref + bnd + shift) - (a.kwidth / 2) - (odd ? 1 : 0) + 1;
@@ -411,7 +409,6 @@ __device__ inline void fit_tile(const GeomArgs& a, const OutDesc* grids, int n_g
     const int kw1 = wave_max_i(ke);
     // polyphase rows of the weight table the sum reads for this tiling
     const double q_rows = lr == 2 ? (double)(1 << pair_q_log2(a.kwidth)) : (double)kPolyphaseRows;
-    const double z_round = lr == 2 ? 0.5 : 0.0;
 #pragma unroll 1
     for (int j = 0; j < lr; ++j) {
         int64_t row = 0, col = 0;
@@ -419,7 +416,7 @@ __device__ inline void fit_tile(const GeomArgs& a, const OutDesc* grids, int n_g
             const int64_t rec = d.rec_base + row * d.map.width + col;
             const PixGeom g = a.geom[rec];
             const bool has_win = g.status == kPixOk && g.kstop > g.kstart;
-            a.split[rec] = fit_pixel(a, g, has_win, kw0, kw1, q_rows, z_round);
+            a.split[rec] = fit_pixel(a, g, has_win, kw0, kw1, q_rows);
         }
     }
 }
@@ -987,13 +984,14 @@ __device__ inline void split_tile(const SumArgs& a, const float* s_poly, char* s
 // preceded (copy 1) by a zero, so that each pixel picks its alignment inside
 // the shared bins by its row offset alone.  This halves the gather traffic
 // per update.  Where the first taps of some lane's pixels are further apart,
-// that pulse is gathered per pixel.  Each pixel reads the table row nearest
-// to its fractional offset (the fit's coordinate offset holds the half row
-// that turns truncation into rounding): with 2^9 rows per bin (2^8 for the
-// widest kernels) the offset error stays below 1/1024 (1/512) bin and no
-// interpolation between rows is needed.
+// that pulse is gathered per pixel.  Each pixel reads table row floor(Q t)
+// of its fractional offset t, so the window of taps is the reference's, and
+// interpolates the weights linearly to t (reading the nearest row instead
+// leaves an offset error of up to 1/(2Q) bin: about 8e-4 relative error per
+// interpolated sample at Q = 512, against 4e-6 interpolated at Q = 256).
 struct PairPulse {
     float ph;   // phase, revolutions in [0, 1)
+    float qf;   // fraction between weight rows
     int row;    // byte offset of the weight row (copy 0)
     int c0;     // first tap's range bin
 };
@@ -1001,6 +999,8 @@ struct PairPulse {
 template<int W>
 struct PairTable {
     static constexpr int kNE = W + 1;                  // entries per row
+    static constexpr int kLo = pair_lerp_lo(W);        // first interpolated entry
+    static constexpr int kNL = pair_lerp_n(W);         // interpolated entries
     static constexpr int kRF = pair_row_floats(W);     // floats per row
     static constexpr int kQL = pair_q_log2(W);
     static constexpr int kRowBytes = 4 * kRF;
@@ -1016,28 +1016,34 @@ __device__ inline PairPulse pair_pulse(const PhaseState& st, float zscale, float
     r.ph = (float)__builtin_amdgcn_fract(st.p);
     const float z = fmaf((float)st.p, zscale, zoff);  // >= 0 over the window
     const uint32_t m = (uint32_t)z;
+    r.qf = __builtin_amdgcn_fractf(z);
     r.row = (int)(m & ((1u << T::kQL) - 1)) * T::kRowBytes;
     r.c0 = cbase - (int)(m >> T::kQL);
     return r;
 }
 
-// Interpolate the W + 1 bins `d` with the weight row at byte offset `row`,
+// Interpolate the W + 1 bins `d` with the weight row at byte offset `row`
+// (its interpolated entries moved the fraction qf towards the next row),
 // rotate by the phase and accumulate.
 template<int W>
-__device__ inline void pair_update(const float* s_pair, int row, float ph,
+__device__ inline void pair_update(const float* s_pair, int row, float qf, float ph,
                                    const float2 (&d)[W + 1], float& cre, float& cim)
 {
     using T = PairTable<W>;
     const float4* r4 = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_pair) + row);
-    float w[T::kRF];
+    float v[T::kRF];
 #pragma unroll
     for (int i = 0; i < T::kRF / 4; ++i) {
-        const float4 v = r4[i];
-        w[4 * i + 0] = v.x;
-        w[4 * i + 1] = v.y;
-        w[4 * i + 2] = v.z;
-        w[4 * i + 3] = v.w;
+        const float4 x = r4[i];
+        v[4 * i + 0] = x.x;
+        v[4 * i + 1] = x.y;
+        v[4 * i + 2] = x.z;
+        v[4 * i + 3] = x.w;
     }
+    float w[T::kNE];
+#pragma unroll
+    for (int e = 0; e < T::kNE; ++e)
+        w[e] = (e >= T::kLo && e < T::kLo + T::kNL) ? fmaf(qf, v[T::kNE + e - T::kLo], v[e]) : v[e];
     float sre = 0.f, sim = 0.f;
 #pragma unroll
     for (int m = 0; m < T::kNE; ++m) {
@@ -1093,12 +1099,12 @@ __device__ inline void pair_masked(const SumArgs& a, const float* s_pair, PairLi
         if (k >= A.ks && k < A.ke) {
             float2 d[W + 1];
             pair_taps_one<W, kGuard>(line, pa.c0, nr, d);
-            pair_update<W>(s_pair, pa.row, pa.ph, d, A.s.cre, A.s.cim);
+            pair_update<W>(s_pair, pa.row, pa.qf, pa.ph, d, A.s.cre, A.s.cim);
         }
         if (k >= B.ks && k < B.ke) {
             float2 d[W + 1];
             pair_taps_one<W, kGuard>(line, pb.c0, nr, d);
-            pair_update<W>(s_pair, pb.row, pb.ph, d, B.s.cre, B.s.cim);
+            pair_update<W>(s_pair, pb.row, pb.qf, pb.ph, d, B.s.cre, B.s.cim);
         }
         if (++A.s.count >= a.chunk) {
             A.s.fold();
@@ -1129,16 +1135,16 @@ __device__ inline void pair_core(const SumArgs& a, const float* s_pair, PairLine
         const uint64_t bad = (__builtin_amdgcn_ballot_w64(oa > 1u) & act_a) |
                              (__builtin_amdgcn_ballot_w64(ob > 1u) & act_b);
         if (bad == 0) {
-            pair_update<W>(s_pair, pa.row + (int)min(oa, 1u) * T::kCopyBytes, pa.ph, d, A.s.cre,
-                           A.s.cim);
-            pair_update<W>(s_pair, pb.row + (int)min(ob, 1u) * T::kCopyBytes, pb.ph, d, B.s.cre,
-                           B.s.cim);
+            pair_update<W>(s_pair, pa.row + (int)min(oa, 1u) * T::kCopyBytes, pa.qf, pa.ph, d,
+                           A.s.cre, A.s.cim);
+            pair_update<W>(s_pair, pb.row + (int)min(ob, 1u) * T::kCopyBytes, pb.qf, pb.ph, d,
+                           B.s.cre, B.s.cim);
         } else {
             float2 e[W + 1];
             pair_taps_one<W, false>(line, pa.c0, nr, e);
-            pair_update<W>(s_pair, pa.row, pa.ph, e, A.s.cre, A.s.cim);
+            pair_update<W>(s_pair, pa.row, pa.qf, pa.ph, e, A.s.cre, A.s.cim);
             pair_taps_one<W, false>(line, pb.c0, nr, e);
-            pair_update<W>(s_pair, pb.row, pb.ph, e, B.s.cre, B.s.cim);
+            pair_update<W>(s_pair, pb.row, pb.qf, pb.ph, e, B.s.cre, B.s.cim);
         }
         if (++A.s.count >= a.chunk) {
             A.s.fold();

@@ -49,8 +49,8 @@ def from_host(sc, host, grid, precision, **kw):
     h = np.zeros(grid.shape, np.float32)
     st = {}
     failed = backproject(out, sc.out_geometry(grid), host, sc.in_geometry(), sc.dem, sc.fc,
-                         sc.ds, sc.kernel, sc.dry_tropo_model, kw.pop("batch", 1024), h,
-                         precision=precision, stats=st, **kw)
+                         sc.ds, sc.kernel, sc.dry_tropo_model, None, None,
+                         kw.pop("batch", 1024), h, precision=precision, stats=st, **kw)
     return out, h, failed, st
 
 

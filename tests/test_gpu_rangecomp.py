@@ -52,12 +52,15 @@ def test_unit_chirp_is_identity(torch_gpu):
 
 def test_compressed_chirp_closed_form(torch_gpu):
     """A replica of the chirp compresses to (T - |t|) sinc(K t (T - |t|))
-    (rangecomp.cpp:84-129; 9601-sample convolution, 2^14-point FFTs)."""
+    (rangecomp.cpp:84-129; 9601-sample convolution, 2^14-point FFTs).  The
+    reported fft_size is the reference's length, the smallest 2^a 3^b 5^c at or
+    above 9601: 2^3 3^5 5 = 9720."""
     from isce3_amd.focus import RangeComp, form_linear_chirp
     chirprate, duration, fs = 100.0, 2.0, 2400.0
     chirp = form_linear_chirp(chirprate, duration, fs)
     rc = RangeComp(chirp, len(chirp))
-    assert rc.fft_size == 10000 and rc.plan == ("fused", 16384, 1)
+    assert 2 * len(chirp) - 1 == 9601
+    assert rc.fft_size == 9720 and rc.plan == ("fused", 16384, 1)
     y = np.zeros(rc.output_size, np.complex64)
     rc.rangecompress(y, chirp)
     y = y / fs

@@ -28,13 +28,42 @@ def declared_functions(header):
 @pytest.mark.parametrize("header,lib", [("isce3_amd_backproject.h", "libisce3_amd.so"),
                                         ("isce3_amd_rangecomp.h", "libisce3_amd.so"),
                                         ("isce3_amd_rslc.h", "libisce3_amd.so"),
+                                        ("isce3_amd_presum.h", "libisce3_amd.so"),
                                         ("isce3_amd_sim.h", "libisce3_amd_sim.so")])
 def test_library_exports_every_declared_symbol(header, lib):
     names = declared_functions(header)
-    assert len(names) >= {"isce3_amd_sim.h": 2, "isce3_amd_rangecomp.h": 4, "isce3_amd_rslc.h": 3}.get(header, 15)
+    assert len(names) >= {"isce3_amd_sim.h": 2, "isce3_amd_rangecomp.h": 4, "isce3_amd_rslc.h": 3,
+                          "isce3_amd_presum.h": 5}.get(header, 15)
     h = C.CDLL(os.path.join(ROOT, "isce3_amd", "lib", lib))
     missing = [n for n in names if not hasattr(h, n)]
     assert not missing, missing
+
+
+def declared_arities(header):
+    src = open(os.path.join(ROOT, "include", header)).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"//[^\n]*", "", src)
+    out = {}
+    for m in re.finditer(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(bp_\w+)\s*\(([^;{]*?)\)\s*;",
+                         src, flags=re.M):
+        params = m.group(2).strip()
+        out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+@pytest.mark.parametrize("header", ["isce3_amd_backproject.h", "isce3_amd_rangecomp.h",
+                                    "isce3_amd_rslc.h", "isce3_amd_presum.h"])
+def test_every_declared_function_has_a_binding_of_its_arity(header):
+    """ctypes converts Python floats and ints only for functions whose argument
+    types are declared: every function of the headers has a binding in the
+    package, with as many arguments as the declaration."""
+    arity = declared_arities(header)
+    assert set(arity) == set(declared_functions(header))
+    unbound = [n for n in arity if n not in _native._SIGNATURES]
+    assert not unbound, unbound
+    wrong = {n: (a, len(_native._SIGNATURES[n][1])) for n, a in arity.items()
+             if len(_native._SIGNATURES[n][1]) != a}
+    assert not wrong, wrong
 
 
 def test_context_create_without_gpu_fails_cleanly():
