@@ -203,6 +203,37 @@ int bp_backproject(bp_context* ctx, const bp_radar_geometry* out_geometry,
                    const bp_options* options, void* stream, int* any_failed,
                    bp_stats* stats);
 
+/*
+ * bp_backproject without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during a
+ * call -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.
+ *
+ * Same arguments, checks, kernels and results as bp_backproject.  Every
+ * device buffer of the call (orbit, Doppler, DEM and kernel tables, the
+ * per-pulse states, the per-pixel records, the flags and the descriptor)
+ * is placed in one fixed scratch array of BP_NOALLOC_SCRATCH_MIB MiB per
+ * device, part of the library and held until the process ends; its device
+ * address is looked up by bp_context_create.  A call whose buffers do not fit
+ * (the records take bp_scratch_bytes_per_pixel(precision) bytes per output
+ * pixel: 48 plain, 112 split, so about 2.7 M plain or 1.1 M split pixels with
+ * small tables) fails with BP_ERR_LENGTH before any launch.  The call makes
+ * no hipMalloc, hipFree, pinned or library allocation on the device; its
+ * table uploads are pageable host-to-device copies on `stream`.
+ *
+ * All calls on one GPU share that GPU's array, and each call's launches read
+ * what the call itself wrote there: calls that overlap in time on one GPU
+ * (other streams, other contexts, other threads) race and must not be issued.
+ */
+#define BP_NOALLOC_SCRATCH_MIB 128
+int bp_backproject_noalloc(bp_context* ctx, const bp_radar_geometry* out_geometry,
+                           const bp_radar_geometry* in_geometry, const bp_dem* dem, double fc,
+                           double ds, const bp_kernel* kernel, int dry_tropo_model,
+                           const bp_rdr2geo_params* rdr2geo, const bp_geo2rdr_params* geo2rdr,
+                           const void* in_dev, void* out_dev, float* height_dev, int precision,
+                           const bp_options* options, void* stream, int* any_failed,
+                           bp_stats* stats);
+
 /* What bp_backproject_host moved (filled when non-null). */
 typedef struct {
     int64_t h2d_bytes;      /* pulse bytes copied host to device */

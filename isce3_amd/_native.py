@@ -121,6 +121,11 @@ _SIGNATURES = {
                                  _P(Rdr2GeoParams_t), _P(Geo2RdrParams_t), C.c_void_p,
                                  C.c_void_p, C.c_void_p, C.c_int, _P(Options_t), C.c_void_p,
                                  _P(C.c_int), _P(Stats_t)]),
+    "bp_backproject_noalloc": (C.c_int, [C.c_void_p, _P(RadarGeometry_t), _P(RadarGeometry_t),
+                                         _P(Dem_t), C.c_double, C.c_double, _P(Kernel_t), C.c_int,
+                                         _P(Rdr2GeoParams_t), _P(Geo2RdrParams_t), C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_int, _P(Options_t),
+                                         C.c_void_p, _P(C.c_int), _P(Stats_t)]),
     "bp_backproject_host": (C.c_int, [C.c_void_p, _P(RadarGeometry_t), _P(RadarGeometry_t),
                                       _P(Dem_t), C.c_double, C.c_double, _P(Kernel_t), C.c_int,
                                       _P(Rdr2GeoParams_t), _P(Geo2RdrParams_t), C.c_void_p,

@@ -100,6 +100,8 @@ struct HostPath {
 struct bp_context {
     int device = 0;
     int n_cu = 0;
+    unsigned char* scratch = nullptr;  // device address of this device's no-alloc scratch
+    std::string scratch_err;           // why it is null
     StaticData st;
     Scratch sc;
     HostPath hp;
@@ -377,6 +379,39 @@ void build_plain_table(const bp_kernel& k, std::vector<double>& out)
     out[2 * (n - 1) + 1] = d;
 }
 
+// The scratch region of bp_backproject_noalloc: one array per device, part of
+// the library's code object, held until the process ends.  Host code never
+// passes `scratch` itself (in host code the name is a stand-in, not a device
+// address): bp_context_create looks up each device's address once, with that
+// device current, and every use goes through that address.
+constexpr size_t kNoAllocScratchBytes = size_t(BP_NOALLOC_SCRATCH_MIB) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device `device`.
+int scratch_address(int device, unsigned char** addr)
+{
+    *addr = nullptr;
+    if (device < 0 || device >= kMaxDevices)
+        return fail(BP_ERR_INVALID_ARGUMENT, "device index beyond the library's device table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr[device] == nullptr) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            (void)hipGetLastError();
+            return fail(BP_ERR_HIP, std::string("hipGetSymbolAddress of the no-alloc scratch failed: ") +
+                                        hipGetErrorString(e));
+        }
+        g_scratch_addr[device] = static_cast<unsigned char*>(p);
+    }
+    *addr = g_scratch_addr[device];
+    return BP_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -405,9 +440,17 @@ int bp_context_create(int device, bp_context** ctx)
         c->n_cu = 256;
     for (auto& ev : c->ev) {
         if (hipEventCreate(&ev) != hipSuccess) {
+            for (auto& e2 : c->ev)
+                if (e2) (void)hipEventDestroy(e2);
             delete c;
             return fail(BP_ERR_HIP, "hipEventCreate failed");
         }
+    }
+    // a failed lookup disables bp_backproject_noalloc only
+    if (scratch_address(device, &c->scratch) != BP_SUCCESS) {
+        c->scratch = nullptr;
+        c->scratch_err = g_msg;
+        g_msg.clear();
     }
     *ctx = c;
     return BP_SUCCESS;
@@ -853,8 +896,15 @@ int finish_step(const int32_t* hflags, int n_grids, const std::vector<int>& bloc
 // output block shares the output orbit and Doppler, the input geometry, DEM,
 // kernel and parameters; the pixels of all blocks are focused by one geometry
 // launch and one coherent-sum launch.
+// `carve`, when given, places the call's buffers in a fixed region instead of
+// the context's grow-only arena (bp_backproject_noalloc).
+struct Carve;
+int carve_call(Carve& cv, const CallArgs& a, int width, int n_blocks, int64_t npix,
+               StaticData& st, Scratch& sc);
+
 int run_blocks(bp_context* c, const CallArgs& a, bp_out_block* blocks, int n_blocks,
-               const bp_options* options, void* stream, int* any_failed, bp_stats* stats)
+               const bp_options* options, void* stream, int* any_failed, bp_stats* stats,
+               Carve* carve = nullptr)
 {
     if (any_failed) *any_failed = 0;
     if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -869,30 +919,36 @@ int run_blocks(bp_context* c, const CallArgs& a, bp_out_block* blocks, int n_blo
     const bp_grid& gi = a.in_geometry->grid;
     if (npix > 0 && gi.length * gi.width > 0 && a.in_dev == nullptr)
         return fail(BP_ERR_INVALID_ARGUMENT, "in is null");
+    StaticData fixed_st;
+    Scratch fixed_sc;
+    if (carve && (rc = carve_call(*carve, a, width, n_blocks, npix, fixed_st, fixed_sc)) != BP_SUCCESS)
+        return rc;
     if (npix == 0) return BP_SUCCESS;
+    StaticData& st = carve ? fixed_st : c->st;
+    Scratch& sc = carve ? fixed_sc : c->sc;
 
     BP_HIP_CHECK(hipSetDevice(c->device));
     hipStream_t s = static_cast<hipStream_t>(stream);
 
     const int n_flag = kFlagWords + n_blocks;
-    if ((rc = ensure(c->sc.flags, sizeof(int32_t) * n_flag))) return rc;
+    if ((rc = ensure(sc.flags, sizeof(int32_t) * n_flag))) return rc;
     c->desc_host.assign(sizeof(bp::BatchHeader) + sizeof(bp::OutDesc) * std::max(n_blocks, 1), 0);
     bp::BatchHeader hdr;
-    if ((rc = plan_batch(blocks, n_blocks, o, static_cast<int32_t*>(c->sc.flags.ptr) + kFlagWords,
+    if ((rc = plan_batch(blocks, n_blocks, o, static_cast<int32_t*>(sc.flags.ptr) + kFlagWords,
                          c->desc_host.data(), n_blocks, &hdr, c->desc_block)) != BP_SUCCESS)
         return rc;
     const int64_t n_rec = npix;
 
-    if ((rc = upload_static(c->st, a, width, s))) return rc;
-    if ((rc = upload(c->sc.desc, c->desc_host.data(), c->desc_host.size(), s))) return rc;
-    if ((rc = ensure(c->sc.geom, sizeof(bp::PixGeom) * n_rec))) return rc;
-    if (a.precision == BP_PRECISION_SPLIT && (rc = ensure(c->sc.split, sizeof(bp::PixSplit) * n_rec)))
+    if ((rc = upload_static(st, a, width, s))) return rc;
+    if ((rc = upload(sc.desc, c->desc_host.data(), c->desc_host.size(), s))) return rc;
+    if ((rc = ensure(sc.geom, sizeof(bp::PixGeom) * n_rec))) return rc;
+    if (a.precision == BP_PRECISION_SPLIT && (rc = ensure(sc.split, sizeof(bp::PixSplit) * n_rec)))
         return rc;
-    BP_HIP_CHECK(bp::launch_clear(static_cast<int32_t*>(c->sc.flags.ptr), n_flag, s));
-    if ((rc = pulse_states(c->st, a, static_cast<int32_t*>(c->sc.flags.ptr), s))) return rc;
+    BP_HIP_CHECK(bp::launch_clear(static_cast<int32_t*>(sc.flags.ptr), n_flag, s));
+    if ((rc = pulse_states(st, a, static_cast<int32_t*>(sc.flags.ptr), s))) return rc;
 
     Step sp;
-    build_step(sp, c->st, c->sc, a, width, o, c->n_cu);
+    build_step(sp, st, sc, a, width, o, c->n_cu);
     sp.geom_grid = (int)hdr.n_geom_groups;
     sp.fit_grid = (int)hdr.n_groups;
     sp.flag_words = 0;  // cleared above, before the pulse states
@@ -901,11 +957,81 @@ int run_blocks(bp_context* c, const CallArgs& a, bp_out_block* blocks, int n_blo
     if ((rc = enqueue_step(sp, (int)hdr.n_groups, s))) return rc;
 
     c->flags_host.assign(n_flag, 0);
-    BP_HIP_CHECK(hipMemcpyAsync(c->flags_host.data(), c->sc.flags.ptr, sizeof(int32_t) * n_flag,
+    BP_HIP_CHECK(hipMemcpyAsync(c->flags_host.data(), sc.flags.ptr, sizeof(int32_t) * n_flag,
                                 hipMemcpyDeviceToHost, s));
     BP_HIP_CHECK(hipStreamSynchronize(s));
     return finish_step(c->flags_host.data(), hdr.n_grids, c->desc_block, blocks, c->ev,
                        any_failed, stats);
+}
+
+// A bump allocator over a fixed device region: every buffer of a call starts
+// on a 256-byte boundary; a call that does not fit fails before any launch.
+struct Carve {
+    unsigned char* base;
+    size_t cap;
+    size_t used = 0;
+    int take(DevBuf& b, size_t bytes)
+    {
+        if (bytes == 0) bytes = 16;  // as ensure()
+        const size_t off = (used + 255) & ~size_t(255);
+        if (off > cap || bytes > cap - off) {
+            used = cap + 1;  // mark the call as too large
+            return BP_ERR_LENGTH;
+        }
+        b.ptr = base + off;
+        b.bytes = bytes;
+        b.fixed = true;
+        used = off + bytes;
+        return BP_SUCCESS;
+    }
+};
+
+// Buffers of one bp_backproject_noalloc call, of the sizes upload_static,
+// pulse_states and run_blocks ask for (a buffer that came out smaller would
+// fail in ensure(), never overrun).
+int carve_call(Carve& cv, const CallArgs& a, int width, int n_blocks, int64_t npix,
+               StaticData& st, Scratch& sc)
+{
+    const bp_orbit& oo = *a.out_orbit;
+    const bp_orbit& io = a.in_geometry->orbit;
+    const bp_lut2d& od = *a.out_doppler;
+    const bp_lut2d& id = a.in_geometry->doppler;
+    const bool split = a.precision == BP_PRECISION_SPLIT;
+    const int64_t na = std::max<int64_t>(a.in_geometry->grid.length, 1);
+    const struct {
+        DevBuf* b;
+        double bytes;  // double: sizes of absurd inputs must not wrap
+    } want[] = {
+        {&sc.flags, 4.0 * (kFlagWords + n_blocks)},
+        {&sc.desc, (double)sizeof(bp::BatchHeader) + (double)sizeof(bp::OutDesc) * std::max(n_blocks, 1)},
+        {&st.out_pos, 24.0 * oo.n},
+        {&st.out_vel, 24.0 * oo.n},
+        {&st.in_pos, 24.0 * io.n},
+        {&st.in_vel, 24.0 * io.n},
+        {&st.out_dop, od.have_data ? 8.0 * od.width * od.length : 0.0},
+        {&st.in_dop, id.have_data ? 8.0 * id.width * id.length : 0.0},
+        {&st.dem, a.dem->have_raster ? 4.0 * a.dem->width * a.dem->length : 0.0},
+        {&st.table, 4.0 * a.kernel->n},
+        {&st.poly, split ? 4.0 * (bp::kPolyphaseRows + 1) * bp::poly_row_floats(width) : 0.0},
+        {&st.poly2, split ? 4.0 * bp::pair_table_floats(width) : 0.0},
+        {&st.table2, split ? 0.0 : 16.0 * a.kernel->n},
+        {&st.pulse_pv, split ? 0.0 : 8.0 * bp::kPulseRecDoubles * na},
+        {&sc.geom, (double)sizeof(bp::PixGeom) * npix},
+        {&sc.split, split ? (double)sizeof(bp::PixSplit) * npix : 0.0},
+    };
+    double total = 0.0;
+    for (const auto& w : want) total += std::ceil(std::max(w.bytes, 16.0) / 256.0) * 256.0;
+    if (total > (double)cv.cap)
+        return fail(BP_ERR_LENGTH,
+                    "bp_backproject_noalloc: the call needs " + std::to_string((long long)total) +
+                        " bytes of scratch (output records " +
+                        std::to_string((long long)bp_scratch_bytes_per_pixel(a.precision)) +
+                        " bytes per pixel, plus orbit, Doppler, DEM and kernel tables); the "
+                        "library's fixed scratch holds " + std::to_string(cv.cap) + " bytes");
+    for (const auto& w : want)
+        if (cv.take(*w.b, (size_t)w.bytes) != BP_SUCCESS)
+            return fail(BP_ERR_LENGTH, "bp_backproject_noalloc: scratch layout overflow");
+    return BP_SUCCESS;
 }
 
 // ------------------------------------------------------------ host input
@@ -1542,6 +1668,30 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
     CallArgs a{&og->orbit, &og->doppler, ig, dem, fc, ds, kernel, dry_tropo_model, r2g, g2r,
                in_dev, precision};
     return run_blocks(c, a, &blk, 1, options, stream, any_failed, stats);
+}
+
+int bp_backproject_noalloc(bp_context* c, const bp_radar_geometry* og,
+                           const bp_radar_geometry* ig, const bp_dem* dem, double fc, double ds,
+                           const bp_kernel* kernel, int dry_tropo_model,
+                           const bp_rdr2geo_params* r2g, const bp_geo2rdr_params* g2r,
+                           const void* in_dev, void* out_dev, float* height_dev, int precision,
+                           const bp_options* options, void* stream, int* any_failed,
+                           bp_stats* stats)
+{
+    if (any_failed) *any_failed = 0;
+    if (!og || !c) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->scratch == nullptr)
+        return fail(BP_ERR_HIP, "bp_backproject_noalloc: no scratch address for this context's "
+                                "device (" + c->scratch_err + ")");
+    bp_out_block blk;
+    std::memset(&blk, 0, sizeof(blk));
+    blk.grid = og->grid;
+    blk.out_dev = out_dev;
+    blk.height_dev = height_dev;
+    CallArgs a{&og->orbit, &og->doppler, ig, dem, fc, ds, kernel, dry_tropo_model, r2g, g2r,
+               in_dev, precision};
+    Carve cv{c->scratch, kNoAllocScratchBytes};
+    return run_blocks(c, a, &blk, 1, options, stream, any_failed, stats, &cv);
 }
 
 int bp_backproject_blocks(bp_context* c, const bp_orbit* out_orbit, const bp_lut2d* out_doppler,
