@@ -675,14 +675,15 @@ def main():
 
     tuner = autotune.Autotuner(args.tune_cache)
     sc, raw_host, results = run_full(args, torch, np, dev, tuner)
+    raw_np = raw_host.numpy()
+    acc = accuracy(args, np, sc, raw_np, results)
     if args.only:
         r = results[args.only]
         print(json.dumps({"only": args.only, "ms_per_step": r["ms_per_step"],
-                          "sum_ms": r["sum_ms"], "updates": r["updates"],
-                          "updates_per_s": r["updates_per_s"]}))
+                          "sum_ms": r["sum_ms"], "geometry_ms": r["geometry_ms"],
+                          "updates": r["updates"], "updates_per_s": r["updates_per_s"],
+                          "accuracy": acc.get(args.only)}))
         return
-    raw_np = raw_host.numpy()
-    acc = accuracy(args, np, sc, raw_np, results)
     cpu = None if args.no_cpu else cpu_baseline(args, np, sc, raw_np)
     del raw_host, raw_np
 

@@ -318,9 +318,10 @@ void build_polyphase(const bp_kernel& k, int width, std::vector<float>& out)
 
 // Two-line weight table of kernel width W: Q = 2^pair_q_log2(W) rows at
 // fractional offsets t0 = tmin + q/Q (tmin = 0 for even W, -1/2 for odd W, as
-// the one-line table), each row pair_row_floats(W) floats, in two copies:
-// copy 0 holds the W weights in entries 0..W-1 and a zero in entry W, copy 1
-// a zero in entry 0 and the weights in entries 1..W.  Entries interpolated in
+// the one-line table), each row pair_row_floats(W) floats, in two copies
+// stored one after the other (copy o of row q at floats (2 q + o) RF): copy 0
+// holds the W weights in entries 0..W-1 and a zero in entry W, copy 1 a zero
+// in entry 0 and the weights in entries 1..W.  Entries interpolated in
 // the sum (pair_lerp_lo .. + pair_lerp_n) hold the weight at t0 and, after the
 // W + 1 entries, its difference to the weight at t0 + 1/Q; the others hold the
 // weight at t0 + 1/(2Q), the middle of the row's interval.
@@ -343,7 +344,7 @@ void build_pair_table(const bp_kernel& k, int W, std::vector<float>& out)
     out.assign((size_t)bp::pair_table_floats(W), 0.f);
     for (int o = 0; o < 2; ++o) {
         for (int q = 0; q < Q; ++q) {
-            float* r = &out[((size_t)o * Q + q) * RF];
+            float* r = &out[((size_t)q * 2 + o) * RF];
             const double t0 = tmin + (double)q / Q;
             for (int e = 0; e <= W; ++e) {
                 const int m = e - o;  // tap of entry e in this copy

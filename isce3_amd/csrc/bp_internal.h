@@ -86,12 +86,22 @@ constexpr int kMaxKernelWidth = 16;
 // the W + 1 entries of a (W + 1)-bin window followed by the differences to the
 // next row of entries [pair_lerp_lo(W), pair_lerp_lo(W) + pair_lerp_n(W)),
 // padded to whole float4s, in two copies -- the W weights followed by a zero
-// (copy 0) or preceded by one (copy 1).  A pixel reads row floor(Q t) and
+// (copy 0) or preceded by one (copy 1), copy 1 of a row right after its copy
+// 0.  A pixel reads row floor(Q t) and
 // interpolates the entries that have differences linearly between the rows;
 // entries without differences hold the weight at the middle of the row's
 // interval.  Q is the largest power of two up to 512 that keeps the table
 // within 64 KB of LDS, so that two 512-thread workgroups fit one CU.
-__host__ __device__ constexpr int pair_lerp_n(int w) { return w + 1; }
+// Entries interpolated between rows: the BP_PAIR_LERP_N central ones (all
+// W + 1 when W + 1 <= BP_PAIR_LERP_N).  The outer taps of the kernel change
+// slowly with the offset, so their half-row offset error costs little: on the
+// config-1 block (width 8, 512 rows) 5 interpolated entries give 5.4e-5
+// relative and 4.5e-5 rad RMS phase error against the FP64 oracle, 7 give
+// 2.0e-5, all 9 (256 rows) 2.7e-6, at 91.1 / 93.4 / 96.3 ms for the sum.
+#ifndef BP_PAIR_LERP_N
+#define BP_PAIR_LERP_N 5
+#endif
+__host__ __device__ constexpr int pair_lerp_n(int w) { return BP_PAIR_LERP_N < w + 1 ? BP_PAIR_LERP_N : w + 1; }
 __host__ __device__ constexpr int pair_lerp_lo(int w) { return (w + 1 - pair_lerp_n(w)) / 2; }
 __host__ __device__ constexpr int pair_row_floats(int w)
 {

@@ -1003,8 +1003,8 @@ struct PairTable {
     static constexpr int kNL = pair_lerp_n(W);         // interpolated entries
     static constexpr int kRF = pair_row_floats(W);     // floats per row
     static constexpr int kQL = pair_q_log2(W);
-    static constexpr int kRowBytes = 4 * kRF;
-    static constexpr int kCopyBytes = (1 << kQL) * kRowBytes;
+    static constexpr int kCopyBytes = 4 * kRF;         // copy 1 of a row follows copy 0
+    static constexpr int kRowBytes = 2 * kCopyBytes;   // stride of the rows
     static constexpr int kFloats = pair_table_floats(W);
 };
 
@@ -1116,37 +1116,60 @@ __device__ inline void pair_masked(const SumArgs& a, const float* s_pair, PairLi
 // Core pulses [k0, k1), every pixel of every lane integrating: one shared
 // (W + 1)-bin gather per lane and pulse (no software prefetch: the two lines'
 // state leaves no registers for a second pulse in flight).
+//
+// A lane whose pixel has no pulses follows its partner (same first tap) or
+// holds zero state (first tap 0, the smallest base), so every lane's offset
+// from the shared base is 0 or 1 except where an integrating pixel's first
+// taps are further apart -- the ballot catches those, and then the pulse is
+// gathered per pixel, outside the loop of shared gathers, which runs up to
+// the next fold or such a pulse and so carries no other branch.
 template<int W>
 __device__ inline void pair_core(const SumArgs& a, const float* s_pair, PairLine& A, PairLine& B,
                                  int k0, int k1, uint64_t act_a, uint64_t act_b)
 {
     using T = PairTable<W>;
     const int nr = (int)a.nr;
-    for (int k = k0; k < k1; ++k) {
-        const PairPulse pa = pair_pulse<W>(A.s.st, a.zscale2, A.s.zoff, A.s.cbase);
-        const PairPulse pb = pair_pulse<W>(B.s.st, a.zscale2, B.s.zoff, B.s.cbase);
-        A.s.st.advance();
-        B.s.st.advance();
-        const int base = min(min(pa.c0, pb.c0), nr - (W + 1));
-        const float2* line = a.rc + (int64_t)k * nr;
-        float2 d[W + 1];
-        pair_taps_shared<W>(line, base, d);
-        const uint32_t oa = (uint32_t)(pa.c0 - base), ob = (uint32_t)(pb.c0 - base);
-        const uint64_t bad = (__builtin_amdgcn_ballot_w64(oa > 1u) & act_a) |
-                             (__builtin_amdgcn_ballot_w64(ob > 1u) & act_b);
-        if (bad == 0) {
-            pair_update<W>(s_pair, pa.row + (int)min(oa, 1u) * T::kCopyBytes, pa.qf, pa.ph, d,
-                           A.s.cre, A.s.cim);
-            pair_update<W>(s_pair, pb.row + (int)min(ob, 1u) * T::kCopyBytes, pb.qf, pb.ph, d,
-                           B.s.cre, B.s.cim);
-        } else {
+    const int base_hi = nr - (W + 1);
+    int k = k0;
+    while (k < k1) {
+        const int kr = min(k1, k + (a.chunk - A.s.count));
+        const int k_run = k;
+        PairPulse pa, pb;
+        bool apart = false;  // pulse k needs per-pixel gathers
+        for (; k < kr; ++k) {
+            pa = pair_pulse<W>(A.s.st, a.zscale2, A.s.zoff, A.s.cbase);
+            pb = pair_pulse<W>(B.s.st, a.zscale2, B.s.zoff, B.s.cbase);
+            const int base = min(min(pa.c0, pb.c0), base_hi);
+            const uint32_t oa = (uint32_t)(pa.c0 - base), ob = (uint32_t)(pb.c0 - base);
+            const uint64_t bad = (__builtin_amdgcn_ballot_w64(oa > 1u) & act_a) |
+                                 (__builtin_amdgcn_ballot_w64(ob > 1u) & act_b);
+            if (bad != 0) {
+                apart = true;
+                break;
+            }
+            A.s.st.advance();
+            B.s.st.advance();
+            float2 d[W + 1];
+            pair_taps_shared<W>(a.rc + (int64_t)k * nr, base, d);
+            // copy oa of the row: the weights aligned to this pixel's first tap
+            pair_update<W>(s_pair, (int)__umul24(oa, (uint32_t)T::kCopyBytes) + pa.row, pa.qf,
+                           pa.ph, d, A.s.cre, A.s.cim);
+            pair_update<W>(s_pair, (int)__umul24(ob, (uint32_t)T::kCopyBytes) + pb.row, pb.qf,
+                           pb.ph, d, B.s.cre, B.s.cim);
+        }
+        if (apart) {
+            A.s.st.advance();
+            B.s.st.advance();
+            const float2* line = a.rc + (int64_t)k * nr;
             float2 e[W + 1];
             pair_taps_one<W, false>(line, pa.c0, nr, e);
             pair_update<W>(s_pair, pa.row, pa.qf, pa.ph, e, A.s.cre, A.s.cim);
             pair_taps_one<W, false>(line, pb.c0, nr, e);
             pair_update<W>(s_pair, pb.row, pb.qf, pb.ph, e, B.s.cre, B.s.cim);
+            ++k;
         }
-        if (++A.s.count >= a.chunk) {
+        A.s.count += k - k_run;
+        if (A.s.count >= a.chunk) {
             A.s.fold();
             B.s.fold();
         }

@@ -121,3 +121,28 @@ def test_two_lines_per_lane_odd_kernel_width(torch_gpu):
     assert err == (rc == 6)
     assert complex_rel_error(out, ref) <= SPLIT_REL
     assert rms(phase_errors(out, ref)) <= SPLIT_PHS
+
+
+@pytest.mark.parametrize("width", [4, 7, 11, 16])
+@pytest.mark.parametrize("precision,lane_rows", [("plain", 1), ("split", 1), ("split", 2)])
+def test_kernel_widths_match_oracle(torch_gpu, width, precision, lane_rows):
+    """Knab kernels of the widths the workflow can ask for (1 + 2 halfwidth and
+    even widths; Backproject.cu:698-754 takes any kernel) against the oracle,
+    in both precisions and both split tilings."""
+    from isce3_amd.focus import backproject
+    import oracle
+    sc, raw = scene_and_raw("tiny", kernel_width=float(width))
+    print(f"[width] {width} {precision} lane_rows={lane_rows}", flush=True)
+    out = np.empty(sc.out_grid.shape, np.complex64)
+    err = backproject(out, sc.out_geometry(), raw, sc.in_geometry(), sc.dem, sc.fc, sc.ds,
+                      sc.kernel, sc.dry_tropo_model, precision=precision,
+                      options={"lane_rows": lane_rows})
+    mode = 0 if precision == "plain" else 1
+    ref, _, _, rc = oracle.backproject(sc.out_geometry(), raw, sc.in_geometry(), sc.dem, sc.fc,
+                                       sc.ds, sc.kernel, sc.dry_tropo_model, mode=mode,
+                                       kernel_bandwidth=0.8)
+    assert err == (rc == 6)
+    rel, ph = complex_rel_error(out, ref), rms(phase_errors(out, ref))
+    print(f"  rel={rel:.3e} rms_phase={ph:.3e}")
+    tol = (PLAIN_REL, PLAIN_PHS) if precision == "plain" else (SPLIT_REL, SPLIT_PHS)
+    assert rel <= tol[0] and ph <= tol[1]
