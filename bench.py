@@ -36,9 +36,6 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# the CPU baseline's OpenMP threads stay on adjacent cores (read when the
-# OpenMP runtime starts, so before anything loads it)
-os.environ.setdefault("OMP_PROC_BIND", "close")
 
 FLOPS_PER_UPDATE = 101  # algorithmic flops per pixel-pulse update (FMA = 2)
 PEAK_FP32_TFLOPS = 157.3
@@ -194,20 +191,52 @@ def host_cpu():
 
 
 def cpu_threads():
-    """OpenMP threads of the CPU baseline: the host's logical CPUs, capped at
-    the 16-core CPU share that a one-GPU box gives this process."""
-    return max(1, min(16, os.cpu_count() or 1))
+    """OpenMP threads of the CPU baseline: the CPUs this process may run on
+    (nproc), capped at the 16-core CPU share that a one-GPU box gives it."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
-def cpu_baseline(args, np, sc, raw_np):
+def cpu_baseline(args):
+    """Time the CPU baseline in a child process (``--cpu-baseline-child``)
+    that starts its OpenMP runtime with OMP_PROC_BIND=close and one thread per
+    CPU.  Binding is an OpenMP runtime setting, fixed when the runtime starts
+    and applied to the starting thread too: set in this process it would pin
+    the main thread to one CPU, and with it every thread the library's host
+    code starts from it."""
+    import subprocess
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(cpu_threads()))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
+           "--cpu-seconds", str(args.cpu_seconds)]
+    res = subprocess.run(cmd, env=env, capture_output=True, text=True)
+    sys.stderr.write(res.stderr)
+    if res.returncode != 0:
+        raise RuntimeError(f"CPU baseline process failed ({res.returncode})")
+    cpu = json.loads(res.stdout.strip().splitlines()[-1])
+    log(f"cpu baseline: {cpu['value']:.3e} updates/s on {cpu['cores']} threads "
+        f"({cpu['host']})")
+    return cpu
+
+
+def cpu_baseline_child(args):
     """The oracle in reference mode (the CPU restatement of the reference
-    backprojector, OpenMP over pixels, OMP_PROC_BIND=close) on a band of output
-    rows at the centre of the grid: one warm-up run, then the median of three
-    timed runs, the band sized so that one run takes about a quarter of
-    --cpu-seconds."""
+    backprojector, OpenMP over pixels) on a band of output rows at the centre
+    of the config-1 grid, whose pulses are simulated here from the same seed:
+    one warm-up run, then the median of three timed runs, the band sized so
+    that one run takes about a quarter of --cpu-seconds.  Prints one JSON
+    line."""
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    nth = cpu_threads()
+    from isce3_amd import sim
+    # the parent's thread count (here the OpenMP runtime has bound this thread
+    # to one CPU, so this process's own affinity no longer shows the CPUs)
+    nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or cpu_threads()
+    sc = sim.make_scene("full")
+    raw_np = sim.simulate(sc)
     W = sc.out_grid.width
     row0 = sc.out_grid.length // 2
 
@@ -238,8 +267,7 @@ def cpu_baseline(args, np, sc, raw_np):
                      f"(OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND', '')}), median of 3 "
                      f"after a warm-up: {', '.join(f'{t:.2f}' for t in times)} s",
            "host": host}
-    log(f"cpu baseline: {cpu['value']:.3e} updates/s on {nth} threads ({host})")
-    return cpu
+    print(json.dumps(cpu))
 
 
 # ------------------------------------------------------------ batched tiles
@@ -637,10 +665,15 @@ def main():
                          "--serve-max-requests)")
     ap.add_argument("--serve-max-requests", type=int, default=400)
     ap.add_argument("--serve-budget", type=float, default=200e9)
+    ap.add_argument("--cpu-baseline-child", action="store_true",
+                    help=argparse.SUPPRESS)  # the CPU baseline's own process
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device-to-device copy of this many MiB first, as the "
                          "known-size reference for HBM byte counters")
     args = ap.parse_args()
+    if args.cpu_baseline_child:
+        cpu_baseline_child(args)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark measures one GPU; runs on several GPUs are "
@@ -684,7 +717,7 @@ def main():
                           "updates": r["updates"], "updates_per_s": r["updates_per_s"],
                           "accuracy": acc.get(args.only)}))
         return
-    cpu = None if args.no_cpu else cpu_baseline(args, np, sc, raw_np)
+    cpu = None if args.no_cpu else cpu_baseline(args)
     del raw_host, raw_np
 
     batch = None if args.no_batch else run_batch(args, torch, np, dev, tuner)

@@ -4,6 +4,7 @@
 // arena, and the three-pass launch (pulse orbit, per-pixel geometry,
 // coherent sum) on the caller's stream.
 #include <hip/hip_runtime.h>
+#include <sched.h>
 
 #include <algorithm>
 #include <cmath>
@@ -31,6 +32,14 @@ int fail(int code, const std::string& msg)
 
 // Shared with the range-compression entry points (rc_api.hip).
 void bp_set_last_error(const std::string& msg) { g_msg = msg; }
+
+int bp::usable_cpus()
+{
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) return std::max(1, CPU_COUNT(&set));
+    return (int)std::max(1u, std::thread::hardware_concurrency());
+}
 
 namespace bp {
 
@@ -1082,7 +1091,7 @@ void host_copy(unsigned char* dst, const unsigned char* src, size_t bytes)
 {
     const size_t kPerThread = size_t(16) << 20;
     unsigned nt = (unsigned)std::min<size_t>(8, (bytes + kPerThread - 1) / kPerThread);
-    nt = std::max(1u, std::min(nt, std::max(1u, std::thread::hardware_concurrency())));
+    nt = std::max(1u, std::min(nt, (unsigned)bp::usable_cpus()));
     if (nt == 1) {
         std::memcpy(dst, src, bytes);
         return;

@@ -220,6 +220,10 @@ hipError_t launch_band_window(const PixGeom* geom, int64_t length, int64_t width
 // as graph kernel nodes report it.
 const void* kernel_fn(int which, int width, bool staged, bool pair);
 
+// CPUs the calling thread may run on (its affinity mask; threads it starts
+// inherit the mask), at least 1.
+int usable_cpus();
+
 // Scene facts the serving queue needs (bp_api.hip).
 int scene_device(const struct bp_scene* scene);
 // Argument rules of one output block (shape, PRF, output buffer), shared by

@@ -40,6 +40,9 @@
 #include "bp_math.h"
 
 void bp_set_last_error(const std::string& msg);
+namespace bp {
+int usable_cpus();  // bp_api.hip
+}
 
 namespace presum {
 namespace {
@@ -750,9 +753,10 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
     }
     first_item[nchunk] = (int64_t)items.size();
     const int64_t nitems = (int64_t)items.size();
+    // planners: one per CPU the calling thread may run on (threads inherit
+    // its affinity, so a caller pinned to one CPU plans alone), at most 16
     const int nworkers = (int)std::max<int64_t>(
-        0, std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
-                              nitems}) - 1);
+        0, std::min<int64_t>({(int64_t)bp::usable_cpus(), 16, nitems}) - 1);
     std::vector<Patterns> item_pat(nitems);
     std::vector<int64_t> done(nchunk, 0);
     std::vector<SolveScratch> scratch(nworkers + 1);   // the last one is this thread's
