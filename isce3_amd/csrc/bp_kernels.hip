@@ -1024,10 +1024,12 @@ __device__ inline PairPulse pair_pulse(const PhaseState& st, float zscale, float
 
 // Interpolate the W + 1 bins `d` with the weight row at byte offset `row`
 // (its interpolated entries moved the fraction qf towards the next row),
-// rotate by the phase and accumulate.
-template<int W>
+// rotate by the phase and accumulate.  kGate: a pixel with `on` false (outside
+// its own pulse window) adds zero, its phasor being zero.
+template<int W, bool kGate = false>
 __device__ inline void pair_update(const float* s_pair, int row, float qf, float ph,
-                                   const float2 (&d)[W + 1], float& cre, float& cim)
+                                   const float2 (&d)[W + 1], float& cre, float& cim,
+                                   bool on = true)
 {
     using T = PairTable<W>;
     const float4* r4 = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_pair) + row);
@@ -1050,12 +1052,28 @@ __device__ inline void pair_update(const float* s_pair, int row, float qf, float
         sre = fmaf(w[m], d[m].x, sre);
         sim = fmaf(w[m], d[m].y, sim);
     }
-    const float sn = __builtin_amdgcn_sinf(ph);
-    const float cs = __builtin_amdgcn_cosf(ph);
+    float sn = __builtin_amdgcn_sinf(ph);
+    float cs = __builtin_amdgcn_cosf(ph);
+    if (kGate) {
+        sn = on ? sn : 0.f;
+        cs = on ? cs : 0.f;
+    }
     cre = fmaf(sre, cs, cre);
     cre = fmaf(-sim, sn, cre);
     cim = fmaf(sre, sn, cim);
     cim = fmaf(sim, cs, cim);
+}
+
+// Byte offset of copy `o` (0 or 1) of the weight row at byte offset `row`:
+// one shift-add for power-of-two copies, else one 24-bit multiply-add.
+template<int W>
+__device__ inline int pair_row(int row, uint32_t o)
+{
+    constexpr int kCB = PairTable<W>::kCopyBytes;
+    if constexpr ((kCB & (kCB - 1)) == 0)
+        return (int)(o << __builtin_ctz((unsigned)kCB)) + row;
+    else
+        return (int)__umul24(o, (uint32_t)kCB) + row;
 }
 
 // The W taps from bin c0 (guarded: zero outside [0, nr)), and a zero.
@@ -1123,7 +1141,13 @@ __device__ inline void pair_masked(const SumArgs& a, const float* s_pair, PairLi
 // taps are further apart -- the ballot catches those, and then the pulse is
 // gathered per pixel, outside the loop of shared gathers, which runs up to
 // the next fold or such a pulse and so carries no other branch.
-template<int W>
+//
+// kRagged: pulses at the ends of the wavefront's window that some pixels do
+// not integrate.  Every pixel's phase fit spans the whole window, so its first
+// tap stays on its range curve there and the shared gather serves it too; a
+// pixel outside its own window adds zero (gated phasor) instead of taking a
+// per-pixel branch.
+template<int W, bool kRagged>
 __device__ inline void pair_core(const SumArgs& a, const float* s_pair, PairLine& A, PairLine& B,
                                  int k0, int k1, uint64_t act_a, uint64_t act_b)
 {
@@ -1152,10 +1176,10 @@ __device__ inline void pair_core(const SumArgs& a, const float* s_pair, PairLine
             float2 d[W + 1];
             pair_taps_shared<W>(a.rc + (int64_t)k * nr, base, d);
             // copy oa of the row: the weights aligned to this pixel's first tap
-            pair_update<W>(s_pair, (int)__umul24(oa, (uint32_t)T::kCopyBytes) + pa.row, pa.qf,
-                           pa.ph, d, A.s.cre, A.s.cim);
-            pair_update<W>(s_pair, (int)__umul24(ob, (uint32_t)T::kCopyBytes) + pb.row, pb.qf,
-                           pb.ph, d, B.s.cre, B.s.cim);
+            pair_update<W, kRagged>(s_pair, pair_row<W>(pa.row, oa), pa.qf, pa.ph, d, A.s.cre,
+                                    A.s.cim, k >= A.ks && k < A.ke);
+            pair_update<W, kRagged>(s_pair, pair_row<W>(pb.row, ob), pb.qf, pb.ph, d, B.s.cre,
+                                    B.s.cim, k >= B.ks && k < B.ke);
         }
         if (apart) {
             A.s.st.advance();
@@ -1163,9 +1187,11 @@ __device__ inline void pair_core(const SumArgs& a, const float* s_pair, PairLine
             const float2* line = a.rc + (int64_t)k * nr;
             float2 e[W + 1];
             pair_taps_one<W, false>(line, pa.c0, nr, e);
-            pair_update<W>(s_pair, pa.row, pa.qf, pa.ph, e, A.s.cre, A.s.cim);
+            pair_update<W, kRagged>(s_pair, pa.row, pa.qf, pa.ph, e, A.s.cre, A.s.cim,
+                                    k >= A.ks && k < A.ke);
             pair_taps_one<W, false>(line, pb.c0, nr, e);
-            pair_update<W>(s_pair, pb.row, pb.qf, pb.ph, e, B.s.cre, B.s.cim);
+            pair_update<W, kRagged>(s_pair, pb.row, pb.qf, pb.ph, e, B.s.cre, B.s.cim,
+                                    k >= B.ks && k < B.ke);
             ++k;
         }
         A.s.count += k - k_run;
@@ -1231,11 +1257,11 @@ __device__ inline void pair_tile(const SumArgs& a, const float* s_pair, const Ou
         if (guard) {
             pair_masked<W, true>(a, s_pair, L[0], L[1], kw0, kw1);
         } else {
-            pair_masked<W, false>(a, s_pair, L[0], L[1], kw0, kf0);
-            if (kf0 < kf1)
-                pair_core<W>(a, s_pair, L[0], L[1], kf0, kf1, __builtin_amdgcn_ballot_w64(win[0]),
-                             __builtin_amdgcn_ballot_w64(win[1]));
-            pair_masked<W, false>(a, s_pair, L[0], L[1], max(kf1, kw0), kw1);
+            const uint64_t act_a = __builtin_amdgcn_ballot_w64(win[0]);
+            const uint64_t act_b = __builtin_amdgcn_ballot_w64(win[1]);
+            pair_core<W, true>(a, s_pair, L[0], L[1], kw0, kf0, act_a, act_b);
+            if (kf0 < kf1) pair_core<W, false>(a, s_pair, L[0], L[1], kf0, kf1, act_a, act_b);
+            pair_core<W, true>(a, s_pair, L[0], L[1], max(kf1, kw0), kw1, act_a, act_b);
         }
         L[0].s.fold();
         L[1].s.fold();
