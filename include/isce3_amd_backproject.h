@@ -56,8 +56,9 @@ enum { BP_INTERP_HERMITE = 0, BP_INTERP_LEGENDRE = 1 };   /* OrbitInterpMethod *
 enum { BP_LOOK_LEFT = 1, BP_LOOK_RIGHT = -1 };            /* isce3 LookSide */
 enum { BP_TROPO_NODELAY = 0, BP_TROPO_TSX = 1 };          /* DryTroposphereModel */
 /* DEM interpolation: isce3::core::dataInterpMethod values (core/Constants.h:23-29);
- * sinc is not offered. */
-enum { BP_DEM_BILINEAR = 1, BP_DEM_BICUBIC = 2, BP_DEM_NEAREST = 3, BP_DEM_BIQUINTIC = 4 };
+ * sinc is the factory's 8-tap, 8192-row Sinc2dInterpolator. */
+enum { BP_DEM_SINC = 0, BP_DEM_BILINEAR = 1, BP_DEM_BICUBIC = 2, BP_DEM_NEAREST = 3,
+       BP_DEM_BIQUINTIC = 4 };
 enum {
     BP_PRECISION_PLAIN = 0, /* FP64 delay/phase/taps index, complex<double> sum */
     BP_PRECISION_SPLIT = 1  /* FP64 per-pixel reference + FP32 deltas, Kahan sum */

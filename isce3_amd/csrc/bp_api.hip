@@ -75,7 +75,7 @@ struct DevBuf {
 // the platform state at every pulse.
 struct StaticData {
     DevBuf table, table2, poly, poly2, in_pos, in_vel, out_pos, out_vel, in_dop, out_dop, dem,
-        pulse_pv;
+        pulse_pv, sinc;
     std::vector<float> poly_host, poly2_host;
     std::vector<double> table2_host;
 };
@@ -247,6 +247,34 @@ bp::Lut2d host_lut(const bp_lut2d& l)
     return d;
 }
 
+// The windowed-sinc kernel of the DEM factory's Sinc2dInterpolator(8, 8192)
+// (core/Sinc2dInterpolator.cpp:14-41, coefficients :105-120): the 65536
+// samples of a sinc under a raised-cosine window, regrouped into 8192
+// fractional rows of 8 taps, each row divided by its sum; stored as float
+// (the interpolator casts each coefficient to the raster's float type).
+// Built once per process.
+const std::vector<float>& sinc_kernel_table()
+{
+    static const std::vector<float> table = [] {
+        constexpr int L = bp::kSincLen, D = bp::kSincSub, n = L * D;
+        const double mid = (n - 1.0) / 2.0;
+        std::vector<double> f(n);
+        for (int i = 0; i < n; ++i) {
+            const double window = 0.5 + 0.5 * std::cos(M_PI * (i - mid) / mid);
+            const double s = std::floor(i - mid) / D;
+            f[i] = window * (s != 0.0 ? std::sin(M_PI * s) / (M_PI * s) : 1.0);
+        }
+        std::vector<float> t((size_t)n);
+        for (int r = 0; r < D; ++r) {
+            double sum = 0.0;
+            for (int j = 0; j < L; ++j) sum += f[r + D * j];
+            for (int j = 0; j < L; ++j) t[(size_t)r * L + j] = (float)(f[r + D * j] / sum);
+        }
+        return t;
+    }();
+    return table;
+}
+
 bp::Dem host_dem(const bp_dem& m)
 {
     bp::Dem d;
@@ -262,6 +290,7 @@ bp::Dem host_dem(const bp_dem& m)
     d.width = m.width;
     d.length = m.length;
     d.data = m.data;
+    d.sinc = m.method == BP_DEM_SINC ? sinc_kernel_table().data() : nullptr;
     if (!bp::proj_init(m.epsg, &d.proj)) bp::proj_init(4326, &d.proj);
     return d;
 }
@@ -269,7 +298,7 @@ bp::Dem host_dem(const bp_dem& m)
 bool dem_method_ok(int m)
 {
     return m == BP_DEM_BILINEAR || m == BP_DEM_BICUBIC || m == BP_DEM_NEAREST ||
-           m == BP_DEM_BIQUINTIC;
+           m == BP_DEM_BIQUINTIC || m == BP_DEM_SINC;
 }
 
 bp::TileMap make_map(int64_t length, int64_t width, int tile_rows, int order, int lane_rows)
@@ -473,7 +502,7 @@ namespace {
 void free_static(StaticData& s)
 {
     DevBuf* bufs[] = {&s.table, &s.table2, &s.poly, &s.poly2, &s.in_pos, &s.in_vel, &s.out_pos,
-                      &s.out_vel, &s.in_dop, &s.out_dop, &s.dem, &s.pulse_pv};
+                      &s.out_vel, &s.in_dop, &s.out_dop, &s.dem, &s.pulse_pv, &s.sinc};
     for (DevBuf* b : bufs)
         if (b->ptr && !b->fixed) (void)hipFree(b->ptr);
 }
@@ -719,6 +748,10 @@ int upload_static(StaticData& st, const CallArgs& a, int width, hipStream_t s)
                      a.dem->have_raster ? sizeof(float) * a.dem->width * a.dem->length : 0, s)))
         return rc;
     if ((rc = upload(st.table, a.kernel->table, sizeof(float) * a.kernel->n, s))) return rc;
+    const bool sinc = a.dem->have_raster && a.dem->method == BP_DEM_SINC;
+    if ((rc = upload(st.sinc, sinc ? sinc_kernel_table().data() : nullptr,
+                     sinc ? sizeof(float) * sinc_kernel_table().size() : 0, s)))
+        return rc;
     if (a.precision == BP_PRECISION_SPLIT) {
         build_polyphase(*a.kernel, width, st.poly_host);
         if ((rc = upload(st.poly, st.poly_host.data(), sizeof(float) * st.poly_host.size(), s)))
@@ -773,6 +806,7 @@ void build_step(Step& sp, const StaticData& st, const Scratch& sc, const CallArg
 
     bp::Dem ddem = host_dem(*a.dem);
     ddem.data = static_cast<const float*>(st.dem.ptr);
+    ddem.sinc = static_cast<const float*>(st.sinc.ptr);
 
     bp::GeomArgs& ga = sp.ga;
     std::memset(&ga, 0, sizeof(ga));
@@ -1021,6 +1055,8 @@ int carve_call(Carve& cv, const CallArgs& a, int width, int n_blocks, int64_t np
         {&st.out_dop, od.have_data ? 8.0 * od.width * od.length : 0.0},
         {&st.in_dop, id.have_data ? 8.0 * id.width * id.length : 0.0},
         {&st.dem, a.dem->have_raster ? 4.0 * a.dem->width * a.dem->length : 0.0},
+        {&st.sinc, a.dem->have_raster && a.dem->method == BP_DEM_SINC
+                       ? 4.0 * bp::kSincLen * bp::kSincSub : 0.0},
         {&st.table, 4.0 * a.kernel->n},
         {&st.poly, split ? 4.0 * (bp::kPolyphaseRows + 1) * bp::poly_row_floats(width) : 0.0},
         {&st.poly2, split ? 4.0 * bp::pair_table_floats(width) : 0.0},

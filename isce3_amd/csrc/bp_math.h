@@ -105,8 +105,13 @@ struct Dem {
     double xstart, ystart, dx, dy;
     int64_t width, length;
     const float* data;
+    const float* sinc;   // kDemSinc: kernel [kSincSub][kSincLen] (sinc_kernel_table)
     Proj proj;
 };
+
+// The DEM factory's windowed-sinc interpolator (core/Interpolator.h:203-221,
+// core/Constants.h:32-35): 8 taps, 8192 fractional rows.
+constexpr int kSincLen = 8, kSincSub = 8192;
 
 // core/Linspace.icc:77-89
 BP_HD int linspace_search(double first, double spacing, int size, double val)
@@ -619,6 +624,32 @@ BP_HD float biquintic_f32(double x, double y, const float* z, int64_t width, int
     return spline_eval(y - i0, hc, r);
 }
 
+// 8 x 8 windowed-sinc interpolation of a float raster at (x, y) with the
+// kernel table k (core/Sinc2dInterpolator.cpp:45-97): zero where the window
+// does not fit the raster; the window's rows run from y's floor + 4 down to
+// floor - 3, its columns likewise; kernel rows picked by the fractions times
+// 8192; products and sum in float.
+BP_HD float sinc_f32(double x, double y, const float* z, int64_t width, int64_t length,
+                     const float* k)
+{
+    constexpr int half = kSincLen / 2;
+    const int ix = static_cast<int>(floor(x));
+    const int iy = static_cast<int>(floor(y));
+    const double fx = x - ix, fy = y - iy;
+    if (ix < half - 1 || ix > width - half - 1) return 0.f;
+    if (iy < half - 1 || iy > length - half - 1) return 0.f;
+    const int fxi = min(max(0, (int)(fx * kSincSub)), kSincSub - 1);
+    const int fyi = min(max(0, (int)(fy * kSincSub)), kSincSub - 1);
+    const float* ky = k + (int64_t)fyi * kSincLen;
+    const float* kx = k + (int64_t)fxi * kSincLen;
+    float ret = 0.f;
+    for (int i = 0; i < kSincLen; ++i) {
+        const float* row = z + (int64_t)(iy + half - i) * width + (ix + half);
+        for (int j = 0; j < kSincLen; ++j) ret += row[-j] * ky[i] * kx[j];
+    }
+    return ret;
+}
+
 // DEM lookup in native raster coordinates
 // (geometry/DEMInterpolator.cpp:615-662).
 BP_HD double dem_interpolate_xy(const Dem& dem, double x, double y)
@@ -644,6 +675,7 @@ BP_HD double dem_interpolate_xy(const Dem& dem, double x, double y)
     case kDemNearest:  // core/NearestNeighborInterpolator.cpp:11-22
         return dem.data[(int64_t)llround(row) * dem.width + llround(col)];
     case kDemBiquintic: return biquintic_f32(col, row, dem.data, dem.width, dem.length);
+    case kDemSinc: return sinc_f32(col, row, dem.data, dem.width, dem.length, dem.sinc);
     default: return bilinear_f32(col, row, dem.data, dem.width);
     }
 }

@@ -17,7 +17,7 @@ import numpy as np
 from . import _native, errors
 
 # isce3::core::dataInterpMethod values (core/Constants.h:23-29)
-_METHODS = {"bilinear": 1, "bicubic": 2, "nearest": 3, "biquintic": 4}
+_METHODS = {"sinc": 0, "bilinear": 1, "bicubic": 2, "nearest": 3, "biquintic": 4}
 
 
 class DEMInterpolator:
@@ -35,10 +35,8 @@ class DEMInterpolator:
 
     @staticmethod
     def _parse_method(method) -> str:
-        # core/Constants.cpp:10-24 (parseDataInterpMethod), sinc not offered
+        # core/Constants.cpp:10-24 (parseDataInterpMethod)
         m = str(getattr(method, "name", method)).lower()
-        if m == "sinc":
-            raise errors.InvalidArgument("sinc DEM interpolation is not supported")
         if m not in _METHODS:
             raise errors.InvalidArgument("Unknown interp method")
         return m

@@ -371,6 +371,64 @@ T spline2d(double x, double y, const T* z, int64_t width, int64_t length, int or
     return spline_value(y - i0, cols, spline_second_derivs(cols));
 }
 
+// Windowed-sinc 2-D interpolation as the DEM factory creates it
+// (core/Interpolator.h:203-221: Sinc2dInterpolator(SINC_LEN = 8,
+// SINC_SUB = 8192), core/Constants.h:32-35).
+constexpr int kSincLen = 8, kSincSub = 8192;
+
+// The interpolator's kernel (core/Sinc2dInterpolator.cpp:14-41, coefficients
+// :105-120): a raised-cosine-windowed sinc of kSincLen x kSincSub samples,
+// each of the kSincSub fractional rows normalised to unit sum, [row][tap].
+const std::vector<double>& sinc2d_kernel()
+{
+    static const std::vector<double> kernel = [] {
+        const int n = kSincLen * kSincSub;
+        const double wgthgt = 0.5;             // (1 - pedestal) / 2, pedestal 0
+        const double soff = (n - 1.0) / 2.0;
+        std::vector<double> f(n);
+        for (int i = 0; i < n; ++i) {
+            const double wgt = (1.0 - wgthgt) + wgthgt * std::cos((M_PI * (i - soff)) / soff);
+            const double s = std::floor(i - soff) / (1.0 * kSincSub);   // beta = 1
+            const double fct = (s != 0.0) ? std::sin(M_PI * s) / (M_PI * s) : 1.0;
+            f[i] = fct * wgt;
+        }
+        std::vector<double> k((size_t)n);
+        for (int i = 0; i < kSincSub; ++i) {
+            double ssum = 0.0;
+            for (int j = 0; j < kSincLen; ++j) ssum += f[i + kSincSub * j];
+            for (int j = 0; j < kSincLen; ++j)
+                k[(size_t)i * kSincLen + j] = f[i + kSincSub * j] / ssum;
+        }
+        return k;
+    }();
+    return kernel;
+}
+
+// core/Sinc2dInterpolator.cpp:45-97: zero where the 8 x 8 window leaves the
+// raster; the sum in the data type, products in the order of the reference.
+template<class T>
+T sinc2d(double x, double y, const T* z, int64_t width, int64_t length)
+{
+    const int half = kSincLen / 2;
+    const int ix = static_cast<int>(std::floor(x));
+    const int iy = static_cast<int>(std::floor(y));
+    const double fx = x - ix;
+    const double fy = y - iy;
+    if ((ix < half - 1) || (ix > width - half - 1)) return T(0);
+    if ((iy < half - 1) || (iy > length - half - 1)) return T(0);
+    const int xx = ix + half, yy = iy + half;
+    const int ifx = std::min(std::max(0, int(fx * kSincSub)), kSincSub - 1);
+    const int ify = std::min(std::max(0, int(fy * kSincSub)), kSincSub - 1);
+    const std::vector<double>& k = sinc2d_kernel();
+    T ret(0);
+    for (int i = 0; i < kSincLen; ++i)
+        for (int j = 0; j < kSincLen; ++j)
+            ret += z[(int64_t)(yy - i) * width + (xx - j)] *
+                   static_cast<T>(k[(size_t)ify * kSincLen + i]) *
+                   static_cast<T>(k[(size_t)ifx * kSincLen + j]);
+    return ret;
+}
+
 template<class T>
 T interp2d(int method, double x, double y, const T* z, int64_t width, int64_t length);
 
@@ -382,6 +440,7 @@ float interp2d<float>(int method, double x, double y, const float* z, int64_t wi
     case ORC_DEM_BICUBIC: return bicubic2d(x, y, z, width, length);
     case ORC_DEM_NEAREST: return nearest2d(x, y, z, width);
     case ORC_DEM_BIQUINTIC: return spline2d(x, y, z, width, length);
+    case ORC_DEM_SINC: return sinc2d(x, y, z, width, length);
     default: return bilinear_f32(x, y, z, width);
     }
 }
@@ -394,6 +453,7 @@ double interp2d<double>(int method, double x, double y, const double* z, int64_t
     case ORC_DEM_BICUBIC: return bicubic2d(x, y, z, width, length);
     case ORC_DEM_NEAREST: return nearest2d(x, y, z, width);
     case ORC_DEM_BIQUINTIC: return spline2d(x, y, z, width, length);
+    case ORC_DEM_SINC: return sinc2d(x, y, z, width, length);
     default: return bilinear_f64(x, y, z, width);
     }
 }
@@ -1096,7 +1156,7 @@ int orc_backproject(const orc_grid* out_grid, const orc_orbit* out_orbit,
             g_err = "Unknown EPSG code (in factory): " + std::to_string(dem->epsg);
             return ORC_ARG_ERROR;
         }
-        if (dem->method < ORC_DEM_BILINEAR || dem->method > ORC_DEM_BIQUINTIC) {
+        if (dem->method < ORC_DEM_SINC || dem->method > ORC_DEM_BIQUINTIC) {
             g_err = "unsupported DEM interpolation method";
             return ORC_ARG_ERROR;
         }

@@ -79,7 +79,8 @@ typedef struct {
 } orc_grid;
 
 /* DEM interpolation methods: isce3::core::dataInterpMethod values */
-enum { ORC_DEM_BILINEAR = 1, ORC_DEM_BICUBIC = 2, ORC_DEM_NEAREST = 3, ORC_DEM_BIQUINTIC = 4 };
+enum { ORC_DEM_SINC = 0, ORC_DEM_BILINEAR = 1, ORC_DEM_BICUBIC = 2, ORC_DEM_NEAREST = 3,
+       ORC_DEM_BIQUINTIC = 4 };
 
 typedef struct {
     int32_t have_raster;

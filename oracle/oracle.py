@@ -47,7 +47,7 @@ class Grid_t(C.Structure):
                 ("pad_", C.c_int32)]
 
 
-DEM_METHODS = {"bilinear": 1, "bicubic": 2, "nearest": 3, "biquintic": 4}
+DEM_METHODS = {"sinc": 0, "bilinear": 1, "bicubic": 2, "nearest": 3, "biquintic": 4}
 
 
 class Dem_t(C.Structure):

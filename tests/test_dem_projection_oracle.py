@@ -20,7 +20,7 @@ import oracle
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 KATS = json.load(open(os.path.join(HERE, "golden", "projection_kats.json")))
-BILINEAR, BICUBIC, NEAREST, BIQUINTIC = 1, 2, 3, 4
+SINC, BILINEAR, BICUBIC, NEAREST, BIQUINTIC = 0, 1, 2, 3, 4
 
 
 def _d3(v):
@@ -114,7 +114,8 @@ def test_oracle_interpolators_meet_reference_tests(dtype):
     M, data, start, delta = _reference_grid()
     Mz = M.astype(dtype)
     n = len(data)
-    err = {BILINEAR: 0.0, BICUBIC: 0.0, BIQUINTIC: 0.0}
+    err = {BILINEAR: 0.0, BICUBIC: 0.0, BIQUINTIC: 0.0, SINC: 0.0}
+    n_sinc = 0
     tol = 1e-8 if dtype == np.float64 else 2e-6
     for row in data:
         x, y = (row[0] - start) / delta, (row[1] - start) / delta
@@ -129,9 +130,14 @@ def test_oracle_interpolators_meet_reference_tests(dtype):
         if x >= 1 and y >= 1 and x < M.shape[1] - 2 and y < M.shape[0] - 2:
             err[BICUBIC] += (_interp(BICUBIC, x, y, Mz) - row[5]) ** 2
         err[BIQUINTIC] += (_interp(BIQUINTIC, x, y, Mz) - row[5]) ** 2
+        # Sinc (8 taps, 8192 fractional rows) where its window fits (:210-236)
+        if x >= 3 and y >= 3 and x <= M.shape[1] - 5 and y <= M.shape[0] - 5:
+            err[SINC] += (_interp(SINC, x, y, Mz) - row[5]) ** 2
+            n_sinc += 1
     assert err[BILINEAR] / n < 0.07
     assert err[BICUBIC] / n < 0.058
     assert err[BIQUINTIC] / n < 0.058
+    assert n_sinc > n // 2 and err[SINC] / n_sinc < 0.0003
 
 
 def test_oracle_interpolators_exact_at_nodes_and_on_planes():
@@ -140,6 +146,17 @@ def test_oracle_interpolators_exact_at_nodes_and_on_planes():
     for method in (BILINEAR, BICUBIC, NEAREST, BIQUINTIC):
         for r, c in ((2, 2), (7, 11), (17, 21)):
             assert _interp(method, float(c), float(r), z) == z[r, c]
+    # sinc: the kernel row of fraction 0 is one unit tap (sin(pi n) / (pi n)
+    # of the other taps is ~1e-16), so float nodes come back exactly; outside
+    # the 8 x 8 window the reference returns zero; a constant field stays
+    # constant (every row of the kernel sums to one)
+    for r, c in ((3, 3), (7, 11), (15, 19)):
+        assert _interp(SINC, float(c), float(r), z) == z[r, c]
+    assert _interp(SINC, 2.5, 8.0, z) == 0.0 and _interp(SINC, 10.0, 16.5, z) == 0.0
+    assert _interp(SINC, 10.0, 15.5, z) != 0.0   # row 15 is the last whose window fits
+    const = np.full((20, 24), 7.25, np.float64)
+    for x, y in ((3.3, 4.7), (10.01, 12.99), (19.0, 15.0)):
+        assert _interp(SINC, x, y, const) == pytest.approx(7.25, abs=1e-12)
     yy, xx = np.mgrid[0:20, 0:24]
     plane = (3.0 + 0.25 * xx - 0.5 * yy).astype(np.float64)
     for method in (BILINEAR, BICUBIC, BIQUINTIC):
@@ -148,7 +165,7 @@ def test_oracle_interpolators_exact_at_nodes_and_on_planes():
                                                                  abs=1e-12)
 
 
-@pytest.mark.parametrize("method", ["bilinear", "bicubic", "nearest", "biquintic"])
+@pytest.mark.parametrize("method", ["bilinear", "bicubic", "nearest", "biquintic", "sinc"])
 @pytest.mark.parametrize("epsg", [4326, 32631, 3413, 6933])
 def test_product_dem_lookup_matches_oracle(method, epsg):
     """DEMInterpolator.interpolate_lonlat / interpolate_xy of the product's

@@ -296,11 +296,11 @@ def _dem_variant(sc, method, epsg):
                                          ("nearest", 4326), ("biquintic", 32631),
                                          ("bilinear", 32631), ("bicubic", 6933),
                                          ("biquintic", 4978), ("bilinear", 3031),
-                                         ("biquintic", 3413)])
+                                         ("biquintic", 3413), ("sinc", 4326), ("sinc", 32631)])
 @pytest.mark.parametrize("precision", ["plain", "split"])
 def test_dem_methods_and_projections(torch_gpu, method, epsg, precision):
     """DEM rasters in UTM / EASE-Grid / geocentric coordinates and the
-    nearest / bicubic / biquintic interpolators (geometry/DEMInterpolator.cpp,
+    nearest / bicubic / biquintic / sinc interpolators (geometry/DEMInterpolator.cpp,
     core/Projections.cpp, core/*Interpolator.cpp) against the oracle."""
     import dataclasses
     sc, raw = scene_and_raw("tiny")
@@ -314,6 +314,15 @@ def test_dem_methods_and_projections(torch_gpu, method, epsg, precision):
     else:
         assert complex_rel_error(z, ref) <= 1e-3
         assert rms(phase_errors(z, ref)) <= 1e-3
-    np.testing.assert_allclose(h, href, atol=1e-3)
+    if method == "sinc":
+        # the sinc interpolant takes the nearest of 8192 fractional kernel rows
+        # (Sinc2dInterpolator.cpp:77-79), so the DEM is piecewise constant on
+        # steps of 1/8192 posting: FP64 coordinates that differ in the last
+        # bits between GPU and CPU put a few Brent roots on the neighbouring
+        # step, millimetres apart on this fractal terrain
+        dh = np.abs(h - href)
+        assert np.mean(dh <= 1e-3) >= 0.99 and dh.max() <= 2e-2, (np.mean(dh <= 1e-3), dh.max())
+    else:
+        np.testing.assert_allclose(h, href, atol=1e-3)
     # the DEM is in play: heights differ from the flat reference height
     assert np.nanstd(h) > 1.0
