@@ -1177,6 +1177,19 @@ int run_host(bp_context* c, const CallArgs& a, bp_out_block* blk, int64_t batch,
     hipStream_t s = static_cast<hipStream_t>(stream);
     HostPath& hp = c->hp;
     if ((rc = host_path_init(hp))) return rc;
+    // a call that fails after enqueueing work drains both streams before it
+    // returns, so that the next call never refills buffers still being read
+    struct Drain {
+        hipStream_t a, b;
+        bool armed = true;
+        ~Drain()
+        {
+            if (armed) {
+                (void)hipStreamSynchronize(a);
+                (void)hipStreamSynchronize(b);
+            }
+        }
+    } drain{s, hp.copy};
 
     const int n_flag = kFlagWords + 1;
     if ((rc = ensure(c->sc.flags, sizeof(int32_t) * n_flag))) return rc;
@@ -1359,6 +1372,7 @@ int run_host(bp_context* c, const CallArgs& a, bp_out_block* blk, int64_t batch,
                                 hipMemcpyDeviceToHost, s));
     BP_HIP_CHECK(hipStreamSynchronize(s));
     BP_HIP_CHECK(hipStreamSynchronize(hp.copy));
+    drain.armed = false;
     double sum_ms = 0., copy_ms = 0.;
     if (timed) {
         for (size_t b = 0; b < bands.size(); ++b) {
