@@ -271,7 +271,7 @@ __device__ inline PixSplit fit_pixel(const GeomArgs& a, const PixGeom& g, bool h
         const double h = fmax(0.5 * (double)(kw1 - 1 - kw0), 0.5);
         // delay at the reference pulse kc
         V3 pc, vc;
-        orbit_interpolate(a.in_orbit, a.in_t0 + kc * a.in_dt, kBorderExtrapolate, &pc, &vc);
+        orbit_interpolate<true>(a.in_orbit, a.in_t0 + kc * a.in_dt, kBorderExtrapolate, &pc, &vc);
         const double tau_c = g.tau_atm + bistatic_delay(pc, vc, x);
         constexpr int M = kFitDegree + 1;
         // Chebyshev nodes cos(pi (m + 1/2) / M) and T_j at the nodes
@@ -282,7 +282,7 @@ __device__ inline PixSplit fit_pixel(const GeomArgs& a, const PixGeom& g, bool h
         for (int m = 0; m < M; ++m) {
             const double km = kc + h * kNode[m];
             V3 pm, vm;
-            orbit_interpolate(a.in_orbit, a.in_t0 + km * a.in_dt, kBorderExtrapolate, &pm, &vm);
+            orbit_interpolate<true>(a.in_orbit, a.in_t0 + km * a.in_dt, kBorderExtrapolate, &pm, &vm);
             const double tau_m = g.tau_atm + bistatic_delay(pm, vm, x);
             gm[m] = a.fc * (tau_m - tau_c);  // cycles relative to the reference pulse
         }
