@@ -853,6 +853,13 @@ void build_step(Step& sp, const StaticData& st, const Scratch& sc, const CallArg
     sa.rdx = (float)(1.0 / (a.kernel->halfwidth / (a.kernel->n - 1.0)));
     sa.ntab = a.kernel->n;
     sa.chunk = o.chunk;
+    {
+        const double n = (double)o.chunk;
+        sa.fold_c[0] = n;
+        sa.fold_c[1] = n * (n - 1.) / 2.;
+        sa.fold_c[2] = n * (n - 1.) * (n - 2.) / 6.;
+        sa.fold_c[3] = n * (n - 1.) * (n - 2.) * (n - 3.) / 24.;
+    }
     sa.rc = static_cast<const float2*>(a.in_dev);
     sa.pulse_pv = static_cast<const double*>(st.pulse_pv.ptr);
     sa.geom = static_cast<const bp::PixGeom*>(sc.geom.ptr);

@@ -32,7 +32,8 @@ struct alignas(16) PixGeom {
 // Per-pixel record of the split-precision sum.  The pulse phase in cycles,
 // P(k), is a degree-4 polynomial in the pulse index fitted in FP64; the sum
 // kernel steps it pulse by pulse from the wavefront's first pulse kw0 by FP64
-// forward differences:  P += d1; d1 += d2; d2 += d3; d3 += d4.
+// forward differences, P += d1; d1 += d2, and every `chunk` pulses adds the
+// terms of d3 and d4 those steps left out (binomial coefficients of chunk).
 // The polyphase coordinate z = fma((float)P, zscale, zoff) >= 0, with the
 // global zscale = -Q/(fc dtau), gives the first tap c0 = cbase - (floor(z) >>
 // log2 Q) and the weight row floor(z) & (Q - 1) + fract(z).
@@ -197,6 +198,7 @@ struct SumArgs {
     int32_t exact_width;        // plain: 2 * halfwidth == W (no tap lies past the half width)
     double inv_dtau;            // 1 / dtau
     const double2* tab2;        // plain: (t_i, t_{i+1} - t_i) per table entry, as doubles
+    double fold_c[4];           // split: n, C(n,2), C(n,3), C(n,4) for n = chunk
 };
 
 // Host launchers (defined in bp_kernels.hip).  `grid` is the number of

@@ -105,6 +105,31 @@ def test_two_lines_per_lane_match_fp64_oracle(torch_gpu, config, tile_rows):
     assert np.array_equal(np.isnan(out), np.isnan(one))
 
 
+@pytest.mark.parametrize("chunk", [16, 32, 64])
+def test_fold_intervals_match_fp64_oracle(torch_gpu, chunk):
+    """The phase steps of the split sum add the third and fourth differences
+    once per fold interval (`chunk` pulses): every interval length the
+    autotuner may pick stays within the split tolerance of the all-FP64
+    oracle in both tilings, and the one-line kernels with staged and
+    gathered taps (which take each pulse's phase one and two pulses ahead of
+    accumulating it) still agree bit for bit."""
+    print(f"[fold interval] chunk={chunk}", flush=True)
+    ref, _, _, rc = oracle_run("small", 1)
+    for lane_rows in (1, 2):
+        out, _, err, _ = run_gpu("small", "split", options={"chunk": chunk, "lane_rows": lane_rows})
+        assert err == (rc == 6)
+        rel = complex_rel_error(out, ref)
+        ph = rms(phase_errors(out, ref))
+        print(f"  lane_rows={lane_rows} rel={rel:.3e} rms_phase={ph:.3e}", flush=True)
+        assert rel <= SPLIT_REL
+        assert ph <= SPLIT_PHS
+    staged, _, _, _ = run_gpu("small", "split", options={"chunk": chunk, "staged": 1,
+                                                          "tile_rows": 2})
+    gathered, _, _, _ = run_gpu("small", "split", options={"chunk": chunk, "staged": 0,
+                                                            "lane_rows": 1, "tile_rows": 2})
+    np.testing.assert_array_equal(staged, gathered)
+
+
 def test_two_lines_per_lane_rejected_where_unsupported(torch_gpu):
     from isce3_amd import errors
     with pytest.raises(errors.InvalidArgument):
