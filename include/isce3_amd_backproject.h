@@ -139,10 +139,19 @@ typedef struct {
     double time_start, time_end;
 } bp_geo2rdr_params;
 
+/* Largest `chunk`: between folds the split sum steps each pixel's phase by
+ * its first two forward differences and adds the third and fourth at the
+ * fold, so the phase it uses inside a fold interval of n pulses is short by
+ * up to C(n,3) d3 (2e-6 cycles at n = 64 for a spaceborne aperture; the term
+ * grows as n^3). */
+#define BP_MAX_CHUNK 64
+
 /* Launch/tiling options; zero-initialise for defaults. */
 typedef struct {
     int32_t tile_rows;     /* output rows per 64-pixel wavefront tile: 1,2,4,8 (0 = 1) */
-    int32_t chunk;         /* pulses per compensated FP32 partial sum (split; 0 = 32) */
+    int32_t chunk;         /* pulses per compensated FP32 partial sum and per
+                              phase-step interval (split; 0 = 32; at most
+                              BP_MAX_CHUNK) */
     int32_t fit_degree;    /* split: delay polynomial degree (0 = 4; only 4 is built) */
     int32_t order;         /* tile walk of the sum kernels: 0 auto, 1 row-major, 2 column-major */
     int32_t blocks_per_cu; /* persistent 512-thread workgroups per CU (0 = occupancy limit) */

@@ -5,7 +5,8 @@ The tunable options are those of ``bp_options`` (include/isce3_amd_backproject.h
     tile_rows      output lines per 64-pixel wavefront tile (1, 2, 4, 8)
     order          tile walk of the persistent split kernel (1 row-major,
                    2 column-major)
-    chunk          pulses per compensated FP32 partial sum (split)
+    chunk          pulses per compensated FP32 partial sum and per phase-step
+                   interval (split; at most 64)
     blocks_per_cu  resident 512-thread workgroups per CU (0 = occupancy limit)
     staged         split taps gathered from global memory (0) or read from
                    pulse windows staged in LDS (1)

@@ -637,7 +637,10 @@ int check_options(const bp_options* options, Opts* o)
     if (o->tile_rows != 1 && o->tile_rows != 2 && o->tile_rows != 4 && o->tile_rows != 8)
         return fail(BP_ERR_INVALID_ARGUMENT, "tile_rows must be 1, 2, 4 or 8");
     o->chunk = opt.chunk == 0 ? 32 : opt.chunk;
-    if (o->chunk < 1 || o->chunk > 4096) return fail(BP_ERR_INVALID_ARGUMENT, "chunk must be in [1, 4096]");
+    // the split sum steps the phase by its first two differences between folds:
+    // the third-difference term it leaves out grows as chunk^3
+    if (o->chunk < 1 || o->chunk > BP_MAX_CHUNK)
+        return fail(BP_ERR_INVALID_ARGUMENT, "chunk must be in [1, 64]");
     if (opt.fit_degree != 0 && opt.fit_degree != bp::kFitDegree)
         return fail(BP_ERR_INVALID_ARGUMENT, "fit_degree must be 4");
     if (opt.order < 0 || opt.order > 2) return fail(BP_ERR_INVALID_ARGUMENT, "order must be 0, 1 or 2");

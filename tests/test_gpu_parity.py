@@ -130,6 +130,16 @@ def test_fold_intervals_match_fp64_oracle(torch_gpu, chunk):
     np.testing.assert_array_equal(staged, gathered)
 
 
+def test_fold_interval_limit(torch_gpu):
+    """Fold intervals beyond 64 pulses (where the phase term the steps leave
+    out would grow as the cube of the interval) are refused."""
+    from isce3_amd import errors
+    with pytest.raises(errors.InvalidArgument, match="chunk"):
+        run_gpu("tiny", "split", options={"chunk": 65})
+    with pytest.raises(errors.InvalidArgument, match="chunk"):
+        run_gpu("tiny", "split", options={"chunk": -1})
+
+
 def test_two_lines_per_lane_rejected_where_unsupported(torch_gpu):
     from isce3_amd import errors
     with pytest.raises(errors.InvalidArgument):
