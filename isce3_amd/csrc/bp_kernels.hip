@@ -59,6 +59,16 @@ __device__ inline int64_t uniform64(int64_t v)
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+// A read-only buffer read at wave-uniform indices as a constant-address-space
+// pointer: its loads go through the scalar data cache into scalar registers
+// (one load per wavefront, no vector memory instruction).  Only for data no
+// kernel writes while the reading kernel runs.
+template<class T>
+__device__ inline const __attribute__((address_space(4))) T* uniform_ptr(const T* p)
+{
+    return (const __attribute__((address_space(4))) T*)(p);
+}
+
 // Output grid owning global wavefront tile s (s uniform across the wavefront).
 __device__ inline int find_grid(const OutDesc* g, int n, int64_t s)
 {
@@ -98,23 +108,24 @@ __device__ inline bool tile_pixel(const TileMap& m, int64_t tile, int lane, int 
 
 // TabulatedKernel<float>::operator() with the reference's FP64 index math
 // (core/Kernels.icc:137-154): axn = |x| * float(1/dx) in FP64, i = floor(axn),
-// K = t_i + (axn - i) (t_{i+1} - t_i) rounded to float.  The table holds each
-// entry as the pair (t_i, t_{i+1} - t_i) in FP64 (the difference formed in
-// float, as the reference does), so one 16-byte LDS read gives both; entry
-// n - 1 holds the value the reference's clamp to i = n - 2 gives at |x| = hw.
-// kCheck: x may lie past the half width (then K = 0, and the entry index is
-// clamped into the table).  Without it |x| <= hw, so axn <= (n - 1)(1 + 2^-23)
-// and floor(axn) <= n - 1 for tables of up to 2^20 entries.
+// K = t_i + (axn - i) (t_{i+1} - t_i) rounded to float.  The caller passes
+// a = x * float(1/dx) (signed); the table holds each entry as the pair
+// (t_i, t_{i+1} - t_i) in FP64 (the difference formed in float, as the
+// reference does), so one 16-byte read gives both; entry n - 1 holds the
+// value the reference's clamp to i = n - 2 gives at |x| = hw.
+// kCheck: x may lie past the half width (|a| > an_max = hw * float(1/dx):
+// then K = 0, and the entry index is clamped into the table).  Without it
+// |x| <= hw, so axn <= (n - 1)(1 + 2^-23) and floor(axn) <= n - 1 for tables
+// of up to 2^20 entries.
 template<bool kCheck>
-__device__ inline float tab_weight(const double2* tab2, int ntab, double hw, double rdx, double x)
+__device__ inline float tab_weight(const double2* tab2, int ntab, double an_max, double a)
 {
-    const double ax = fabs(x);
-    const double axn = ax * rdx;
+    const double axn = fabs(a);
     int i = (int)axn;  // axn >= 0: truncation is floor
     if (kCheck) i = min(i, ntab - 1);
     const double2 e = tab2[i];
     const float w = (float)fma(__builtin_amdgcn_fract(axn), e.y, e.x);
-    return kCheck && ax > hw ? 0.f : w;
+    return kCheck && axn > an_max ? 0.f : w;
 }
 
 }  // namespace
@@ -251,11 +262,37 @@ __global__ __launch_bounds__(256) void geometry_kernel(GeomArgs a)
     geometry_tile<kGenericDem>(a, grids, n_grids, n_geom, lane_rows, gw, lane);
 }
 
+constexpr int kFitNodes = kFitDegree + 1;
+
+// Chebyshev nodes cos(pi (m + 1/2) / M), M = 5
+__device__ inline double fit_node(int m)
+{
+    return m == 0 ? 0.95105651629515353
+                  : m == 1 ? 0.58778525229247314
+                           : m == 2 ? 0.0 : m == 3 ? -0.58778525229247314 : -0.95105651629515353;
+}
+
+__device__ inline double readlane_f64(double x, int lane)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)u, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ inline V3 readlane_v3(V3 x, int lane)
+{
+    return {readlane_f64(x.x, lane), readlane_f64(x.y, lane), readlane_f64(x.z, lane)};
+}
+
 // Split-precision record of one pixel: the pulse phase in cycles fitted
 // over the sum wavefront's union window [kw0, kw1) with a degree-4 Chebyshev
 // interpolant evaluated in FP64, re-expanded as forward differences at kw0.
+// tau_c: the delay at the reference pulse kc; gm[m]: the phase in cycles at
+// Chebyshev node m relative to the reference pulse's.
 __device__ inline PixSplit fit_pixel(const GeomArgs& a, const PixGeom& g, bool has_win, int kw0,
-                                     int kw1, double q_rows)
+                                     int kw1, double q_rows, double tau_c,
+                                     const double (&gm)[kFitNodes])
 {
     const V3 x{g.x[0], g.x[1], g.x[2]};
     PixSplit sp;
@@ -269,23 +306,10 @@ __device__ inline PixSplit fit_pixel(const GeomArgs& a, const PixGeom& g, bool h
     if (has_win) {
         const double kc = 0.5 * ((double)kw0 + (double)(kw1 - 1));
         const double h = fmax(0.5 * (double)(kw1 - 1 - kw0), 0.5);
-        // delay at the reference pulse kc
-        V3 pc, vc;
-        orbit_interpolate<true>(a.in_orbit, a.in_t0 + kc * a.in_dt, kBorderExtrapolate, &pc, &vc);
-        const double tau_c = g.tau_atm + bistatic_delay(pc, vc, x);
-        constexpr int M = kFitDegree + 1;
+        constexpr int M = kFitNodes;
         // Chebyshev nodes cos(pi (m + 1/2) / M) and T_j at the nodes
         constexpr double kNode[M] = {0.95105651629515353, 0.58778525229247314, 0.0,
                                      -0.58778525229247314, -0.95105651629515353};
-        double gm[M];
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-            const double km = kc + h * kNode[m];
-            V3 pm, vm;
-            orbit_interpolate<true>(a.in_orbit, a.in_t0 + km * a.in_dt, kBorderExtrapolate, &pm, &vm);
-            const double tau_m = g.tau_atm + bistatic_delay(pm, vm, x);
-            gm[m] = a.fc * (tau_m - tau_c);  // cycles relative to the reference pulse
-        }
         // Chebyshev coefficients c_j = 2/M sum_m g_m T_j(s_m), c_0 halved,
         // with T_j(s_m) = cos(j pi (m + 1/2) / M).
         double cc[M];
@@ -393,32 +417,69 @@ __device__ inline void fit_tile(const GeomArgs& a, const OutDesc* grids, int n_g
 {
     const bool in_seq = gseq < n_seq;
     const OutDesc d = load_desc(grids, in_seq ? find_grid(grids, n_grids, gseq) : 0);
-    const int lr = d.map.lane_rows;
+    const int lr = d.map.lane_rows;  // 1 or 2
+    bool valid[2] = {false, false}, has_win[2] = {false, false};
+    int64_t rec[2] = {0, 0};
+    PixGeom g[2];
     int ks = INT_MAX, ke = INT_MIN;
-    for (int j = 0; j < lr; ++j) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
         int64_t row = 0, col = 0;
-        if (in_seq && tile_pixel(d.map, gseq - d.seq_base, lane, j, &row, &col)) {
-            const PixGeom& g = a.geom[d.rec_base + row * d.map.width + col];
-            if (g.status == kPixOk && g.kstop > g.kstart) {
-                ks = min(ks, g.kstart);
-                ke = max(ke, g.kstop);
+        valid[j] = j < lr && in_seq && tile_pixel(d.map, gseq - d.seq_base, lane, j, &row, &col);
+        rec[j] = d.rec_base + row * d.map.width + col;
+        if (valid[j]) {
+            g[j] = a.geom[rec[j]];
+            has_win[j] = g[j].status == kPixOk && g[j].kstop > g[j].kstart;
+            if (has_win[j]) {
+                ks = min(ks, g[j].kstart);
+                ke = max(ke, g[j].kstop);
+            }
+        } else {
+            g[j].x[0] = g[j].x[1] = g[j].x[2] = 0.;
+            g[j].tau_atm = 0.;
+        }
+    }
+    const int kw0 = __builtin_amdgcn_readfirstlane(wave_min_i(ks));
+    const int kw1 = __builtin_amdgcn_readfirstlane(wave_max_i(ke));
+    double tau_c[2] = {0., 0.};
+    double gm[2][kFitNodes] = {};
+    if (kw0 < kw1) {
+        // The platform states at the reference pulse kc and the Chebyshev
+        // nodes kc + h s_m are the same for every pixel of the tile: one orbit
+        // evaluation per lane on lanes 0-5 (lane m < 5: node m, lane 5: kc),
+        // each state read back from its lane in turn.
+        const double kc = 0.5 * ((double)kw0 + (double)(kw1 - 1));
+        const double h = fmax(0.5 * (double)(kw1 - 1 - kw0), 0.5);
+        const int m = min(lane, kFitNodes);
+        const double km = m == kFitNodes ? kc : kc + h * fit_node(m);
+        V3 p, v;
+        orbit_interpolate<true>(a.in_orbit, a.in_t0 + km * a.in_dt, kBorderExtrapolate, &p, &v);
+        {
+            const V3 pc = readlane_v3(p, kFitNodes), vc = readlane_v3(v, kFitNodes);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                if (j < lr)
+                    tau_c[j] =
+                        g[j].tau_atm + bistatic_delay(pc, vc, V3{g[j].x[0], g[j].x[1], g[j].x[2]});
+        }
+#pragma unroll
+        for (int n = 0; n < kFitNodes; ++n) {
+            const V3 pm = readlane_v3(p, n), vm = readlane_v3(v, n);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                if (j >= lr) continue;
+                const double tau_m =
+                    g[j].tau_atm + bistatic_delay(pm, vm, V3{g[j].x[0], g[j].x[1], g[j].x[2]});
+                gm[j][n] = a.fc * (tau_m - tau_c[j]);  // cycles relative to the reference pulse
             }
         }
     }
-    const int kw0 = wave_min_i(ks);
-    const int kw1 = wave_max_i(ke);
     // polyphase rows of the weight table the sum reads for this tiling
     const double q_rows = lr == 2 ? (double)(1 << pair_q_log2(a.kwidth)) : (double)kPolyphaseRows;
-#pragma unroll 1
-    for (int j = 0; j < lr; ++j) {
-        int64_t row = 0, col = 0;
-        if (in_seq && tile_pixel(d.map, gseq - d.seq_base, lane, j, &row, &col)) {
-            const int64_t rec = d.rec_base + row * d.map.width + col;
-            const PixGeom g = a.geom[rec];
-            const bool has_win = g.status == kPixOk && g.kstop > g.kstart;
-            a.split[rec] = fit_pixel(a, g, has_win, kw0, kw1, q_rows);
-        }
-    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+        if (valid[j])
+            a.split[rec[j]] = fit_pixel(a, g[j], has_win[j], kw0, kw1, q_rows, tau_c[j], gm[j]);
 }
 
 // Split-precision records of a batch, one wavefront per sum tile.
@@ -487,11 +548,15 @@ __device__ inline void for_each_tile(const BatchHeader* batch, Body&& body)
 //
 // Per pulse, everything the W taps share is formed once: the delay from the
 // pulse record (its 2/(|v|^2 - c^2) factor precomputed), u by a product with
-// 1/dtau, and the first tap's offset x0 = low - u; tap m is then at x0 + m
-// (exact: both terms are within a few bins of each other for u >= W, so the
-// differences are exact, as the reference's (m + low) - u is).  The phase
-// 2 pi fc tau comes from sincos_turns of fc tau in FP64.
+// 1/dtau, and the first tap's offset x0 = low - u (exact, as the reference's
+// (m + low) - u: both terms are within a few bins of each other for u >= W);
+// tap m's table coordinate is then x0 * float(1/dx) + m * float(1/dx) (the
+// reference rounds (x0 + m) * float(1/dx) once instead of twice: the index
+// and fraction agree to an ulp of the coordinate).  The phase 2 pi fc tau
+// comes from sincos_turns of fc tau in FP64.
 // kExact: 2 * halfwidth == W, so no tap lies past the half width.
+// `tab2` is the table in LDS or in global memory: each caller passes one of
+// the two, so that the reads compile to LDS or global loads.
 template<int W, bool kExact>
 __device__ inline void plain_tile(const SumArgs& a, const double2* tab2, const OutDesc& od,
                                   int64_t tile, int lane)
@@ -510,28 +575,30 @@ __device__ inline void plain_tile(const SumArgs& a, const double2* tab2, const O
     const V3 x{g.x[0], g.x[1], g.x[2]};
     const double tau_atm = ok ? g.tau_atm : 0.;
     const double nr_d = (double)a.nr;
-    const double hw = a.halfwidth, rdx = (double)a.rdx;
+    const double rdx = (double)a.rdx;
+    const double an_max = a.halfwidth * rdx;
     const int ntab = a.ntab;
 
     double acc_re = 0., acc_im = 0.;
     for (int k = kw0; k < kw1; ++k) {
         if (k < ks || k >= ke) continue;
-        const double* pv = a.pulse_pv + kPulseRecDoubles * (int64_t)k;
+        // the pulse record (k is wave-uniform) through the scalar cache
+        const auto* pv = uniform_ptr(a.pulse_pv) + kPulseRecDoubles * (int64_t)k;
         const V3 r{x.x - pv[0], x.y - pv[1], x.z - pv[2]};
         const double rv = r.x * pv[3] + r.y * pv[4] + r.z * pv[5];
-        const double rn = sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+        const double rn = sqrt_normal(r.x * r.x + r.y * r.y + r.z * r.z);
         const double tau = tau_atm + (rv - kC * rn) * pv[6];
         const double u = (tau - a.swst) * a.inv_dtau;
         const double low = ((W % 2 == 0) ? ceil(u) : round(u)) - (double)(W / 2);
-        const double x0 = low - u;
+        const double a0 = (low - u) * rdx;
         float w[W];
 #pragma unroll
         for (int m = 0; m < W; ++m) {
-            const double xm = x0 + (double)m;
+            const double am = a0 + (double)m * rdx;  // m * rdx exact
             // only the end taps can reach past a half width below W / 2
             w[m] = (!kExact && (m == 0 || m == W - 1))
-                       ? tab_weight<true>(tab2, ntab, hw, rdx, xm)
-                       : tab_weight<false>(tab2, ntab, hw, rdx, xm);
+                       ? tab_weight<true>(tab2, ntab, an_max, am)
+                       : tab_weight<false>(tab2, ntab, an_max, am);
         }
         float sre = 0.f, sim = 0.f;
         if (low >= 0. && low + (double)W <= nr_d) {
@@ -582,7 +649,6 @@ __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
     if (tab_in_lds)
         for (int i = threadIdx.x; i < a.ntab; i += blockDim.x) s_tab2[i] = a.tab2[i];
     __syncthreads();
-    const double2* tab2 = tab_in_lds ? s_tab2 : a.tab2;
     const OutDesc* grids = batch_grids(a.batch);
     const int n_grids = __builtin_amdgcn_readfirstlane(a.batch->n_grids);
     const int64_t n_seq = uniform64(a.batch->n_seq);
@@ -590,10 +656,18 @@ __global__ __launch_bounds__(256) void sum_plain_kernel(SumArgs a)
     const int64_t gseq = group_tile(blockIdx.x, ng, &lane);
     if (gseq < n_seq) {
         const OutDesc d = load_desc(grids, find_grid(grids, n_grids, gseq));
-        if (a.exact_width)
-            plain_tile<W, true>(a, tab2, d, gseq - d.seq_base, lane);
-        else
-            plain_tile<W, false>(a, tab2, d, gseq - d.seq_base, lane);
+        const int64_t t = gseq - d.seq_base;
+        if (tab_in_lds) {
+            if (a.exact_width)
+                plain_tile<W, true>(a, s_tab2, d, t, lane);
+            else
+                plain_tile<W, false>(a, s_tab2, d, t, lane);
+        } else {
+            if (a.exact_width)
+                plain_tile<W, true>(a, a.tab2, d, t, lane);
+            else
+                plain_tile<W, false>(a, a.tab2, d, t, lane);
+        }
     }
 }
 

@@ -970,6 +970,43 @@ BP_HD double bistatic_delay(V3 p, V3 v, V3 x)
     return 2. * (dot(r, v) - kC * norm(r)) / (dot(v, v) - (kC * kC));
 }
 
+// fma(a, b, c) as one v_fma_f64 on the device.  The compiler's two-address
+// form of an FMA (v_fmac_f64, the result written over the addend) costs a copy
+// of the addend when the addend is a constant kept in registers across loop
+// trips, as the coefficients of a polynomial in a loop are: one 64-bit move
+// per term.
+BP_HD double fma_untied(double a, double b, double c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+#else
+    return fma(a, b, c);
+#endif
+}
+
+// sqrt(x) for x well inside the normal range (x > 2^-900, finite): the
+// reciprocal square root seed and two Newton steps of the correctly rounded
+// FP64 square root, without its rescaling of tiny and special inputs.
+BP_HD double sqrt_normal(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = 0.5 * y;
+    const double e = fma(-h, g, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+#else
+    return sqrt(x);
+#endif
+}
+
 // sin(2 pi t) and cos(2 pi t) in FP64 for a phase t in turns.  t is reduced
 // to r = t - q/4 with q the nearest integer to 4t (exact for |t| < 2^50), so
 // that |2 pi r| <= pi/4; sin and cos of x = 2 pi r come from their Taylor
@@ -983,28 +1020,28 @@ BP_HD void sincos_turns(double t, double* s, double* c)
     const double x = r * 6.283185307179586476925286766559;
     const double x2 = x * x;
     // sin x = x (1 + x^2 (-1/3! + x^2 (1/5! - ...)))
-    double ps = 2.8114572543455207632e-15;          //  1/17!
-    ps = fma(ps, x2, -7.6471637318198164759e-13);   // -1/15!
-    ps = fma(ps, x2, 1.6059043836821614599e-10);    //  1/13!
-    ps = fma(ps, x2, -2.5052108385441718775e-08);   // -1/11!
-    ps = fma(ps, x2, 2.7557319223985890653e-06);    //  1/9!
-    ps = fma(ps, x2, -1.9841269841269841270e-04);   // -1/7!
-    ps = fma(ps, x2, 8.3333333333333333333e-03);    //  1/5!
-    ps = fma(ps, x2, -1.6666666666666666667e-01);   // -1/3!
+    double ps = 2.8114572543455207632e-15;                  //  1/17!
+    ps = fma_untied(ps, x2, -7.6471637318198164759e-13);   // -1/15!
+    ps = fma_untied(ps, x2, 1.6059043836821614599e-10);    //  1/13!
+    ps = fma_untied(ps, x2, -2.5052108385441718775e-08);   // -1/11!
+    ps = fma_untied(ps, x2, 2.7557319223985890653e-06);    //  1/9!
+    ps = fma_untied(ps, x2, -1.9841269841269841270e-04);   // -1/7!
+    ps = fma_untied(ps, x2, 8.3333333333333333333e-03);    //  1/5!
+    ps = fma_untied(ps, x2, -1.6666666666666666667e-01);   // -1/3!
     const double sx = fma(x * x2, ps, x);
     // cos x = 1 + x^2 (-1/2! + x^2 (1/4! - ...))
-    double pc = 1.5619206968586225686e-16;          //  1/18!
-    pc = fma(pc, x2, -4.7794773323873852974e-14);   // -1/16!
-    pc = fma(pc, x2, 1.1470745597729724714e-11);    //  1/14!
-    pc = fma(pc, x2, -2.0876756987868098979e-09);   // -1/12!
-    pc = fma(pc, x2, 2.7557319223985890653e-07);    //  1/10!
-    pc = fma(pc, x2, -2.4801587301587301587e-05);   // -1/8!
-    pc = fma(pc, x2, 1.3888888888888888889e-03);    //  1/6!
-    pc = fma(pc, x2, -4.1666666666666666667e-02);   //  1/4!
-    pc = fma(pc, x2, 0.5);                          // (sign folded below)
+    double pc = 1.5619206968586225686e-16;                  //  1/18!
+    pc = fma_untied(pc, x2, -4.7794773323873852974e-14);   // -1/16!
+    pc = fma_untied(pc, x2, 1.1470745597729724714e-11);    //  1/14!
+    pc = fma_untied(pc, x2, -2.0876756987868098979e-09);   // -1/12!
+    pc = fma_untied(pc, x2, 2.7557319223985890653e-07);    //  1/10!
+    pc = fma_untied(pc, x2, -2.4801587301587301587e-05);   // -1/8!
+    pc = fma_untied(pc, x2, 1.3888888888888888889e-03);    //  1/6!
+    pc = fma_untied(pc, x2, -4.1666666666666666667e-02);   //  1/4!
+    pc = fma(pc, x2, 0.5);                                  // (sign folded below)
     const double cx = fma(-x2, pc, 1.0);
-    // quadrant: (sin, cos) of x + q pi/2
-    const int64_t iq = (int64_t)q & 3;
+    // quadrant: (sin, cos) of x + q pi/2, q mod 4 formed exactly in FP64
+    const int iq = (int)fma(-4.0, floor(0.25 * q), q);
     const double sa = (iq & 1) ? cx : sx;
     const double ca = (iq & 1) ? sx : cx;
     *s = (iq & 2) ? -sa : sa;
