@@ -17,8 +17,10 @@
 // pybind focus/Presum.cpp:26-66 do (the deramp from the phase in turns,
 // reduced in FP64, by the hardware sine and cosine, where the reference
 // rounds the phase in radians to float and calls sinf/cosf).  Consecutive
-// output pulses share most of their input pulses, so a thread's pulses
-// re-read them from cache.
+// output pulses share most of their input pulses: a thread loads its bin of
+// every input pulse its output pulses reach (up to 12) at once, into its own
+// column of an LDS array, and reads the windows from there (longer spans
+// read them from global memory).
 //
 // The output pulses go in launches of 512, then 2048: host threads plan the
 // next launches' patterns and weights while the GPU resamples the earlier
@@ -209,6 +211,10 @@ constexpr int64_t kChunk = 2048;       // of the others (multiples of kPulsesPer
 constexpr int64_t kItem = 32;          // output pulses a host thread plans at a time
 constexpr int kTapGroup = 4;           // window taps loaded together (longer windows)
 constexpr int kFixedTaps = 8;          // windows up to this many taps: unrolled per count
+constexpr int kStageSpan = 12;         // input pulses a thread stages in LDS
+// readable bytes after each uploaded table: the sub-swath bounds of
+// kStageSpan pulses from the last one
+constexpr size_t kPartSlack = 2 * kStageSpan * sizeof(int32_t);
 
 // complex64 product and sum rounded term by term (no fused multiply-add)
 __device__ inline float2 cmul(float2 a, float2 b)
@@ -290,7 +296,7 @@ __device__ inline float2 window_sum(const float2* row, int64_t stride, uint32_t 
     double tk[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-        x[k] = (row + k * stride)[jj];
+        x[k] = row[k * stride + jj];
         wv[k] = uniform_pat ? w0[k] : w[k];
         tk[k] = tp[k];
     }
@@ -330,10 +336,44 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
         pend = max(pend, desc[i].offset + (int64_t)desc[i].nw);
     }
     const int64_t span = pend - pfirst;
+    // this thread's bin of every input pulse of the span, staged in LDS (its
+    // own column: no other thread reads it, no barrier) with all loads in
+    // flight together, so that the windows read it at LDS latency
+    __shared__ float2 s_raw[kStageSpan * kThreads];
+    const bool staged = span <= kStageSpan;
+    if (staged) {
+        const float2* col = a.raw + pfirst * a.width + j;
+        float2 v[kStageSpan];
+#pragma unroll
+        for (int p = 0; p < kStageSpan; ++p)
+            if (p < span) v[p] = col[(int64_t)p * a.width];
+#pragma unroll
+        for (int p = 0; p < kStageSpan; ++p)
+            if (p < span) s_raw[p * kThreads + threadIdx.x] = v[p];
+    }
     uint64_t bits = 0;
-    if (span <= 64)
+    if (staged) {
+        // each sub-swath's bounds for the span in two wide scalar loads (the
+        // table is padded so that kStageSpan pulses from any pulse are
+        // readable; bounds past the span are ignored)
+        const int32_t j32 = (int32_t)jj;
+        uint32_t b = 0;
+        for (int64_t sub = 0; sub < a.ns; ++sub) {
+            const auto* sw = uniform_ptr(a.swaths) + (sub * a.nt + pfirst) * 2;
+            int32_t lim[2 * kStageSpan];
+#pragma unroll
+            for (int k = 0; k < 2 * kStageSpan; ++k) lim[k] = sw[k];
+#pragma unroll
+            for (int p = 0; p < kStageSpan; ++p) {
+                const bool in = (j32 >= lim[2 * p]) & (j32 < lim[2 * p + 1]);
+                b |= in ? 1u << p : 0u;
+            }
+        }
+        bits = b & ((1u << span) - 1u);
+    } else if (span <= 64) {
         for (int64_t p = 0; p < span; ++p)
             if (bin_valid(a, j, pfirst + p)) bits |= (uint64_t)1 << p;
+    }
     for (int64_t i = i0; i < i1; ++i) {
         const PulseDesc d = desc[i];
         // pattern of this range bin over the window
@@ -371,7 +411,19 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a)
         const auto* tp = uniform_ptr(d.t_row);
         const float* w = d.w + (int64_t)max(pat, 0) * d.nw;
         const auto* w0 = uniform_ptr(d.w) + (int64_t)max(pat0, 0) * d.nw;
-        if (pat >= 0 && d.nw <= kFixedTaps) {
+        if (pat >= 0 && d.nw <= kFixedTaps && staged) {
+            const float2* srow = s_raw + (d.offset - pfirst) * kThreads;
+            switch (d.nw) {
+#define PRESUM_TAPS(N)                                                                        \
+    case N:                                                                                   \
+        sum = window_sum<N>(srow, kThreads, threadIdx.x, w, w0, uniform_pat, tp, d.tout, fd); \
+        break;
+                PRESUM_TAPS(1) PRESUM_TAPS(2) PRESUM_TAPS(3) PRESUM_TAPS(4)
+                PRESUM_TAPS(5) PRESUM_TAPS(6) PRESUM_TAPS(7) PRESUM_TAPS(8)
+#undef PRESUM_TAPS
+            default: break;
+            }
+        } else if (pat >= 0 && d.nw <= kFixedTaps) {
             // windows of up to kFixedTaps pulses with the tap count fixed at
             // compile time
             switch (d.nw) {
@@ -533,15 +585,17 @@ struct Staging {
     size_t capacity = 0;
     size_t pending = 0;   // bytes placed by the last reserve()
     hipEvent_t ready = nullptr;   // the last upload's completion
-    // Places the parts at 256-byte aligned offsets (into off), growing the
-    // buffers if needed (the current device is `device`); then send() packs
-    // and enqueues the upload on s.
+    // Places the parts at 256-byte aligned offsets (into off), each followed
+    // by at least kPartSlack bytes inside the buffer (so that a fixed-size
+    // read may run past the end of a table), growing the buffers if needed
+    // (the current device is `device`); then send() packs and enqueues the
+    // upload on s.
     hipError_t reserve(const Part* parts, int n, size_t* off)
     {
         size_t total = 0;
         for (int k = 0; k < n; ++k) {
             off[k] = total;
-            total += (parts[k].bytes + 255) / 256 * 256;
+            total += (parts[k].bytes + kPartSlack + 255) / 256 * 256;
         }
         pending = total;
         if (total == 0 || total <= capacity) return hipSuccess;
