@@ -573,11 +573,16 @@ def run_presum(args, torch, np, dev):
     gen.manual_seed(19)
     raw = torch.randn((nt, nr), dtype=torch.complex64, device=dev, generator=gen)
     stream = torch.cuda.current_stream()
+    # each call's result is dropped before the next call, as a pipeline would,
+    # so that every call reuses the same output block of torch's allocator
+    out = None
     for _ in range(max(args.warmup, 1)):
+        out = None
         out = resample(raw, t, grid, sw, orbit, lut)
     torch.cuda.synchronize()
     times = []
     for _ in range(args.steps):
+        out = None
         t0 = time.perf_counter()
         out = resample(raw, t, grid, sw, orbit, lut)
         torch.cuda.synchronize()
