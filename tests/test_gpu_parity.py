@@ -158,6 +158,28 @@ def test_two_lines_per_lane_odd_kernel_width(torch_gpu):
     assert rms(phase_errors(out, ref)) <= SPLIT_PHS
 
 
+@pytest.mark.parametrize("entries", [4096, 8192])
+def test_plain_kernel_tables_in_lds_and_global_memory(torch_gpu, entries):
+    """The plain sum copies tables of up to 4096 entries to LDS and reads
+    larger ones from global memory (its two instantiations): both against the
+    oracle with the same table."""
+    from isce3_amd.core import KnabKernel, TabulatedKernelF32
+    from isce3_amd.focus import backproject
+    import oracle
+    sc, raw = scene_and_raw("tiny")
+    kern = TabulatedKernelF32(KnabKernel(8.0, 0.8), entries)
+    print(f"[plain table] entries={entries}", flush=True)
+    out = np.empty(sc.out_grid.shape, np.complex64)
+    err = backproject(out, sc.out_geometry(), raw, sc.in_geometry(), sc.dem, sc.fc, sc.ds,
+                      kern, sc.dry_tropo_model, precision="plain")
+    ref, _, _, rc = oracle.backproject(sc.out_geometry(), raw, sc.in_geometry(), sc.dem, sc.fc,
+                                       sc.ds, kern, sc.dry_tropo_model, mode=0)
+    assert err == (rc == 6)
+    rel, ph = complex_rel_error(out, ref), rms(phase_errors(out, ref))
+    print(f"  rel={rel:.3e} rms_phase={ph:.3e}")
+    assert rel <= PLAIN_REL and ph <= PLAIN_PHS
+
+
 @pytest.mark.parametrize("width", [4, 7, 11, 16])
 @pytest.mark.parametrize("precision,lane_rows", [("plain", 1), ("split", 1), ("split", 2)])
 def test_kernel_widths_match_oracle(torch_gpu, width, precision, lane_rows):

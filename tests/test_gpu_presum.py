@@ -105,6 +105,24 @@ def test_resample_matches_oracle(torch_gpu, dither, gap, ns, walk):
         assert np.all(ref[:, 120:120 + gap] == 0)
 
 
+@pytest.mark.parametrize("L", [6.0, 24.0])
+def test_resample_window_lengths_match_oracle(torch_gpu, L):
+    """Short windows (L = 6 m: ~3 pulses, every thread's input span staged in
+    LDS) and long ones (L = 24 m: ~10 pulses, spans longer than the 12 pulses
+    the kernel stages, taps read from global memory in groups) against the
+    oracle."""
+    from isce3_amd.presum import resample
+    grid, orbit, lut, t, sw, raw, fd, speed = _dithered_scene(dither=0.2, gap=16, ns=2,
+                                                              seed=int(L), walk=True)
+    out = resample(raw, t, grid, sw, orbit, lut, L=L)
+    ref = po.resample(raw, t, grid.prf, grid.starting_range, grid.range_pixel_spacing, sw,
+                      lambda tout: speed, fd, L=L)
+    e = rel(out, ref)
+    print(f"[presum window] L={L} rel={e:.2e}", flush=True)
+    assert e <= 1e-5
+    assert np.array_equal(out == 0, ref == 0)
+
+
 def test_resample_across_launch_chunks(torch_gpu):
     """4500 output pulses run as launches of 512, 2048 and 1940 pulses
     (planned on host threads while the GPU works): the pulses either side of
