@@ -22,9 +22,10 @@
 // column of an LDS array, and reads the windows from there (longer spans
 // read them from global memory).
 //
-// The output pulses go in launches of 512, then 2048: host threads plan the
-// next launches' patterns and weights while the GPU resamples the earlier
-// ones, and the launches' tables go up on a copy stream beside them.
+// The output pulses go in launches of 512, then 2048: host threads (a pool
+// that lives across calls) plan the next launches' patterns and weights
+// while the GPU resamples the earlier ones, and the launches' tables go up
+// on a copy stream beside them.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,6 +33,8 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -678,6 +681,86 @@ StagingPool& staging_pool()
     return *p;
 }
 
+// Host threads that plan the items of every call: created when a call first
+// wants them (each inherits that caller's CPU affinity), never freed; between
+// calls they wait on a condition variable.  A call posts its job; every idle
+// thread takes items of the oldest job that has items left, by the job's
+// atomic counter, alongside the calling thread.  Before it returns, the call
+// retracts its job: no thread starts one of its items afterwards, and the
+// call waits for the threads still inside one.
+struct PlanJob {
+    std::function<void(int64_t, SolveScratch&)> run;
+    std::atomic<int64_t>* next = nullptr;
+    int64_t nitems = 0;
+    int active = 0;        // pool threads inside the job (guarded by the pool's mutex)
+    bool queued = false;   // in the pool's queue (guarded by the pool's mutex)
+};
+
+struct PlanPool {
+    std::mutex mu;
+    std::condition_variable work;   // a job was posted
+    std::condition_variable left;   // a thread left a job
+    std::deque<PlanJob*> jobs;
+    int threads = 0;
+
+    // At least n threads (fewer if no more can start).
+    void ensure(int n)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        while (threads < n) {
+            try {
+                std::thread(&PlanPool::loop, this).detach();
+            } catch (...) {
+                break;
+            }
+            ++threads;
+        }
+    }
+    void post(PlanJob* j)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            j->queued = true;
+            jobs.push_back(j);
+        }
+        work.notify_all();
+    }
+    void unqueue(PlanJob* j)  // mu held
+    {
+        if (!j->queued) return;
+        jobs.erase(std::find(jobs.begin(), jobs.end(), j));
+        j->queued = false;
+    }
+    void retract(PlanJob* j)
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        unqueue(j);
+        left.wait(lk, [&] { return j->active == 0; });
+    }
+    void loop()
+    {
+        SolveScratch sc;
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            work.wait(lk, [&] { return !jobs.empty(); });
+            PlanJob* j = jobs.front();
+            ++j->active;
+            lk.unlock();
+            for (int64_t k = (*j->next)++; k < j->nitems; k = (*j->next)++) j->run(k, sc);
+            lk.lock();
+            unqueue(j);  // no items left
+            --j->active;
+            left.notify_all();
+        }
+    }
+};
+
+PlanPool& plan_pool()
+{
+    static auto* p = new PlanPool();   // lives until the process ends
+    return *p;
+}
+
 // The staging buffers one call holds; they go back to the pool once the
 // call's stream and the copy stream have finished with them.
 struct Leases {
@@ -807,13 +890,14 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
     }
     first_item[nchunk] = (int64_t)items.size();
     const int64_t nitems = (int64_t)items.size();
-    // planners: one per CPU the calling thread may run on (threads inherit
-    // its affinity, so a caller pinned to one CPU plans alone), at most 16
+    // planners: the calling thread and pool threads, as many as the CPUs
+    // the calling thread may run on, at most 16 (a caller pinned to one CPU
+    // plans alone; pool threads started for one call serve later ones too)
     const int nworkers = (int)std::max<int64_t>(
         0, std::min<int64_t>({(int64_t)bp::usable_cpus(), 16, nitems}) - 1);
     std::vector<Patterns> item_pat(nitems);
     std::vector<int64_t> done(nchunk, 0);
-    std::vector<SolveScratch> scratch(nworkers + 1);   // the last one is this thread's
+    SolveScratch own_scratch;   // this thread's
     std::mutex mu;
     std::condition_variable cv;
     std::atomic<bool> abort{false};
@@ -830,28 +914,27 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
         }
         cv.notify_all();
     };
-    std::vector<std::thread> pool;
-    for (int p = 0; p < nworkers; ++p) {
-        try {
-            pool.emplace_back([&, p] {
-                for (int64_t k = next++; k < nitems; k = next++) run_item(k, scratch[p]);
-            });
-        } catch (...) {
-            break;  // fewer threads: this thread takes more items
-        }
+    PlanPool& planners = plan_pool();
+    PlanJob job;
+    job.run = run_item;
+    job.next = &next;
+    job.nitems = nitems;
+    if (nworkers > 0) {
+        planners.ensure(nworkers);
+        planners.post(&job);
     }
 
     // every failure below stops the planners and waits for the launches
     // already made; the staging buffers go back to the pool after the wait
-    // (destructors), and an exception still stops and joins the planners
-    struct Joiner {
-        std::vector<std::thread>& pool;
+    // (destructors), and an exception still retracts the job
+    struct Retractor {
+        PlanPool& pool;
+        PlanJob& job;
         std::atomic<bool>& abort;
-        ~Joiner()
+        ~Retractor()
         {
             abort = true;
-            for (auto& th : pool)
-                if (th.joinable()) th.join();
+            pool.retract(&job);
         }
     };
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -859,11 +942,10 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
     hipStream_t copy = device_set ? staging_pool().copy_stream(device) : nullptr;
     if (copy == nullptr) copy = s;
     Leases leases{s, copy, {}};
-    Joiner joiner{pool, abort};
+    Retractor retractor{planners, job, abort};
     auto finish = [&](int code, const std::string& msg) {
         abort = true;
-        for (auto& th : pool) th.join();
-        pool.clear();
+        planners.retract(&job);
         const hipError_t e = hipStreamSynchronize(s);
         if (code == BP_SUCCESS && e != hipSuccess)
             return fail(BP_ERR_HIP, std::string("presum resample: ") + hipGetErrorString(e));
@@ -908,7 +990,7 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
             }
             const int64_t k = next++;
             if (k < nitems) {
-                run_item(k, scratch[nworkers]);
+                run_item(k, own_scratch);
                 continue;
             }
             std::unique_lock<std::mutex> lk(mu);
