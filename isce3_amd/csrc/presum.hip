@@ -687,7 +687,8 @@ StagingPool& staging_pool()
 // thread takes items of the oldest job that has items left, by the job's
 // atomic counter, alongside the calling thread.  Before it returns, the call
 // retracts its job: no thread starts one of its items afterwards, and the
-// call waits for the threads still inside one.
+// call waits for the threads still inside one.  (A process forked after the
+// pool started has no pool threads: its calls plan on the calling thread.)
 struct PlanJob {
     std::function<void(int64_t, SolveScratch&)> run;
     std::atomic<int64_t>* next = nullptr;
