@@ -297,7 +297,6 @@ __device__ inline PixSplit fit_pixel(const GeomArgs& a, const PixGeom& g, bool h
                                      int kw1, double q_rows, double tau_c,
                                      const double (&gm)[kFitNodes])
 {
-    const V3 x{g.x[0], g.x[1], g.x[2]};
     PixSplit sp;
     sp.p0 = sp.d1 = sp.d2 = sp.d3 = sp.d4 = 0.;
     sp.zoff = 0.f;

@@ -22,7 +22,7 @@
 // column of an LDS array, and reads the windows from there (longer spans
 // read them from global memory).
 //
-// The output pulses go in launches of 512, then 2048: host threads (a pool
+// The output pulses go in launches of 512, 1024, then 2048: host threads (a pool
 // that lives across calls) plan the next launches' patterns and weights
 // while the GPU resamples the earlier ones, and the launches' tables go up
 // on a copy stream beside them.
@@ -35,6 +35,7 @@
 #include <cstring>
 #include <deque>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -210,7 +211,7 @@ constexpr int kMaxWindow = 63;
 constexpr int kThreads = 256;
 constexpr int kPulsesPerThread = 8;    // output pulses per thread
 constexpr int64_t kFirstChunk = 512;   // output pulses of the first launch,
-constexpr int64_t kChunk = 2048;       // of the others (multiples of kPulsesPerThread)
+constexpr int64_t kChunk = 2048;       // at most of the others (multiples of kPulsesPerThread)
 constexpr int64_t kItem = 32;          // output pulses a host thread plans at a time
 constexpr int kTapGroup = 4;           // window taps loaded together (longer windows)
 constexpr int kFixedTaps = 8;          // windows up to this many taps: unrolled per count
@@ -509,8 +510,9 @@ int plan_pulses(const bp::Orbit& ho, const double* t, int64_t nt, const int32_t*
                 int64_t W, double prf, double antenna_length, const bp_lut2d& lut, int64_t i0,
                 int64_t i1, PulseDesc* desc, Patterns& out, SolveScratch& scratch)
 {
-    std::vector<double> sub_t, wsub;
+    std::vector<double> wsub, gfull, rfull;
     std::vector<int64_t> cuts;
+    int idx[kMaxWindow + 1];
     for (int64_t i = i0; i < i1; ++i) {
         const double tout = t[0] + (double)i / prf;
         bp::V3 pos, vel;
@@ -541,6 +543,16 @@ int plan_pulses(const bp::Orbit& ho, const double* t, int64_t nt, const int32_t*
             }
         std::sort(cuts.begin(), cuts.end());
         cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+        // autocorrelations among the window's pulses and with tout, computed
+        // once: each pattern's system is the submatrix of its valid pulses
+        // (the values getPresumWeights forms from those pulses alone)
+        gfull.resize((size_t)nw * nw);
+        rfull.resize(nw);
+        for (int k = 0; k < nw; ++k) {
+            const double xk = t[offset + k];
+            for (int l = k; l < nw; ++l) gfull[(size_t)l * nw + k] = acorr_eval(acor, t[offset + l] - xk);
+            rfull[k] = acorr_eval(acor, tout - xk);
+        }
         for (int64_t c : cuts) {
             int64_t id = 0;
             for (int k = 0; k < nw; ++k)
@@ -554,13 +566,19 @@ int plan_pulses(const bp::Orbit& ho, const double* t, int64_t nt, const int32_t*
             bool seen = false;
             for (int q = 0; q < d.npat; ++q) seen |= out.id[d.pbase + q] == id;
             if (seen) continue;
-            // weights from the valid window pulses only, zeros elsewhere
-            sub_t.clear();
+            // weights from the valid window pulses only, zeros elsewhere (all
+            // of them lie within the half width of tout: the window's do)
+            int m = 0;
             for (int k = 0; k < nw; ++k)
-                if (id >> k & 1) sub_t.push_back(t[offset + k]);
-            int64_t joff = 0;
-            presum_weights(acor, sub_t.data(), (int64_t)sub_t.size(), tout, &joff, wsub, scratch);
-            if (joff != 0 || wsub.size() != sub_t.size()) return BP_ERR_RUNTIME;
+                if (id >> k & 1) idx[m++] = k;
+            scratch.g.assign((size_t)m * m, 0.0);
+            scratch.r.resize(m);
+            for (int a = 0; a < m; ++a) {
+                for (int b = a; b < m; ++b)
+                    scratch.g[(size_t)b * m + a] = gfull[(size_t)idx[b] * nw + idx[a]];
+                scratch.r[a] = rfull[idx[a]];
+            }
+            ldlt_solve(scratch, m, wsub);
             out.id.push_back(id);
             const size_t base = out.w.size();
             out.w.resize(base + nw, 0.f);
@@ -868,15 +886,24 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
 
     bp::Orbit ho{orbit->t0, orbit->dt, orbit->n, orbit->method, orbit->pos, orbit->vel};
     const int64_t L = grid->length, W = grid->width;
-    std::vector<PulseDesc> desc(L);
-    // Output pulses in launches (chunks) of kFirstChunk, then kChunk pulses,
-    // each split into items of kItem pulses.  Host threads -- and this thread
+    // every field of every descriptor is written by the planning or before
+    // its launch's upload: no zero fill of the L descriptors
+    std::unique_ptr<PulseDesc[]> desc_buf(new PulseDesc[L]);
+    PulseDesc* const desc = desc_buf.get();
+    // Output pulses in launches (chunks) of kFirstChunk pulses, then twice as
+    // many each time up to kChunk (planning a chunk takes longer than the
+    // GPU's work on the one before only while the chunks still grow), each
+    // split into items of kItem pulses.  Host threads -- and this thread
     // while it waits for a chunk -- take the items in order, each item with
     // its own pattern tables, while the GPU resamples the chunks already
     // planned.
     std::vector<int64_t> chunk_start{0};
-    for (int64_t c0 = std::min(L, kFirstChunk); c0 < L; c0 = std::min(L, c0 + kChunk))
+    for (int64_t c0 = 0, n = kFirstChunk;;) {
+        c0 = std::min(L, c0 + n);
+        if (c0 >= L) break;
         chunk_start.push_back(c0);
+        n = std::min(2 * n, kChunk);
+    }
     chunk_start.push_back(L);
     const int64_t nchunk = (int64_t)chunk_start.size() - 1;
     struct Item {
@@ -908,7 +935,7 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
         Patterns& pt = item_pat[k];
         pt.status = abort ? BP_ERR_RUNTIME
                           : plan_pulses(ho, t, nt, swaths, ns, W, grid->prf, antenna_length,
-                                        *doppler, it.i0, it.i1, desc.data(), pt, sc);
+                                        *doppler, it.i0, it.i1, desc, pt, sc);
         {
             std::lock_guard<std::mutex> lk(mu);
             ++done[it.chunk];
@@ -1008,7 +1035,7 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
                 return finish(BP_ERR_LENGTH, "Hash algorithm requires number of pulses less than "
                                              "number of bits in identifier (64)");
             if (pt.status != BP_SUCCESS)
-                return finish(BP_ERR_RUNTIME, "presum weights of a mask pattern left its window");
+                return finish(BP_ERR_RUNTIME, "presum planning of an output pulse failed");
             // the chunk's tables: its items' tables one after the other
             const int32_t base = (int32_t)pat_id.size(), wb = (int32_t)pat_w.size();
             for (int64_t i = items[k].i0; i < items[k].i1; ++i) {
@@ -1019,7 +1046,7 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
             pat_w.insert(pat_w.end(), pt.w.begin(), pt.w.end());
         }
         const int64_t c0 = chunk_start[c], rows = chunk_start[c + 1] - c0;
-        const Part parts[3] = {{desc.data() + c0, sizeof(PulseDesc) * rows},
+        const Part parts[3] = {{desc + c0, sizeof(PulseDesc) * rows},
                                {pat_id.data(), sizeof(int64_t) * pat_id.size()},
                                {pat_w.data(), sizeof(float) * pat_w.size()}};
         size_t off[3];
