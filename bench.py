@@ -192,7 +192,8 @@ def host_cpu():
 
 def cpu_threads():
     """OpenMP threads of the CPU baseline: the CPUs this process may run on
-    (nproc), capped at the 16-core CPU share that a one-GPU box gives it."""
+    (nproc), at most 16 (the host cores that go with one GPU of an 8-GPU,
+    128-core node)."""
     try:
         n = len(os.sched_getaffinity(0))
     except (AttributeError, OSError):
