@@ -10,7 +10,8 @@
  * Return codes: values 0..11 mirror isce3::error::ErrorCode
  * (cxx/isce3/error/ErrorCode.h:8-21).  Negative values are argument/runtime
  * errors that a host binding raises as exceptions; bp_last_error_message()
- * returns the message for the calling thread.
+ * returns the message for the calling thread.  No C++ exception leaves an
+ * entry point: a failed host allocation returns BP_ERR_OUT_OF_MEMORY.
  *
  * Threading: every call is synchronous with respect to the host unless a
  * non-null HIP stream is passed, in which case device work is enqueued on

@@ -10,8 +10,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -32,6 +34,30 @@ int fail(int code, const std::string& msg)
 
 // Shared with the range-compression entry points (rc_api.hip).
 void bp_set_last_error(const std::string& msg) { g_msg = msg; }
+
+// Every int-valued entry point ends in a catch clause that returns this, so
+// no C++ exception crosses the C ABI: a failed host allocation gives
+// BP_ERR_OUT_OF_MEMORY, any other exception BP_ERR_RUNTIME, with the message
+// set (or cleared, if even the message cannot be stored).
+int bp_host_exception() noexcept
+{
+    int code = BP_ERR_RUNTIME;
+    try {
+        try {
+            throw;
+        } catch (const std::bad_alloc&) {
+            code = BP_ERR_OUT_OF_MEMORY;
+            g_msg = "host memory allocation failed";
+        } catch (const std::exception& e) {
+            g_msg = std::string("host exception: ") + e.what();
+        } catch (...) {
+            g_msg = "unknown host exception";
+        }
+    } catch (...) {
+        g_msg.clear();
+    }
+    return code;
+}
 
 int bp::usable_cpus()
 {
@@ -464,7 +490,7 @@ int64_t bp_scratch_bytes_per_pixel(int precision)
 }
 
 int bp_context_create(int device, bp_context** ctx)
-{
+try {
     if (ctx == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "ctx is null");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
@@ -493,6 +519,8 @@ int bp_context_create(int device, bp_context** ctx)
     }
     *ctx = c;
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 }  // extern "C"
@@ -533,7 +561,7 @@ void free_host_path(HostPath& h)
 }  // namespace
 
 extern "C" int bp_context_destroy(bp_context* c)
-{
+try {
     if (c == nullptr) return BP_SUCCESS;
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
@@ -544,6 +572,8 @@ extern "C" int bp_context_destroy(bp_context* c)
         if (ev) (void)hipEventDestroy(ev);
     delete c;
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 namespace {
@@ -1420,7 +1450,7 @@ extern "C" int bp_backproject_host(bp_context* c, const bp_radar_geometry* og,
                                    void* out_dev, float* height_dev, int precision,
                                    const bp_options* options, void* stream, int* any_failed,
                                    bp_stats* stats, bp_host_stats* host_stats)
-{
+try {
     if (any_failed) *any_failed = 0;
     if (!og) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     bp_out_block blk;
@@ -1432,6 +1462,8 @@ extern "C" int bp_backproject_host(bp_context* c, const bp_radar_geometry* og,
                in_host, precision};
     return run_host(c, a, &blk, batch, max_window_bytes, options, stream, any_failed, stats,
                     host_stats);
+} catch (...) {
+    return bp_host_exception();
 }
 
 // ------------------------------------------------------------------ scenes
@@ -1591,7 +1623,7 @@ int bp_scene_create(bp_context* c, const bp_orbit* out_orbit, const bp_lut2d* ou
                     const bp_rdr2geo_params* rdr2geo, const bp_geo2rdr_params* geo2rdr,
                     const void* in_dev, int precision, int64_t max_pixels, int max_blocks,
                     bp_scene** scene)
-{
+try {
     if (scene == nullptr || c == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     *scene = nullptr;
     CallArgs a{out_orbit, out_doppler, in_geometry, dem, fc, ds, kernel, dry_tropo_model,
@@ -1651,15 +1683,19 @@ int bp_scene_create(bp_context* c, const bp_orbit* out_orbit, const bp_lut2d* ou
     if (oor) return bail(fail(BP_ERR_OUT_OF_RANGE, orbit_err_string(BP_ORBIT_INTERP_DOMAIN_ERROR)));
     *scene = sc;
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_scene_destroy(bp_scene* sc)
-{
+try {
     if (sc == nullptr) return BP_SUCCESS;
     (void)hipSetDevice(sc->ctx->device);
     (void)hipStreamSynchronize(sc->cap);
     destroy_scene(sc);
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 }  // extern "C"
@@ -1674,7 +1710,7 @@ extern "C" {
 
 int bp_scene_focus(bp_scene* sc, bp_out_block* blocks, int n_blocks, const bp_options* options,
                    void* stream, int* any_failed, bp_stats* stats)
-{
+try {
     if (any_failed) *any_failed = 0;
     if (stats) std::memset(stats, 0, sizeof(*stats));
     if (sc == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
@@ -1719,6 +1755,8 @@ int bp_scene_focus(bp_scene* sc, bp_out_block* blocks, int n_blocks, const bp_op
     BP_HIP_CHECK(hipStreamSynchronize(s));
     const hipEvent_t ev[3] = {sc->ev[0], nullptr, sc->ev[2]};
     return finish_step(sc->h_flags, hdr.n_grids, sc->block_of_grid, blocks, ev, any_failed, stats);
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_geometry* ig,
@@ -1727,7 +1765,7 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
                    const bp_geo2rdr_params* g2r, const void* in_dev, void* out_dev,
                    float* height_dev, int precision, const bp_options* options, void* stream,
                    int* any_failed, bp_stats* stats)
-{
+try {
     if (any_failed) *any_failed = 0;
     if (!og) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     bp_out_block blk;
@@ -1738,6 +1776,8 @@ int bp_backproject(bp_context* c, const bp_radar_geometry* og, const bp_radar_ge
     CallArgs a{&og->orbit, &og->doppler, ig, dem, fc, ds, kernel, dry_tropo_model, r2g, g2r,
                in_dev, precision};
     return run_blocks(c, a, &blk, 1, options, stream, any_failed, stats);
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_backproject_noalloc(bp_context* c, const bp_radar_geometry* og,
@@ -1747,7 +1787,7 @@ int bp_backproject_noalloc(bp_context* c, const bp_radar_geometry* og,
                            const void* in_dev, void* out_dev, float* height_dev, int precision,
                            const bp_options* options, void* stream, int* any_failed,
                            bp_stats* stats)
-{
+try {
     if (any_failed) *any_failed = 0;
     if (!og || !c) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     if (c->scratch == nullptr)
@@ -1762,6 +1802,8 @@ int bp_backproject_noalloc(bp_context* c, const bp_radar_geometry* og,
                in_dev, precision};
     Carve cv{c->scratch, kNoAllocScratchBytes};
     return run_blocks(c, a, &blk, 1, options, stream, any_failed, stats, &cv);
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_backproject_blocks(bp_context* c, const bp_orbit* out_orbit, const bp_lut2d* out_doppler,
@@ -1771,22 +1813,26 @@ int bp_backproject_blocks(bp_context* c, const bp_orbit* out_orbit, const bp_lut
                           const bp_geo2rdr_params* g2r, const void* in_dev, int precision,
                           const bp_options* options, void* stream, int* any_failed,
                           bp_stats* stats)
-{
+try {
     CallArgs a{out_orbit, out_doppler, ig, dem, fc, ds, kernel, dry_tropo_model, r2g, g2r,
                in_dev, precision};
     return run_blocks(c, a, blocks, n_blocks, options, stream, any_failed, stats);
+} catch (...) {
+    return bp_host_exception();
 }
 
 // ------------------------------------------------------------ host helpers
 
 int bp_orbit_interpolate(const bp_orbit* orbit, double t, int border, double* pos, double* vel)
-{
+try {
     if (!orbit || !pos || !vel) return BP_NULL_DEREFERENCE;
     bp::V3 p{bp::qnan(), bp::qnan(), bp::qnan()}, v = p;
     int st = bp::orbit_interpolate(host_orbit(*orbit), t, border, &p, &v);
     pos[0] = p.x; pos[1] = p.y; pos[2] = p.z;
     vel[0] = v.x; vel[1] = v.y; vel[2] = v.z;
     return st;
+} catch (...) {
+    return bp_host_exception();
 }
 
 void bp_lonlat_to_xyz(double a, double e2, const double* llh, double* xyz)
@@ -1812,7 +1858,7 @@ double bp_dem_interpolate_xy(const bp_dem* dem, double x, double y)
 }
 
 int bp_proj_forward(int epsg, const double* llh, double* xyz)
-{
+try {
     bp::Proj p;
     if (llh == nullptr || xyz == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     if (!bp::proj_init(epsg, &p))
@@ -1821,10 +1867,12 @@ int bp_proj_forward(int epsg, const double* llh, double* xyz)
     if (!bp::proj_forward(p, bp::ld3(llh), &out)) return 1;
     xyz[0] = out.x; xyz[1] = out.y; xyz[2] = out.z;
     return 0;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_proj_inverse(int epsg, const double* xyz, double* llh)
-{
+try {
     bp::Proj p;
     if (llh == nullptr || xyz == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     if (!bp::proj_init(epsg, &p))
@@ -1833,6 +1881,8 @@ int bp_proj_inverse(int epsg, const double* xyz, double* llh)
     if (!bp::proj_inverse(p, bp::ld3(xyz), &out)) return 1;
     llh[0] = out.x; llh[1] = out.y; llh[2] = out.z;
     return 0;
+} catch (...) {
+    return bp_host_exception();
 }
 
 double bp_lut2d_eval(const bp_lut2d* lut, double y, double x)
@@ -1854,7 +1904,7 @@ double bp_knab_eval(double width, double bandwidth, double x)
 }
 
 int bp_tabulate_knab(double width, double bandwidth, int n, float* table)
-{
+try {
     if (n < 2 || table == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "Require table size >= 2.");
     if (!(bandwidth > 0.0 && bandwidth < 1.0))
         return fail(BP_ERR_RUNTIME, "Require 0 < bandwidth < 1");
@@ -1862,24 +1912,28 @@ int bp_tabulate_knab(double width, double bandwidth, int n, float* table)
     const double dx = hw / (n - 1.0);
     for (int i = 0; i < n; ++i) table[i] = (float)bp::knab_eval(i * dx, hw, bandwidth);
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_rdr2geo_bracket(double aztime, double slant_range, double doppler, const bp_orbit* orbit,
                        const bp_dem* dem, double wavelength, int side,
                        const bp_rdr2geo_params* params, double* xyz)
-{
+try {
     bp::V3 x{bp::qnan(), bp::qnan(), bp::qnan()};
     bp::R2GParams p{params->tol_height, params->look_min, params->look_max};
     int st = bp::rdr2geo_bracket(aztime, slant_range, doppler, host_orbit(*orbit), host_dem(*dem),
                                  wavelength, side, p, &x);
     xyz[0] = x.x; xyz[1] = x.y; xyz[2] = x.z;
     return st;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_geo2rdr_bracket(const double* xyz, const bp_orbit* orbit, const bp_lut2d* doppler,
                        double wavelength, int side, const bp_geo2rdr_params* params,
                        double* aztime, double* range)
-{
+try {
     bp::G2RParams p;
     p.tol_aztime = params->tol_aztime;
     p.have_time_start = params->have_time_start;
@@ -1888,6 +1942,8 @@ int bp_geo2rdr_bracket(const double* xyz, const bp_orbit* orbit, const bp_lut2d*
     p.time_end = params->time_end;
     return bp::geo2rdr_bracket(bp::V3{xyz[0], xyz[1], xyz[2]}, host_orbit(*orbit),
                                host_lut(*doppler), wavelength, side, p, aztime, range);
+} catch (...) {
+    return bp_host_exception();
 }
 
 }  // extern "C"

@@ -23,6 +23,7 @@
 #include "bp_internal.h"
 
 void bp_set_last_error(const std::string& msg);
+int bp_host_exception() noexcept;
 
 namespace {
 
@@ -107,7 +108,7 @@ extern "C" {
 
 int bp_server_create(bp_scene* scene, const bp_options* options, int max_pending,
                      bp_server** server)
-{
+try {
     if (server == nullptr || scene == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     *server = nullptr;
     if (max_pending < 1) return fail(BP_ERR_INVALID_ARGUMENT, "max_pending must be > 0");
@@ -132,10 +133,12 @@ int bp_server_create(bp_scene* scene, const bp_options* options, int max_pending
     }
     *server = s;
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_server_destroy(bp_server* s)
-{
+try {
     if (s == nullptr) return BP_SUCCESS;
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -154,10 +157,12 @@ int bp_server_destroy(bp_server* s)
     (void)hipStreamDestroy(s->stream);
     delete s;
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_serve_submit(bp_server* s, const bp_out_block* block, uint64_t* ticket)
-{
+try {
     if (s == nullptr || block == nullptr || ticket == nullptr)
         return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     // the per-block rules of the batched call, checked here so that a bad
@@ -180,10 +185,12 @@ int bp_serve_submit(bp_server* s, const bp_out_block* block, uint64_t* ticket)
     }
     s->work_cv.notify_one();
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_serve_poll(bp_server* s, uint64_t ticket, int* done, int* failed)
-{
+try {
     if (s == nullptr || done == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->results.find(ticket);
@@ -196,10 +203,12 @@ int bp_serve_poll(bp_server* s, uint64_t ticket, int* done, int* failed)
     s->results.erase(it);
     if (code < 0) return fail(code, "focusing the request's batch failed: " + msg);
     return code;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_serve_wait(bp_server* s, uint64_t ticket, int* failed)
-{
+try {
     if (s == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->results.find(ticket);
@@ -211,7 +220,9 @@ int bp_serve_wait(bp_server* s, uint64_t ticket, int* failed)
         auto f = s->results.find(ticket);
         return f == s->results.end() || f->second.done;
     });
-    --s->waiters;
+    // destroy, woken here, still needs the mutex this thread holds until it
+    // no longer touches the server
+    if (--s->waiters == 0) s->left_cv.notify_all();
     it = s->results.find(ticket);
     int code = BP_ERR_INVALID_ARGUMENT;
     std::string msg = "unknown ticket (reported to another caller)";
@@ -221,10 +232,11 @@ int bp_serve_wait(bp_server* s, uint64_t ticket, int* failed)
         msg = "focusing the request's batch failed: " + it->second.msg;
         s->results.erase(it);
     }
-    if (s->waiters == 0) s->left_cv.notify_all();
     lk.unlock();
     if (code < 0) return fail(code, msg);
     return code;
+} catch (...) {
+    return bp_host_exception();
 }
 
 }  // extern "C"

@@ -37,6 +37,7 @@
 #include <functional>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -46,6 +47,7 @@
 #include "bp_math.h"
 
 void bp_set_last_error(const std::string& msg);
+int bp_host_exception() noexcept;
 namespace bp {
 int usable_cpus();  // bp_api.hip
 }
@@ -813,7 +815,7 @@ double bp_acorr_eval(const bp_acorr* acorr, double x)
 
 int bp_presum_weights(const bp_acorr* acorr, const double* t, int64_t n, double tout,
                       int64_t* offset, double* w, int64_t max_w, int64_t* nw)
-{
+try {
     using presum::fail;
     if (acorr == nullptr || offset == nullptr || nw == nullptr || (n > 0 && t == nullptr))
         return fail(BP_ERR_INVALID_ARGUMENT, "no storage provided for index offset");
@@ -826,12 +828,14 @@ int bp_presum_weights(const bp_acorr* acorr, const double* t, int64_t n, double 
     if (*nw > max_w) return fail(BP_ERR_LENGTH, "weight buffer too short");
     if (*nw > 0) std::memcpy(w, ww.data(), sizeof(double) * ww.size());
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_presum_apply_weights(void* out, const double* pulse_times, const double* doppler,
                             const double* weights, const void* pulses, int64_t m, int64_t n,
                             void* stream)
-{
+try {
     using presum::fail;
     if (m < 0 || n < 0) return fail(BP_ERR_LENGTH, "negative dimensions");
     if (n == 0) return BP_SUCCESS;
@@ -846,10 +850,12 @@ int bp_presum_apply_weights(void* out, const double* pulse_times, const double* 
     if (e == hipSuccess && s == nullptr) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("apply_presum_weights: ") + hipGetErrorString(e));
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_presum_compute_ids(const uint8_t* mask, int64_t nr, int64_t np, int64_t* ids)
-{
+try {
     using presum::fail;
     if (np > 63)
         return fail(BP_ERR_LENGTH, "Hash algorithm requires number of pulses less than number "
@@ -862,13 +868,15 @@ int bp_presum_compute_ids(const uint8_t* mask, int64_t nr, int64_t np, int64_t* 
         ids[i] = id;
     }
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* swaths,
                        int64_t ns, const bp_grid* grid, const bp_orbit* orbit,
                        const bp_lut2d* doppler, double antenna_length, const void* raw,
                        void* out, void* stream)
-{
+try {
     using namespace presum;
     if (!grid || !orbit || !doppler || !t || (ns > 0 && !swaths) || nt < 1)
         return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
@@ -933,28 +941,25 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
     auto run_item = [&](int64_t k, SolveScratch& sc) {
         const Item& it = items[k];
         Patterns& pt = item_pat[k];
-        pt.status = abort ? BP_ERR_RUNTIME
-                          : plan_pulses(ho, t, nt, swaths, ns, W, grid->prf, antenna_length,
-                                        *doppler, it.i0, it.i1, desc, pt, sc);
+        // an item always counts as done (an exception, on a pool thread too,
+        // becomes its status), so no waiter is left waiting for it
+        try {
+            pt.status = abort ? BP_ERR_RUNTIME
+                              : plan_pulses(ho, t, nt, swaths, ns, W, grid->prf, antenna_length,
+                                            *doppler, it.i0, it.i1, desc, pt, sc);
+        } catch (const std::bad_alloc&) {
+            pt.status = BP_ERR_OUT_OF_MEMORY;
+        } catch (...) {
+            pt.status = BP_ERR_RUNTIME;
+        }
         {
             std::lock_guard<std::mutex> lk(mu);
             ++done[it.chunk];
         }
         cv.notify_all();
     };
-    PlanPool& planners = plan_pool();
-    PlanJob job;
-    job.run = run_item;
-    job.next = &next;
-    job.nitems = nitems;
-    if (nworkers > 0) {
-        planners.ensure(nworkers);
-        planners.post(&job);
-    }
-
-    // every failure below stops the planners and waits for the launches
-    // already made; the staging buffers go back to the pool after the wait
-    // (destructors), and an exception still retracts the job
+    // the job is retracted (no pool thread inside it) before anything it
+    // refers to on this stack goes away, an exception included
     struct Retractor {
         PlanPool& pool;
         PlanJob& job;
@@ -965,12 +970,23 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
             pool.retract(&job);
         }
     };
+    PlanPool& planners = plan_pool();
+    PlanJob job;
+    job.run = run_item;
+    job.next = &next;
+    job.nitems = nitems;
+    if (nworkers > 0) planners.ensure(nworkers);
+    Retractor retractor{planners, job, abort};
+    if (nworkers > 0) planners.post(&job);
+
+    // every failure below stops the planners and waits for the launches
+    // already made; the staging buffers go back to the pool after the wait
+    // (destructors)
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool device_set = hipSetDevice(device) == hipSuccess;
     hipStream_t copy = device_set ? staging_pool().copy_stream(device) : nullptr;
     if (copy == nullptr) copy = s;
     Leases leases{s, copy, {}};
-    Retractor retractor{planners, job, abort};
     auto finish = [&](int code, const std::string& msg) {
         abort = true;
         planners.retract(&job);
@@ -1031,6 +1047,8 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
             const Patterns& pt = item_pat[k];
             if (pt.status == BP_ERR_OUT_OF_RANGE)
                 return finish(BP_ERR_OUT_OF_RANGE, "output pulse time outside the orbit");
+            if (pt.status == BP_ERR_OUT_OF_MEMORY)
+                return finish(BP_ERR_OUT_OF_MEMORY, "host memory allocation failed in presum planning");
             if (pt.status == BP_ERR_LENGTH)
                 return finish(BP_ERR_LENGTH, "Hash algorithm requires number of pulses less than "
                                              "number of bits in identifier (64)");
@@ -1080,6 +1098,8 @@ int bp_presum_resample(int device, const double* t, int64_t nt, const int32_t* s
     }
     // the staging buffers go back to the pool on return, after the kernels
     return finish(BP_SUCCESS, "");
+} catch (...) {
+    return bp_host_exception();
 }
 
 }  // extern "C"

@@ -22,6 +22,7 @@
 
 // Message of the last failing call of this thread (shared with bp_api.hip).
 void bp_set_last_error(const std::string& msg);
+int bp_host_exception() noexcept;
 
 namespace {
 
@@ -170,7 +171,7 @@ extern "C" {
 
 int bp_rangecomp_create(int device, const float* chirp, int chirp_size, int input_size,
                         int max_batch, int mode, bp_rangecomp** rc)
-{
+try {
     if (rc == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     *rc = nullptr;
     if (chirp_size < 1 || chirp == nullptr)
@@ -252,10 +253,12 @@ int bp_rangecomp_create(int device, const float* chirp, int chirp_size, int inpu
     }
     *rc = r;
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_rangecomp_destroy(bp_rangecomp* rc)
-{
+try {
     if (rc == nullptr) return BP_SUCCESS;
     (void)hipSetDevice(rc->device);
     if (rc->H) (void)hipFree(rc->H);
@@ -264,10 +267,12 @@ int bp_rangecomp_destroy(bp_rangecomp* rc)
     if (rc->work) (void)hipFree(rc->work);
     delete rc;
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_rangecomp_sizes(const bp_rangecomp* rc, int32_t* sizes)
-{
+try {
     if (rc == nullptr || sizes == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     sizes[0] = rc->chirp_size;
     sizes[1] = rc->input_size;
@@ -277,20 +282,24 @@ int bp_rangecomp_sizes(const bp_rangecomp* rc, int32_t* sizes)
     sizes[5] = rc->out_size;
     sizes[6] = first_valid(rc->chirp_size, rc->mode);
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_rangecomp_plan(const bp_rangecomp* rc, int32_t* plan)
-{
+try {
     if (rc == nullptr || plan == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     plan[0] = rc->method;
     plan[1] = 1 << rc->log2l;
     plan[2] = rc->method == 2 ? 1 : rc->nblk;
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_rangecomp_run(bp_rangecomp* rc, void* out_dev, const void* in_dev, int batch,
                      void* stream)
-{
+try {
     if (rc == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     if (batch > rc->max_batch) return fail(BP_ERR_LENGTH, "batch size exceeds max batch");
     if (batch < 0) return fail(BP_ERR_INVALID_ARGUMENT, "negative batch size");
@@ -335,6 +344,8 @@ int bp_rangecomp_run(bp_rangecomp* rc, void* out_dev, const void* in_dev, int ba
     if (e == hipSuccess && s == nullptr) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return fail(BP_ERR_HIP, std::string("range compression launch: ") + hipGetErrorString(e));
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 }  // extern "C"

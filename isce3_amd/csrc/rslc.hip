@@ -26,6 +26,7 @@
 #include "../../include/isce3_amd_rslc.h"
 
 void bp_set_last_error(const std::string& msg);
+int bp_host_exception() noexcept;
 
 namespace rslc {
 namespace {
@@ -317,7 +318,7 @@ struct bp_rslc_finisher {
 extern "C" {
 
 int bp_rslc_finisher_create(int device, int64_t max_pixels, bp_rslc_finisher** f)
-{
+try {
     using rslc::fail;
     if (f == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     *f = nullptr;
@@ -342,10 +343,12 @@ int bp_rslc_finisher_create(int device, int64_t max_pixels, bp_rslc_finisher** f
     }
     *f = r;
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_rslc_finisher_destroy(bp_rslc_finisher* f)
-{
+try {
     if (f == nullptr) return BP_SUCCESS;
     (void)hipSetDevice(f->device);
     if (f->partials) (void)hipFree(f->partials);
@@ -353,13 +356,15 @@ int bp_rslc_finisher_destroy(bp_rslc_finisher* f)
     if (f->h_stats) (void)hipHostFree(f->h_stats);
     delete f;
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 int bp_rslc_finish_block(bp_rslc_finisher* f, void* z, int64_t rows, int64_t cols,
                          const void* range_cor, const void* prior, int data_type,
                          int mantissa_nbits, void* encoded, bp_stats_real_imag_f32* stats,
                          void* stream)
-{
+try {
     using rslc::fail;
     if (f == nullptr || stats == nullptr) return fail(BP_ERR_INVALID_ARGUMENT, "null argument");
     if (rows < 0 || cols < 0) return fail(BP_ERR_INVALID_ARGUMENT, "negative block shape");
@@ -425,6 +430,8 @@ int bp_rslc_finish_block(bp_rslc_finisher* f, void* z, int64_t rows, int64_t col
         return fail(BP_ERR_HIP, std::string("RSLC block finishing: ") + hipGetErrorString(e));
     *stats = *f->h_stats;
     return BP_SUCCESS;
+} catch (...) {
+    return bp_host_exception();
 }
 
 }  // extern "C"

@@ -117,3 +117,16 @@ def test_oracle_apply_weights_formula():
     j = 3
     want = sum(w[i, j] * np.exp(-2j * np.pi * t[i] * fd[j]) * x[i, j] for i in range(5))
     assert abs(out[j] - want) < 1e-12
+
+
+def test_host_allocation_failure_is_an_error_not_an_abort():
+    """A window of 2^23 samples asks the weight solve for a 2^23 x 2^23
+    system (512 TiB, beyond any process's address space): the C-ABI call
+    returns BP_ERR_OUT_OF_MEMORY (MemoryError here) instead of letting the
+    C++ exception end the process, and the next call works."""
+    t = np.arange(1 << 23) * 1e-3
+    with pytest.raises(MemoryError, match="host memory allocation failed"):
+        get_presum_weights(core.BartlettKernel(1e5), t, 10.0)
+    off, w = get_presum_weights(core.BartlettKernel(2.5e-3), t[:40], 10e-3)
+    assert off == 9 and len(w) == 3
+    npt.assert_allclose(w, [0.0, 1.0, 0.0], atol=1e-12)
