@@ -604,7 +604,9 @@ __device__ inline void plain_tile(const SumArgs& a, const double2* tab2, const O
         }
         float sre = 0.f, sim = 0.f;
         if (low >= 0. && low + (double)W <= nr_d) {
-            const float2* p = a.rc + (int64_t)k * a.nr + (int64_t)low;
+            // 0 <= low < nr, an int: one conversion (a double-to-int64 one
+            // takes six instructions)
+            const float2* p = a.rc + (int64_t)k * a.nr + (int)low;
 #pragma unroll
             for (int m = 0; m < W; ++m) {
                 const float2 d = p[m];
@@ -616,7 +618,7 @@ __device__ inline void plain_tile(const SumArgs& a, const double2* tab2, const O
 #pragma unroll
             for (int m = 0; m < W; ++m) {
                 const double j = low + (double)m;
-                const float2 d = (j >= 0. && j < nr_d) ? line[(int64_t)j] : make_float2(0.f, 0.f);
+                const float2 d = (j >= 0. && j < nr_d) ? line[(int)j] : make_float2(0.f, 0.f);
                 sre = fmaf(w[m], d.x, sre);
                 sim = fmaf(w[m], d.y, sim);
             }

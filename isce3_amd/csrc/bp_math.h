@@ -1007,15 +1007,22 @@ BP_HD double sqrt_normal(double x)
 #endif
 }
 
-// sin(2 pi t) and cos(2 pi t) in FP64 for a phase t in turns.  t is reduced
-// to r = t - q/4 with q the nearest integer to 4t (exact for |t| < 2^50), so
-// that |2 pi r| <= pi/4; sin and cos of x = 2 pi r come from their Taylor
-// series through x^17 and x^18 (truncation below 1e-16 on that interval), and
-// the quadrant q mod 4 rotates the pair.  Error about 2 ulp; no table, no
-// division.
+// sin(2 pi t) and cos(2 pi t) in FP64 for a phase t in turns, |t| < 2^49
+// (where a double still resolves 1/16 turn; a phase fc tau is below 1e12).
+// t is reduced to r = t - q/4 with q the nearest integer to 4t (ties to
+// even), so that |2 pi r| <= pi/4; sin and cos of x = 2 pi r come from their
+// Taylor series through x^17 and x^18 (truncation below 1e-16 on that
+// interval), and the quadrant q mod 4 rotates the pair.  q comes from one
+// FMA: 4t + 1.5 * 2^52 lies in [2^52, 2^53), where doubles are the integers,
+// so the sum rounds 4t to the nearest integer (ties to even, as 1.5 * 2^52 is
+// even), subtracting 1.5 * 2^52 again is exact, and the low bits of the sum's
+// mantissa are those of q + 2^51, i.e. q mod 4.  Error about 2 ulp; no table,
+// no division.
 BP_HD void sincos_turns(double t, double* s, double* c)
 {
-    const double q = rint(4.0 * t);
+    constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
+    const double tq = fma(4.0, t, kRound);
+    const double q = tq - kRound;
     const double r = fma(-0.25, q, t);
     const double x = r * 6.283185307179586476925286766559;
     const double x2 = x * x;
@@ -1040,8 +1047,8 @@ BP_HD void sincos_turns(double t, double* s, double* c)
     pc = fma_untied(pc, x2, -4.1666666666666666667e-02);   //  1/4!
     pc = fma(pc, x2, 0.5);                                  // (sign folded below)
     const double cx = fma(-x2, pc, 1.0);
-    // quadrant: (sin, cos) of x + q pi/2, q mod 4 formed exactly in FP64
-    const int iq = (int)fma(-4.0, floor(0.25 * q), q);
+    // quadrant: (sin, cos) of x + q pi/2, q mod 4 from the sum's low bits
+    const int iq = (int)(__builtin_bit_cast(unsigned long long, tq) & 3u);
     const double sa = (iq & 1) ? cx : sx;
     const double ca = (iq & 1) ? sx : cx;
     *s = (iq & 2) ? -sa : sa;
