@@ -387,9 +387,14 @@ BP_HD V3 nvector(double lon, double lat)
 }
 
 // Bilinear on a float raster with float weights
-// (core/BilinearInterpolator.cpp:13-47, U = float).
+// (core/BilinearInterpolator.cpp:13-47, U = float).  floor and ceil of a
+// coordinate differ by 0 or exactly 1, so the formula's divisions by x2 - x1,
+// y2 - y1 and their product, where it reaches them, divide by 1 and are left
+// out (the same bits; products are not fused into the sums, as the
+// divisions between them kept them apart).
 BP_HD float bilinear_f32(double x, double y, const float* z, int64_t width)
 {
+#pragma clang fp contract(off)
     int x1 = static_cast<int>(floor(x));
     int x2 = static_cast<int>(ceil(x));
     int y1 = static_cast<int>(floor(y));
@@ -400,21 +405,20 @@ BP_HD float bilinear_f32(double x, double y, const float* z, int64_t width)
     float q22 = z[(int64_t)y2 * width + x2];
     if ((y1 == y2) && (x1 == x2)) return q11;
     if (y1 == y2)
-        return (static_cast<float>((x2 - x) / (x2 - x1)) * q11) +
-               (static_cast<float>((x - x1) / (x2 - x1)) * q21);
+        return (static_cast<float>(x2 - x) * q11) + (static_cast<float>(x - x1) * q21);
     if (x1 == x2)
-        return (static_cast<float>((y2 - y) / (y2 - y1)) * q11) +
-               (static_cast<float>((y - y1) / (y2 - y1)) * q12);
-    const float den = static_cast<float>((x2 - x1) * (y2 - y1));
-    return ((q11 * static_cast<float>((x2 - x) * (y2 - y))) / den) +
-           ((q21 * static_cast<float>((x - x1) * (y2 - y))) / den) +
-           ((q12 * static_cast<float>((x2 - x) * (y - y1))) / den) +
-           ((q22 * static_cast<float>((x - x1) * (y - y1))) / den);
+        return (static_cast<float>(y2 - y) * q11) + (static_cast<float>(y - y1) * q12);
+    return (q11 * static_cast<float>((x2 - x) * (y2 - y))) +
+           (q21 * static_cast<float>((x - x1) * (y2 - y))) +
+           (q12 * static_cast<float>((x2 - x) * (y - y1))) +
+           (q22 * static_cast<float>((x - x1) * (y - y1)));
 }
 
-// Same formula on a double table (LUT2d<double>).
+// Same formula on a double table (LUT2d<double>), the divisions by 1 left
+// out in the same way.
 BP_HD double bilinear_f64(double x, double y, const double* z, int64_t width)
 {
+#pragma clang fp contract(off)
     int x1 = static_cast<int>(floor(x));
     int x2 = static_cast<int>(ceil(x));
     int y1 = static_cast<int>(floor(y));
@@ -424,11 +428,10 @@ BP_HD double bilinear_f64(double x, double y, const double* z, int64_t width)
     double q21 = z[(int64_t)y1 * width + x2];
     double q22 = z[(int64_t)y2 * width + x2];
     if ((y1 == y2) && (x1 == x2)) return q11;
-    if (y1 == y2) return (((x2 - x) / (x2 - x1)) * q11) + (((x - x1) / (x2 - x1)) * q21);
-    if (x1 == x2) return (((y2 - y) / (y2 - y1)) * q11) + (((y - y1) / (y2 - y1)) * q12);
-    const double den = static_cast<double>((x2 - x1) * (y2 - y1));
-    return ((q11 * ((x2 - x) * (y2 - y))) / den) + ((q21 * ((x - x1) * (y2 - y))) / den) +
-           ((q12 * ((x2 - x) * (y - y1))) / den) + ((q22 * ((x - x1) * (y - y1))) / den);
+    if (y1 == y2) return ((x2 - x) * q11) + ((x - x1) * q21);
+    if (x1 == x2) return ((y2 - y) * q11) + ((y - y1) * q12);
+    return (q11 * ((x2 - x) * (y2 - y))) + (q21 * ((x - x1) * (y2 - y))) +
+           (q12 * ((x2 - x) * (y - y1))) + (q22 * ((x - x1) * (y - y1)));
 }
 
 // ------------------------------------------------------------ projections
