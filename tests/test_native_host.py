@@ -219,8 +219,12 @@ def test_sincos_turns_against_long_double():
     magnitudes fc * tau takes (about 7e6 turns at 850 km and L band) and
     across the quadrant boundaries of the reduction."""
     rng = np.random.Generator(np.random.PCG64(5))
+    # the reduction rounds 4t to an integer with one FMA against 1.5 * 2^52,
+    # valid for |t| < 2^49: up to that bound, including ties at odd eighths
+    big = rng.uniform(2.0 ** 40, 2.0 ** 48, 400) * rng.choice([-1.0, 1.0], 400)
+    ties = np.concatenate([2.0 ** 47 + np.arange(-16, 17) / 8.0, -(2.0 ** 46) + np.arange(9) / 8.0])
     t = np.concatenate([rng.uniform(-1, 1, 4000), rng.uniform(7.0e6, 7.3e6, 4000),
-                        np.arange(-8, 9) / 8.0, np.arange(-8, 9) / 8.0 + 1e-12,
+                        np.arange(-8, 9) / 8.0, np.arange(-8, 9) / 8.0 + 1e-12, big, ties,
                         np.array([0.0, 0.125, 0.375, 1e-300, -0.25 + 2 ** -40])])
     s = C.c_double()
     c = C.c_double()
